@@ -1,0 +1,115 @@
+"""ctypes binding of libcrava_hip.so (include/crava_hip.h).
+
+The library is the only compute path: if it is missing, importing this
+module raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIBPATH = Path(os.environ.get("CRAVA_HIP_LIB", _PKG / "libcrava_hip.so"))
+
+CRV_OK = 0
+CRV_FP32, CRV_FP64 = 32, 64
+CRV_CTMISSING, CRV_DATA, CRV_PARAMETER, CRV_COVARIANCE, CRV_VELOCITY, CRV_OPERATOR = range(6)
+CRV_MODE_PLAIN, CRV_MODE_FP32_REFINE = 0, 1
+CRV_NOISE_REFERENCE, CRV_NOISE_COUNTER, CRV_NOISE_HOST = 0, 1, 2
+
+# every symbol declared in include/crava_hip.h
+EXPORTS = [
+    "crv_ctx_create", "crv_ctx_destroy", "crv_last_error", "crv_ctx_synchronize", "crv_ctx_set_stream",
+    "crv_version", "crv_grid_create", "crv_grid_destroy", "crv_grid_info_get", "crv_grid_set_cubetype",
+    "crv_grid_set_transformed", "crv_grid_device_ptr", "crv_grid_upload_real", "crv_grid_upload_complex",
+    "crv_grid_download", "crv_grid_copy", "crv_grid_add", "crv_grid_multiply_by_scalar", "crv_fft_r2c",
+    "crv_fft_c2r", "crv_fft_r2c_from", "crv_fft_c2r_add", "crv_posterior", "crv_simulate",
+    "crv_fill_complex_noise", "crv_noise_draws", "crv_find_closest_factorable", "crv_padding_size",
+]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("nxp", C.c_int), ("nyp", C.c_int), ("nzp", C.c_int),
+                ("cnxp", C.c_int), ("rnxp", C.c_int),
+                ("precision", C.c_int), ("cubetype", C.c_int), ("is_transformed", C.c_int),
+                ("rsize", C.c_size_t), ("csize", C.c_size_t)]
+
+
+_dp = C.POINTER(C.c_double)
+
+
+class PostParams(C.Structure):
+    _fields_ = [("ntheta", C.c_int), ("nz", C.c_int), ("lz", C.c_double), ("min_rel_thick", C.c_double),
+                ("low_cut", C.c_double), ("high_cut", C.c_double), ("wnc", C.c_double),
+                ("A", C.POINTER(C.c_float)), ("err_theta_cov", _dp),
+                ("kwd", _dp), ("em1", _dp), ("em2", _dp), ("em3", _dp),
+                ("mode", C.c_int), ("cond_threshold", C.c_double), ("lowk_fraction", C.c_double),
+                ("separable", C.c_int), ("sigma0", _dp), ("cxy", _dp), ("ct", _dp),
+                ("skip_mean_inverse", C.c_int)]
+
+
+class PostStats(C.Structure):
+    _fields_ = [("n_wavenumbers", C.c_longlong), ("n_inverted", C.c_longlong),
+                ("n_chol_fail", C.c_longlong), ("n_fp64", C.c_longlong), ("max_cond", C.c_double)]
+
+
+class NoiseSpec(C.Structure):
+    _fields_ = [("mode", C.c_int), ("lcg_state", C.POINTER(C.c_uint)), ("seed", C.c_ulonglong),
+                ("realization", C.c_ulonglong), ("host_noise", C.c_void_p * 3)]
+
+
+def load(path: Path | None = None) -> C.CDLL:
+    p = Path(path) if path else LIBPATH
+    if not p.exists():
+        raise RuntimeError(f"libcrava_hip.so not found at {p}: run `python -m crava_amd.build` first")
+    lib = C.CDLL(str(p))
+    vp = C.c_void_p
+    i = C.c_int
+    sig = {
+        "crv_ctx_create": (i, [i, C.POINTER(vp)]),
+        "crv_ctx_destroy": (i, [vp]),
+        "crv_last_error": (C.c_char_p, [vp]),
+        "crv_ctx_synchronize": (i, [vp]),
+        "crv_ctx_set_stream": (i, [vp, vp]),
+        "crv_version": (C.c_char_p, []),
+        "crv_grid_create": (i, [vp, i, i, i, i, i, i, i, i, C.POINTER(vp)]),
+        "crv_grid_destroy": (i, [vp]),
+        "crv_grid_info_get": (i, [vp, C.POINTER(GridInfo)]),
+        "crv_grid_set_cubetype": (i, [vp, i]),
+        "crv_grid_set_transformed": (i, [vp, i]),
+        "crv_grid_device_ptr": (i, [vp, C.POINTER(vp)]),
+        "crv_grid_upload_real": (i, [vp, vp, C.c_size_t]),
+        "crv_grid_upload_complex": (i, [vp, vp, C.c_size_t]),
+        "crv_grid_download": (i, [vp, vp, C.c_size_t]),
+        "crv_grid_copy": (i, [vp, vp]),
+        "crv_grid_add": (i, [vp, vp]),
+        "crv_grid_multiply_by_scalar": (i, [vp, C.c_double]),
+        "crv_fft_r2c": (i, [vp]),
+        "crv_fft_c2r": (i, [vp]),
+        "crv_fft_r2c_from": (i, [vp, vp]),
+        "crv_fft_c2r_add": (i, [vp, vp, vp]),
+        "crv_posterior": (i, [vp, C.POINTER(PostParams), C.POINTER(vp), C.POINTER(vp), vp, C.POINTER(vp),
+                              C.POINTER(PostStats)]),
+        "crv_simulate": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(NoiseSpec), i, C.POINTER(vp)]),
+        "crv_fill_complex_noise": (i, [vp, C.POINTER(C.c_uint)]),
+        "crv_noise_draws": (C.c_longlong, [i, i, i]),
+        "crv_find_closest_factorable": (i, [i]),
+        "crv_padding_size": (i, [i, C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load()
+    return _LIB

@@ -1,0 +1,282 @@
+// C ABI: contexts, grids, transforms and padding helpers.
+#include "internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+int crv_set_error(crv_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+// Scale factors of FFTGrid::fftInPlace / invFFTInPlace (fftgrid.cpp:1796-1797,
+// 1827-1830).  FP32 grids use the reference's float expressions; FP64 grids
+// the exact values.
+double crv_scale_forward(const crv_grid* g) {
+  if (g->cubetype == CRV_COVARIANCE || g->cubetype == CRV_OPERATOR) return 1.0;
+  const long long n = (long long)g->nxp * g->nyp * g->nzp;
+  if (g->precision == CRV_FP64) return 1.0 / std::sqrt((double)n);
+  return (double)(1.0f / std::sqrt((float)n));
+}
+double crv_scale_inverse(const crv_grid* g) {
+  const long long n = (long long)g->nxp * g->nyp * g->nzp;
+  if (g->cubetype == CRV_COVARIANCE || g->cubetype == CRV_OPERATOR)
+    return g->precision == CRV_FP64 ? 1.0 / (double)n : (double)(float)(1.0 / (double)n);
+  if (g->precision == CRV_FP64) return 1.0 / std::sqrt((double)n);
+  return (double)(float)(1.0 / (double)std::sqrt((float)n));
+}
+
+extern "C" {
+
+const char* crv_version(void) { return "crava_hip 1.0 gfx950"; }
+
+int crv_ctx_create(int device, crv_ctx** out) {
+  if (!out) return CRV_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return CRV_E_HIP;
+  if (hipSetDevice(device) != hipSuccess) return CRV_E_HIP;
+  crv_ctx* c = new (std::nothrow) crv_ctx();
+  if (!c) return CRV_E_NOMEM;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CRV_E_HIP; }
+  c->own_stream = true;
+  *out = c;
+  return CRV_OK;
+}
+
+int crv_ctx_destroy(crv_ctx* ctx) {
+  if (!ctx) return CRV_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->plans) crv_free_plan(kv.second);
+  if (ctx->lcg_scratch) (void)hipFree(ctx->lcg_scratch);
+  if (ctx->normals) (void)hipFree(ctx->normals);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CRV_OK;
+}
+
+const char* crv_last_error(const crv_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "NULL context"; }
+
+int crv_ctx_synchronize(crv_ctx* ctx) {
+  if (!ctx) return CRV_E_INVALID;
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return CRV_OK;
+}
+
+int crv_ctx_set_stream(crv_ctx* ctx, void* s) {
+  if (!ctx) return CRV_E_INVALID;
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (s) {
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    ctx->stream = (hipStream_t)s;
+    ctx->own_stream = false;
+  } else if (!ctx->own_stream) {
+    CRV_HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  return CRV_OK;
+}
+
+int crv_grid_create(crv_ctx* ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int precision, int cubetype,
+                    crv_grid** out) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!out) return crv_set_error(ctx, CRV_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (nxp < 1 || nyp < 1 || nzp < 1 || nx < 0 || ny < 0 || nz < 0 || nx > nxp || ny > nyp || nz > nzp)
+    return crv_set_error(ctx, CRV_E_INVALID, "bad grid dimensions");
+  if (precision != CRV_FP32 && precision != CRV_FP64) return crv_set_error(ctx, CRV_E_INVALID, "bad precision");
+  if (cubetype < CRV_CTMISSING || cubetype > CRV_OPERATOR) return crv_set_error(ctx, CRV_E_INVALID, "bad cubetype");
+  FftPlan* plan = nullptr;
+  int rc = crv_get_plan(ctx, nxp, nyp, nzp, precision, &plan);
+  if (rc != CRV_OK) return rc;
+  crv_grid* g = new (std::nothrow) crv_grid();
+  if (!g) return crv_set_error(ctx, CRV_E_NOMEM, "host allocation");
+  g->ctx = ctx;
+  g->nx = nx; g->ny = ny; g->nz = nz; g->nxp = nxp; g->nyp = nyp; g->nzp = nzp;
+  g->cnxp = nxp / 2 + 1; g->rnxp = 2 * g->cnxp;
+  g->precision = precision; g->cubetype = cubetype; g->transformed = false;
+  g->rsize = (size_t)g->rnxp * nyp * nzp;
+  g->csize = (size_t)g->cnxp * nyp * nzp;
+  const hipError_t e = hipMalloc(&g->d, g->rsize * g->elem_bytes());
+  if (e != hipSuccess) {
+    delete g;
+    return crv_set_error(ctx, CRV_E_NOMEM, std::string("grid allocation: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return CRV_OK;
+}
+
+int crv_grid_destroy(crv_grid* g) {
+  if (!g) return CRV_E_INVALID;
+  if (g->d) {
+    (void)hipStreamSynchronize(g->ctx->stream);
+    (void)hipFree(g->d);
+  }
+  delete g;
+  return CRV_OK;
+}
+
+int crv_grid_info_get(const crv_grid* g, crv_grid_info* info) {
+  if (!g || !info) return CRV_E_INVALID;
+  info->nx = g->nx; info->ny = g->ny; info->nz = g->nz;
+  info->nxp = g->nxp; info->nyp = g->nyp; info->nzp = g->nzp;
+  info->cnxp = g->cnxp; info->rnxp = g->rnxp;
+  info->precision = g->precision; info->cubetype = g->cubetype;
+  info->is_transformed = g->transformed ? 1 : 0;
+  info->rsize = g->rsize; info->csize = g->csize;
+  return CRV_OK;
+}
+
+int crv_grid_set_cubetype(crv_grid* g, int cubetype) {
+  if (!g) return CRV_E_INVALID;
+  if (cubetype < CRV_CTMISSING || cubetype > CRV_OPERATOR) return crv_set_error(g->ctx, CRV_E_INVALID, "bad cubetype");
+  g->cubetype = cubetype;
+  return CRV_OK;
+}
+
+int crv_grid_set_transformed(crv_grid* g, int t) {
+  if (!g) return CRV_E_INVALID;
+  g->transformed = t != 0;
+  return CRV_OK;
+}
+
+int crv_grid_device_ptr(const crv_grid* g, void** dptr) {
+  if (!g || !dptr) return CRV_E_INVALID;
+  *dptr = g->d;
+  return CRV_OK;
+}
+
+int crv_grid_upload_real(crv_grid* g, const void* host, size_t nelem) {
+  if (!g || !host) return CRV_E_INVALID;
+  if (nelem != g->rsize) return crv_set_error(g->ctx, CRV_E_INVALID, "upload_real: nelem != rsize");
+  CRV_HIP_CHECK(g->ctx, hipMemcpyAsync(g->d, host, nelem * g->elem_bytes(), hipMemcpyHostToDevice, g->ctx->stream));
+  CRV_HIP_CHECK(g->ctx, hipStreamSynchronize(g->ctx->stream));
+  g->transformed = false;
+  return CRV_OK;
+}
+
+int crv_grid_upload_complex(crv_grid* g, const void* host, size_t nelem) {
+  if (!g || !host) return CRV_E_INVALID;
+  if (nelem != 2 * g->csize) return crv_set_error(g->ctx, CRV_E_INVALID, "upload_complex: nelem != 2*csize");
+  CRV_HIP_CHECK(g->ctx, hipMemcpyAsync(g->d, host, nelem * g->elem_bytes(), hipMemcpyHostToDevice, g->ctx->stream));
+  CRV_HIP_CHECK(g->ctx, hipStreamSynchronize(g->ctx->stream));
+  g->transformed = true;
+  return CRV_OK;
+}
+
+int crv_grid_download(crv_grid* g, void* host, size_t nelem) {
+  if (!g || !host) return CRV_E_INVALID;
+  if (nelem != g->rsize && nelem != 2 * g->csize)
+    return crv_set_error(g->ctx, CRV_E_INVALID, "download: nelem must be rsize or 2*csize");
+  CRV_HIP_CHECK(g->ctx, hipMemcpyAsync(host, g->d, nelem * g->elem_bytes(), hipMemcpyDeviceToHost, g->ctx->stream));
+  CRV_HIP_CHECK(g->ctx, hipStreamSynchronize(g->ctx->stream));
+  return CRV_OK;
+}
+
+int crv_grid_copy(crv_grid* dst, const crv_grid* src) {
+  if (!dst || !src) return CRV_E_INVALID;
+  if (dst->nxp != src->nxp || dst->nyp != src->nyp || dst->nzp != src->nzp || dst->precision != src->precision)
+    return crv_set_error(dst->ctx, CRV_E_INVALID, "copy: size/precision mismatch");
+  CRV_HIP_CHECK(dst->ctx, hipMemcpyAsync(dst->d, src->d, src->rsize * src->elem_bytes(), hipMemcpyDeviceToDevice,
+                                         dst->ctx->stream));
+  dst->transformed = src->transformed;
+  dst->cubetype = src->cubetype;
+  return CRV_OK;
+}
+
+int crv_grid_add(crv_grid* dst, const crv_grid* src) {
+  if (!dst || !src) return CRV_E_INVALID;
+  if (dst->nxp != src->nxp || dst->nyp != src->nyp || dst->nzp != src->nzp || dst->precision != src->precision)
+    return crv_set_error(dst->ctx, CRV_E_INVALID, "add: size/precision mismatch");
+  // FFTGrid::add adds over csize complex (transformed) or rsize reals; both
+  // cover the same buffer.
+  const size_t n = dst->transformed ? 2 * dst->csize : dst->rsize;
+  return crv_launch_axpy(dst->ctx, dst->precision, dst->d, src->d, n);
+}
+
+int crv_grid_multiply_by_scalar(crv_grid* g, double s) {
+  if (!g) return CRV_E_INVALID;
+  return crv_launch_scale(g->ctx, g->precision, g->d, s, g->rsize);
+}
+
+int crv_fft_r2c(crv_grid* g) {
+  if (!g) return CRV_E_INVALID;
+  return crv_fft_r2c_from(g, g);
+}
+
+int crv_fft_r2c_from(crv_grid* dst, const crv_grid* src) {
+  if (!dst || !src) return CRV_E_INVALID;
+  crv_ctx* ctx = dst->ctx;
+  if (src->transformed) return crv_set_error(ctx, CRV_E_STATE, "fftInPlace: grid already transformed");
+  if (src->cubetype == CRV_CTMISSING) return crv_set_error(ctx, CRV_E_STATE, "fftInPlace: cubetype missing");
+  if (dst->nxp != src->nxp || dst->nyp != src->nyp || dst->nzp != src->nzp || dst->precision != src->precision)
+    return crv_set_error(ctx, CRV_E_INVALID, "r2c: size/precision mismatch");
+  FftPlan* plan = nullptr;
+  int rc = crv_get_plan(ctx, src->nxp, src->nyp, src->nzp, src->precision, &plan);
+  if (rc != CRV_OK) return rc;
+  rc = crv_launch_r2c(ctx, plan, dst->d, src->d, crv_scale_forward(src));
+  if (rc != CRV_OK) return rc;
+  dst->cubetype = src->cubetype;
+  dst->transformed = true;
+  return CRV_OK;
+}
+
+int crv_fft_c2r(crv_grid* g) {
+  if (!g) return CRV_E_INVALID;
+  return crv_fft_c2r_add(g, g, nullptr);
+}
+
+int crv_fft_c2r_add(crv_grid* dst, crv_grid* src, const crv_grid* add) {
+  if (!dst || !src) return CRV_E_INVALID;
+  crv_ctx* ctx = dst->ctx;
+  if (!src->transformed) return crv_set_error(ctx, CRV_E_STATE, "invFFTInPlace: grid not transformed");
+  if (src->cubetype == CRV_CTMISSING) return crv_set_error(ctx, CRV_E_STATE, "invFFTInPlace: cubetype missing");
+  if (dst->nxp != src->nxp || dst->nyp != src->nyp || dst->nzp != src->nzp || dst->precision != src->precision)
+    return crv_set_error(ctx, CRV_E_INVALID, "c2r: size/precision mismatch");
+  if (add) {
+    if (add->nxp != src->nxp || add->nyp != src->nyp || add->nzp != src->nzp || add->precision != src->precision)
+      return crv_set_error(ctx, CRV_E_INVALID, "c2r: add grid size/precision mismatch");
+    if (add->transformed) return crv_set_error(ctx, CRV_E_STATE, "c2r: add grid must be real");
+  }
+  FftPlan* plan = nullptr;
+  int rc = crv_get_plan(ctx, src->nxp, src->nyp, src->nzp, src->precision, &plan);
+  if (rc != CRV_OK) return rc;
+  rc = crv_launch_c2r(ctx, plan, src->d, dst->d, crv_scale_inverse(src), add ? add->d : nullptr);
+  if (rc != CRV_OK) return rc;
+  dst->cubetype = src->cubetype;
+  dst->transformed = false;
+  return CRV_OK;
+}
+
+// FFTGrid::findClosestFactorableNumber (fftgrid.cpp:2857-2885): the bounds
+// are the reference's float-log expressions; the search is exact.
+int crv_find_closest_factorable(int leastint) {
+  if (leastint < 1) return -1;
+  const int maxant2 = (int)std::ceil((double)std::log((float)leastint) / std::log(2.0f));
+  const int maxant3 = (int)std::ceil((double)std::log((float)leastint) / std::log(3.0f));
+  const int maxant5 = (int)std::ceil((double)std::log((float)leastint) / std::log(5.0f));
+  const int maxant7 = (int)std::ceil((double)std::log((float)leastint) / std::log(7.0f));
+  int closest = (int)std::pow(2.0, maxant2);
+  // products are exact integers in double; the reference's int cast of a
+  // product beyond INT_MAX yields a negative number, which never qualifies
+  for (int i = 0; i <= maxant2; ++i)
+    for (int j = 0; j <= maxant3; ++j)
+      for (int k = 0; k <= maxant5; ++k)
+        for (int l = 0; l <= maxant7; ++l) {
+          const double f = std::pow(2.0, i) * std::pow(3.0, j) * std::pow(5.0, k) * std::pow(7.0, l);
+          if (f > 2147483647.0) continue;
+          if ((int)f >= leastint && (int)f < closest) closest = (int)f;
+        }
+  return closest;
+}
+
+// ModelGeneral::setPaddingSize (modelgeneral.cpp:616-622)
+int crv_padding_size(int n, double pad_fac) {
+  const int leastint = (int)std::ceil(n * (1.0f + pad_fac));
+  return crv_find_closest_factorable(leastint);
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Internal declarations shared by the HIP translation units of libcrava_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/crava_hip.h"
+
+// ---------------------------------------------------------------------------
+// complex helpers (float2 / double2 as interleaved re, im)
+template <typename T> struct cplx_of;
+template <> struct cplx_of<float>  { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+template <typename T> using cplx_t = typename cplx_of<T>::type;
+
+template <typename C> __host__ __device__ inline C cmake(decltype(C::x) a, decltype(C::x) b) { C r; r.x = a; r.y = b; return r; }
+template <typename C> __host__ __device__ inline C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __host__ __device__ inline C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __host__ __device__ inline C cmul(C a, C b) { return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// a * conj(b)
+template <typename C> __host__ __device__ inline C cmulc(C a, C b) { return cmake<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+template <typename C> __host__ __device__ inline C cconj(C a) { return cmake<C>(a.x, -a.y); }
+template <typename C> __host__ __device__ inline C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
+
+// ---------------------------------------------------------------------------
+// 1D transform plan: Stockham stages over LDS, twiddles in device memory.
+constexpr int kMaxStages = 16;
+struct Fft1D {
+  int L;                 // transform length
+  int nst;               // number of radix stages
+  int R[kMaxStages];     // radix of each stage
+  int Ns[kMaxStages];    // product of the radices before the stage
+  int tw[kMaxStages];    // offset of the stage's twiddles in the table
+};
+
+struct FftPlan {
+  int nxp = 0, nyp = 0, nzp = 0, prec = 0;
+  bool xeven = true;     // nxp even: x transform via a length nxp/2 complex FFT
+  Fft1D fx{}, fy{}, fz{};
+  int post_off = 0;      // offset of exp(-2 pi i k / nxp), k = 0..nxp/2
+  void* tw = nullptr;    // device twiddle table (float2 or double2)
+  // launch shapes (chosen by crv_plan_tune or defaults)
+  int rows_per_block = 0;
+  int tx_y = 0, tx_z = 0;
+  // reference noise: per-row prefix of draws (nyp*nzp+1 int64), lazily made
+  long long* noise_prefix = nullptr;
+  long long noise_draws = 0;
+};
+
+struct crv_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string last_error;
+  std::map<std::tuple<int, int, int, int>, FftPlan*> plans;
+  // small device scratch: statistics accumulators
+  void* dstats = nullptr;
+  // reference-noise scratch (grown on demand)
+  void*  lcg_scratch = nullptr;
+  size_t lcg_scratch_bytes = 0;
+  double* normals = nullptr;
+  size_t  normals_cap = 0;
+};
+
+struct crv_grid {
+  crv_ctx* ctx = nullptr;
+  int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0, cnxp = 0, rnxp = 0;
+  int precision = CRV_FP32;
+  int cubetype = CRV_CTMISSING;
+  bool transformed = false;
+  size_t rsize = 0, csize = 0;
+  void* d = nullptr;
+  size_t elem_bytes() const { return precision == CRV_FP64 ? 8 : 4; }
+};
+
+// error helpers -------------------------------------------------------------
+int crv_set_error(crv_ctx* ctx, int code, const std::string& msg);
+#define CRV_HIP_CHECK(ctx, expr)                                                     \
+  do {                                                                               \
+    hipError_t e__ = (expr);                                                         \
+    if (e__ != hipSuccess)                                                           \
+      return crv_set_error((ctx), CRV_E_HIP, std::string(#expr) + ": " +             \
+                                                 hipGetErrorString(e__));            \
+  } while (0)
+
+// plan cache (fft.hip) -------------------------------------------------------
+int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** out);
+void crv_free_plan(FftPlan* p);
+
+// transform launchers (fft.hip) ------------------------------------------------
+// Forward R2C of src (real view) into dst (may alias), scale applied on load.
+int crv_launch_r2c(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, double scale);
+// Inverse C2R of buf in place (complex view in, real view out); result written
+// to dst (may alias buf) times scale, plus add (real grid) if non-null.
+int crv_launch_c2r(crv_ctx* ctx, FftPlan* plan, void* buf, void* dst, double scale, const void* add);
+
+// elementwise (simulate.hip) ---------------------------------------------------
+int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n);
+int crv_launch_scale(crv_ctx* ctx, int prec, void* dst, double s, size_t n);
+
+// noise (noise.hip) ------------------------------------------------------------
+int crv_noise_prefix(crv_ctx* ctx, FftPlan* plan);
+// Decode nnormals reference normals from *state into ctx->normals; updates *state.
+int crv_reference_normals(crv_ctx* ctx, unsigned int* state, long long nnormals);
+// Fill one grid from ctx->normals starting at draw offset `first`.
+int crv_launch_fill_noise(crv_ctx* ctx, FftPlan* plan, crv_grid* g, long long first);
+
+// FFT scale factors by grid type and precision (capi.hip)
+double crv_scale_forward(const crv_grid* g);
+double crv_scale_inverse(const crv_grid* g);
+
+// Host-side LCG skip-ahead: state after `steps` applications of s <- 69069 s + 1.
+unsigned int crv_lcg_skip(unsigned int s, unsigned long long steps);

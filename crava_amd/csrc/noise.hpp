@@ -1,0 +1,132 @@
+// Device-side noise values on the half spectrum (shared by the fill and the
+// simulation kernels).  The Hermitian structure is that of
+// FFTGrid::fillInComplexNoise (reference src/fftgrid.cpp:1191-1240):
+//  * element i = x + cnxp*(j + nyp*k) is "special" when x == 0, or when
+//    x == cnxp-1 and i is odd (the reference tests the parity of the global
+//    index, not of nxp);
+//  * a special element whose conjugate partner (nyp-j, nzp-k) is itself gets
+//    a real N(0,1); one whose partner comes later gets (N, N)/sqrt(2); one
+//    whose partner came earlier copies the partner's conjugate;
+//  * every other element gets (N, N)/sqrt(2).
+#pragma once
+#include "internal.hpp"
+
+struct NoiseGeom {
+  int cnxp, nyp, nzp;
+  const long long* prefix;  // draws before each row (reference stream only)
+};
+
+inline NoiseGeom make_noise_geom(const FftPlan* p) {
+  NoiseGeom g;
+  g.cnxp = p->nxp / 2 + 1; g.nyp = p->nyp; g.nzp = p->nzp; g.prefix = p->noise_prefix;
+  return g;
+}
+
+// classification of element i: 0 ordinary, 1 self-conjugate, 2 owns draws
+// (partner later), 3 copies the partner (returned in partner)
+__device__ __forceinline__ int noise_kind(const NoiseGeom& g, long long i, long long& partner, long long& jk, int& x) {
+  x = (int)(i % g.cnxp);
+  jk = i / g.cnxp;
+  const bool special = (x == 0) || (x == g.cnxp - 1 && (i & 1));
+  if (!special) return 0;
+  const int j = (int)(jk % g.nyp), k = (int)(jk / g.nyp);
+  const int jcc = (j == 0) ? 0 : g.nyp - j, kcc = (k == 0) ? 0 : g.nzp - k;
+  const long long jkcc = jcc + (long long)kcc * g.nyp;
+  if (jkcc == jk) return 1;
+  if (jkcc > jk) return 2;
+  partner = jkcc * g.cnxp + x;
+  return 3;
+}
+
+// float(1/sqrt(2.0)) of fftgrid.cpp:1195, applied as float*double -> float
+__device__ __forceinline__ double noise_std() { return (double)(float)0.70710678118654752440; }
+
+// draw offset of element (jk, x) inside the reference stream of one grid
+__device__ __forceinline__ long long noise_ref_offset(const NoiseGeom& g, long long jk, int x) {
+  const long long r = g.prefix[jk];
+  if (x == 0) return r;
+  // draws of element 0 of this row
+  const int j = (int)(jk % g.nyp), k = (int)(jk / g.nyp);
+  const int jcc = (j == 0) ? 0 : g.nyp - j, kcc = (k == 0) ? 0 : g.nzp - k;
+  const long long jkcc = jcc + (long long)kcc * g.nyp;
+  const int e0 = jkcc == jk ? 1 : (jkcc > jk ? 2 : 0);
+  return r + e0 + 2LL * (x - 1);
+}
+
+// reference-stream value (re, im) of element i; normals indexed from the
+// grid's first draw
+__device__ __forceinline__ void noise_ref_value(const NoiseGeom& g, const double* normals, long long i, double& re,
+                                                double& im) {
+  long long partner = 0, jk;
+  int x;
+  const int kind = noise_kind(g, i, partner, jk, x);
+  const double sd = noise_std();
+  if (kind == 1) {
+    re = (double)(float)normals[noise_ref_offset(g, jk, x)];
+    im = 0.0;
+  } else if (kind == 3) {
+    const long long pjk = partner / g.cnxp;
+    const long long o = noise_ref_offset(g, pjk, x);
+    re = (double)(float)(sd * normals[o]);
+    im = -(double)(float)(sd * normals[o + 1]);
+  } else {
+    const long long o = noise_ref_offset(g, jk, x);
+    re = (double)(float)(sd * normals[o]);
+    im = (double)(float)(sd * normals[o + 1]);
+  }
+}
+
+// ---- counter-based stream ---------------------------------------------------
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): multipliers and Weyl key increments as
+// published there.
+__host__ __device__ inline void philox10(unsigned c[4], unsigned k0, unsigned k1) {
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+}
+
+// two N(0,1) for (seed, realization, grid, element): Box-Muller on the first
+// two Philox words, uniforms (w + 0.5) 2^-32 in (0, 1)
+__device__ __forceinline__ void counter_pair(unsigned long long seed, unsigned long long real, int grid, long long i,
+                                             double& n0, double& n1) {
+  unsigned c[4] = {(unsigned)i, (unsigned)((unsigned long long)i >> 32), (unsigned)real,
+                   ((unsigned)(real >> 32) & 0x00FFFFFFu) | ((unsigned)grid << 24)};
+  philox10(c, (unsigned)seed, (unsigned)(seed >> 32));
+  const double u1 = ((double)c[0] + 0.5) * (1.0 / 4294967296.0);
+  const double u2 = ((double)c[1] + 0.5) * (1.0 / 4294967296.0);
+  const double r = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
+  n0 = r * cs;
+  n1 = r * sn;
+}
+
+__device__ __forceinline__ void noise_counter_value(const NoiseGeom& g, unsigned long long seed,
+                                                    unsigned long long real, int grid, long long i, double& re,
+                                                    double& im) {
+  long long partner = 0, jk;
+  int x;
+  const int kind = noise_kind(g, i, partner, jk, x);
+  const double sd = noise_std();
+  double n0, n1;
+  if (kind == 1) {
+    counter_pair(seed, real, grid, i, n0, n1);
+    re = (double)(float)n0;
+    im = 0.0;
+  } else if (kind == 3) {
+    counter_pair(seed, real, grid, partner, n0, n1);
+    re = (double)(float)(sd * n0);
+    im = -(double)(float)(sd * n1);
+  } else {
+    counter_pair(seed, real, grid, i, n0, n1);
+    re = (double)(float)(sd * n0);
+    im = (double)(float)(sd * n1);
+  }
+}

@@ -1,0 +1,227 @@
+"""Python view of the C ABI with the reference's names.
+
+``FFTGrid`` mirrors the reference class for the hot path (src/fftgrid.h):
+``fftInPlace`` / ``invFFTInPlace`` / ``add`` / ``multiplyByScalar`` /
+``fillInComplexNoise`` / ``getIsTransformed``; ``computePostMeanResidAndFFTCov``
+and ``simulate`` mirror the Crava members (src/crava.h:43-64, 154).  Errors
+raise ``CravaError`` carrying the library's message; the reference asserts
+instead (same conditions).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+class CravaError(RuntimeError):
+    pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = L.lib()
+        h = C.c_void_p()
+        rc = self.lib.crv_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise CravaError(f"crv_ctx_create failed ({rc}): no usable GPU")
+        self.h = h
+
+    def check(self, rc: int, what: str = ""):
+        if rc != 0:
+            msg = self.lib.crv_last_error(self.h)
+            raise CravaError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def synchronize(self):
+        self.check(self.lib.crv_ctx_synchronize(self.h), "synchronize")
+
+    def close(self):
+        if self.h:
+            self.lib.crv_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FFTGrid:
+    DATA, PARAMETER, COVARIANCE, OPERATOR = L.CRV_DATA, L.CRV_PARAMETER, L.CRV_COVARIANCE, L.CRV_OPERATOR
+
+    def __init__(self, ctx: Context, nxp, nyp, nzp, precision=32, cubetype=L.CRV_PARAMETER, nx=None, ny=None, nz=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.lib.crv_grid_create(ctx.h, nx or nxp, ny or nyp, nz or nzp, nxp, nyp, nzp, precision,
+                                          cubetype, C.byref(h)), "crv_grid_create")
+        self.h = h
+        self.nxp, self.nyp, self.nzp = nxp, nyp, nzp
+        self.cnxp = nxp // 2 + 1
+        self.rnxp = 2 * self.cnxp
+        self.precision = precision
+        self.dtype = np.float64 if precision == 64 else np.float32
+
+    # --- state -------------------------------------------------------------
+    def info(self) -> L.GridInfo:
+        i = L.GridInfo()
+        self.ctx.check(self.ctx.lib.crv_grid_info_get(self.h, C.byref(i)), "info")
+        return i
+
+    def getIsTransformed(self) -> bool:
+        return bool(self.info().is_transformed)
+
+    def getType(self) -> int:
+        return self.info().cubetype
+
+    def setType(self, t: int):
+        self.ctx.check(self.ctx.lib.crv_grid_set_cubetype(self.h, t), "set_cubetype")
+
+    def setTransformedStatus(self, t: bool):
+        self.ctx.check(self.ctx.lib.crv_grid_set_transformed(self.h, int(t)), "set_transformed")
+
+    # --- data movement -------------------------------------------------------
+    def upload_real(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype).reshape(-1)
+        self.ctx.check(self.ctx.lib.crv_grid_upload_real(self.h, a.ctypes.data, a.size), "upload_real")
+
+    def upload_complex(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if np.iscomplexobj(a):
+            a = np.stack([a.real, a.imag], axis=-1)
+        a = np.ascontiguousarray(a, dtype=self.dtype).reshape(-1)
+        self.ctx.check(self.ctx.lib.crv_grid_upload_complex(self.h, a.ctypes.data, a.size), "upload_complex")
+
+    def download_real(self) -> np.ndarray:
+        out = np.empty((self.nzp, self.nyp, self.rnxp), dtype=self.dtype)
+        self.ctx.check(self.ctx.lib.crv_grid_download(self.h, out.ctypes.data, out.size), "download")
+        return out
+
+    def download_complex(self) -> np.ndarray:
+        out = np.empty((self.nzp, self.nyp, self.cnxp, 2), dtype=self.dtype)
+        self.ctx.check(self.ctx.lib.crv_grid_download(self.h, out.ctypes.data, out.size), "download")
+        return out[..., 0] + 1j * out[..., 1]
+
+    # --- reference operations ------------------------------------------------
+    def fftInPlace(self):
+        self.ctx.check(self.ctx.lib.crv_fft_r2c(self.h), "fftInPlace")
+
+    def invFFTInPlace(self):
+        self.ctx.check(self.ctx.lib.crv_fft_c2r(self.h), "invFFTInPlace")
+
+    def fftFrom(self, src: "FFTGrid"):
+        self.ctx.check(self.ctx.lib.crv_fft_r2c_from(self.h, src.h), "fft_r2c_from")
+
+    def invFFTAdd(self, src: "FFTGrid", add: "FFTGrid | None"):
+        self.ctx.check(self.ctx.lib.crv_fft_c2r_add(self.h, src.h, add.h if add else None), "fft_c2r_add")
+
+    def add(self, other: "FFTGrid"):
+        self.ctx.check(self.ctx.lib.crv_grid_add(self.h, other.h), "add")
+
+    def multiplyByScalar(self, s: float):
+        self.ctx.check(self.ctx.lib.crv_grid_multiply_by_scalar(self.h, float(s)), "multiplyByScalar")
+
+    def copyFrom(self, other: "FFTGrid"):
+        self.ctx.check(self.ctx.lib.crv_grid_copy(self.h, other.h), "copy")
+
+    def fillInComplexNoise(self, state: int) -> int:
+        st = C.c_uint(state & 0xFFFFFFFF)
+        self.ctx.check(self.ctx.lib.crv_fill_complex_noise(self.h, C.byref(st)), "fillInComplexNoise")
+        return st.value
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.crv_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _arr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+@dataclass
+class PostResult:
+    n_wavenumbers: int
+    n_inverted: int
+    n_chol_fail: int
+    n_fp64: int
+    max_cond: float
+
+
+def computePostMeanResidAndFFTCov(ctx: Context, mean, data, err_corr, cov, *, A, theta, tables, nz, lz,
+                                  low_cut=5.0, high_cut=55.0, wnc=0.1, min_rel_thick=1.0, sigma0,
+                                  mode=L.CRV_MODE_PLAIN, cond_threshold=1e5, lowk_fraction=0.02,
+                                  separable=False, cxy=None, ct=None, skip_mean_inverse=False) -> PostResult:
+    """Crava::computePostMeanResidAndFFTCov over FFTGrid objects (crava.h:43-64)."""
+    n = len(data)
+    keep = []
+    A_ = np.ascontiguousarray(A, dtype=np.float32); keep.append(A_)
+    th = np.ascontiguousarray(theta, dtype=np.float64); keep.append(th)
+    kwd, em1, em2, em3 = tables.as_interleaved(); keep += [kwd, em1, em2, em3]
+    s0 = np.ascontiguousarray(sigma0, dtype=np.float64); keep.append(s0)
+    p = L.PostParams()
+    p.ntheta = n; p.nz = int(nz); p.lz = float(lz); p.min_rel_thick = float(min_rel_thick)
+    p.low_cut = float(np.float32(low_cut)); p.high_cut = float(np.float32(high_cut)); p.wnc = float(np.float32(wnc))
+    p.A = _arr(A_, C.c_float); p.err_theta_cov = _arr(th, C.c_double)
+    p.kwd = _arr(kwd, C.c_double); p.em1 = _arr(em1, C.c_double)
+    p.em2 = _arr(em2, C.c_double); p.em3 = _arr(em3, C.c_double)
+    p.mode = mode; p.cond_threshold = cond_threshold; p.lowk_fraction = lowk_fraction
+    p.sigma0 = _arr(s0, C.c_double)
+    p.separable = int(separable)
+    if separable:
+        cx = np.ascontiguousarray(cxy, dtype=np.float64); c_t = np.ascontiguousarray(ct, dtype=np.float64)
+        keep += [cx, c_t]
+        p.cxy = _arr(cx, C.c_double); p.ct = _arr(c_t, C.c_double)
+    p.skip_mean_inverse = int(skip_mean_inverse)
+    means = (C.c_void_p * 3)(*[g.h.value for g in mean])
+    datas = (C.c_void_p * n)(*[g.h.value for g in data])
+    covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+    st = L.PostStats()
+    ctx.check(ctx.lib.crv_posterior(ctx.h, C.byref(p), means, datas, err_corr.h if err_corr is not None else None,
+                                    covs, C.byref(st)), "computePostMeanResidAndFFTCov")
+    return PostResult(st.n_wavenumbers, st.n_inverted, st.n_chol_fail, st.n_fp64, st.max_cond)
+
+
+def simulate(ctx: Context, cov, post_mean, out, *, noise_mode=L.CRV_NOISE_COUNTER, lcg_state=None, seed=0,
+             realization=0, host_noise=None, fp64_chol=False):
+    """One realization of Crava::simulate (crava.cpp:1599-1708).  Returns the
+    updated LCG state for the reference stream."""
+    spec = L.NoiseSpec()
+    spec.mode = noise_mode
+    st = C.c_uint((lcg_state or 0) & 0xFFFFFFFF)
+    spec.lcg_state = C.pointer(st)
+    spec.seed = seed
+    spec.realization = realization
+    keep = []
+    if host_noise is not None:
+        for g in range(3):
+            a = np.ascontiguousarray(np.stack([host_noise[g].real, host_noise[g].imag], -1),
+                                     dtype=cov[0].dtype)
+            keep.append(a)
+            spec.host_noise[g] = a.ctypes.data
+    covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+    pm = (C.c_void_p * 3)(*[g.h.value if g is not None else None for g in post_mean]) if post_mean else None
+    outs = (C.c_void_p * 3)(*[g.h.value for g in out])
+    ctx.check(ctx.lib.crv_simulate(ctx.h, covs, pm, C.byref(spec), int(fp64_chol), outs), "simulate")
+    return st.value
+
+
+def noise_draws(nxp, nyp, nzp) -> int:
+    return int(L.lib().crv_noise_draws(nxp, nyp, nzp))
+
+
+def find_closest_factorable(n: int) -> int:
+    return int(L.lib().crv_find_closest_factorable(n))
+
+
+def padding_size(n: int, frac: float) -> int:
+    return int(L.lib().crv_padding_size(n, frac))

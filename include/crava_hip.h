@@ -1,0 +1,259 @@
+/*
+ * crava_hip.h -- C ABI of the MI355X (gfx950) implementation of CRAVA's
+ * Fourier-domain Bayesian AVO inversion hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes.  Every entry point
+ * returns CRV_OK (0) or a negative CRV_E_* code; the text of the last error
+ * on a context is returned by crv_last_error().  Numerical failures of a
+ * single wavenumber (a Cholesky factorisation that does not succeed) are not
+ * errors: they reproduce the reference's fallback and are counted in
+ * crv_post_stats.
+ *
+ * Grid storage follows FFTGrid exactly (reference src/fftgrid.h:284-322,
+ * src/fftgrid.cpp:40-70): one buffer of rsize = rnxp*nyp*nzp reals with
+ * rnxp = 2*(nxp/2+1), x fastest; real index i + rnxp*j + rnxp*nyp*k; after the
+ * forward transform the same buffer holds cnxp*nyp*nzp complex values,
+ * complex index i + cnxp*j + cnxp*nyp*k (src/fftgrid.cpp:1544,1677).
+ * Host buffers passed to upload/download use that same layout, pad columns
+ * included; they are borrowed only for the duration of the call.
+ *
+ * Threading: one host thread per crv_ctx; all device work of a context is
+ * issued on the context's HIP stream in call order.
+ */
+#ifndef CRAVA_HIP_H
+#define CRAVA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define CRV_OK              0
+#define CRV_E_INVALID      -1  /* bad argument / inconsistent dimensions     */
+#define CRV_E_STATE        -2  /* grid in the wrong (transformed) state      */
+#define CRV_E_HIP          -3  /* HIP runtime error                          */
+#define CRV_E_NOMEM        -4  /* device allocation failed                   */
+#define CRV_E_UNSUPPORTED  -5  /* size or option not supported               */
+
+/* ---- precision --------------------------------------------------------- */
+#define CRV_FP32 32
+#define CRV_FP64 64
+
+/* ---- grid types: same values as FFTGrid::gridTypes (src/fftgrid.h:193) - */
+#define CRV_CTMISSING  0
+#define CRV_DATA       1
+#define CRV_PARAMETER  2
+#define CRV_COVARIANCE 3
+#define CRV_VELOCITY   4
+#define CRV_OPERATOR   5
+
+typedef struct crv_ctx  crv_ctx;
+typedef struct crv_grid crv_grid;
+
+typedef struct crv_grid_info {
+  int nx, ny, nz;        /* un-padded sizes                                  */
+  int nxp, nyp, nzp;     /* padded sizes                                     */
+  int cnxp, rnxp;        /* nxp/2+1 and 2*cnxp                               */
+  int precision;         /* CRV_FP32 | CRV_FP64                              */
+  int cubetype;          /* CRV_DATA ...                                     */
+  int is_transformed;    /* FFTGrid::istransformed_                          */
+  size_t rsize;          /* rnxp*nyp*nzp reals                               */
+  size_t csize;          /* cnxp*nyp*nzp complex values                      */
+} crv_grid_info;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the global FFTW plan creation of FFTGrid::fftInPlace
+ * (src/fftgrid.cpp:1799-1804): a context owns the device, the stream and the
+ * cached transform plans (twiddle tables) for every grid size it has seen.  */
+int         crv_ctx_create(int device, crv_ctx** out);
+int         crv_ctx_destroy(crv_ctx* ctx);
+const char* crv_last_error(const crv_ctx* ctx);
+int         crv_ctx_synchronize(crv_ctx* ctx);
+/* Use an external HIP stream (hipStream_t passed as void*; NULL = own). */
+int         crv_ctx_set_stream(crv_ctx* ctx, void* hip_stream);
+/* Library version string, e.g. "crava_hip 1.0 gfx950". */
+const char* crv_version(void);
+
+/* ---- grids --------------------------------------------------------------
+ * crv_grid_create replaces FFTGrid::FFTGrid + createRealGrid
+ * (src/fftgrid.cpp:40-70,1243-1295).  The buffer is uninitialised.          */
+int crv_grid_create(crv_ctx* ctx, int nx, int ny, int nz,
+                    int nxp, int nyp, int nzp,
+                    int precision, int cubetype, crv_grid** out);
+int crv_grid_destroy(crv_grid* g);
+int crv_grid_info_get(const crv_grid* g, crv_grid_info* info);
+int crv_grid_set_cubetype(crv_grid* g, int cubetype);
+/* FFTGrid::setTransformedStatus (src/fftgrid.h:185). */
+int crv_grid_set_transformed(crv_grid* g, int is_transformed);
+/* Device address of the grid buffer (hipDeviceptr as void*). */
+int crv_grid_device_ptr(const crv_grid* g, void** dptr);
+
+/* Host <-> device copies of the whole rsize buffer (real view) or csize
+ * buffer (complex view, interleaved re/im).  The element type is float for
+ * CRV_FP32 grids and double for CRV_FP64 grids.  nelem must equal rsize
+ * (real) or 2*csize (complex).  upload_real clears the transformed flag,
+ * upload_complex sets it; download copies whatever the buffer holds.        */
+int crv_grid_upload_real(crv_grid* g, const void* host, size_t nelem);
+int crv_grid_upload_complex(crv_grid* g, const void* host, size_t nelem);
+int crv_grid_download(crv_grid* g, void* host, size_t nelem);
+/* Device-to-device copy of buffer, flag and type (same sizes/precision). */
+int crv_grid_copy(crv_grid* dst, const crv_grid* src);
+/* FFTGrid::add (src/fftgrid.cpp:1870-1891): dst += src over the whole real
+ * buffer (untransformed) or complex buffer (transformed). */
+int crv_grid_add(crv_grid* dst, const crv_grid* src);
+/* FFTGrid::multiplyByScalar (src/fftgrid.cpp:1985-1995): all rsize reals. */
+int crv_grid_multiply_by_scalar(crv_grid* g, double scalar);
+
+/* ---- transforms ---------------------------------------------------------
+ * crv_fft_r2c replaces FFTGrid::fftInPlace (src/fftgrid.cpp:1783-1808):
+ * requires !is_transformed and cubetype != CTMISSING; scales by
+ * 1/sqrt(nxp*nyp*nzp) unless COVARIANCE or OPERATOR; unnormalised forward
+ * real-to-complex 3D DFT with sign -1 (FFTW_FORWARD) over (nzp, nyp, nxp),
+ * x last; sets is_transformed.
+ * crv_fft_c2r replaces FFTGrid::invFFTInPlace (src/fftgrid.cpp:1811-1841):
+ * requires is_transformed; unnormalised inverse (sign +1) complex-to-real,
+ * then scales by 1/N (COVARIANCE, OPERATOR) or 1/sqrt(N) (others); the
+ * imaginary parts of the kx = 0 and kx = nxp/2 planes are ignored, as a
+ * half-complex inverse does.  Pad columns i >= nxp hold unspecified values
+ * afterwards, as in the reference.                                          */
+int crv_fft_r2c(crv_grid* g);
+int crv_fft_c2r(crv_grid* g);
+/* Out-of-place forms: src is left untouched, dst (same sizes, precision)
+ * receives the result and src's cubetype.  Same arithmetic as in place.     */
+int crv_fft_r2c_from(crv_grid* dst, const crv_grid* src);
+/* C2R of src into dst followed by dst += add_real (untransformed grid of the
+ * same size): FFTGrid::invFFTInPlace + FFTGrid::add fused (crava.cpp:1672-1708).
+ * add_real may be NULL.  src is consumed (holds unspecified values after). */
+int crv_fft_c2r_add(crv_grid* dst, crv_grid* src, const crv_grid* add_real);
+
+/* ---- posterior ----------------------------------------------------------
+ * crv_posterior replaces Crava::computePostMeanResidAndFFTCov
+ * (src/crava.cpp:975-1285; signature src/crava.h:43-64) for the
+ * constant-thickness branch.
+ *
+ * Inputs/outputs, as in the reference:
+ *   mean[3]   in: prior means (real, untransformed, PARAMETER) -- they are
+ *             transformed here if not yet transformed (crava.cpp:177-184);
+ *             out: posterior means, inverse transformed (crava.cpp:1262-1264)
+ *   data[n]   in: seismic data spectra or real data (transformed here if
+ *             needed); out: residual spectra (crava.cpp:1193-1195)
+ *   err_corr  in: real errCorr grid (COVARIANCE), transformed here
+ *             (crava.cpp:1062); may be NULL when separable != 0
+ *   cov[6]    order aa, bb, rr, ab, ar, br.  in: prior covariance grids
+ *             (real, COVARIANCE; transformed here if not yet transformed,
+ *             seismicparametersholder.cpp:299-322) or ignored when
+ *             separable != 0; out: posterior covariance spectra
+ *             (crava.cpp:1196-1205).
+ * The per-kz tables follow Crava::fillErrMultVectors (crava.cpp:1336-1364)
+ * for constant thickness: kwd[k][l] = conj(W_l(k)) * D1(k) (complex),
+ * em1[k][l] = conj(W_l(k))/|W_l|, em2[k][l] = conj(E3_l(k))/|E3_l|,
+ * em3[k][l] = 1 -- all interleaved re/im doubles, nzp*n complex each.       */
+
+#define CRV_MODE_PLAIN        0   /* arithmetic in the grids' precision      */
+#define CRV_MODE_FP32_REFINE  1   /* FP32 grids; FP32 solve + 1 refinement
+                                     step, FP64 for ill-conditioned and
+                                     low-|k| wavenumbers                     */
+
+typedef struct crv_post_params {
+  int           ntheta;        /* n, 1..8                                    */
+  int           nz;            /* un-padded nz (frequency axis scaling)      */
+  double        lz;            /* simbox lz (ms), frequency = nz*1000/(lz*nzp)*min(k,nzp-k) */
+  double        min_rel_thick; /* simbox->getMinRelThick()                   */
+  double        low_cut;       /* Hz                                         */
+  double        high_cut;      /* Hz                                         */
+  double        wnc;           /* white-noise component                      */
+  const float*  A;             /* ntheta x 3, row-major (Aki-Richards rows)  */
+  const double* err_theta_cov; /* ntheta x ntheta                            */
+  const double* kwd;           /* nzp x ntheta complex                       */
+  const double* em1;           /* nzp x ntheta complex                       */
+  const double* em2;           /* nzp x ntheta complex                       */
+  const double* em3;           /* nzp x ntheta complex                       */
+  int           mode;          /* CRV_MODE_*                                 */
+  double        cond_threshold;/* FP64 re-solve above this (default 1e5)     */
+  double        lowk_fraction; /* FP64 re-solve for |k|/Nyquist below this   */
+  /* Separable prior (gradI = gradJ = 0): prior covariance spectrum
+   * Sigma0[a][b] * |Re(cxy[kx,ky] * ct[kz])| and error spectrum
+   * |Re(cxy[kx,ky])|, with cxy the 2D spectrum of the lateral correlation
+   * (cnxp x nyp reals) and ct the 1D spectrum of the temporal correlation
+   * (nzp reals).  When separable == 0 these are ignored and cov/err_corr
+   * grids are read as the reference does.                                   */
+  int           separable;
+  const double* sigma0;        /* 3x3 row-major                              */
+  const double* cxy;           /* cnxp*nyp                                   */
+  const double* ct;            /* nzp                                        */
+  int           skip_mean_inverse; /* 1: leave posterior means as spectra     */
+} crv_post_params;
+
+typedef struct crv_post_stats {
+  long long n_wavenumbers;     /* cnxp*nyp*nzp                               */
+  long long n_inverted;        /* wavenumbers inside the frequency band      */
+  long long n_chol_fail;       /* fell back to the prior (crava.cpp:1421)    */
+  long long n_fp64;            /* re-solved in FP64 (CRV_MODE_FP32_REFINE)   */
+  double    max_cond;          /* largest condition estimate seen            */
+} crv_post_stats;
+
+int crv_posterior(crv_ctx* ctx, const crv_post_params* p,
+                  crv_grid* mean[3], crv_grid** data, crv_grid* err_corr,
+                  crv_grid* cov[6], crv_post_stats* stats);
+
+/* ---- simulation ---------------------------------------------------------
+ * crv_simulate replaces one realization of Crava::simulate
+ * (src/crava.cpp:1550-1734, loop body 1599-1708): complex white noise as
+ * FFTGrid::fillInComplexNoise (src/fftgrid.cpp:1191-1240) for seed0, seed1,
+ * seed2 in that order; per wavenumber the 3x3 posterior covariance (upper
+ * triangle from cov[], lower = conjugate) is Cholesky factored
+ * (lib_matrCholCpx) and the noise multiplied by L (lib_matrProdCholVec);
+ * on failure the noise is zeroed; the three grids are inverse transformed
+ * with PARAMETER scaling and the real posterior means added.
+ *
+ * Noise sources:
+ *   CRV_NOISE_REFERENCE  the reference RandomGen stream (32-bit LCG
+ *                        69069*s+1 and Marsaglia-Bray normals,
+ *                        libs/lib/random.cpp:38-119), continued from
+ *                        *lcg_state, which is updated on return;
+ *   CRV_NOISE_COUNTER    a counter-based generator (Philox-4x32-10 with
+ *                        Box-Muller), keyed by (seed, realization); same
+ *                        Hermitian structure as the reference fill;
+ *   CRV_NOISE_HOST       three complex noise grids supplied by the caller
+ *                        (host arrays, csize complex each, grid precision). */
+#define CRV_NOISE_REFERENCE 0
+#define CRV_NOISE_COUNTER   1
+#define CRV_NOISE_HOST      2
+
+typedef struct crv_noise_spec {
+  int                mode;
+  unsigned int*      lcg_state;    /* CRV_NOISE_REFERENCE (in/out)            */
+  unsigned long long seed;         /* CRV_NOISE_COUNTER                       */
+  unsigned long long realization;  /* CRV_NOISE_COUNTER                       */
+  const void*        host_noise[3];/* CRV_NOISE_HOST                          */
+} crv_noise_spec;
+
+/* cov: posterior covariance spectra (aa, bb, rr, ab, ar, br), transformed.
+ * post_mean: posterior means in real space (untransformed) or NULL.
+ * out[3]: destination grids (alpha, beta, rho realization), real on return.
+ * fp64_chol: 1 = factor the 3x3 matrices in FP64 (FP32 grids only).        */
+int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[3],
+                 const crv_noise_spec* noise, int fp64_chol, crv_grid* out[3]);
+
+/* FFTGrid::fillInComplexNoise alone (src/fftgrid.cpp:1191-1240): fills g
+ * (becomes transformed, PARAMETER) with the reference noise stream from
+ * *lcg_state (updated).  Used to check the stream against the reference.   */
+int crv_fill_complex_noise(crv_grid* g, unsigned int* lcg_state);
+
+/* Number of N(0,1) draws FFTGrid::fillInComplexNoise makes on one grid of
+ * this padded size (a function of nxp, nyp, nzp only). */
+long long crv_noise_draws(int nxp, int nyp, int nzp);
+
+/* ---- padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622) -- */
+/* Smallest 2^a 3^b 5^c 7^d >= leastint (FFTGrid::findClosestFactorableNumber). */
+int crv_find_closest_factorable(int leastint);
+/* ModelGeneral::setPaddingSize: findClosestFactorable(ceil(n*(1+pad_fac))). */
+int crv_padding_size(int n, double pad_fac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRAVA_HIP_H */

@@ -1,0 +1,311 @@
+"""ORACLE -- test infrastructure, not product code.
+
+Python side of the CPU oracle: ctypes access to liboracle.so (the plain-C
+restatement in crava_oracle.c) and, where it was built, to
+_ref/libcravaref.so (the reference's own FFTW 2.1.2 / lib_matr / RandomGen
+compiled from its sources), plus FP64 transforms on CRAVA's padded layout
+through numpy's pocketfft (an independent FFT implementation).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, and only to check or to time the reference algorithm.
+
+Reference conventions restated here:
+  * FFTGrid::fftInPlace / invFFTInPlace scaling by grid type
+    (src/fftgrid.cpp:1783-1841): forward x 1/sqrt(N) unless COVARIANCE or
+    OPERATOR; inverse x 1/N for those, 1/sqrt(N) otherwise; the half-complex
+    inverse ignores the imaginary parts of the kx = 0 and kx = nxp/2 planes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libcravaref.so"
+
+COVARIANCE, OPERATOR = 3, 5
+
+
+def build() -> None:
+    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", str(HERE), "-j8"], check=True)
+
+
+_O = None
+_R = None
+
+
+def olib() -> C.CDLL:
+    global _O
+    if _O is None:
+        if not ORACLE_SO.exists():
+            build()
+        o = C.CDLL(str(ORACLE_SO))
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        o.or_rnorm_seq.argtypes = [C.POINTER(C.c_uint32), C.c_long, dp]
+        o.or_fill_complex_noise.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), fp]
+        o.or_fill_complex_noise_d.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), dp]
+        o.or_noise_draws.restype = C.c_long
+        o.or_noise_draws.argtypes = [C.c_int] * 3
+        o.or_chol_flat_f.argtypes = [C.c_int, fp]
+        o.or_chol_flat_d.argtypes = [C.c_int, dp]
+        o.or_axeqb_flat_f.argtypes = [C.c_int, fp, fp, C.c_int]
+        for nm, pt in (("or_posterior_f", fp), ("or_posterior_d", dp)):
+            f = getattr(o, nm)
+            f.restype = C.c_long
+            f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(pt), C.POINTER(pt), pt, C.POINTER(pt),
+                          pt, pt, pt, pt, fp, dp, dp, C.POINTER(C.c_int), C.c_float, C.c_int]
+        for nm, pt in (("or_simulate_scale_f", fp), ("or_simulate_scale_d", dp)):
+            f = getattr(o, nm)
+            f.restype = C.c_long
+            f.argtypes = [C.c_long, C.POINTER(pt), C.POINTER(pt)]
+        o.or_find_closest_factorable.argtypes = [C.c_int]
+        o.or_find_closest_factorable.restype = C.c_int
+        _O = o
+    return _O
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def rlib() -> C.CDLL:
+    global _R
+    if _R is None:
+        if not REF_SO.exists():
+            raise RuntimeError("oracle/_ref/libcravaref.so not built (needs the reference tree)")
+        r = C.CDLL(str(REF_SO))
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        r.ref_rnorm_seq.restype = C.c_uint
+        r.ref_rnorm_seq.argtypes = [C.c_uint, C.c_long, dp]
+        r.ref_unif_seq.argtypes = [C.c_uint, C.c_long, dp]
+        r.ref_fill_complex_noise.restype = C.c_uint
+        r.ref_fill_complex_noise.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint, fp]
+        r.ref_chol_flat.argtypes = [C.c_int, fp]
+        r.ref_axeqb_flat.argtypes = [C.c_int, fp, fp, C.c_int]
+        r.ref_posterior.restype = C.c_long
+        r.ref_posterior.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(fp), C.POINTER(fp), fp,
+                                    C.POINTER(fp), fp, fp, fp, fp, fp, dp, dp, C.POINTER(C.c_int), C.c_float, C.c_int]
+        r.ref_simulate_scale.restype = C.c_long
+        r.ref_simulate_scale.argtypes = [C.c_long, C.POINTER(fp), C.POINTER(fp)]
+        r.ref_fft3d_r2c.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.c_int]
+        r.ref_fft3d_c2r.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.c_int]
+        _R = r
+    return _R
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+# ------------------------------------------------------------------ RNG
+def rnorm_seq(seed: int, n: int):
+    """(normals, final state) of the restated RandomGen."""
+    st = C.c_uint32(seed & 0xFFFFFFFF)
+    out = np.empty(n, np.float64)
+    olib().or_rnorm_seq(C.byref(st), n, _p(out, C.c_double))
+    return out, st.value
+
+
+def fill_complex_noise(nxp, nyp, nzp, seed: int, dtype=np.float32):
+    cnxp = nxp // 2 + 1
+    out = np.empty((nzp, nyp, cnxp, 2), dtype)
+    st = C.c_uint32(seed & 0xFFFFFFFF)
+    if dtype == np.float32:
+        olib().or_fill_complex_noise(nxp, nyp, nzp, C.byref(st), _p(out, C.c_float))
+    else:
+        olib().or_fill_complex_noise_d(nxp, nyp, nzp, C.byref(st), _p(out, C.c_double))
+    return out[..., 0] + 1j * out[..., 1], st.value
+
+
+def noise_draws(nxp, nyp, nzp) -> int:
+    return int(olib().or_noise_draws(nxp, nyp, nzp))
+
+
+def chol(a: np.ndarray, dtype=np.float32):
+    """lib_matrCholCpx restated on an n x n complex matrix; (flag, L)."""
+    n = a.shape[0]
+    buf = np.ascontiguousarray(np.stack([a.real, a.imag], -1), dtype=dtype)
+    if dtype == np.float32:
+        flag = olib().or_chol_flat_f(n, _p(buf, C.c_float))
+    else:
+        flag = olib().or_chol_flat_d(n, _p(buf, C.c_double))
+    return flag, buf[..., 0] + 1j * buf[..., 1]
+
+
+def find_closest_factorable(n: int) -> int:
+    return int(olib().or_find_closest_factorable(n))
+
+
+# ------------------------------------------------------------ transforms
+def fft_r2c(real: np.ndarray, nxp: int, cubetype: int) -> np.ndarray:
+    """FP64 FFTGrid::fftInPlace on a (nzp, nyp, rnxp) real grid."""
+    x = np.asarray(real, np.float64)[:, :, :nxp]
+    N = x.size
+    s = 1.0 if cubetype in (COVARIANCE, OPERATOR) else 1.0 / math.sqrt(N)
+    return np.fft.rfftn(x * s)
+
+
+def fft_c2r(spec: np.ndarray, nxp: int, cubetype: int) -> np.ndarray:
+    """FP64 FFTGrid::invFFTInPlace; returns (nzp, nyp, rnxp) with zero pads."""
+    nzp, nyp, cnxp = spec.shape
+    N = nxp * nyp * nzp
+    sp = np.array(spec, dtype=np.complex128)
+    # complex inverse over (z, y) first, then the half-complex inverse along
+    # x, which keeps only the real parts of the kx = 0 and kx = nxp/2 bins
+    t = np.fft.ifft(np.fft.ifft(sp, axis=0), axis=1) * (nzp * nyp)
+    t[:, :, 0] = t[:, :, 0].real
+    if nxp % 2 == 0:
+        t[:, :, nxp // 2] = t[:, :, nxp // 2].real
+    x = np.fft.irfft(t, n=nxp, axis=2) * nxp
+    s = 1.0 / N if cubetype in (COVARIANCE, OPERATOR) else 1.0 / math.sqrt(N)
+    out = np.zeros((nzp, nyp, 2 * cnxp))
+    out[:, :, :nxp] = x * s
+    return out
+
+
+def ref_fft_r2c(real: np.ndarray, nxp: int, cubetype: int) -> np.ndarray:
+    """The reference's own FFTW 2.1.2 float transform (needs _ref)."""
+    nzp, nyp, rnxp = real.shape
+    buf = np.ascontiguousarray(real, np.float32).copy()
+    rlib().ref_fft3d_r2c(nxp, nyp, nzp, _p(buf, C.c_float), cubetype)
+    v = buf.reshape(nzp, nyp, rnxp // 2, 2)
+    return v[..., 0] + 1j * v[..., 1]
+
+
+def ref_fft_c2r(spec: np.ndarray, nxp: int, cubetype: int) -> np.ndarray:
+    nzp, nyp, cnxp = spec.shape
+    buf = np.ascontiguousarray(np.stack([spec.real, spec.imag], -1), np.float32).copy()
+    rlib().ref_fft3d_c2r(nxp, nyp, nzp, _p(buf, C.c_float), cubetype)
+    return buf.reshape(nzp, nyp, 2 * cnxp)
+
+
+# ------------------------------------------------------------- posterior
+def _inter(a, dtype):
+    return np.ascontiguousarray(np.stack([a.real, a.imag], -1), dtype=dtype)
+
+
+def posterior_loop(mean, data, errcorr, cov, *, kwd, em1, em2, em3, A, theta, sigma0, invert, wnc,
+                   dtype=np.float32, nthreads=1, backend="oracle"):
+    """Run the per-wavenumber loop on half spectra (complex arrays of shape
+    (nzp, nyp, cnxp)).  Returns (mean, data(residual), cov, nfail) as complex.
+    backend: "oracle" (restated lib_matr) or "reference" (compiled lib_matr)."""
+    nzp, nyp, cnxp = mean[0].shape
+    nxp = 2 * (cnxp - 1)
+    n = len(data)
+    ct = C.c_float if dtype == np.float32 else C.c_double
+    pt = C.POINTER(ct)
+    M = [_inter(m, dtype) for m in mean]
+    D = [_inter(d, dtype) for d in data]
+    E = _inter(errcorr, dtype)
+    V = [_inter(c, dtype) for c in cov]
+    tabs = [_inter(t, dtype) for t in (kwd, em1, em2, em3)]
+    A_ = np.ascontiguousarray(A, np.float32)
+    th = np.ascontiguousarray(theta, np.float64)
+    s0 = np.ascontiguousarray(sigma0, np.float64)
+    inv = np.ascontiguousarray(invert, np.int32)
+    mp = (pt * 3)(*[_p(m, ct) for m in M])
+    dp_ = (pt * n)(*[_p(d, ct) for d in D])
+    vp = (pt * 6)(*[_p(v, ct) for v in V])
+    if backend == "reference":
+        if dtype != np.float32:
+            raise ValueError("the reference loop is float only")
+        fn = rlib().ref_posterior
+    else:
+        fn = olib().or_posterior_f if dtype == np.float32 else olib().or_posterior_d
+    nfail = fn(nxp, nyp, nzp, n, mp, dp_, _p(E, ct), vp, *[_p(t, ct) for t in tabs], _p(A_, C.c_float),
+               _p(th, C.c_double), _p(s0, C.c_double), _p(inv, C.c_int), float(np.float32(wnc)), nthreads)
+    cz = lambda a: a[..., 0] + 1j * a[..., 1]
+    return [cz(m) for m in M], [cz(d) for d in D], [cz(v) for v in V], int(nfail)
+
+
+def simulate_scale(cov, seeds, dtype=np.float32, backend="oracle"):
+    """Crava::simulate's per-wavenumber Cholesky scaling on half spectra."""
+    ct = C.c_float if dtype == np.float32 else C.c_double
+    pt = C.POINTER(ct)
+    V = [_inter(c, dtype) for c in cov]
+    S = [_inter(s, dtype) for s in seeds]
+    vp = (pt * 6)(*[_p(v, ct) for v in V])
+    sp = (pt * 3)(*[_p(s, ct) for s in S])
+    csize = V[0].size // 2
+    if backend == "reference":
+        nfail = rlib().ref_simulate_scale(csize, vp, sp)
+    else:
+        fn = olib().or_simulate_scale_f if dtype == np.float32 else olib().or_simulate_scale_d
+        nfail = fn(csize, vp, sp)
+    cz = lambda a: a[..., 0] + 1j * a[..., 1]
+    return [cz(s) for s in S], int(nfail)
+
+
+def gaussian_update(Sm, K, Se, mu, d):
+    """Closed-form Gaussian posterior (FP64): mu + Sm K^H (K Sm K^H + Se)^-1 (d - K mu),
+    Sm - Sm K^H (K Sm K^H + Se)^-1 K Sm; residual d - K mu_post."""
+    M = K @ Sm @ K.conj().T + Se
+    G = np.linalg.solve(M, K @ Sm)            # M^-1 K Sm
+    mp = mu + G.conj().T @ (d - K @ mu)
+    Sp = Sm - (K @ Sm).conj().T @ G
+    return mp, Sp, d - K @ mp
+
+
+# ----------------------------------------------- counter-based noise (FP64)
+M32 = 0xFFFFFFFF
+
+
+def philox10(c0, c1, c2, c3, k0, k1):
+    """Philox-4x32-10 (Salmon et al., SC 2011) on numpy uint64 arrays."""
+    c0, c1, c2, c3 = (np.asarray(v, np.uint64) & M32 for v in (c0, c1, c2, c3))
+    k0 = np.uint64(k0 & M32)
+    k1 = np.uint64(k1 & M32)
+    for _ in range(10):
+        p0 = np.uint64(0xD2511F53) * c0
+        p1 = np.uint64(0xCD9E8D57) * c2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & np.uint64(M32)
+        hi1, lo1 = p1 >> np.uint64(32), p1 & np.uint64(M32)
+        c0, c1, c2, c3 = hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0
+        k0 = (k0 + np.uint64(0x9E3779B9)) & np.uint64(M32)
+        k1 = (k1 + np.uint64(0xBB67AE85)) & np.uint64(M32)
+    return c0, c1, c2, c3
+
+
+def counter_noise(nxp, nyp, nzp, seed, realization, grid):
+    """Counter-stream noise laid out with the reference Hermitian rules."""
+    cnxp = nxp // 2 + 1
+    n = cnxp * nyp * nzp
+    i = np.arange(n, dtype=np.int64)
+    x = i % cnxp
+    jk = i // cnxp
+    j = jk % nyp
+    k = jk // nyp
+    jcc = np.where(j == 0, 0, nyp - j)
+    kcc = np.where(k == 0, 0, nzp - k)
+    jkcc = jcc + kcc * nyp
+    special = (x == 0) | ((x == cnxp - 1) & (i % 2 == 1))
+    src = np.where(special & (jkcc < jk), jkcc * cnxp + x, i)
+
+    def pair(idx):
+        hi = (np.uint64(realization >> 32) & np.uint64(0xFFFFFF)) | np.uint64(grid << 24)
+        c = philox10(idx.astype(np.uint64) & np.uint64(M32), idx.astype(np.uint64) >> np.uint64(32),
+                     np.full(idx.shape, realization & M32, np.uint64), np.full(idx.shape, hi, np.uint64),
+                     seed & M32, (seed >> 32) & M32)
+        u1 = (c[0].astype(np.float64) + 0.5) / 4294967296.0
+        u2 = (c[1].astype(np.float64) + 0.5) / 4294967296.0
+        r = np.sqrt(-2.0 * np.log(u1))
+        return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
+
+    n0, n1 = pair(src)
+    sd = float(np.float32(1.0 / math.sqrt(2.0)))
+    re = (sd * n0).astype(np.float32).astype(np.float64)
+    im = (sd * n1).astype(np.float32).astype(np.float64)
+    selfc = special & (jkcc == jk)
+    re = np.where(selfc, n0.astype(np.float32).astype(np.float64), re)
+    im = np.where(selfc, 0.0, im)
+    copy = special & (jkcc < jk)
+    im = np.where(copy, -im, im)
+    return (re + 1j * im).reshape(nzp, nyp, cnxp)

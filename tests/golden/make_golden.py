@@ -1,0 +1,143 @@
+"""Write tests/golden/golden_ref.npz from the reference's own numerics.
+
+Runs in a container that holds the reference tree: oracle/Makefile compiles
+FFTW 2.1.2 (float), lib_matr.c and random.cpp from it into
+oracle/_ref/libcravaref.so, and this script records inputs and outputs of
+those routines on small cases.  The GPU tests and the oracle tests compare
+against the recorded vectors, so neither needs the reference at run time.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from oracle import oracle as O  # noqa: E402
+from crava_amd import model  # noqa: E402
+
+OUT = Path(__file__).resolve().parent / "golden_ref.npz"
+
+
+def main() -> None:
+    O.build()
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libcravaref.so is missing: the reference tree is needed")
+    r = O.rlib()
+    import ctypes as C
+    g = {}
+    rng = np.random.default_rng(20240611)
+
+    # RandomGen: first 20000 normals of several seeds, the state after them
+    for seed in (0, 1, 231171, 12345):
+        out = np.empty(20000, np.float64)
+        st = r.ref_rnorm_seq(seed, out.size, out.ctypes.data_as(C.POINTER(C.c_double)))
+        g[f"rng_{seed}"] = out
+        g[f"rng_{seed}_state"] = np.uint32(st)
+    u = np.empty(1000, np.float64)
+    r.ref_unif_seq(0, u.size, u.ctypes.data_as(C.POINTER(C.c_double)))
+    g["unif_0"] = u
+
+    # FFTGrid::fillInComplexNoise, whole grids (cnxp odd and even) and the
+    # 64x64x128 grid as checksums plus samples
+    for (nx, ny, nz), seed in (((8, 6, 10), 3), ((10, 7, 12), 5), ((9, 4, 6), 7)):
+        cnx = nx // 2 + 1
+        buf = np.zeros((nz, ny, cnx, 2), np.float32)
+        st = r.ref_fill_complex_noise(nx, ny, nz, seed, buf.ctypes.data_as(C.POINTER(C.c_float)))
+        g[f"noise_{nx}x{ny}x{nz}"] = buf
+        g[f"noise_{nx}x{ny}x{nz}_seed"] = np.uint32(seed)
+        g[f"noise_{nx}x{ny}x{nz}_state"] = np.uint32(st)
+    nx, ny, nz = 64, 64, 128
+    buf = np.zeros((nz, ny, nx // 2 + 1, 2), np.float32)
+    st = r.ref_fill_complex_noise(nx, ny, nz, 0, buf.ctypes.data_as(C.POINTER(C.c_float)))
+    flat = buf.reshape(-1, 2)
+    idx = np.sort(rng.choice(flat.shape[0], 4000, replace=False))
+    g["noise_c0_idx"] = idx
+    g["noise_c0_val"] = flat[idx]
+    g["noise_c0_sum"] = np.array([np.abs(flat.astype(np.float64)).sum(), flat.astype(np.float64).sum()])
+    g["noise_c0_state"] = np.uint32(st)
+
+    # FFTW 2.1.2 float transforms in CRAVA layout
+    for (nx, ny, nz) in ((8, 6, 10), (9, 7, 12), (12, 10, 16), (6, 5, 7), (16, 9, 15)):
+        rnx = 2 * (nx // 2 + 1)
+        x = rng.standard_normal((nz, ny, rnx)).astype(np.float32)
+        x[:, :, nx:] = 0.0
+        for ct in (1, 3):
+            X = O.ref_fft_r2c(x, nx, ct)
+            g[f"r2c_{nx}x{ny}x{nz}_{ct}_in"] = x
+            g[f"r2c_{nx}x{ny}x{nz}_{ct}_out"] = np.stack([X.real, X.imag], -1).astype(np.float32)
+        spec = (rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1)))
+        for ct in (2, 3):
+            y = O.ref_fft_c2r(spec.astype(np.complex64), nx, ct)
+            g[f"c2r_{nx}x{ny}x{nz}_{ct}_in"] = np.stack([spec.real, spec.imag], -1).astype(np.float32)
+            g[f"c2r_{nx}x{ny}x{nz}_{ct}_out"] = y
+
+    # lib_matrCholCpx and lib_matrAXeqBMatCpx
+    mats, flags, Ls, Bs, Xs = [], [], [], [], []
+    for t in range(120):
+        n = (3, 5)[t % 2]
+        G = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        A = G @ G.conj().T + (0.1 if t % 7 else -3.0) * np.eye(n)
+        A *= 10.0 ** rng.uniform(-4, 4)
+        a = np.zeros((5, 5, 2), np.float32)
+        a[:n, :n, 0], a[:n, :n, 1] = A.real, A.imag
+        mats.append(a.copy())
+        L = np.ascontiguousarray(a[:n, :n]).copy()
+        f = r.ref_chol_flat(n, L.ctypes.data_as(C.POINTER(C.c_float)))
+        flags.append(f)
+        l5 = np.zeros((5, 5, 2), np.float32)
+        l5[:n, :n] = L
+        Ls.append(l5)
+        B = rng.standard_normal((n, 3, 2)).astype(np.float32)
+        b5 = np.zeros((5, 3, 2), np.float32)
+        b5[:n] = B
+        Bs.append(b5.copy())
+        if f == 0:
+            r.ref_axeqb_flat(n, L.ctypes.data_as(C.POINTER(C.c_float)), B.ctypes.data_as(C.POINTER(C.c_float)), 3)
+        x5 = np.zeros((5, 3, 2), np.float32)
+        x5[:n] = B
+        Xs.append(x5)
+    g["chol_in"], g["chol_flag"], g["chol_L"] = np.array(mats), np.array(flags, np.int32), np.array(Ls)
+    g["axeqb_B"], g["axeqb_X"] = np.array(Bs), np.array(Xs)
+
+    # posterior loop and simulation scaling on a small synthetic survey
+    s = model.config_small(12, 10, 16, 3)
+    syn = model.make_synthetic(s)
+    pin = model.posterior_inputs(s, syn.data_var)
+    mean = [O.fft_r2c(m.astype(np.float32), s.nxp, 2).astype(np.complex64) for m in syn.mean]
+    data = [O.fft_r2c(d.astype(np.float32), s.nxp, 1).astype(np.complex64) for d in syn.data]
+    errc = O.fft_r2c(model.err_corr_grid(s), s.nxp, 3).astype(np.complex64)
+    cov = [O.fft_r2c(c, s.nxp, 3).astype(np.complex64) for c in model.prior_cov_grids(s)]
+    t = pin.tables
+    args = dict(kwd=t.kwd.astype(np.complex64), em1=t.em1.astype(np.complex64), em2=t.em2.astype(np.complex64),
+                em3=t.em3.astype(np.complex64), A=pin.A, theta=pin.theta, sigma0=pin.sigma0,
+                invert=pin.invert, wnc=s.wnc)
+    pm, pd, pc, nf = O.posterior_loop(mean, data, errc, cov, backend="reference", **args)
+    cz = lambda L_: np.array([np.stack([a.real, a.imag], -1) for a in L_], np.float32)
+    g["post_in_mean"], g["post_in_data"], g["post_in_err"], g["post_in_cov"] = cz(mean), cz(data), cz([errc]), cz(cov)
+    for k in ("kwd", "em1", "em2", "em3"):
+        g[f"post_{k}"] = np.stack([args[k].real, args[k].imag], -1).astype(np.float32)
+    g["post_A"], g["post_theta"], g["post_sigma0"] = pin.A, pin.theta, pin.sigma0
+    g["post_invert"], g["post_wnc"] = pin.invert, np.float32(s.wnc)
+    g["post_out_mean"], g["post_out_data"], g["post_out_cov"] = cz(pm), cz(pd), cz(pc)
+    g["post_nfail"] = np.int64(nf)
+    seeds = []
+    st = 0
+    for _ in range(3):
+        buf = np.zeros((s.nzp, s.nyp, s.cnxp, 2), np.float32)
+        st = r.ref_fill_complex_noise(s.nxp, s.nyp, s.nzp, st, buf.ctypes.data_as(C.POINTER(C.c_float)))
+        seeds.append(buf[..., 0] + 1j * buf[..., 1])
+    sc, nf2 = O.simulate_scale(pc, seeds, backend="reference")
+    g["sim_in_seed"], g["sim_out"], g["sim_nfail"] = cz(seeds), cz(sc), np.int64(nf2)
+
+    np.savez_compressed(OUT, **g)
+    print(f"wrote {OUT} ({OUT.stat().st_size / 1024:.0f} KiB, {len(g)} arrays)")
+
+
+if __name__ == "__main__":
+    main()

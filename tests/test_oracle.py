@@ -1,0 +1,276 @@
+"""The oracle pinned against the reference's own numerics (golden vectors
+written from oracle/_ref by tests/golden/make_golden.py), against closed
+forms and against brute force.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import cplx, rel_rms
+from oracle import oracle as O
+from crava_amd import model
+
+
+# ---------------------------------------------------------------- RandomGen
+@pytest.mark.parametrize("seed", [0, 1, 231171, 12345])
+def test_rng_stream_bit_exact(golden, seed):
+    ref = golden[f"rng_{seed}"]
+    got, st = O.rnorm_seq(seed, ref.size)
+    assert np.array_equal(got, ref)
+    assert st == int(golden[f"rng_{seed}_state"])
+
+
+def test_rng_stream_continues(golden):
+    # two calls with the state carried equal one long call
+    a, st = O.rnorm_seq(231171, 7000)
+    b, _ = O.rnorm_seq(st, 13000)
+    assert np.array_equal(np.concatenate([a, b]), golden["rng_231171"])
+
+
+def test_rng_distribution():
+    x, _ = O.rnorm_seq(99, 200000)
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1.0) < 0.01
+    # the Marsaglia-Bray tails are exact normals beyond |x| > 3
+    assert 0.0015 < np.mean(np.abs(x) > 3.0) < 0.0040
+
+
+# ---------------------------------------------------- fillInComplexNoise
+@pytest.mark.parametrize("shape", [(8, 6, 10), (10, 7, 12), (9, 4, 6)])
+def test_noise_fill_bit_exact(golden, shape):
+    nx, ny, nz = shape
+    key = f"noise_{nx}x{ny}x{nz}"
+    got, st = O.fill_complex_noise(nx, ny, nz, int(golden[key + "_seed"]))
+    ref = cplx(golden[key])
+    assert np.array_equal(got.astype(np.complex64), ref.astype(np.complex64))
+    assert st == int(golden[key + "_state"])
+
+
+def test_noise_fill_c0_samples(golden):
+    got, st = O.fill_complex_noise(64, 64, 128, 0)
+    flat = np.stack([got.real, got.imag], -1).reshape(-1, 2).astype(np.float32)
+    assert np.array_equal(flat[golden["noise_c0_idx"]], golden["noise_c0_val"])
+    assert st == int(golden["noise_c0_state"])
+
+
+def test_noise_draw_count_matches_stream(golden):
+    for nx, ny, nz in [(8, 6, 10), (10, 7, 12), (9, 4, 6), (64, 64, 128)]:
+        seed = 5
+        _, st = O.fill_complex_noise(nx, ny, nz, seed)
+        x, st2 = O.rnorm_seq(seed, O.noise_draws(nx, ny, nz))
+        assert st == st2
+
+
+def test_noise_hermitian_where_paired():
+    # x = 0 plane: c(j,k) = conj(c(nyp-j, nzp-k)), so its 2D inverse is real
+    nx, ny, nz = 10, 7, 12
+    c, _ = O.fill_complex_noise(nx, ny, nz, 11, np.float64)
+    p = c[:, :, 0]
+    jj = (-np.arange(ny)) % ny
+    kk = (-np.arange(nz)) % nz
+    assert np.allclose(p, np.conj(p[kk][:, jj]))
+
+
+# ------------------------------------------------------------- lib_matr
+def test_cholesky_bit_exact(golden):
+    for a, f, L in zip(golden["chol_in"], golden["chol_flag"], golden["chol_L"]):
+        n = 3 if not np.any(a[3:, :, :]) and not np.any(a[:, 3:, :]) else 5
+        A = cplx(a[:n, :n])
+        flag, Lg = O.chol(A.astype(np.complex64))
+        assert flag == f
+        if f == 0:
+            ref = cplx(L[:n, :n])
+            low = np.tril(np.ones((n, n), bool))
+            assert np.array_equal(Lg[low].astype(np.complex64), ref[low].astype(np.complex64))
+
+
+def test_cholesky_closed_form():
+    rng = np.random.default_rng(3)
+    for n in (1, 3, 5, 8):
+        G = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        A = G @ G.conj().T + np.eye(n)
+        flag, L = O.chol(A, np.float64)
+        L = np.tril(L)
+        assert flag == 0
+        assert np.allclose(L @ L.conj().T, A, rtol=1e-12, atol=1e-12)
+        assert np.allclose(L, np.linalg.cholesky(A), rtol=1e-10, atol=1e-12)
+
+
+def test_cholesky_rejects():
+    assert O.chol(np.diag([0.0, 1.0, 1.0]).astype(complex), np.float64)[0] == 1     # A00 <= 0
+    assert O.chol(np.diag([-1.0, 1.0, 1.0]).astype(complex), np.float64)[0] == 1
+    A = np.array([[1, 1, 0], [1, 1, 0], [0, 0, 1]], complex)                          # singular pivot
+    assert O.chol(A, np.float64)[0] == 1
+
+
+# ------------------------------------------------------ posterior loop
+def _post_args(golden):
+    return dict(kwd=cplx(golden["post_kwd"]), em1=cplx(golden["post_em1"]), em2=cplx(golden["post_em2"]),
+                em3=cplx(golden["post_em3"]), A=golden["post_A"], theta=golden["post_theta"],
+                sigma0=golden["post_sigma0"], invert=golden["post_invert"], wnc=float(golden["post_wnc"]))
+
+
+def test_posterior_loop_bit_exact(golden):
+    mean = [cplx(a) for a in golden["post_in_mean"]]
+    data = [cplx(a) for a in golden["post_in_data"]]
+    err = cplx(golden["post_in_err"][0])
+    cov = [cplx(a) for a in golden["post_in_cov"]]
+    pm, pd, pc, nf = O.posterior_loop(mean, data, err, cov, **_post_args(golden))
+    c64 = lambda x: np.asarray(x).astype(np.complex64)
+    for got, ref in zip(pm, golden["post_out_mean"]):
+        assert np.array_equal(c64(got), c64(cplx(ref)))
+    for got, ref in zip(pd, golden["post_out_data"]):
+        assert np.array_equal(c64(got), c64(cplx(ref)))
+    for got, ref in zip(pc, golden["post_out_cov"]):
+        assert np.array_equal(c64(got), c64(cplx(ref)))
+    assert nf == int(golden["post_nfail"])
+
+
+def test_posterior_fp64_matches_closed_form(golden):
+    """FP64 loop == mu + Sm K^H (K Sm K^H + Se)^-1 (d - K mu) per wavenumber."""
+    a = _post_args(golden)
+    mean = [cplx(x) for x in golden["post_in_mean"]]
+    data = [cplx(x) for x in golden["post_in_data"]]
+    err = cplx(golden["post_in_err"][0])
+    cov = [cplx(x) for x in golden["post_in_cov"]]
+    pm, pd, pc, nf = O.posterior_loop(mean, data, err, cov, dtype=np.float64, **a)
+    assert nf == 0
+    nzp, nyp, cnxp = mean[0].shape
+    n = len(data)
+    S0 = a["sigma0"]
+    A = a["A"].astype(np.float64)
+    th = a["theta"]
+    wnc = float(np.float32(a["wnc"]))
+    rng = np.random.default_rng(0)
+    checked = 0
+    for _ in range(300):
+        k, j, i = rng.integers(nzp), rng.integers(nyp), rng.integers(cnxp)
+        if not a["invert"][k]:
+            continue
+        pa, pb = (0, 1, 2, 0, 0, 1), (0, 1, 2, 1, 2, 2)
+        Sm = np.zeros((3, 3))
+        for q in range(6):
+            v = abs(cov[q][k, j, i].real / S0[pa[q], pb[q]]) * S0[pa[q], pb[q]]
+            Sm[pa[q], pb[q]] = Sm[pb[q], pa[q]] = v
+        K = a["kwd"][k][:, None] * A
+        lam = abs(err[k, j, i].real)
+        e1, e2, e3 = a["em1"][k], a["em2"][k], a["em3"][k]
+        Se = 0.5 * (1 - wnc) * th * lam * (np.outer(e1, e1.conj()) + np.outer(e2, e2.conj()))
+        Se = Se + np.diag(wnc * np.diag(th) * e3.real ** 2)
+        Se[np.diag_indices(n)] = Se[np.diag_indices(n)].real
+        mu = np.array([m[k, j, i] for m in mean])
+        d = np.array([x[k, j, i] for x in data])
+        mp, Sp, res = O.gaussian_update(Sm, K, Se, mu, d)
+        assert np.allclose([m[k, j, i] for m in pm], mp, rtol=1e-9, atol=1e-12)
+        assert np.allclose([x[k, j, i] for x in pd], res, rtol=1e-9, atol=1e-12)
+        got = np.array([c[k, j, i] for c in pc])
+        want = np.array([Sp[pa[q], pb[q]] for q in range(6)])
+        assert np.allclose(got, want, rtol=1e-8, atol=1e-14 * np.abs(Sm).max())
+        checked += 1
+    assert checked > 50
+
+
+def test_posterior_fp32_close_to_fp64(golden):
+    a = _post_args(golden)
+    ins = [[cplx(x) for x in golden["post_in_mean"]], [cplx(x) for x in golden["post_in_data"]],
+           cplx(golden["post_in_err"][0]), [cplx(x) for x in golden["post_in_cov"]]]
+    r32 = O.posterior_loop(*ins, dtype=np.float32, **a)
+    r64 = O.posterior_loop(*ins, dtype=np.float64, **a)
+    for g32, g64 in zip(r32[0] + r32[2], r64[0] + r64[2]):
+        assert rel_rms(g32, g64) < 1e-4
+
+
+def test_forced_cholesky_failure_keeps_prior(golden):
+    """A zero prior and zero error spectrum at one wavenumber make x00 <= 0:
+    posterior = prior, residual = data (crava.cpp:1421-1437)."""
+    a = _post_args(golden)
+    mean = [cplx(x) for x in golden["post_in_mean"]]
+    data = [cplx(x) for x in golden["post_in_data"]]
+    err = cplx(golden["post_in_err"][0])
+    cov = [cplx(x) for x in golden["post_in_cov"]]
+    k = int(np.nonzero(a["invert"])[0][0])
+    for c in cov:
+        c[k, 2, 3] = 0.0
+    err[k, 2, 3] = 0.0
+    a["em3"] = a["em3"].copy()
+    a["em3"][k] = 0.0
+    pm, pd, pc, nf = O.posterior_loop(mean, data, err, cov, dtype=np.float64, **a)
+    assert nf >= 1
+    for m0, m1 in zip(mean, pm):
+        assert m1[k, 2, 3] == m0[k, 2, 3]
+    for d0, d1 in zip(data, pd):
+        assert d1[k, 2, 3] == d0[k, 2, 3]
+
+
+# ------------------------------------------------------- simulate scaling
+def test_simulate_scale_bit_exact(golden):
+    cov = [cplx(x) for x in golden["post_out_cov"]]
+    seeds = [cplx(x) for x in golden["sim_in_seed"]]
+    got, nf = O.simulate_scale(cov, seeds)
+    for g, r in zip(got, golden["sim_out"]):
+        assert np.array_equal(g.astype(np.complex64), cplx(r).astype(np.complex64))
+    assert nf == int(golden["sim_nfail"])
+
+
+def test_simulate_scale_closed_form():
+    rng = np.random.default_rng(1)
+    G = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+    P = G @ G.conj().T + np.eye(3)
+    cov = [np.array([[[P[a, b]]]]) for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+    z = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    seeds = [np.array([[[v]]]) for v in z]
+    out, nf = O.simulate_scale(cov, seeds, dtype=np.float64)
+    assert nf == 0
+    assert np.allclose([o[0, 0, 0] for o in out], np.linalg.cholesky(P) @ z, rtol=1e-12)
+
+
+# ------------------------------------------------------------ transforms
+@pytest.mark.parametrize("shape", [(8, 6, 10), (9, 7, 12), (12, 10, 16), (6, 5, 7), (16, 9, 15)])
+@pytest.mark.parametrize("ct", [1, 3])
+def test_fp64_r2c_matches_fftw(golden, shape, ct):
+    nx, ny, nz = shape
+    x = golden[f"r2c_{nx}x{ny}x{nz}_{ct}_in"]
+    ref = cplx(golden[f"r2c_{nx}x{ny}x{nz}_{ct}_out"])
+    got = O.fft_r2c(x, nx, ct)
+    assert rel_rms(got, ref) < 2e-6
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 10), (9, 7, 12), (12, 10, 16), (6, 5, 7), (16, 9, 15)])
+@pytest.mark.parametrize("ct", [2, 3])
+def test_fp64_c2r_matches_fftw(golden, shape, ct):
+    nx, ny, nz = shape
+    spec = cplx(golden[f"c2r_{nx}x{ny}x{nz}_{ct}_in"])
+    ref = golden[f"c2r_{nx}x{ny}x{nz}_{ct}_out"][:, :, :nx]
+    got = O.fft_c2r(spec, nx, ct)[:, :, :nx]
+    assert rel_rms(got, ref) < 2e-6
+
+
+def test_brute_force_dft_tiny():
+    rng = np.random.default_rng(2)
+    nx, ny, nz = 5, 3, 4
+    x = rng.standard_normal((nz, ny, nx))
+    X = O.fft_r2c(np.pad(x, ((0, 0), (0, 0), (0, 1))), nx, 3)
+    for kz in range(nz):
+        for ky in range(ny):
+            for kx in range(nx // 2 + 1):
+                s = 0
+                for z in range(nz):
+                    for y in range(ny):
+                        for xx in range(nx):
+                            s += x[z, y, xx] * np.exp(-2j * np.pi * (kx * xx / nx + ky * y / ny + kz * z / nz))
+                assert abs(X[kz, ky, kx] - s) < 1e-10
+
+
+# --------------------------------------------------------------- padding
+def test_padding_brute_force():
+    def brute(n):
+        best = None
+        for f in range(n, 4 * n + 2):
+            m = f
+            for p in (2, 3, 5, 7):
+                while m % p == 0:
+                    m //= p
+            if m == 1:
+                return f
+        return best
+    for n in list(range(1, 600)) + [1000, 1234, 2047, 4096, 4097]:
+        assert O.find_closest_factorable(n) == brute(n) == model.find_closest_factorable(n)
