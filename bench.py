@@ -1,0 +1,318 @@
+#!/usr/bin/env python3
+"""Inverted voxels/s of CRAVA's Fourier-domain AVO inversion hot path on one
+MI355X: the 450x450x320 (padded) 5-angle survey, FP32 with one step of
+iterative refinement (FP64 for ill-conditioned and low-|k| wavenumbers),
+checked against the plain FP64 GPU path on the same inputs.
+
+One step = forward transforms of the 5 data cubes and 3 prior means ->
+per-wavenumber posterior (mean, residual, 3x3 covariance) -> inverse
+transforms of the 3 means -> one realization (noise, Cholesky scaling, 3
+inverse transforms with the posterior mean added) -> inverse transforms of
+the 6 covariance grids (Crava::computePostMeanResidAndFFTCov, simulate and
+invFFTCovGrids; reference src/crava.cpp:205-300).  Inputs are resident in
+HBM when the timed region starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c0]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_grids(ctx, s, prec):
+    from crava_amd import _lib as L
+    from crava_amd.hip import FFTGrid
+    mk = lambda ct: FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=prec, cubetype=ct, nx=s.nx, ny=s.ny, nz=s.nz)
+    return dict(
+        mean_in=[mk(L.CRV_PARAMETER) for _ in range(3)],
+        data_in=[mk(L.CRV_DATA) for _ in range(s.n)],
+        mean=[mk(L.CRV_PARAMETER) for _ in range(3)],
+        data=[mk(L.CRV_DATA) for _ in range(s.n)],
+        cov=[mk(L.CRV_COVARIANCE) for _ in range(6)],
+        sim=[mk(L.CRV_PARAMETER) for _ in range(3)],
+    )
+
+
+def step(ctx, G, kw, mode, realization, seed):
+    """One pass of the hot path (see module docstring)."""
+    from crava_amd import _lib as L
+    from crava_amd import hip
+    for w, src in zip(G["data"], G["data_in"]):
+        w.fftFrom(src)
+    for w, src in zip(G["mean"], G["mean_in"]):
+        w.fftFrom(src)
+    hip.computePostMeanResidAndFFTCov(ctx, G["mean"], G["data"], None, G["cov"], mode=mode, separable=True,
+                                      want_stats=False, **kw)
+    hip.simulate(ctx, G["cov"], G["mean"], G["sim"], noise_mode=L.CRV_NOISE_COUNTER, seed=seed,
+                 realization=realization, fp64_chol=(mode == L.CRV_MODE_FP32_REFINE))
+    for c in G["cov"]:
+        c.invFFTInPlace()
+
+
+def timed(ctx, G, kw, mode, steps, warmup, seed):
+    import torch
+    for r in range(warmup):
+        step(ctx, G, kw, mode, r, seed)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(steps):
+        step(ctx, G, kw, mode, 1000 + r, seed)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_bytes(name, s, prec, ninv):
+    """Algorithmic HBM bytes of one launch of a kernel (reads + writes of the
+    data it must touch), for the FP32 or FP64 grids of survey s."""
+    e = 4 if prec == 32 else 8
+    N = s.nxp * s.nyp * s.nzp
+    Nc = s.cnxp * s.nyp * s.nzp
+    n = s.n
+    if name in ("fft_y_fwd", "fft_z_fwd", "fft_y_inv", "fft_z_inv"):
+        return 2 * 2 * e * Nc
+    if name == "fft_x_r2c":
+        return e * N + 2 * e * Nc
+    if name == "fft_x_c2r":
+        return 2 * e * Nc + e * N            # (+ e*N when the mean is added)
+    if name in ("posterior", "posterior_refine"):
+        return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e
+    if name == "simulate_scale":
+        return Nc * (6 + 3) * 2 * e
+    return None
+
+
+def pipeline_bytes_per_voxel(n, prec):
+    """Algorithmic bytes per voxel of one step: (n + 15) transforms at one read
+    and one write of the real grid, the posterior's (8n + 58) and the mean add
+    (12); FP64 doubles it."""
+    b = 8 * (n + 15) + (8 * n + 58) + 12
+    return b * (2 if prec == 64 else 1)
+
+
+def compare(a64, b32, nx, mu=0.0):
+    """rel RMS and max relative error of b32 against a64 (real grids)."""
+    x = a64.download_real()[:, :, :nx].astype(np.float64) - mu
+    y = b32.download_real()[:, :, :nx].astype(np.float64) - mu
+    d = y - x
+    rms = float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(x * x)), 1e-300))
+    mx = float(np.max(np.abs(d)) / max(np.max(np.abs(x)), 1e-300))
+    return rms, mx
+
+
+def cpu_baseline(s, nthreads):
+    """The reference's CPU path on a bounded sample, scaled to one step.
+
+    With oracle/_ref (FFTW 2.1.2 float + lib_matr + RandomGen compiled from
+    the reference's sources): one R2C and one C2R at full size, the posterior
+    loop on 8 inverted and 8 non-inverted kz planes (OpenMP over kz as the
+    reference), noise fill and Cholesky scaling on 8 planes; extrapolated to
+    (n+3) R2C + 12 C2R and all planes.  Without it, the C restatement
+    (liboracle) with numpy's FFT in place of FFTW."""
+    from crava_amd import model
+    from oracle import oracle as O
+    import ctypes as C
+    kind = "reference" if O.ref_available() else "port"
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((s.nzp, s.nyp, s.rnxp)).astype(np.float32)
+    if kind == "reference":
+        r = O.rlib()
+        t = time.perf_counter(); r.ref_fft3d_r2c(s.nxp, s.nyp, s.nzp, x.ctypes.data_as(C.POINTER(C.c_float)), 1)
+        t_r2c = time.perf_counter() - t
+        t = time.perf_counter(); r.ref_fft3d_c2r(s.nxp, s.nyp, s.nzp, x.ctypes.data_as(C.POINTER(C.c_float)), 2)
+        t_c2r = time.perf_counter() - t
+    else:
+        t = time.perf_counter(); X = np.fft.rfftn(x[:, :, : s.nxp]); t_r2c = time.perf_counter() - t
+        t = time.perf_counter(); np.fft.irfftn(X, s=(s.nzp, s.nyp, s.nxp)); t_c2r = time.perf_counter() - t
+        del X
+    del x
+    inv = model.invert_mask(s)
+    kin = np.nonzero(inv)[0][:8]
+    kout = np.nonzero(inv == 0)[0][:8]
+    tab = model.kz_tables(s)
+    A = model.reflection_matrix(s.angles_deg, s.vsvp)
+    theta = np.eye(s.n) * 1e-3
+    S0 = s.sigma0()
+    times = {}
+    for tag, ks in (("in", kin), ("out", kout)):
+        nk = len(ks)
+        shp = (nk, s.nyp, s.cnxp)
+        cr = lambda: (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(np.complex64)
+        mean = [cr() for _ in range(3)]
+        data = [cr() for _ in range(s.n)]
+        err = np.abs(cr()).astype(np.complex64) + 1
+        cov = [np.full(shp, S0[a, b] * 50.0, np.complex64) for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+        sub = lambda t: t[ks].astype(np.complex64)
+        args = dict(kwd=sub(tab.kwd), em1=sub(tab.em1), em2=sub(tab.em2), em3=sub(tab.em3), A=A, theta=theta,
+                    sigma0=S0, invert=inv[ks], wnc=s.wnc, nthreads=nthreads)
+        t = time.perf_counter()
+        O.posterior_loop(mean, data, err, cov, backend="reference" if kind == "reference" else "oracle", **args)
+        times[tag] = (time.perf_counter() - t) / max(nk, 1)
+    t_post = times["in"] * int(inv.sum()) + times["out"] * int((inv == 0).sum())
+    nk = 8
+    t = time.perf_counter()
+    seeds = []
+    st = 0
+    for _ in range(3):
+        z, st = O.fill_complex_noise(s.nxp, s.nyp, nk, st)
+        seeds.append(z)
+    t_noise = (time.perf_counter() - t) * s.nzp / nk
+    covs = [np.full(seeds[0].shape, S0[a, b] * 50.0, np.complex64) for a, b in
+            ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+    t = time.perf_counter()
+    O.simulate_scale(covs, seeds, backend="reference" if kind == "reference" else "oracle")
+    t_scale = (time.perf_counter() - t) * s.nzp / nk
+    a = np.zeros((s.nzp, s.nyp, s.rnxp), np.float32)
+    b = np.ones_like(a)
+    t = time.perf_counter(); a += b; t_add = 3 * (time.perf_counter() - t)
+    total = (s.n + 3) * t_r2c + 12 * t_c2r + t_post + t_noise + t_scale + t_add
+    sample = (f"{'FFTW 2.1.2 float' if kind == 'reference' else 'numpy pocketfft'} R2C {t_r2c:.2f}s + C2R {t_c2r:.2f}s "
+              f"at {s.nxp}x{s.nyp}x{s.nzp} (x{s.n + 3} and x12); posterior loop ({'lib_matr' if kind == 'reference' else 'C restatement'}, "
+              f"OpenMP {nthreads} threads over kz) on 8 inverted + 8 non-inverted kz planes scaled to "
+              f"{int(inv.sum())}+{int((inv == 0).sum())}; noise fill + Cholesky scaling on 8 planes scaled to {s.nzp}; "
+              f"one step = {total:.1f} s")
+    return dict(value=s.voxels / total, unit="voxels/s", cores=nthreads, kind=kind, sample=sample)
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary, when present."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get(kernel)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus} GPUs, WORLD_SIZE={world}); "
+              "the multi-GPU layout is described, not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from crava_amd import _lib as L
+    from crava_amd import hip, model
+    from crava_amd.survey import GpuSurvey
+
+    s = model.config1() if a.config == "c1" else model.config0()
+    ctx = hip.Context(0)
+    log(f"[bench] survey {s.nxp}x{s.nyp}x{s.nzp}, {s.n} angles; generating inputs on the GPU")
+    sv = GpuSurvey(ctx, s)
+    kw = sv.posterior_kwargs()
+    ninv = int(sv.invert.sum()) * s.cnxp * s.nyp
+    res = {}
+    for prec, mode in ((32, L.CRV_MODE_FP32_REFINE), (64, L.CRV_MODE_PLAIN)):
+        if prec == 64 and a.no_fp64:
+            continue
+        G = make_grids(ctx, s, prec)
+        for g, src in zip(G["mean_in"], sv.mean):
+            g.copyFrom(src)
+        for g, src in zip(G["data_in"], sv.data):
+            g.copyFrom(src)
+        if prec == 64:
+            # the FP64 path sees exactly the FP32 path's inputs
+            for g in G["mean_in"] + G["data_in"]:
+                tmp = hip.FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=32, cubetype=g.getType())
+                tmp.copyFrom(g)
+                g.copyFrom(tmp)
+                tmp.close()
+        log(f"[bench] FP{prec} mode {mode}: {a.warmup} warmup + {a.steps} timed steps")
+        sec = timed(ctx, G, kw, mode, a.steps, a.warmup, seed=s.seed_sim)
+        # one profiled step for per-kernel durations (HIP events on the library stream)
+        ctx.profile(True)
+        for r in range(3):
+            step(ctx, G, kw, mode, 5000 + r, s.seed_sim)
+        ctx.profile(False)
+        prof = ctx.profile_report()
+        # reference step for accuracy (same realization for both precisions)
+        step(ctx, G, kw, mode, 77, s.seed_sim)
+        ctx.synchronize()
+        res[prec] = dict(sec=sec, prof=prof, grids=G)
+        log(f"[bench] FP{prec}: {sec * 1e3:.2f} ms/step, {s.voxels / sec:.3e} voxels/s")
+
+    # error table of FP32 + refinement against FP64
+    errors = None
+    if 64 in res:
+        G32, G64 = res[32]["grids"], res[64]["grids"]
+        errors = {}
+        for a_, nm in enumerate(("mean_vp", "mean_vs", "mean_rho")):
+            errors[nm] = compare(G64["mean"][a_], G32["mean"][a_], s.nxp, s.mu[a_])
+        for q, nm in enumerate(("cov_aa", "cov_bb", "cov_rr", "cov_ab", "cov_ar", "cov_br")):
+            errors[nm] = compare(G64["cov"][q], G32["cov"][q], s.nxp)
+        for a_, nm in enumerate(("real_vp", "real_vs", "real_rho")):
+            errors[nm] = compare(G64["sim"][a_], G32["sim"][a_], s.nxp, s.mu[a_])
+        errors = {k: {"rel_rms": v[0], "max_rel": v[1]} for k, v in errors.items()}
+        log("[bench] FP32+refine vs FP64: " + ", ".join(f"{k} {v['rel_rms']:.1e}" for k, v in errors.items()))
+
+    sec = res[32]["sec"]
+    value = s.voxels / sec
+    prof = res[32]["prof"]
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    name, (cnt, tot, mn, mx) = dom
+    avg_s = tot / cnt / 1e3
+    kb = kernel_bytes(name, s, 32, ninv)
+    roof = {"kernel": name, "bound": "hbm", "achieved": (kb / avg_s / 1e9) if kb else None, "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": (kb / avg_s / PEAK_HBM) if kb else None, "traffic": read_traffic(name),
+            "avg_ms": avg_s * 1e3, "bytes_per_launch": kb}
+    kernels = {k: {"launches": v[0], "avg_ms": v[1] / v[0],
+                   "GB/s": (kernel_bytes(k, s, 32, ninv) / (v[1] / v[0] / 1e3) / 1e9)
+                   if kernel_bytes(k, s, 32, ninv) else None} for k, v in prof.items()}
+    bpv = pipeline_bytes_per_voxel(s.n, 32)
+    out = {
+        "metric": "inverted voxels/s (mean+cov+1 sim), FP32-refined vs FP64 GPU at <=1e-4 rel RMS",
+        "value": value, "unit": "voxels/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64-refine", "data": "synthetic (GPU-generated survey: prior draw from RandomGen seed 1, "
+                                           "Aki-Richards + Ricker forward model, white noise at S/N 4)",
+        "config": {"workload": f"AVO inversion {s.nxp}x{s.nyp}x{s.nzp} padded, {s.n} angles "
+                               f"({'/'.join(str(int(x)) for x in s.angles_deg)} deg), separable prior, "
+                               "posterior mean + 6 cov + 1 realization",
+                   "voxels": s.voxels, "angles": s.n},
+        "roofline": roof,
+        "pipeline": {"bytes_per_voxel": bpv, "GB/s": bpv * value / 1e9, "frac": bpv * value / PEAK_HBM},
+        "fp64": ({"value": s.voxels / res[64]["sec"], "ms_per_step": res[64]["sec"] * 1e3,
+                  "speedup_fp32_refine": res[64]["sec"] / sec} if 64 in res else None),
+        "errors_vs_fp64": errors,
+        "kernels": kernels,
+    }
+    if not a.no_cpu:
+        log("[bench] CPU baseline (bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(s, a.cpu_threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -26,6 +26,8 @@ EXPORTS = [
     "crv_grid_download", "crv_grid_copy", "crv_grid_add", "crv_grid_multiply_by_scalar", "crv_fft_r2c",
     "crv_fft_c2r", "crv_fft_r2c_from", "crv_fft_c2r_add", "crv_posterior", "crv_simulate",
     "crv_fill_complex_noise", "crv_noise_draws", "crv_find_closest_factorable", "crv_padding_size",
+    "crv_ctx_profile", "crv_ctx_profile_report", "crv_fill_prior_spectra", "crv_grid_fill",
+    "crv_forward_model", "crv_add_white_noise", "crv_grid_moments",
 ]
 
 
@@ -97,6 +99,13 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_noise_draws": (C.c_longlong, [i, i, i]),
         "crv_find_closest_factorable": (i, [i]),
         "crv_padding_size": (i, [i, C.c_double]),
+        "crv_ctx_profile": (i, [vp, i]),
+        "crv_ctx_profile_report": (i, [vp, C.c_char_p, C.c_size_t]),
+        "crv_fill_prior_spectra": (i, [vp, _dp, _dp, _dp, C.POINTER(vp)]),
+        "crv_grid_fill": (i, [vp, C.c_double]),
+        "crv_forward_model": (i, [vp, i, C.POINTER(C.c_float), _dp, C.POINTER(vp), C.POINTER(vp)]),
+        "crv_add_white_noise": (i, [vp, C.c_double, C.c_ulonglong, C.c_ulonglong]),
+        "crv_grid_moments": (i, [vp, _dp, _dp, C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <algorithm>
+#include <cstdio>
 
 int crv_set_error(crv_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->last_error = msg;
@@ -178,10 +180,15 @@ int crv_grid_download(crv_grid* g, void* host, size_t nelem) {
 
 int crv_grid_copy(crv_grid* dst, const crv_grid* src) {
   if (!dst || !src) return CRV_E_INVALID;
-  if (dst->nxp != src->nxp || dst->nyp != src->nyp || dst->nzp != src->nzp || dst->precision != src->precision)
-    return crv_set_error(dst->ctx, CRV_E_INVALID, "copy: size/precision mismatch");
-  CRV_HIP_CHECK(dst->ctx, hipMemcpyAsync(dst->d, src->d, src->rsize * src->elem_bytes(), hipMemcpyDeviceToDevice,
-                                         dst->ctx->stream));
+  if (dst->nxp != src->nxp || dst->nyp != src->nyp || dst->nzp != src->nzp)
+    return crv_set_error(dst->ctx, CRV_E_INVALID, "copy: size mismatch");
+  if (dst->precision != src->precision) {
+    const int rc = crv_launch_convert(dst->ctx, dst->precision, dst->d, src->precision, src->d, src->rsize);
+    if (rc != CRV_OK) return rc;
+  } else {
+    CRV_HIP_CHECK(dst->ctx, hipMemcpyAsync(dst->d, src->d, src->rsize * src->elem_bytes(), hipMemcpyDeviceToDevice,
+                                           dst->ctx->stream));
+  }
   dst->transformed = src->transformed;
   dst->cubetype = src->cubetype;
   return CRV_OK;
@@ -280,3 +287,41 @@ int crv_padding_size(int n, double pad_fac) {
 }
 
 }  // extern "C"
+
+extern "C" int crv_ctx_profile(crv_ctx* ctx, int enable) {
+  if (!ctx) return CRV_E_INVALID;
+  ctx->prof_on = enable != 0;
+  return CRV_OK;
+}
+
+extern "C" int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len) {
+  if (!ctx) return CRV_E_INVALID;
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  struct Acc { long long n = 0; double tot = 0, mn = 1e30, mx = 0; };
+  std::map<std::string, Acc> acc;
+  std::vector<std::string> order;
+  for (auto& e : ctx->prof) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      if (!acc.count(e.name)) order.push_back(e.name);
+      Acc& a = acc[e.name];
+      a.n++; a.tot += ms; a.mn = std::min(a.mn, (double)ms); a.mx = std::max(a.mx, (double)ms);
+    }
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  ctx->prof.clear();
+  std::string out;
+  char line[256];
+  for (auto& nm : order) {
+    const Acc& a = acc[nm];
+    snprintf(line, sizeof(line), "%s %lld %.6f %.6f %.6f\n", nm.c_str(), a.n, a.tot, a.mn, a.mx);
+    out += line;
+  }
+  if (buf && len) {
+    const size_t k = std::min(len - 1, out.size());
+    memcpy(buf, out.data(), k);
+    buf[k] = 0;
+  }
+  return (int)out.size() + 1;
+}

@@ -442,6 +442,7 @@ int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale) {
     a.divL = make_fastdiv(p->fx.L); a.divL1 = make_fastdiv(cnxp);
     a.scale = (T)scale;
     const int nb = (nrows + p->rows_per_block - 1) / p->rows_per_block;
+    ProfScope ps(ctx, "fft_x_r2c");
     hipLaunchKernelGGL(k_rows_r2c<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }
@@ -451,6 +452,7 @@ int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale) {
     a.data = cd; a.f = p->fy; a.tw = tw; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp;
     a.ncols = cnxp; a.cshift = ilog2(p->tx_y); a.ntx = (cnxp + p->tx_y - 1) / p->tx_y;
     const int nb = a.ntx * p->nzp;
+    ProfScope ps(ctx, "fft_y_fwd");
     hipLaunchKernelGGL((k_cols<T, false>), dim3(nb), dim3(kThreads), (size_t)p->fy.L * p->tx_y * sizeof(C), ctx->stream, a);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }
@@ -459,6 +461,7 @@ int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale) {
     a.data = cd; a.f = p->fz; a.tw = tw; a.estride = (long long)cnxp * p->nyp; a.ostride = cnxp;
     a.ncols = cnxp; a.cshift = ilog2(p->tx_z); a.ntx = (cnxp + p->tx_z - 1) / p->tx_z;
     const int nb = a.ntx * p->nyp;
+    ProfScope ps(ctx, "fft_z_fwd");
     hipLaunchKernelGGL((k_cols<T, false>), dim3(nb), dim3(kThreads), (size_t)p->fz.L * p->tx_z * sizeof(C), ctx->stream, a);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }
@@ -477,6 +480,7 @@ int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const T* 
     a.data = cd; a.f = p->fz; a.tw = tw; a.estride = (long long)cnxp * p->nyp; a.ostride = cnxp;
     a.ncols = cnxp; a.cshift = ilog2(p->tx_z); a.ntx = (cnxp + p->tx_z - 1) / p->tx_z;
     const int nb = a.ntx * p->nyp;
+    ProfScope ps(ctx, "fft_z_inv");
     hipLaunchKernelGGL((k_cols<T, true>), dim3(nb), dim3(kThreads), (size_t)p->fz.L * p->tx_z * sizeof(C), ctx->stream, a);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }
@@ -485,6 +489,7 @@ int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const T* 
     a.data = cd; a.f = p->fy; a.tw = tw; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp;
     a.ncols = cnxp; a.cshift = ilog2(p->tx_y); a.ntx = (cnxp + p->tx_y - 1) / p->tx_y;
     const int nb = a.ntx * p->nzp;
+    ProfScope ps(ctx, "fft_y_inv");
     hipLaunchKernelGGL((k_cols<T, true>), dim3(nb), dim3(kThreads), (size_t)p->fy.L * p->tx_y * sizeof(C), ctx->stream, a);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }
@@ -497,6 +502,7 @@ int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const T* 
     a.divL = make_fastdiv(p->fx.L); a.divL1 = make_fastdiv(cnxp);
     a.scale = (T)scale;
     const int nb = (nrows + p->rows_per_block - 1) / p->rows_per_block;
+    ProfScope ps(ctx, "fft_x_c2r");
     hipLaunchKernelGGL(k_rows_c2r<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }

@@ -51,7 +51,14 @@ struct FftPlan {
   long long noise_draws = 0;
 };
 
+struct ProfEvent {
+  const char* name;
+  hipEvent_t a, b;
+};
+
 struct crv_ctx {
+  bool prof_on = false;
+  std::vector<ProfEvent> prof;
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -87,6 +94,23 @@ int crv_set_error(crv_ctx* ctx, int code, const std::string& msg);
                                                  hipGetErrorString(e__));            \
   } while (0)
 
+// Brackets the launches of its scope with events when profiling is on.
+struct ProfScope {
+  crv_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(crv_ctx* ctx, const char* n) : c(ctx), name(n) {
+    if (c->prof_on && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, c->stream);
+  }
+  ~ProfScope() {
+    if (a && b) {
+      (void)hipEventRecord(b, c->stream);
+      c->prof.push_back(ProfEvent{name, a, b});
+    }
+  }
+};
+
 // plan cache (fft.hip) -------------------------------------------------------
 int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** out);
 void crv_free_plan(FftPlan* p);
@@ -101,6 +125,7 @@ int crv_launch_c2r(crv_ctx* ctx, FftPlan* plan, void* buf, void* dst, double sca
 // elementwise (simulate.hip) ---------------------------------------------------
 int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n);
 int crv_launch_scale(crv_ctx* ctx, int prec, void* dst, double s, size_t n);
+int crv_launch_convert(crv_ctx* ctx, int dprec, void* dst, int sprec, const void* src, size_t n);
 
 // noise (noise.hip) ------------------------------------------------------------
 int crv_noise_prefix(crv_ctx* ctx, FftPlan* plan);

@@ -318,6 +318,7 @@ int crv_launch_fill_noise(crv_ctx* ctx, FftPlan* plan, crv_grid* g, long long fi
   NoiseGeom geo = make_noise_geom(plan);
   const long long n = (long long)geo.cnxp * geo.nyp * geo.nzp;
   const int nb = (int)std::min<long long>((n + kThreads - 1) / kThreads, 65536);
+  ProfScope ps(ctx, "noise_fill");
   if (g->precision == CRV_FP64)
     hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(kThreads), 0, ctx->stream, (double2*)g->d, geo, ctx->normals, first);
   else

@@ -379,6 +379,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior_list64(PostArgs<float> a
 
 template <typename T>
 int launch_plain(crv_ctx* ctx, int n, const PostArgs<T>& a, int nblocks) {
+  ProfScope ps(ctx, "posterior");
   switch (n) {
 #define CASE(NN) case NN: hipLaunchKernelGGL((k_posterior<T, NN>), dim3(nblocks), dim3(kThreads), 0, ctx->stream, a); break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
@@ -393,8 +394,14 @@ int launch_refine(crv_ctx* ctx, int n, const PostArgs<float>& a, int nblocks) {
   switch (n) {
 #define CASE(NN)                                                                                   \
   case NN:                                                                                         \
-    hipLaunchKernelGGL((k_posterior_refine<NN>), dim3(nblocks), dim3(kThreads), 0, ctx->stream, a); \
-    hipLaunchKernelGGL((k_posterior_list64<NN>), dim3(1024), dim3(kThreads), 0, ctx->stream, a);    \
+    {                                                                                              \
+      ProfScope p1(ctx, "posterior_refine");                                                       \
+      hipLaunchKernelGGL((k_posterior_refine<NN>), dim3(nblocks), dim3(kThreads), 0, ctx->stream, a); \
+    }                                                                                              \
+    {                                                                                              \
+      ProfScope p2(ctx, "posterior_fp64_list");                                                    \
+      hipLaunchKernelGGL((k_posterior_list64<NN>), dim3(1024), dim3(kThreads), 0, ctx->stream, a);  \
+    }                                                                                              \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
@@ -446,8 +453,7 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
   for (int l = 0; l < n; ++l) if ((rc = ensure_transformed(data[l])) != CRV_OK) return rc;
   if (!p->separable) {
     for (int q = 0; q < 6; ++q) if ((rc = ensure_transformed(cov[q])) != CRV_OK) return rc;
-    if (err_corr->transformed) return crv_set_error(ctx, CRV_E_STATE, "err_corr already transformed");
-    if ((rc = crv_fft_r2c(err_corr)) != CRV_OK) return rc;
+    if ((rc = ensure_transformed(err_corr)) != CRV_OK) return rc;
   }
 
   // ---- kz tables (crava.cpp:1104-1121, 1336-1364, 1441-1469) -------------
@@ -455,8 +461,10 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
   std::vector<int> inv(nzp);
   const double wnc = p->wnc;
   for (int k = 0; k < nzp; ++k) {
-    const float fr = (float)((p->nz * 1000.0f) / ((float)p->lz * nzp) * (float)std::min(k, nzp - k));
-    inv[k] = (fr > (float)p->low_cut * (float)p->min_rel_thick && fr < (float)p->high_cut) ? 1 : 0;
+    // crava.cpp:1104-1105: float numerator over a double product, compared
+    // with lowCut * minRelThick in double and with highCut in float
+    const float fr = (float)((double)(p->nz * 1000.0f) / (p->lz * nzp) * (double)std::min(k, nzp - k));
+    inv[k] = ((double)fr > p->low_cut * p->min_rel_thick && fr < (float)p->high_cut) ? 1 : 0;
     for (int l = 0; l < n; ++l) {
       const double wr = p->kwd[((size_t)k * n + l) * 2], wi = p->kwd[((size_t)k * n + l) * 2 + 1];
       for (int m = 0; m < 3; ++m) {

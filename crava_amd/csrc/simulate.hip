@@ -90,11 +90,32 @@ __global__ void k_scale(T* d, T s, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] *= s;
 }
 
+template <typename S, typename D>
+__global__ void k_convert(D* __restrict__ d, const S* __restrict__ s, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = (D)s[i];
+}
+
 int nblocks_for(size_t n) { return (int)std::min<size_t>((n + kThreads - 1) / kThreads, 65536); }
 
 }  // namespace
 
+int crv_launch_convert(crv_ctx* ctx, int dprec, void* dst, int sprec, const void* src, size_t n) {
+  ProfScope ps(ctx, "convert");
+  if (dprec == CRV_FP64 && sprec == CRV_FP32)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks_for(n)), dim3(kThreads), 0, ctx->stream, (double*)dst,
+                       (const float*)src, n);
+  else if (dprec == CRV_FP32 && sprec == CRV_FP64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks_for(n)), dim3(kThreads), 0, ctx->stream, (float*)dst,
+                       (const double*)src, n);
+  else
+    return crv_set_error(ctx, CRV_E_INVALID, "convert: unsupported precision pair");
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
 int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n) {
+  ProfScope ps(ctx, "axpy");
   if (prec == CRV_FP64)
     hipLaunchKernelGGL(k_axpy<double>, dim3(nblocks_for(n)), dim3(kThreads), 0, ctx->stream, (double*)dst, (const double*)src, n);
   else
@@ -104,6 +125,7 @@ int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n
 }
 
 int crv_launch_scale(crv_ctx* ctx, int prec, void* dst, double s, size_t n) {
+  ProfScope ps(ctx, "scale");
   if (prec == CRV_FP64)
     hipLaunchKernelGGL(k_scale<double>, dim3(nblocks_for(n)), dim3(kThreads), 0, ctx->stream, (double*)dst, s, n);
   else
@@ -158,6 +180,8 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
   }
   const int nb = (int)std::min<long long>((nc + kThreads - 1) / kThreads, 256LL * 64);
   const size_t gb = (size_t)nc * 2 * g0->elem_bytes();
+  {
+  ProfScope ps(ctx, "simulate_scale");
   if (f64) {
     SimArgs<double> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const double2*)cov[q]->d;
@@ -177,6 +201,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
       hipLaunchKernelGGL((k_simulate<float, double>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
     else
       hipLaunchKernelGGL((k_simulate<float, float>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+  }
   }
   CRV_HIP_CHECK(ctx, hipGetLastError());
   if (hostbuf) CRV_HIP_CHECK(ctx, hipFreeAsync(hostbuf, ctx->stream));

@@ -38,6 +38,25 @@ class Context:
     def synchronize(self):
         self.check(self.lib.crv_ctx_synchronize(self.h), "synchronize")
 
+    def set_stream(self, stream_handle: int | None):
+        """Run on an external HIP stream (e.g. torch.cuda.current_stream().cuda_stream)."""
+        self.check(self.lib.crv_ctx_set_stream(self.h, stream_handle), "set_stream")
+
+    def profile(self, on: bool):
+        self.check(self.lib.crv_ctx_profile(self.h, int(on)), "profile")
+
+    def profile_report(self) -> dict:
+        """{kernel: (launches, total_ms, min_ms, max_ms)} of the recorded launches."""
+        buf = C.create_string_buffer(1 << 16)
+        n = self.lib.crv_ctx_profile_report(self.h, buf, len(buf))
+        if n < 0:
+            self.check(n, "profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, tot, mn, mx = line.split()
+            out[name] = (int(cnt), float(tot), float(mn), float(mx))
+        return out
+
     def close(self):
         if self.h:
             self.lib.crv_ctx_destroy(self.h)
@@ -160,7 +179,8 @@ class PostResult:
 def computePostMeanResidAndFFTCov(ctx: Context, mean, data, err_corr, cov, *, A, theta, tables, nz, lz,
                                   low_cut=5.0, high_cut=55.0, wnc=0.1, min_rel_thick=1.0, sigma0,
                                   mode=L.CRV_MODE_PLAIN, cond_threshold=1e5, lowk_fraction=0.02,
-                                  separable=False, cxy=None, ct=None, skip_mean_inverse=False) -> PostResult:
+                                  separable=False, cxy=None, ct=None, skip_mean_inverse=False,
+                                  want_stats=True) -> PostResult | None:
     """Crava::computePostMeanResidAndFFTCov over FFTGrid objects (crava.h:43-64)."""
     n = len(data)
     keep = []
@@ -187,7 +207,9 @@ def computePostMeanResidAndFFTCov(ctx: Context, mean, data, err_corr, cov, *, A,
     covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
     st = L.PostStats()
     ctx.check(ctx.lib.crv_posterior(ctx.h, C.byref(p), means, datas, err_corr.h if err_corr is not None else None,
-                                    covs, C.byref(st)), "computePostMeanResidAndFFTCov")
+                                    covs, C.byref(st) if want_stats else None), "computePostMeanResidAndFFTCov")
+    if not want_stats:
+        return None
     return PostResult(st.n_wavenumbers, st.n_inverted, st.n_chol_fail, st.n_fp64, st.max_cond)
 
 
@@ -213,6 +235,38 @@ def simulate(ctx: Context, cov, post_mean, out, *, noise_mode=L.CRV_NOISE_COUNTE
     outs = (C.c_void_p * 3)(*[g.h.value for g in out])
     ctx.check(ctx.lib.crv_simulate(ctx.h, covs, pm, C.byref(spec), int(fp64_chol), outs), "simulate")
     return st.value
+
+
+def fill_prior_spectra(ctx: Context, sigma0, cxy, ct, cov):
+    s0 = np.ascontiguousarray(sigma0, np.float64)
+    cx = np.ascontiguousarray(cxy, np.float64)
+    c_t = np.ascontiguousarray(ct, np.float64)
+    covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+    ctx.check(ctx.lib.crv_fill_prior_spectra(ctx.h, _arr(s0, C.c_double), _arr(cx, C.c_double),
+                                             _arr(c_t, C.c_double), covs), "fill_prior_spectra")
+
+
+def grid_fill(g: FFTGrid, value: float):
+    g.ctx.check(g.ctx.lib.crv_grid_fill(g.h, float(value)), "grid_fill")
+
+
+def forward_model(ctx: Context, A, kwd_interleaved, model, data):
+    A_ = np.ascontiguousarray(A, np.float32)
+    K = np.ascontiguousarray(kwd_interleaved, np.float64)
+    ms = (C.c_void_p * 3)(*[g.h.value for g in model])
+    ds = (C.c_void_p * len(data))(*[g.h.value for g in data])
+    ctx.check(ctx.lib.crv_forward_model(ctx.h, len(data), _arr(A_, C.c_float), _arr(K, C.c_double), ms, ds),
+              "forward_model")
+
+
+def add_white_noise(g: FFTGrid, sigma: float, seed: int, stream: int):
+    g.ctx.check(g.ctx.lib.crv_add_white_noise(g.h, float(sigma), seed, stream), "add_white_noise")
+
+
+def moments(g: FFTGrid):
+    s, s2, n = C.c_double(), C.c_double(), C.c_longlong()
+    g.ctx.check(g.ctx.lib.crv_grid_moments(g.h, C.byref(s), C.byref(s2), C.byref(n)), "moments")
+    return s.value, s2.value, n.value
 
 
 def noise_draws(nxp, nyp, nzp) -> int:

@@ -76,6 +76,14 @@ int         crv_ctx_synchronize(crv_ctx* ctx);
 int         crv_ctx_set_stream(crv_ctx* ctx, void* hip_stream);
 /* Library version string, e.g. "crava_hip 1.0 gfx950". */
 const char* crv_version(void);
+/* Kernel timing with HIP events on the context's stream (no counterpart in
+ * the reference, whose phase timers are src/timings.h:11-81).  While enabled
+ * every kernel launch is bracketed by a pair of events.  The report
+ * synchronises, then writes one line per kernel name,
+ * "name launches total_ms min_ms max_ms", clears the record and returns the
+ * number of bytes needed (buf may be NULL to query).                        */
+int crv_ctx_profile(crv_ctx* ctx, int enable);
+int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
 
 /* ---- grids --------------------------------------------------------------
  * crv_grid_create replaces FFTGrid::FFTGrid + createRealGrid
@@ -99,7 +107,8 @@ int crv_grid_device_ptr(const crv_grid* g, void** dptr);
 int crv_grid_upload_real(crv_grid* g, const void* host, size_t nelem);
 int crv_grid_upload_complex(crv_grid* g, const void* host, size_t nelem);
 int crv_grid_download(crv_grid* g, void* host, size_t nelem);
-/* Device-to-device copy of buffer, flag and type (same sizes/precision). */
+/* Device-to-device copy of buffer, flag and type (same sizes); when the
+ * precisions differ every element is converted (rounded to nearest). */
 int crv_grid_copy(crv_grid* dst, const crv_grid* src);
 /* FFTGrid::add (src/fftgrid.cpp:1870-1891): dst += src over the whole real
  * buffer (untransformed) or complex buffer (transformed). */
@@ -246,6 +255,32 @@ int crv_fill_complex_noise(crv_grid* g, unsigned int* lcg_state);
 /* Number of N(0,1) draws FFTGrid::fillInComplexNoise makes on one grid of
  * this padded size (a function of nxp, nyp, nzp only). */
 long long crv_noise_draws(int nxp, int nyp, int nzp);
+
+/* ---- inputs of the path generated on the GPU ----------------------------
+ * crv_fill_prior_spectra: the six prior covariance spectra that
+ * SeismicParametersHolder::FFTCovGrids + getParameterCovariance yield for
+ * zero correlation gradients (seismicparametersholder.cpp:198-222, 299-322,
+ * 408-426): cov[q] = Sigma0[a][b] |cxy[kx,ky] ct[kz]| (im 0), transformed,
+ * COVARIANCE.  cxy is cnxp*nyp doubles (row ky), ct is nzp doubles.        */
+int crv_fill_prior_spectra(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct,
+                           crv_grid* cov[6]);
+/* Real grid set to value in columns i < nxp, 0 in the pad (untransformed).
+ * FFTGrid::fillInConstant (fftgrid.cpp:875-892) with a zero pad.           */
+int crv_grid_fill(crv_grid* g, double value);
+/* Crava::computeSyntSeismic in the Fourier domain (crava.cpp:1767-1894):
+ * data[l](k) = kwd[kz][l] * sum_m A[l][m] model[m](k), with kwd as in
+ * crv_post_params (kD * conj(W_l)); model transformed; data become
+ * transformed DATA grids.                                                   */
+int crv_forward_model(crv_ctx* ctx, int ntheta, const float* A, const double* kwd, crv_grid* model[3],
+                      crv_grid** data);
+/* Adds N(0, sigma^2) white noise (counter stream keyed by seed, stream) to
+ * the columns i < nxp of a real grid (the seismic noise of the synthetic
+ * surveys).                                                                 */
+int crv_add_white_noise(crv_grid* g, double sigma, unsigned long long seed, unsigned long long stream);
+/* Sum, sum of squares and count of the non-zero samples of the un-padded box
+ * (i < nx, j < ny, k < nz): Crava::computeDataVariance (crava.cpp:407-449)
+ * is sum2 / count.                                                          */
+int crv_grid_moments(crv_grid* g, double* sum, double* sum2, long long* count);
 
 /* ---- padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622) -- */
 /* Smallest 2^a 3^b 5^c 7^d >= leastint (FFTGrid::findClosestFactorableNumber). */

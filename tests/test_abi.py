@@ -1,0 +1,57 @@
+"""The C ABI library loads without a GPU and exports every entry point that
+include/crava_hip.h declares; the host-only entry points work on the CPU."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    txt = (ROOT / "include" / "crava_hip.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(crv_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree():
+    from crava_amd import _lib
+    assert sorted(_lib.EXPORTS) == _declared()
+
+
+def test_library_exports_every_symbol():
+    from crava_amd import _lib
+    lib = ctypes.CDLL(str(_lib.LIBPATH))
+    missing = [s for s in _declared() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_host_entry_points_without_gpu():
+    from crava_amd import _lib
+    from oracle import oracle as O
+    lib = _lib.lib()
+    assert lib.crv_version().startswith(b"crava_hip")
+    for n in (1, 7, 11, 13, 97, 450, 451, 1000, 4097):
+        assert lib.crv_find_closest_factorable(n) == O.find_closest_factorable(n)
+    for dims in ((8, 6, 10), (9, 4, 6), (64, 64, 128), (450, 450, 320)):
+        assert lib.crv_noise_draws(*dims) == O.noise_draws(*dims)
+    assert lib.crv_padding_size(320, 0.0625) == 343
+
+
+def test_context_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from crava_amd import hip
+    with pytest.raises(hip.CravaError):
+        hip.Context(0)
+
+
+def test_product_does_not_import_oracle():
+    for p in (ROOT / "crava_amd").rglob("*.py"):
+        src = p.read_text()
+        assert "import oracle" not in src and "from oracle" not in src, p
+    for p in list((ROOT / "crava_amd" / "csrc").rglob("*")):
+        if p.is_file():
+            assert "oracle" not in p.read_text(errors="ignore").lower(), p

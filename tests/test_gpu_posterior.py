@@ -1,0 +1,141 @@
+"""Crava::computePostMeanResidAndFFTCov on the GPU against the reference's
+own float loop (golden vectors from the compiled lib_matr), the FP64 oracle,
+and the closed-form special cases."""
+import numpy as np
+import pytest
+
+from conftest import cplx, rel_rms
+from crava_amd import model
+from pipeline import Case, gpu_posterior, oracle_posterior
+
+pytestmark = pytest.mark.gpu
+
+
+def _golden_run(ctx, golden, prec, mode=0):
+    from crava_amd.hip import FFTGrid, computePostMeanResidAndFFTCov
+    from crava_amd.model import KzTables
+    s = model.config_small(12, 10, 16, 3)
+    mk = lambda ct: FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=prec, cubetype=ct)
+    mean = [mk(2) for _ in range(3)]
+    data = [mk(1) for _ in range(3)]
+    cov = [mk(3) for _ in range(6)]
+    err = mk(3)
+    for g, a in zip(mean, golden["post_in_mean"]):
+        g.upload_complex(cplx(a))
+    for g, a in zip(data, golden["post_in_data"]):
+        g.upload_complex(cplx(a))
+    for g, a in zip(cov, golden["post_in_cov"]):
+        g.upload_complex(cplx(a))
+    err.upload_complex(cplx(golden["post_in_err"][0]))
+    tab = KzTables(cplx(golden["post_kwd"]), cplx(golden["post_em1"]), cplx(golden["post_em2"]),
+                   cplx(golden["post_em3"]))
+    st = computePostMeanResidAndFFTCov(ctx, mean, data, err, cov, A=golden["post_A"], theta=golden["post_theta"],
+                                       tables=tab, nz=s.nz, lz=s.lz, sigma0=golden["post_sigma0"], mode=mode,
+                                       skip_mean_inverse=True)
+    return mean, data, cov, st
+
+
+@pytest.mark.parametrize("prec,mode,tol", [(32, 0, 2e-5), (64, 0, 2e-5), (32, 1, 2e-5)])
+def test_matches_reference_float_loop(ctx, golden, prec, mode, tol):
+    """Same spectra in, compared with the reference's compiled float loop."""
+    mean, data, cov, st = _golden_run(ctx, golden, prec, mode)
+    assert st.n_chol_fail == int(golden["post_nfail"])
+    assert st.n_inverted == int(golden["post_invert"].sum()) * 10 * 7
+    for g, r in zip(mean, golden["post_out_mean"]):
+        assert rel_rms(g.download_complex(), cplx(r)) < tol
+    for g, r in zip(data, golden["post_out_data"]):
+        assert rel_rms(g.download_complex(), cplx(r)) < tol
+    for g, r in zip(cov, golden["post_out_cov"]):
+        assert rel_rms(g.download_complex(), cplx(r)) < tol
+
+
+@pytest.fixture(scope="module")
+def case_small():
+    return Case(model.config_small(20, 18, 30, 5))
+
+
+@pytest.fixture(scope="module")
+def oracle_small(case_small):
+    return oracle_posterior(case_small, np.float64)
+
+
+def test_fp64_gpu_equals_fp64_oracle(ctx, case_small, oracle_small):
+    r = gpu_posterior(ctx, case_small, prec=64)
+    o = oracle_small
+    for g, m in zip(r["mean"], o["mean"]):
+        assert rel_rms(g.download_real()[:, :, :20], m[:, :, :20]) < 1e-12
+    for g, d in zip(r["data"], o["data"]):
+        assert rel_rms(g.download_complex(), d) < 1e-11
+    for g, c in zip(r["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-11
+    assert r["stats"].n_chol_fail == o["nfail"] == 0
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_separable_prior_equals_grids(ctx, case_small, prec):
+    a = gpu_posterior(ctx, case_small, prec=prec)
+    b = gpu_posterior(ctx, case_small, prec=prec, separable=True)
+    tol = 2e-6 if prec == 32 else 1e-11
+    for x, y in zip(a["mean"], b["mean"]):
+        assert rel_rms(x.download_real()[:, :, :20], y.download_real()[:, :, :20]) < tol
+    for x, y in zip(a["cov"], b["cov"]):
+        assert rel_rms(x.download_complex(), y.download_complex()) < tol
+
+
+def test_fp32_refine_close_to_fp64(ctx, case_small, oracle_small):
+    r = gpu_posterior(ctx, case_small, prec=32, mode=1)
+    o = oracle_small
+    for g, m in zip(r["mean"], o["mean"]):
+        assert rel_rms(g.download_real()[:, :, :20], m[:, :, :20]) < 1e-6
+    for g, c in zip(r["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-5
+    st = r["stats"]
+    assert st.n_inverted > 0 and st.max_cond > 1.0
+
+
+def test_non_inverted_band_keeps_cleaned_prior(ctx, case_small):
+    """Outside the frequency band the means/residuals are untouched and the
+    covariance is the cleaned prior Sigma0 |Re c / Sigma0| (im = 0)."""
+    s = case_small.s
+    r = gpu_posterior(ctx, case_small, prec=64, skip_mean_inverse=True)
+    inv = case_small.pin.invert.astype(bool)
+    from oracle import oracle as O
+    data0 = O.fft_r2c(case_small.syn.data[0], s.nxp, 1)
+    cov0 = O.fft_r2c(case_small.cov_real[0], s.nxp, 3)
+    d = r["data"][0].download_complex()
+    c = r["cov"][0].download_complex()
+    assert np.allclose(d[~inv], data0[~inv], rtol=1e-12, atol=1e-12)
+    want = case_small.pin.sigma0[0, 0] * np.abs(cov0[~inv].real / case_small.pin.sigma0[0, 0])
+    assert np.allclose(c[~inv].real, want, rtol=1e-12) and np.all(c[~inv].imag == 0)
+
+
+def test_forced_cholesky_failure_is_prior(ctx):
+    """wnc = 0 and a zero prior at one wavenumber: x00 = 0 fails the
+    factorisation, so posterior = prior and residual = data there."""
+    case = Case(model.config_small(8, 6, 16, 3))
+    case.s.wnc = 0.0
+    k = int(np.nonzero(case.pin.invert)[0][0])
+    # zero the temporal correlation spectrum at kz = k through the grids
+    from oracle import oracle as O
+    for c in case.cov_real:
+        spec = O.fft_r2c(c, 8, 3)
+        spec[k] = 0.0
+        c[:] = O.fft_c2r(spec, 8, 3)
+    case.err_real[:] = 0.0
+    r = gpu_posterior(ctx, case, prec=64, skip_mean_inverse=True)
+    assert r["stats"].n_chol_fail >= 6 * 5
+    data0 = O.fft_r2c(case.syn.data[0], 8, 1)
+    assert np.allclose(r["data"][0].download_complex()[k], data0[k], rtol=1e-12, atol=1e-13)
+
+
+def test_c0_end_to_end_fp32_vs_fp64_oracle(ctx):
+    """Config 0 (64x64x128, 3 angles): the FP32 GPU pipeline (plain, the
+    reference's precision) against the FP64 oracle."""
+    case = Case(model.config0())
+    o = oracle_posterior(case, np.float64, nthreads=8)
+    r = gpu_posterior(ctx, case, prec=32)
+    for a, (g, m) in enumerate(zip(r["mean"], o["mean"])):
+        mu = case.s.mu[a]
+        assert rel_rms(g.download_real()[:, :, :64] - mu, m[:, :, :64] - mu) < 1e-4
+    for g, c in zip(r["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-4
