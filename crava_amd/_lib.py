@@ -27,8 +27,9 @@ EXPORTS = [
     "crv_fft_c2r", "crv_fft_r2c_from", "crv_fft_c2r_add", "crv_posterior", "crv_simulate",
     "crv_fill_complex_noise", "crv_noise_draws", "crv_find_closest_factorable", "crv_padding_size",
     "crv_ctx_profile", "crv_ctx_profile_report", "crv_fill_prior_spectra", "crv_grid_fill",
-    "crv_forward_model", "crv_add_white_noise", "crv_grid_moments",
+    "crv_forward_model", "crv_add_white_noise", "crv_grid_moments", "crv_ctx_tune",
 ]
+CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS = 1, 2, 3, 4
 
 
 class GridInfo(C.Structure):
@@ -106,6 +107,7 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_forward_model": (i, [vp, i, C.POINTER(C.c_float), _dp, C.POINTER(vp), C.POINTER(vp)]),
         "crv_add_white_noise": (i, [vp, C.c_double, C.c_ulonglong, C.c_ulonglong]),
         "crv_grid_moments": (i, [vp, _dp, _dp, C.POINTER(C.c_longlong)]),
+        "crv_ctx_tune": (i, [vp, i, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -325,3 +325,21 @@ extern "C" int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len) {
   }
   return (int)out.size() + 1;
 }
+
+extern "C" int crv_ctx_tune(crv_ctx* ctx, int key, int value) {
+  if (!ctx) return CRV_E_INVALID;
+  if (key < CRV_TUNE_CHUNK_MB || key > CRV_TUNE_ROWS || value < 0)
+    return crv_set_error(ctx, CRV_E_INVALID, "crv_ctx_tune: bad key or value");
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const int old = ctx->tune[key];
+  ctx->tune[key] = value;
+  for (auto& kv : ctx->plans) {
+    const int rc = crv_plan_shapes(ctx, kv.second);
+    if (rc != CRV_OK) {
+      ctx->tune[key] = old;
+      for (auto& kv2 : ctx->plans) (void)crv_plan_shapes(ctx, kv2.second);
+      return rc;
+    }
+  }
+  return CRV_OK;
+}

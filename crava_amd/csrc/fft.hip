@@ -13,6 +13,7 @@
 // from a per-size table computed on the host in long double.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -428,83 +429,95 @@ template <typename T>
 size_t row_lds_bytes(const FftPlan* p) { return (size_t)((p->rows_per_block + 1) * p->fx.L) * sizeof(cplx_t<T>); }
 
 template <typename T>
-int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale) {
-  using C = cplx_t<T>;
-  const C* tw = reinterpret_cast<const C*>(p->tw);
+int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const T* add, int row_begin,
+                int row_end, double scale) {
   const int cnxp = p->nxp / 2 + 1, rnxp = 2 * cnxp;
-  const int nrows = p->nyp * p->nzp;
-  {
-    RowArgs<T> a{};
-    a.dst = dst; a.src = src; a.add = nullptr;
-    a.f = p->fx; a.tw = tw; a.wpost = tw + p->post_off;
-    a.nrows = nrows; a.rnxp = rnxp; a.nxp = p->nxp; a.cnxp = cnxp; a.xeven = p->xeven ? 1 : 0;
-    a.pshift = ilog2(p->rows_per_block);
-    a.divL = make_fastdiv(p->fx.L); a.divL1 = make_fastdiv(cnxp);
-    a.scale = (T)scale;
-    const int nb = (nrows + p->rows_per_block - 1) / p->rows_per_block;
+  const size_t off = (size_t)row_begin * rnxp;
+  RowArgs<T> a{};
+  a.dst = dst + off; a.src = src + off; a.add = add ? add + off : nullptr;
+  a.f = p->fx; a.tw = reinterpret_cast<const cplx_t<T>*>(p->tw);
+  a.wpost = a.tw + p->post_off;
+  a.nrows = row_end - row_begin; a.rnxp = rnxp; a.nxp = p->nxp; a.cnxp = cnxp; a.xeven = p->xeven ? 1 : 0;
+  a.pshift = ilog2(p->rows_per_block);
+  a.divL = make_fastdiv(p->fx.L); a.divL1 = make_fastdiv(cnxp);
+  a.scale = (T)scale;
+  const int nb = (a.nrows + p->rows_per_block - 1) / p->rows_per_block;
+  if (inv) {
+    ProfScope ps(ctx, "fft_x_c2r");
+    hipLaunchKernelGGL(k_rows_c2r<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
+  } else {
     ProfScope ps(ctx, "fft_x_r2c");
     hipLaunchKernelGGL(k_rows_r2c<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
   }
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
+// y pass over z planes [k0, k1) or z pass over y rows [j0, j1)
+template <typename T>
+int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data, int o0, int o1) {
+  using C = cplx_t<T>;
+  const int cnxp = p->nxp / 2 + 1;
+  ColArgs<T> a{};
+  a.tw = reinterpret_cast<const C*>(p->tw);
+  a.ncols = cnxp;
+  int tx;
+  if (zaxis) {
+    a.f = p->fz; a.estride = (long long)cnxp * p->nyp; a.ostride = cnxp; tx = p->tx_z;
+  } else {
+    a.f = p->fy; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp; tx = p->tx_y;
+  }
+  a.data = data + (size_t)o0 * a.ostride;
+  a.cshift = ilog2(tx);
+  a.ntx = (cnxp + tx - 1) / tx;
+  const int nb = a.ntx * (o1 - o0);
+  const size_t lds = (size_t)a.f.L * tx * sizeof(C);
+  static const char* names[4] = {"fft_y_fwd", "fft_y_inv", "fft_z_fwd", "fft_z_inv"};
+  ProfScope ps(ctx, names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
+  if (inv)
+    hipLaunchKernelGGL((k_cols<T, true>), dim3(nb), dim3(kThreads), lds, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((k_cols<T, false>), dim3(nb), dim3(kThreads), lds, ctx->stream, a);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
+// z planes per chunk when the x and y passes are interleaved so that the
+// second pass of a chunk finds the first pass's output still in the
+// memory-side cache; 0 = whole grid per pass
+int chunk_planes(const FftPlan* p, size_t elem) {
+  if (p->chunk_mb <= 0) return p->nzp;
+  const size_t plane = (size_t)(p->nxp / 2 + 1) * 2 * p->nyp * elem;
+  const size_t want = (size_t)p->chunk_mb << 20;
+  return (int)std::max<size_t>(1, std::min<size_t>((size_t)p->nzp, want / std::max<size_t>(plane, 1)));
+}
+
+template <typename T>
+int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale) {
+  using C = cplx_t<T>;
   C* cd = reinterpret_cast<C*>(dst);
-  if (p->nyp > 1) {
-    ColArgs<T> a{};
-    a.data = cd; a.f = p->fy; a.tw = tw; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp;
-    a.ncols = cnxp; a.cshift = ilog2(p->tx_y); a.ntx = (cnxp + p->tx_y - 1) / p->tx_y;
-    const int nb = a.ntx * p->nzp;
-    ProfScope ps(ctx, "fft_y_fwd");
-    hipLaunchKernelGGL((k_cols<T, false>), dim3(nb), dim3(kThreads), (size_t)p->fy.L * p->tx_y * sizeof(C), ctx->stream, a);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
+  const int cp = chunk_planes(p, sizeof(T));
+  int rc;
+  for (int k0 = 0; k0 < p->nzp; k0 += cp) {
+    const int k1 = std::min(p->nzp, k0 + cp);
+    if ((rc = launch_rows<T>(ctx, p, false, dst, src, nullptr, k0 * p->nyp, k1 * p->nyp, scale)) != CRV_OK) return rc;
+    if (p->nyp > 1 && (rc = launch_cols<T>(ctx, p, false, false, cd, k0, k1)) != CRV_OK) return rc;
   }
-  if (p->nzp > 1) {
-    ColArgs<T> a{};
-    a.data = cd; a.f = p->fz; a.tw = tw; a.estride = (long long)cnxp * p->nyp; a.ostride = cnxp;
-    a.ncols = cnxp; a.cshift = ilog2(p->tx_z); a.ntx = (cnxp + p->tx_z - 1) / p->tx_z;
-    const int nb = a.ntx * p->nyp;
-    ProfScope ps(ctx, "fft_z_fwd");
-    hipLaunchKernelGGL((k_cols<T, false>), dim3(nb), dim3(kThreads), (size_t)p->fz.L * p->tx_z * sizeof(C), ctx->stream, a);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
-  }
+  if (p->nzp > 1 && (rc = launch_cols<T>(ctx, p, false, true, cd, 0, p->nyp)) != CRV_OK) return rc;
   return CRV_OK;
 }
 
 template <typename T>
 int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const T* add) {
   using C = cplx_t<T>;
-  const C* tw = reinterpret_cast<const C*>(p->tw);
-  const int cnxp = p->nxp / 2 + 1, rnxp = 2 * cnxp;
-  const int nrows = p->nyp * p->nzp;
   C* cd = reinterpret_cast<C*>(buf);
-  if (p->nzp > 1) {
-    ColArgs<T> a{};
-    a.data = cd; a.f = p->fz; a.tw = tw; a.estride = (long long)cnxp * p->nyp; a.ostride = cnxp;
-    a.ncols = cnxp; a.cshift = ilog2(p->tx_z); a.ntx = (cnxp + p->tx_z - 1) / p->tx_z;
-    const int nb = a.ntx * p->nyp;
-    ProfScope ps(ctx, "fft_z_inv");
-    hipLaunchKernelGGL((k_cols<T, true>), dim3(nb), dim3(kThreads), (size_t)p->fz.L * p->tx_z * sizeof(C), ctx->stream, a);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
-  }
-  if (p->nyp > 1) {
-    ColArgs<T> a{};
-    a.data = cd; a.f = p->fy; a.tw = tw; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp;
-    a.ncols = cnxp; a.cshift = ilog2(p->tx_y); a.ntx = (cnxp + p->tx_y - 1) / p->tx_y;
-    const int nb = a.ntx * p->nzp;
-    ProfScope ps(ctx, "fft_y_inv");
-    hipLaunchKernelGGL((k_cols<T, true>), dim3(nb), dim3(kThreads), (size_t)p->fy.L * p->tx_y * sizeof(C), ctx->stream, a);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
-  }
-  {
-    RowArgs<T> a{};
-    a.dst = dst; a.src = buf; a.add = add;
-    a.f = p->fx; a.tw = tw; a.wpost = tw + p->post_off;
-    a.nrows = nrows; a.rnxp = rnxp; a.nxp = p->nxp; a.cnxp = cnxp; a.xeven = p->xeven ? 1 : 0;
-    a.pshift = ilog2(p->rows_per_block);
-    a.divL = make_fastdiv(p->fx.L); a.divL1 = make_fastdiv(cnxp);
-    a.scale = (T)scale;
-    const int nb = (nrows + p->rows_per_block - 1) / p->rows_per_block;
-    ProfScope ps(ctx, "fft_x_c2r");
-    hipLaunchKernelGGL(k_rows_c2r<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
+  int rc;
+  if (p->nzp > 1 && (rc = launch_cols<T>(ctx, p, true, true, cd, 0, p->nyp)) != CRV_OK) return rc;
+  const int cp = chunk_planes(p, sizeof(T));
+  for (int k0 = 0; k0 < p->nzp; k0 += cp) {
+    const int k1 = std::min(p->nzp, k0 + cp);
+    if (p->nyp > 1 && (rc = launch_cols<T>(ctx, p, true, false, cd, k0, k1)) != CRV_OK) return rc;
+    if ((rc = launch_rows<T>(ctx, p, true, dst, buf, add, k0 * p->nyp, k1 * p->nyp, scale)) != CRV_OK) return rc;
   }
   return CRV_OK;
 }
@@ -539,17 +552,10 @@ int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** ou
     tr.push_back(cosl(ang));
     ti.push_back(sinl(ang));
   }
-  const int maxe = (prec == CRV_FP64) ? 4096 : 8192;
-  const int txmax = (prec == CRV_FP64) ? 8 : 16;
-  // rows per block: (P+1)*Lx <= maxe, P <= 64
-  int P = 1;
-  while (P < 64 && (2 * P + 1) * Lx <= maxe) P *= 2;
-  if ((P + 1) * Lx > maxe) { delete p; return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nxp too large"); }
-  p->rows_per_block = P;
-  auto pick_tx = [&](int L) { int t = txmax; while (t > 1 && L * t > maxe) t /= 2; return (L * t <= maxe) ? t : 0; };
-  p->tx_y = pick_tx(nyp);
-  p->tx_z = pick_tx(nzp);
-  if (!p->tx_y || !p->tx_z) { delete p; return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nyp or nzp too large"); }
+  {
+    const int rc = crv_plan_shapes(ctx, p);
+    if (rc != CRV_OK) { delete p; return rc; }
+  }
   const size_t n = tr.size();
   hipError_t e;
   if (prec == CRV_FP64) {
@@ -569,6 +575,32 @@ int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** ou
   }
   ctx->plans[key] = p;
   *out = p;
+  return CRV_OK;
+}
+
+// Launch shapes: a tile holds at most 8192 (FP32) / 4096 (FP64) complex
+// values (64 KB of LDS); x rows per block P with (P+1) L <= that; y/z tiles
+// of TX consecutive x columns, 16 (FP32) or 8 (FP64) by default so that a
+// tile row is one 128-byte line.  CRV_TUNE_* values override, within those
+// limits.
+int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
+  const int maxe = (p->prec == CRV_FP64) ? 4096 : 8192;
+  const int txdef = (p->prec == CRV_FP64) ? 8 : 16;
+  const int Lx = p->fx.L;
+  int P = 1;
+  const int pmax = ctx->tune[CRV_TUNE_ROWS] > 0 ? ctx->tune[CRV_TUNE_ROWS] : 64;
+  while (2 * P <= pmax && (2 * P + 1) * Lx <= maxe) P *= 2;
+  if ((P + 1) * Lx > maxe) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nxp too large");
+  p->rows_per_block = P;
+  auto pick_tx = [&](int L, int want) {
+    int t = want > 0 ? pow2_floor(want) : txdef;
+    while (t > 1 && L * t > maxe) t /= 2;
+    return (L * t <= maxe) ? t : 0;
+  };
+  p->tx_y = pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y]);
+  p->tx_z = pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z]);
+  if (!p->tx_y || !p->tx_z) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nyp or nzp too large");
+  p->chunk_mb = ctx->tune[CRV_TUNE_CHUNK_MB];
   return CRV_OK;
 }
 

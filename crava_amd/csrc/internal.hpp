@@ -46,6 +46,7 @@ struct FftPlan {
   // launch shapes (chosen by crv_plan_tune or defaults)
   int rows_per_block = 0;
   int tx_y = 0, tx_z = 0;
+  int chunk_mb = 0;      // x/y pass interleave chunk (MB of grid), 0 = off
   // reference noise: per-row prefix of draws (nyp*nzp+1 int64), lazily made
   long long* noise_prefix = nullptr;
   long long noise_draws = 0;
@@ -57,6 +58,7 @@ struct ProfEvent {
 };
 
 struct crv_ctx {
+  int tune[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // CRV_TUNE_* overrides (0 = default)
   bool prof_on = false;
   std::vector<ProfEvent> prof;
   int device = 0;
@@ -114,6 +116,8 @@ struct ProfScope {
 // plan cache (fft.hip) -------------------------------------------------------
 int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** out);
 void crv_free_plan(FftPlan* p);
+// (re)select the launch shapes of a plan from the context's tuning values
+int crv_plan_shapes(crv_ctx* ctx, FftPlan* p);
 
 // transform launchers (fft.hip) ------------------------------------------------
 // Forward R2C of src (real view) into dst (may alias), scale applied on load.

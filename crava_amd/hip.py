@@ -42,6 +42,9 @@ class Context:
         """Run on an external HIP stream (e.g. torch.cuda.current_stream().cuda_stream)."""
         self.check(self.lib.crv_ctx_set_stream(self.h, stream_handle), "set_stream")
 
+    def tune(self, key: int, value: int):
+        self.check(self.lib.crv_ctx_tune(self.h, key, value), "tune")
+
     def profile(self, on: bool):
         self.check(self.lib.crv_ctx_profile(self.h, int(on)), "profile")
 

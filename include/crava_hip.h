@@ -84,6 +84,19 @@ const char* crv_version(void);
  * number of bytes needed (buf may be NULL to query).                        */
 int crv_ctx_profile(crv_ctx* ctx, int enable);
 int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
+/* Launch-shape tuning of the transforms (no counterpart in the reference,
+ * which plans FFTW_ESTIMATE per call, src/fftgrid.cpp:1801-1804).  Applies to
+ * every plan of the context, present and future; 0 restores the default.
+ *   CRV_TUNE_CHUNK_MB  interleave the x and y passes per chunk of z planes of
+ *                      about this many MB, so the second pass reads the first
+ *                      pass's output from the memory-side cache (0 = off)
+ *   CRV_TUNE_TX_Y/Z    x columns per y / z pencil tile (power of two)
+ *   CRV_TUNE_ROWS      maximum x rows per block (power of two)            */
+#define CRV_TUNE_CHUNK_MB 1
+#define CRV_TUNE_TX_Y     2
+#define CRV_TUNE_TX_Z     3
+#define CRV_TUNE_ROWS     4
+int crv_ctx_tune(crv_ctx* ctx, int key, int value);
 
 /* ---- grids --------------------------------------------------------------
  * crv_grid_create replaces FFTGrid::FFTGrid + createRealGrid
@@ -281,6 +294,30 @@ int crv_add_white_noise(crv_grid* g, double sigma, unsigned long long seed, unsi
  * (i < nx, j < ny, k < nz): Crava::computeDataVariance (crava.cpp:407-449)
  * is sum2 / count.                                                          */
 int crv_grid_moments(crv_grid* g, double* sum, double* sum2, long long* count);
+
+/* ---- rock physics (the model behind the learned surrogate) ---------------
+ * crv_rock_forward replaces, per sample, the rplib chain used to generate
+ * facies-conditional elastic samples (DistributionsRockDEM ->
+ * DEMTools::CalcEffectiveBulkAndShearModulus -> DEM::CalcEffectiveModulus ->
+ * OrdDiffEqSolver::Ode45, then RockGassmann::ComputeSeismicAndElasticParams;
+ * rplib/dem.cpp:39-189, rplib/orddiffeqsolver.cpp:17-167,
+ * rplib/demmodelling.cpp:29-55,178-236,274-354, rplib/rockgassmann.cpp:79-97):
+ * Hill-mixed quartz/clay mineral, DEM dry frame with empty pores, Batzle-Wang
+ * brine and gas (Reuss mix by water saturation), Gassmann.  One thread per
+ * sample in FP64.  d_in: n x 5 doubles (porosity, clay fraction, water
+ * saturation, pore aspect ratio, effective pressure MPa); d_out: n x 3
+ * (ln vp, ln vs in m/s, ln rho in kg/m3) as float (out_fp32) or double;
+ * d_status (optional): 0 ok, 1 DEM integration failed.  Device pointers.    */
+typedef struct crv_rock_params {
+  double k_quartz, g_quartz, rho_quartz;   /* kPa, kPa, g/cm3 */
+  double k_clay, g_clay, rho_clay;
+  double temperature;                      /* deg C */
+  double salinity;                         /* weight fraction */
+  double gas_gravity;
+} crv_rock_params;
+
+int crv_rock_forward(crv_ctx* ctx, const crv_rock_params* p, long long n, const double* d_in, void* d_out,
+                     int out_fp32, int* d_status);
 
 /* ---- padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622) -- */
 /* Smallest 2^a 3^b 5^c 7^d >= leastint (FFTGrid::findClosestFactorableNumber). */

@@ -36,7 +36,7 @@ def test_noise_fill_c0_matches_reference(ctx, golden):
     assert np.array_equal(flat[golden["noise_c0_idx"]], golden["noise_c0_val"])
 
 
-@pytest.mark.parametrize("shape,seed", [((256, 256, 200), 7), ((450, 30, 64), 231171), ((33, 17, 9), 4)])
+@pytest.mark.parametrize("shape,seed", [((256, 256, 200), 7), ((450, 30, 64), 231171), ((35, 21, 9), 4)])
 def test_noise_fill_long_stream(ctx, shape, seed):
     """Millions of draws: every value equal to the oracle's sequential stream
     and the final LCG state identical (the decode is exact)."""
