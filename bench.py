@@ -212,6 +212,9 @@ def main():
     ap.add_argument("--config", default="c1", choices=["c1", "c0"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp32-mode", choices=["refine", "plain"], default="refine",
+                    help="FP32 path: refinement + FP64 fallback (the metric) or the reference's plain FP32")
+    ap.add_argument("--chunk-mb", type=int, default=0, help="CRV_TUNE_CHUNK_MB for the transforms")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -232,7 +235,10 @@ def main():
     kw = sv.posterior_kwargs()
     ninv = int(sv.invert.sum()) * s.cnxp * s.nyp
     res = {}
-    for prec, mode in ((32, L.CRV_MODE_FP32_REFINE), (64, L.CRV_MODE_PLAIN)):
+    if a.chunk_mb:
+        ctx.tune(L.CRV_TUNE_CHUNK_MB, a.chunk_mb)
+    mode32 = L.CRV_MODE_FP32_REFINE if a.fp32_mode == "refine" else L.CRV_MODE_PLAIN
+    for prec, mode in ((32, mode32), (64, L.CRV_MODE_PLAIN)):
         if prec == 64 and a.no_fp64:
             continue
         G = make_grids(ctx, s, prec)
@@ -259,6 +265,10 @@ def main():
         step(ctx, G, kw, mode, 77, s.seed_sim)
         ctx.synchronize()
         res[prec] = dict(sec=sec, prof=prof, grids=G)
+        for k_, v_ in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+            b_ = kernel_bytes(k_, s, prec, ninv)
+            gbs = f"{b_ / (v_[1] / v_[0] / 1e3) / 1e9:7.0f} GB/s" if b_ else ""
+            log(f"[bench]   FP{prec} {k_:22s} x{v_[0] // 3:3d}/step  avg {v_[1] / v_[0]:.4f} ms  {gbs}")
         log(f"[bench] FP{prec}: {sec * 1e3:.2f} ms/step, {s.voxels / sec:.3e} voxels/s")
 
     # error table of FP32 + refinement against FP64
@@ -305,6 +315,8 @@ def main():
                   "speedup_fp32_refine": res[64]["sec"] / sec} if 64 in res else None),
         "errors_vs_fp64": errors,
         "kernels": kernels,
+        "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}
+                         if 64 in res else None),
     }
     if not a.no_cpu:
         log("[bench] CPU baseline (bounded sample)")

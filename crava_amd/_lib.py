@@ -27,7 +27,7 @@ EXPORTS = [
     "crv_fft_c2r", "crv_fft_r2c_from", "crv_fft_c2r_add", "crv_posterior", "crv_simulate",
     "crv_fill_complex_noise", "crv_noise_draws", "crv_find_closest_factorable", "crv_padding_size",
     "crv_ctx_profile", "crv_ctx_profile_report", "crv_fill_prior_spectra", "crv_grid_fill",
-    "crv_forward_model", "crv_add_white_noise", "crv_grid_moments", "crv_ctx_tune",
+    "crv_forward_model", "crv_add_white_noise", "crv_grid_moments", "crv_ctx_tune", "crv_rock_forward",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS = 1, 2, 3, 4
 
@@ -56,6 +56,11 @@ class PostParams(C.Structure):
 class PostStats(C.Structure):
     _fields_ = [("n_wavenumbers", C.c_longlong), ("n_inverted", C.c_longlong),
                 ("n_chol_fail", C.c_longlong), ("n_fp64", C.c_longlong), ("max_cond", C.c_double)]
+
+
+class RockParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("k_quartz", "g_quartz", "rho_quartz", "k_clay", "g_clay", "rho_clay",
+                                           "temperature", "salinity", "gas_gravity")]
 
 
 class NoiseSpec(C.Structure):
@@ -108,6 +113,7 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_add_white_noise": (i, [vp, C.c_double, C.c_ulonglong, C.c_ulonglong]),
         "crv_grid_moments": (i, [vp, _dp, _dp, C.POINTER(C.c_longlong)]),
         "crv_ctx_tune": (i, [vp, i, i]),
+        "crv_rock_forward": (i, [vp, C.POINTER(RockParams), C.c_longlong, vp, vp, i, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -19,11 +19,14 @@
 
 namespace {
 
-constexpr int kThreads = 512;
+// 256-thread blocks with tiles of at most 4096 (FP32) / 2048 (FP64) complex
+// values (32 KB of LDS): three blocks share a CU, so one block's global
+// loads overlap another's LDS stages.
+constexpr int kThreads = 256;
 
 template <typename T> struct Ept;
-template <> struct Ept<float>  { static constexpr int value = 16; };  // 8192 complex / block
-template <> struct Ept<double> { static constexpr int value = 8; };   // 4096 complex / block
+template <> struct Ept<float>  { static constexpr int value = 16; };  // 4096 complex / block
+template <> struct Ept<double> { static constexpr int value = 8; };   // 2048 complex / block
 
 // Unsigned division by a runtime constant (n < 2^31): q = (umulhi(n,m)+n)>>s.
 struct FastDiv {
@@ -252,20 +255,29 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   const int nr = min(P, a.nrows - row0);
   const int L = a.f.L;
   const int tid = threadIdx.x;
-  // load
-  for (int idx = tid; idx < P * L; idx += kThreads) {
-    const int c = fdiv(idx, a.divL), e = idx - c * L;
-    C z = cmake<C>((T)0, (T)0);
-    if (c < nr) {
-      const T* row = a.src + (size_t)(row0 + c) * a.rnxp;
-      if (a.xeven) {
-        z = *reinterpret_cast<const C*>(row + 2 * e);
-        z = cscale(z, a.scale);
-      } else {
-        z = cmake<C>(row[e] * a.scale, (T)0);
+  // load: every load of the thread is issued before the first LDS store, so
+  // a wave keeps Ept loads in flight
+  constexpr int EPT = Ept<T>::value;
+  C buf[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = tid + q * kThreads;
+    buf[q] = cmake<C>((T)0, (T)0);
+    if (idx < P * L) {
+      const int c = fdiv(idx, a.divL), e = idx - c * L;
+      if (c < nr) {
+        const T* row = a.src + (size_t)(row0 + c) * a.rnxp;
+        buf[q] = a.xeven ? *reinterpret_cast<const C*>(row + 2 * e) : cmake<C>(row[e], (T)0);
       }
     }
-    lds[e * S + c] = z;
+  }
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = tid + q * kThreads;
+    if (idx < P * L) {
+      const int c = fdiv(idx, a.divL), e = idx - c * L;
+      lds[e * S + c] = cscale(buf[q], a.scale);
+    }
   }
   __syncthreads();
   run_stages<T, false>(lds, S, a.pshift, a.f, a.tw);
@@ -301,15 +313,34 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   const int nr = min(P, a.nrows - row0);
   const int L = a.f.L;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < P * L; idx += kThreads) {
-    const int c = fdiv(idx, a.divL), k = idx - c * L;
-    C z = cmake<C>((T)0, (T)0);
-    if (c < nr) {
-      const C* X = reinterpret_cast<const C*>(a.src + (size_t)(row0 + c) * a.rnxp);
+  constexpr int EPT = Ept<T>::value;
+  C ba[EPT], bb[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = tid + q * kThreads;
+    ba[q] = cmake<C>((T)0, (T)0);
+    bb[q] = ba[q];
+    if (idx < P * L) {
+      const int c = fdiv(idx, a.divL), k = idx - c * L;
+      if (c < nr) {
+        const C* X = reinterpret_cast<const C*>(a.src + (size_t)(row0 + c) * a.rnxp);
+        if (a.xeven) {
+          ba[q] = X[k];
+          bb[q] = X[L - k];
+        } else {
+          ba[q] = (k < a.cnxp) ? X[k] : cconj(X[L - k]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = tid + q * kThreads;
+    if (idx < P * L) {
+      const int c = fdiv(idx, a.divL), k = idx - c * L;
+      C z;
       if (a.xeven) {
-        const int M = L;
-        C xa = X[k];
-        C xb = X[M - k];
+        C xa = ba[q], xb = bb[q];
         if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
         xb = cconj(xb);
         C d = csub(xa, xb);
@@ -317,15 +348,11 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
         // Z = (xa + xb) + i t
         z = cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
       } else {
-        if (k < a.cnxp) {
-          z = X[k];
-          if (k == 0) z.y = (T)0;
-        } else {
-          z = cconj(X[L - k]);
-        }
+        z = ba[q];
+        if (k == 0) z.y = (T)0;
       }
+      lds[k * S + c] = z;
     }
-    lds[k * S + c] = z;
   }
   __syncthreads();
   run_stages<T, true>(lds, S, a.pshift, a.f, a.tw);
@@ -374,9 +401,18 @@ __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   const int nc = min(TX, a.ncols - x0);
   C* base = a.data + (size_t)o * a.ostride + x0;
   const int n = a.f.L << a.cshift;
-  for (int idx = threadIdx.x; idx < n; idx += kThreads) {
+  constexpr int EPT = Ept<T>::value;
+  C buf[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
     const int c = idx & cm, e = idx >> a.cshift;
-    lds[idx] = (c < nc) ? base[(size_t)e * a.estride + c] : cmake<C>((T)0, (T)0);
+    buf[q] = (idx < n && c < nc) ? base[(size_t)e * a.estride + c] : cmake<C>((T)0, (T)0);
+  }
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    if (idx < n) lds[idx] = buf[q];
   }
   __syncthreads();
   run_stages<T, INV>(lds, TX, a.cshift, a.f, a.tw);
@@ -578,13 +614,12 @@ int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** ou
   return CRV_OK;
 }
 
-// Launch shapes: a tile holds at most 8192 (FP32) / 4096 (FP64) complex
-// values (64 KB of LDS); x rows per block P with (P+1) L <= that; y/z tiles
-// of TX consecutive x columns, 16 (FP32) or 8 (FP64) by default so that a
-// tile row is one 128-byte line.  CRV_TUNE_* values override, within those
-// limits.
+// Launch shapes: a tile holds at most 4096 (FP32) / 2048 (FP64) complex
+// values (32 KB of LDS); x rows per block P with (P+1) L <= that; y/z tiles
+// of TX consecutive x columns, 16 (FP32) or 8 (FP64) at most, halved until
+// the pencil tile fits.  CRV_TUNE_* values override, within those limits.
 int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
-  const int maxe = (p->prec == CRV_FP64) ? 4096 : 8192;
+  const int maxe = kThreads * ((p->prec == CRV_FP64) ? Ept<double>::value : Ept<float>::value);
   const int txdef = (p->prec == CRV_FP64) ? 8 : 16;
   const int Lx = p->fx.L;
   int P = 1;

@@ -284,65 +284,65 @@ __global__ __launch_bounds__(kThreads) void k_posterior_refine(PostArgs<float> a
             for (int m = 0; m < 3; ++m) x = csub(x, cmul(K[l][m], mu[m]));
             B[l][3] = x;
           }
-          C X[N][4];
+          // One right-hand side at a time (columns of K Sm, then d - K mu):
+          // x = M^-1 b in FP32, residual b - M x in FP64, x += M^-1 residual.
+          // Each refined column is folded into the outputs at once, so only
+          // one solution vector is live.
+          C cv[6];
+          C mp[3];
 #pragma unroll
-          for (int l = 0; l < N; ++l)
+          for (int c = 0; c < 4; ++c) {
+            C x[N][1], y[N][1];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) X[l][c] = B[l][c];
-          lsolve<float, N, 4>(A, X);
-          lhsolve<float, N, 4>(A, X);
-          // refinement: R = B - M X in FP64, X += M^-1 R
-          C R[N][4];
+            for (int l = 0; l < N; ++l) x[l][0] = B[l][c];
+            lsolve<float, N, 1>(A, x);
+            lhsolve<float, N, 1>(A, x);
 #pragma unroll
-          for (int l = 0; l < N; ++l)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-              D x = cmake<D>((double)B[l][c].x, (double)B[l][c].y);
+            for (int l = 0; l < N; ++l) {
+              D r = cmake<D>((double)B[l][c].x, (double)B[l][c].y);
 #pragma unroll
               for (int m = 0; m < N; ++m) {
                 const C mm = (m <= l) ? M[l * (l + 1) / 2 + m] : cconj(M[m * (m + 1) / 2 + l]);
-                const D md = cmake<D>((double)mm.x, (double)mm.y);
-                const D xd = cmake<D>((double)X[m][c].x, (double)X[m][c].y);
-                x = csub(x, cmul(md, xd));
+                r = csub(r, cmul(cmake<D>((double)mm.x, (double)mm.y), cmake<D>((double)x[m][0].x, (double)x[m][0].y)));
               }
-              R[l][c] = cmake<C>((float)x.x, (float)x.y);
+              y[l][0] = cmake<C>((float)r.x, (float)r.y);
             }
-          lsolve<float, N, 4>(A, R);
-          lhsolve<float, N, 4>(A, R);
-          // outputs, accumulated in FP64 from the refined X
-          D mp[3];
+            lsolve<float, N, 1>(A, y);
+            lhsolve<float, N, 1>(A, y);
 #pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            D x = cmake<D>((double)mu[m].x, (double)mu[m].y);
+            for (int l = 0; l < N; ++l) x[l][0] = cadd(x[l][0], y[l][0]);
+            if (c < 3) {
+              // S_post[a][c] = Sm[a][c] - sum_l conj(KS[l][a]) x[l], a <= c
 #pragma unroll
-            for (int l = 0; l < N; ++l) {
-              const D xr = cmake<D>((double)X[l][3].x + (double)R[l][3].x, (double)X[l][3].y + (double)R[l][3].y);
-              const D bs = cmake<D>((double)B[l][m].x, (double)B[l][m].y);
-              x = cadd(x, cmulc(xr, bs));  // KS^H M^-1 r
+              for (int aa = 0; aa <= c; ++aa) {
+                C acc = cmake<C>(0.f, 0.f);
+#pragma unroll
+                for (int l = 0; l < N; ++l) acc = cadd(acc, cmulc(x[l][0], B[l][aa]));
+                const int q = (aa == c) ? c : (c == 1 ? 3 : (aa == 0 ? 4 : 5));
+                cv[q] = acc;
+              }
+            } else {
+#pragma unroll
+              for (int m = 0; m < 3; ++m) {
+                C acc = mu[m];
+#pragma unroll
+                for (int l = 0; l < N; ++l) acc = cadd(acc, cmulc(x[l][0], B[l][m]));
+                mp[m] = acc;
+              }
             }
-            mp[m] = x;
-            a.mean[m][idx] = cmake<C>((float)x.x, (float)x.y);
           }
+#pragma unroll
+          for (int m = 0; m < 3; ++m) a.mean[m][idx] = mp[m];
 #pragma unroll
           for (int l = 0; l < N; ++l) {
-            D x = cmake<D>((double)d[l].x, (double)d[l].y);
+            C x = d[l];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) x = csub(x, cmul(cmake<D>((double)K[l][m].x, (double)K[l][m].y), mp[m]));
-            a.data[l][idx] = cmake<C>((float)x.x, (float)x.y);
+            for (int m = 0; m < 3; ++m) x = csub(x, cmul(K[l][m], mp[m]));
+            a.data[l][idx] = x;
           }
-          const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
-          for (int q = 0; q < 6; ++q) {
-            D x = cmake<D>(s[q], 0.0);
-#pragma unroll
-            for (int l = 0; l < N; ++l) {
-              const D xr = cmake<D>((double)X[l][pb[q]].x + (double)R[l][pb[q]].x,
-                                    (double)X[l][pb[q]].y + (double)R[l][pb[q]].y);
-              const D bs = cmake<D>((double)B[l][pa[q]].x, (double)B[l][pa[q]].y);
-              x = csub(x, cmulc(xr, bs));  // (KS^H M^-1 KS)[a][b]
-            }
-            a.cov[q][idx] = cmake<C>((float)x.x, (float)x.y);
-          }
+          for (int q = 0; q < 6; ++q)
+            a.cov[q][idx] = cmake<C>((float)(s[q] - (double)cv[q].x), (float)(-(double)cv[q].y));
           done = true;
         } else {
           const unsigned int slot = atomicAdd(a.list_count, 1u);

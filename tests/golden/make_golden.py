@@ -135,6 +135,30 @@ def main() -> None:
     sc, nf2 = O.simulate_scale(pc, seeds, backend="reference")
     g["sim_in_seed"], g["sim_out"], g["sim_nfail"] = cz(seeds), cz(sc), np.int64(nf2)
 
+    # rplib: DEM effective moduli (empty pores in a quartz/clay host) and
+    # Batzle-Wang brine, from the reference's compiled dem.cpp /
+    # orddiffeqsolver.cpp / demmodelling.cpp
+    dp = C.POINTER(C.c_double)
+    r.ref_dem_effective.argtypes = [C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp]
+    for nm in ("ref_brine_velocity", "ref_brine_density", "ref_brine_modulus"):
+        getattr(r, nm).restype = C.c_double
+        getattr(r, nm).argtypes = [C.c_double] * 3
+    dem_in, dem_out = [], []
+    for _ in range(300):
+        asp = 10 ** rng.uniform(-2, 0)
+        phi = rng.uniform(0.02, 0.4)
+        kh, gh = rng.uniform(2.0e7, 3.8e7), rng.uniform(0.7e7, 4.5e7)
+        one = lambda v: (C.c_double * 1)(v)
+        k1, g1 = C.c_double(), C.c_double()
+        rc = r.ref_dem_effective(1, one(0.0), one(0.0), one(asp), one(phi), kh, gh, C.byref(k1), C.byref(g1))
+        dem_in.append([asp, phi, kh, gh])
+        dem_out.append([k1.value, g1.value, rc])
+    g["dem_in"], g["dem_out"] = np.array(dem_in), np.array(dem_out)
+    tps = np.column_stack([rng.uniform(20, 150, 200), rng.uniform(5, 60, 200), rng.uniform(0, 0.2, 200)])
+    g["brine_tps"] = tps
+    g["brine_out"] = np.array([[r.ref_brine_velocity(*t), r.ref_brine_density(*t), r.ref_brine_modulus(*t)]
+                               for t in tps])
+
     np.savez_compressed(OUT, **g)
     print(f"wrote {OUT} ({OUT.stat().st_size / 1024:.0f} KiB, {len(g)} arrays)")
 

@@ -274,3 +274,83 @@ def test_padding_brute_force():
         return best
     for n in list(range(1, 600)) + [1000, 1234, 2047, 4096, 4097]:
         assert O.find_closest_factorable(n) == brute(n) == model.find_closest_factorable(n)
+
+
+# ------------------------------------------------------------ rplib model
+def _rp():
+    import ctypes as C
+    o = O.olib()
+    dp = C.POINTER(C.c_double)
+    o.or_dem_effective.argtypes = [C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp, C.POINTER(C.c_int)]
+    for nm in ("or_brine_velocity", "or_brine_density", "or_brine_modulus_kpa"):
+        getattr(o, nm).restype = C.c_double
+        getattr(o, nm).argtypes = [C.c_double] * 3
+    o.or_gas_props.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
+    return o, C
+
+
+def test_dem_bit_exact_vs_reference(golden):
+    o, C = _rp()
+    one = lambda v: (C.c_double * 1)(v)
+    for (asp, phi, kh, gh), (k_ref, g_ref, rc) in zip(golden["dem_in"], golden["dem_out"]):
+        k, g, ns = C.c_double(), C.c_double(), C.c_int()
+        r = o.or_dem_effective(1, one(0.0), one(0.0), one(asp), one(phi), kh, gh, C.byref(k), C.byref(g), C.byref(ns))
+        assert r == int(rc)
+        assert k.value == k_ref and g.value == g_ref
+
+
+def test_brine_bit_exact_vs_reference(golden):
+    o, _ = _rp()
+    for (t, p, s), (v, rho, kmpa) in zip(golden["brine_tps"], golden["brine_out"]):
+        assert o.or_brine_velocity(t, p, s) == v
+        assert o.or_brine_density(t, p, s) == rho
+        assert o.or_brine_modulus_kpa(t, p, s) == pytest.approx(kmpa * 1e3, rel=1e-15)
+
+
+def test_dem_against_high_accuracy_ode():
+    """RKF45 at tol 1e-5 agrees with scipy's DOP853 at rtol 1e-11 on the same
+    right-hand side (dilute-limit P, Q checked through the ODE's slope at 0)."""
+    from scipy.integrate import solve_ivp
+    o, C = _rp()
+    one = lambda v: (C.c_double * 1)(v)
+    for asp, phi in ((0.05, 0.2), (0.3, 0.3), (0.9, 0.1)):
+        kh, gh = 3.7e7, 4.4e7
+        k, g, ns = C.c_double(), C.c_double(), C.c_int()
+        o.or_dem_effective(1, one(0.0), one(0.0), one(asp), one(phi), kh, gh, C.byref(k), C.byref(g), C.byref(ns))
+
+        def rhs(t, y):
+            K, mu = y
+            th = (asp / (1 - asp ** 2) ** 1.5) * (math.acos(asp) - asp * math.sqrt(1 - asp ** 2))
+            fn = (asp ** 2 / (1 - asp ** 2)) * (3 * th - 2)
+            nu = (3 * K - 2 * mu) / (2 * (3 * K + mu))
+            r = (1 - 2 * nu) / (2 * (1 - nu))
+            a, b = -1.0, (1 / 3) * (0 - 0)
+            f1 = 1 + a * (1.5 * (fn + th) - r * (1.5 * fn + 2.5 * th - 4 / 3))
+            f2 = 1 + a * (1 + 1.5 * (fn + th) - (r / 2) * (3 * fn + 5 * th)) + b * (3 - 4 * r)
+            f2 += (a / 2) * (a + 3 * b) * (3 - 4 * r) * (fn + th - r * (fn - th + 2 * th * th))
+            f3 = 1 + a * (1 - (fn + 1.5 * th) + r * (fn + th))
+            f4 = 1 + (a / 4) * (fn + 3 * th - r * (fn - th))
+            f5 = a * (-fn + r * (fn + th - 4 / 3)) + b * th * (3 - 4 * r)
+            f6 = 1 + a * (1 + fn - r * (fn + th)) + b * (1 - th) * (3 - 4 * r)
+            f7 = 2 + (a / 4) * (3 * fn + 9 * th - r * (3 * fn + 5 * th)) + b * th * (3 - 4 * r)
+            f8 = a * (1 - 2 * r + (fn / 2) * (r - 1) + (th / 2) * (5 * r - 3)) + b * (1 - th) * (3 - 4 * r)
+            f9 = a * ((r - 1) * fn - r * th) + b * th * (3 - 4 * r)
+            P = f1 / f2
+            Q = ((2 / f3) + (1 / f4) + ((f4 * f5 + f6 * f7 - f8 * f9) / (f2 * f4))) / 5
+            return [(0 - K) * P / (1 - t), (0 - mu) * Q / (1 - t)]
+        sol = solve_ivp(rhs, (0, phi), [kh, gh], method="DOP853", rtol=1e-11, atol=1e-3)
+        assert k.value == pytest.approx(sol.y[0, -1], rel=2e-4)
+        assert g.value == pytest.approx(sol.y[1, -1], rel=2e-4)
+
+
+def test_gassmann_and_gas_limits():
+    import ctypes as C
+    o, _ = _rp()
+    rho, k = C.c_double(), C.c_double()
+    o.or_gas_props(80.0, 25.0, 0.6, C.byref(rho), C.byref(k))
+    # Batzle & Wang (1992) Fig. 6-7 range for G = 0.6 at 25 MPa, 80 C
+    assert 0.15 < rho.value < 0.25 and 3e4 < k.value < 1.5e5
+    # rho, K of the gas rise with pressure
+    r2, k2 = C.c_double(), C.c_double()
+    o.or_gas_props(80.0, 40.0, 0.6, C.byref(r2), C.byref(k2))
+    assert r2.value > rho.value and k2.value > k.value
