@@ -80,9 +80,31 @@ def timed(ctx, G, kw, mode, steps, warmup, seed):
     return (time.perf_counter() - t0) / steps
 
 
+def transforms_per_step(name, s):
+    """3D transforms per step whose passes carry this kernel name: forward
+    for the n data and 3 means, inverse for 3 means, 3 realization grids
+    and 6 covariance grids."""
+    if name in ("fft_x_r2c", "fft_y_fwd", "fft_z_fwd"):
+        return s.n + 3
+    if name in ("fft_x_c2r", "fft_y_inv", "fft_z_inv"):
+        return 12
+    return None
+
+
+def launch_bytes(name, s, prec, ninv, launches_per_step):
+    """Algorithmic bytes of one launch: a transform pass split into chunks
+    moves its share of the whole-grid figure."""
+    b = kernel_bytes(name, s, prec, ninv)
+    t = transforms_per_step(name, s)
+    if b is None or t is None:
+        return b
+    return b * t / launches_per_step
+
+
 def kernel_bytes(name, s, prec, ninv):
-    """Algorithmic HBM bytes of one launch of a kernel (reads + writes of the
-    data it must touch), for the FP32 or FP64 grids of survey s."""
+    """Algorithmic HBM bytes of one kernel over the whole grid (reads +
+    writes of the data it must touch: one transform pass, or one posterior
+    or simulation launch), for the FP32 or FP64 grids of survey s."""
     e = 4 if prec == 32 else 8
     N = s.nxp * s.nyp * s.nzp
     Nc = s.cnxp * s.nyp * s.nzp
@@ -93,7 +115,7 @@ def kernel_bytes(name, s, prec, ninv):
         return e * N + 2 * e * Nc
     if name == "fft_x_c2r":
         return 2 * e * Nc + e * N            # (+ e*N when the mean is added)
-    if name in ("posterior", "posterior_refine"):
+    if name in ("posterior", "posterior_fp32"):
         return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e
     if name == "simulate_scale":
         return Nc * (6 + 3) * 2 * e
@@ -192,16 +214,30 @@ def cpu_baseline(s, nthreads):
     return dict(value=s.voxels / total, unit="voxels/s", cores=nthreads, kind=kind, sample=sample)
 
 
+# profiled launch name -> kernel symbol prefix in the rocprofv3 summaries
+KERNEL_SYMBOLS = {
+    "fft_x_r2c": "k_rows_r2c<float>", "fft_x_c2r": "k_rows_c2r<float>",
+    "fft_y_fwd": "k_cols<float, false>", "fft_z_fwd": "k_cols<float, false>",
+    "fft_y_inv": "k_cols<float, true>", "fft_z_inv": "k_cols<float, true>",
+    "posterior_fp32": "k_posterior<float, ", "simulate_scale": "k_simulate<float",
+}
+
+
 def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary, when present."""
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a
+    plain copy's reading) from the committed rocprofv3 PMC summary, or None."""
     p = ROOT / "profiles" / "traffic.json"
-    if not p.exists():
+    sym = KERNEL_SYMBOLS.get(kernel)
+    if not p.exists() or sym is None:
         return None
     try:
-        return json.loads(p.read_text()).get(kernel)
-    except Exception:
+        tab = json.loads(p.read_text())
+    except (OSError, ValueError):
         return None
+    for k, v in tab.items():
+        if k.startswith(sym) and isinstance(v, dict):
+            return v.get("bytes")
+    return None
 
 
 def main():
@@ -216,6 +252,8 @@ def main():
                     help="FP32 path: refinement + FP64 fallback (the metric) or the reference's plain FP32")
     ap.add_argument("--chunk-mb", type=int, default=0, help="CRV_TUNE_CHUNK_MB for the transforms")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 plain 1 GiB device copies (the reference point for HBM counter readings)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -224,6 +262,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if a.copy_calibration:
+        n = 1 << 28  # 1 GiB of float32
+        x = torch.ones(n, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"read_bytes": 4 * n, "write_bytes": 4 * n, "launches": 3}}))
+        return
     from crava_amd import _lib as L
     from crava_amd import hip, model
     from crava_amd.survey import GpuSurvey
@@ -266,9 +313,10 @@ def main():
         ctx.synchronize()
         res[prec] = dict(sec=sec, prof=prof, grids=G)
         for k_, v_ in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-            b_ = kernel_bytes(k_, s, prec, ninv)
+            b_ = launch_bytes(k_, s, prec, ninv, v_[0] / 3)
             gbs = f"{b_ / (v_[1] / v_[0] / 1e3) / 1e9:7.0f} GB/s" if b_ else ""
-            log(f"[bench]   FP{prec} {k_:22s} x{v_[0] // 3:3d}/step  avg {v_[1] / v_[0]:.4f} ms  {gbs}")
+            log(f"[bench]   FP{prec} {k_:22s} x{v_[0] // 3:3d}/step  avg {v_[1] / v_[0]:.4f} ms  "
+                f"{v_[1] / 3:.3f} ms/step {gbs}")
         log(f"[bench] FP{prec}: {sec * 1e3:.2f} ms/step, {s.voxels / sec:.3e} voxels/s")
 
     # error table of FP32 + refinement against FP64
@@ -291,13 +339,15 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (cnt, tot, mn, mx) = dom
     avg_s = tot / cnt / 1e3
-    kb = kernel_bytes(name, s, 32, ninv)
+    kb = launch_bytes(name, s, 32, ninv, cnt / 3)
     roof = {"kernel": name, "bound": "hbm", "achieved": (kb / avg_s / 1e9) if kb else None, "peak": PEAK_HBM / 1e9,
             "unit": "GB/s", "frac": (kb / avg_s / PEAK_HBM) if kb else None, "traffic": read_traffic(name),
             "avg_ms": avg_s * 1e3, "bytes_per_launch": kb}
-    kernels = {k: {"launches": v[0], "avg_ms": v[1] / v[0],
-                   "GB/s": (kernel_bytes(k, s, 32, ninv) / (v[1] / v[0] / 1e3) / 1e9)
-                   if kernel_bytes(k, s, 32, ninv) else None} for k, v in prof.items()}
+    kernels = {}
+    for k, v in prof.items():
+        b_ = launch_bytes(k, s, 32, ninv, v[0] / 3)
+        kernels[k] = {"launches_per_step": v[0] / 3, "avg_ms": v[1] / v[0], "ms_per_step": v[1] / 3,
+                      "GB/s": (b_ / (v[1] / v[0] / 1e3) / 1e9) if b_ else None}
     bpv = pipeline_bytes_per_voxel(s.n, 32)
     out = {
         "metric": "inverted voxels/s (mean+cov+1 sim), FP32-refined vs FP64 GPU at <=1e-4 rel RMS",

@@ -42,6 +42,9 @@ int crv_ctx_create(int device, crv_ctx** out) {
   crv_ctx* c = new (std::nothrow) crv_ctx();
   if (!c) return CRV_E_NOMEM;
   c->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    c->num_cu = ncu;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CRV_E_HIP; }
   c->own_stream = true;
   *out = c;

@@ -62,6 +62,7 @@ struct crv_ctx {
   bool prof_on = false;
   std::vector<ProfEvent> prof;
   int device = 0;
+  int num_cu = 256;      // compute units of the device (persistent grids)
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;

@@ -50,6 +50,74 @@ __device__ __forceinline__ bool chol_ref(cplx_t<U>* A, U& dmin, U& dmax) {
   return true;
 }
 
+__device__ __forceinline__ float rsqrt_any(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_any(double x) { return 1.0 / sqrt(x); }
+
+// The same factorisation and acceptance rule without the scaling pass and
+// without divisions.  Scaling A by 1/A00 only moves the thresholds:
+// a scaled pivot p / A00 <= 1e-20 is the unscaled pivot p <= 1e-20 * A00, so
+// the rule is applied to unscaled pivots and the factor keeps its reciprocal
+// diagonal rd[i] = 1 / L[i][i].  Off-diagonal L entries are stored in A;
+// the diagonal of A is left as it was.  pmin / pmax are the smallest and
+// largest pivots L[i][i]^2, so (dmax/dmin)^2 of the diagonal is pmax / pmin.
+template <typename U, int N>
+__device__ __forceinline__ bool chol_rcp(cplx_t<U>* A, U* rd, U& pmin, U& pmax) {
+  using C = cplx_t<U>;
+  const U factor = A[0].x;
+  if (factor <= (U)0) return false;
+  const U tol = (U)kCholTol * factor;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int ri = i * (i + 1) / 2;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      const int rj = j * (j + 1) / 2;
+      C x = A[ri + j];
+#pragma unroll
+      for (int q = 0; q < j; ++q) x = csub(x, cmulc(A[ri + q], A[rj + q]));
+      A[ri + j] = cscale(x, rd[j]);
+    }
+    U p = A[ri + i].x;
+#pragma unroll
+    for (int q = 0; q < i; ++q) p -= A[ri + q].x * A[ri + q].x + A[ri + q].y * A[ri + q].y;
+    if (p <= tol) return false;
+    rd[i] = rsqrt_any(p);
+    if (i == 0) { pmin = p; pmax = p; } else { pmin = fmin(pmin, p); pmax = fmax(pmax, p); }
+  }
+  return true;
+}
+
+// L y = b with L from chol_rcp (strict lower part in L, 1/diag in rd)
+template <typename U, int N, int NC>
+__device__ __forceinline__ void lsolve_r(const cplx_t<U>* L, const U* rd, cplx_t<U> (*b)[NC]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int ri = i * (i + 1) / 2;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      cplx_t<U> x = b[i][c];
+#pragma unroll
+      for (int j = 0; j < i; ++j) x = csub(x, cmul(b[j][c], L[ri + j]));
+      b[i][c] = cscale(x, rd[i]);
+    }
+  }
+}
+
+// L^H x = y with L from chol_rcp
+template <typename U, int N, int NC>
+__device__ __forceinline__ void lhsolve_r(const cplx_t<U>* L, const U* rd, cplx_t<U> (*b)[NC]) {
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      cplx_t<U> x = b[i][c];
+#pragma unroll
+      for (int j = N - 1; j > i; --j) x = csub(x, cmulc(b[j][c], L[j * (j + 1) / 2 + i]));
+      b[i][c] = cscale(x, rd[i]);
+    }
+  }
+}
+
 // forward substitution L y = b (b overwritten), N x NC right-hand sides
 template <typename U, int N, int NC>
 __device__ __forceinline__ void lsolve(const cplx_t<U>* L, cplx_t<U> (*b)[NC]) {

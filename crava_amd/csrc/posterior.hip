@@ -7,176 +7,70 @@
 // 339-426) and lib_matrCholCpx (libs/lib/lib_matr.c:78-167).
 //
 // One thread owns one wavenumber (i, j, k) of the half spectrum; it gathers
-// 3 means, n data, the error and prior spectra, factors the n x n Hermitian
-// system, and scatters the posterior mean, the n residuals and the 6 entries
-// of the 3 x 3 posterior covariance.  The kz-only quantities (forward matrix
-// K = conj(W_l) D1 A, error shape, white-noise diagonal, band mask) are
-// tabulated once per call on the host, so a wavenumber costs one pass over
-// its own grid entries: the kernel is bound by HBM bandwidth.
+// 3 means, n data and the prior, factors the n x n Hermitian system, and
+// scatters the posterior mean, the n residuals and the 6 entries of the
+// 3 x 3 posterior covariance.  Everything that depends on kz only is
+// tabulated once per call on the host in FP64: the forward matrix
+// K = conj(W_l) D1 A (wavelet times Aki-Richards), the error shape E, the
+// white-noise diagonal D and the band mask.  For the separable prior the
+// prior and error spectra at (i, j, k) are |cxy(i,j)| * |ct(k)| * Sigma0 and
+// |cxy(i,j)|, so with lam = |cxy(i,j)|
+//   M = K S K^H + lam E + D = lam (|ct| K Sigma0 K^H + E) + D,  K S = lam |ct| K Sigma0,
+// and the bracketed per-kz matrices are tabulated too: a wavenumber then
+// costs one Cholesky factorisation and its solves, no matrix products.
 //
-// Algebra.  With M = K Sm K^H + Se = L L^H, Y = L^-1 K Sm and w = L^-1 (d - K mu):
-//   mu_post = mu + Y^H w,  S_post = Sm - Y^H Y,  res = d - K mu_post,
-// which equals the reference's KScc (M^-1 KS) and mu + (M^-1 KS)^H (d - K mu).
+// Algebra.  With M = L L^H, Y = L^-1 K S and w = L^-1 (d - K mu):
+//   mu_post = mu + Y^H w,  S_post = S - Y^H Y,  res = d - K mu_post,
+// which equals the reference's KScc (M^-1 K S) and mu + (M^-1 K S)^H (d - K mu).
+//
+// Precision.  CRV_MODE_PLAIN computes in the grids' precision.
+// CRV_MODE_FP32_REFINE (FP32 grids) solves in FP32 with FP32 accumulation
+// and sends a wavenumber to an FP64 solve with one step of iterative
+// refinement when the FP32 factorisation is rejected, when its condition
+// estimate (largest over smallest Cholesky pivot) exceeds the threshold, or
+// when |k| is below the low-wavenumber fraction of Nyquist.
 #include "internal.hpp"
 #include "linalg.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace {
 
 constexpr int kThreads = 256;
 
+// Per-kz tables in precision U; P = n (n + 1) / 2 packed lower triangle.
+template <typename U>
+struct KzTab {
+  const cplx_t<U>* K;   // [nzp][n][3]  forward matrix
+  const cplx_t<U>* E;   // [nzp][P]     error shape; separable: |ct| K Sigma0 K^H + E
+  const cplx_t<U>* B;   // [nzp][n][3]  separable: |ct| K Sigma0
+  const U* D;           // [nzp][n]     white-noise diagonal
+  const U* S;           // [nzp][6]     separable: |ct| Sigma0 (aa bb rr ab ar br)
+  const U* lam;         // [nyp][cnxp]  separable: |cxy|
+};
+
 template <typename T>
 struct PostArgs {
   cplx_t<T>* mean[3];
   cplx_t<T>* data[8];
-  const cplx_t<T>* err;       // errCorr spectrum (non-separable)
-  cplx_t<T>* cov[6];          // aa bb rr ab ar br
-  const cplx_t<T>* Kt;        // [nzp][n][3]
-  const cplx_t<T>* Et;        // [nzp][n][n]
-  const T* Dt;                // [nzp][n]
-  const int* inv;             // [nzp]
-  // FP64 copies of the tables (refine mode fallback)
-  const double2* Kt64;
-  const double2* Et64;
-  const double* Dt64;
-  const double* cxy;          // separable prior: [nyp][cnxp]
-  const double* ct;           // [nzp]
-  double sig0[6];
-  int separable;
+  const cplx_t<T>* err;   // errCorr spectrum (general prior)
+  cplx_t<T>* cov[6];      // aa bb rr ab ar br
+  KzTab<T> t;
+  KzTab<double> t64;      // FP64 copies (refine mode)
+  const int* inv;         // [nzp] band mask
+  T sig0[6];
+  double sig0d[6];
   int nxp, cnxp, nyp, nzp;
   long long nc;
-  unsigned long long* stats;  // [0] inverted [1] chol fail [2] fp64 [3] max cond (bits)
-  unsigned int* list;         // refine mode: indices to redo in FP64
+  unsigned long long* stats;  // [0] rejected factorisations [1] max condition (double bits)
+  unsigned int* list;         // refine mode: wavenumbers for the FP64 solve
   unsigned int* list_count;
-  double cond_thr;
+  float cond_thr;
   double lowk2;
 };
-
-// Gather the cleaned prior (all entries real: getParameterCovariance keeps
-// Sigma0[a][b] * |Re(c)/Sigma0[a][b]|) and the error-spectrum multiplier.
-template <typename T>
-__device__ __forceinline__ void load_prior(const PostArgs<T>& a, long long idx, int i, int j, int k,
-                                           double s[6], double& lam) {
-  if (a.separable) {
-    const double cx = a.cxy[(size_t)j * a.cnxp + i];
-    const double sc = fabs(cx * a.ct[k]);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) s[q] = a.sig0[q] * sc;
-    lam = fabs(cx);
-  } else {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const double re = (double)a.cov[q][idx].x;
-      const double sg = a.sig0[q];
-      s[q] = (sg != 0.0) ? sg * fabs(re / sg) : 0.0;
-    }
-    lam = fabs((double)a.err[idx].x);
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void store_cov_real(const PostArgs<T>& a, long long idx, const double s[6]) {
-  using C = cplx_t<T>;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) a.cov[q][idx] = cmake<C>((T)s[q], (T)0);
-}
-
-__device__ __forceinline__ void stat_add(unsigned long long* p, unsigned long long v) {
-  // one atomic per wavefront
-  unsigned long long s = v;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(p, s);
-}
-__device__ __forceinline__ void stat_max(unsigned long long* p, double v) {
-  double m = v;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(m));
-}
-
-// Full solve for one wavenumber in precision U (storage T).  Returns false
-// when the factorisation is rejected (posterior = prior).
-template <typename T, typename U, int N>
-__device__ __forceinline__ bool solve_one(const PostArgs<T>& a, long long idx, int k, const double s[6],
-                                          double lam, const cplx_t<U>* Kt, const cplx_t<U>* Et, const U* Dt,
-                                          double& cond) {
-  using C = cplx_t<U>;
-  using CT = cplx_t<T>;
-  C K[N][3];
-#pragma unroll
-  for (int l = 0; l < N; ++l)
-#pragma unroll
-    for (int m = 0; m < 3; ++m) K[l][m] = Kt[((size_t)k * N + l) * 3 + m];
-  const U S[3][3] = {{(U)s[0], (U)s[3], (U)s[4]}, {(U)s[3], (U)s[1], (U)s[5]}, {(U)s[4], (U)s[5], (U)s[2]}};
-  C KS[N][4];  // columns 0..2: K Sm ; column 3: d - K mu
-#pragma unroll
-  for (int l = 0; l < N; ++l)
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      C x = cmake<C>((U)0, (U)0);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) x = cadd(x, cscale(K[l][q], S[q][m]));
-      KS[l][m] = x;
-    }
-  C A[N * (N + 1) / 2];
-  const U lm = (U)lam;
-#pragma unroll
-  for (int l = 0; l < N; ++l)
-#pragma unroll
-    for (int m = 0; m <= l; ++m) {
-      C x = cscale(Et[((size_t)k * N + l) * N + m], lm);
-      if (l == m) x.x += Dt[(size_t)k * N + l];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) x = cadd(x, cmulc(KS[l][q], K[m][q]));
-      A[l * (l + 1) / 2 + m] = x;
-    }
-  U dmin, dmax;
-  if (!chol_ref<U, N>(A, dmin, dmax)) return false;
-  cond = (double)(dmax / dmin) * (double)(dmax / dmin);
-  C mu[3];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) { CT v = a.mean[m][idx]; mu[m] = cmake<C>((U)v.x, (U)v.y); }
-  C d[N];
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    CT v = a.data[l][idx];
-    d[l] = cmake<C>((U)v.x, (U)v.y);
-    C x = d[l];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) x = csub(x, cmul(K[l][m], mu[m]));
-    KS[l][3] = x;
-  }
-  lsolve<U, N, 4>(A, KS);
-  // mu_post = mu + Y^H w ; S_post = Sm - Y^H Y
-  C mp[3];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    C x = mu[m];
-#pragma unroll
-    for (int l = 0; l < N; ++l) x = cadd(x, cmulc(KS[l][3], KS[l][m]));
-    mp[m] = x;
-    a.mean[m][idx] = cmake<CT>((T)x.x, (T)x.y);
-  }
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    C x = d[l];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) x = csub(x, cmul(K[l][m], mp[m]));
-    a.data[l][idx] = cmake<CT>((T)x.x, (T)x.y);
-  }
-  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    C x = cmake<C>((U)s[q], (U)0);
-#pragma unroll
-    for (int l = 0; l < N; ++l) x = csub(x, cmulc(KS[l][pb[q]], KS[l][pa[q]]));
-    a.cov[q][idx] = cmake<CT>((T)x.x, (T)x.y);
-  }
-  return true;
-}
 
 __device__ __forceinline__ void decode(long long idx, int cnxp, int nyp, int& i, int& j, int& k) {
   const long long plane = (long long)cnxp * nyp;
@@ -186,229 +80,320 @@ __device__ __forceinline__ void decode(long long idx, int cnxp, int nyp, int& i,
   i = rem - j * cnxp;
 }
 
-// Plain mode: everything in the grids' precision T.
-template <typename T, int N>
+// Cleaned prior from the covariance grids (getParameterCovariance keeps
+// Sigma0[a][b] * |Re(c) / Sigma0[a][b]|, a real value) and the error
+// multiplier |Re errCorr|.
+template <typename T, typename U>
+__device__ __forceinline__ void gather_general(const PostArgs<T>& a, long long idx, const U* sig0, U s[6], U& lam) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const U re = (U)a.cov[q][idx].x;
+    const U sg = sig0[q];
+    s[q] = (sg != (U)0) ? sg * fabs(re / sg) : (U)0;
+  }
+  lam = fabs((U)a.err[idx].x);
+}
+
+template <typename T, typename U>
+__device__ __forceinline__ void store_prior(const PostArgs<T>& a, long long idx, const U s[6]) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a.cov[q][idx] = cmake<cplx_t<T>>((T)s[q], (T)0);
+}
+
+// M (packed lower) and the right-hand sides B = K S (n x 3), in precision U.
+// Separable: B holds |ct| K Sigma0 (not yet multiplied by lam).
+template <typename U, int N, bool SEP>
+__device__ __forceinline__ void build_system(const KzTab<U>& t, int k, U lam, const U s[6], cplx_t<U>* M,
+                                             cplx_t<U> (*B)[4]) {
+  using C = cplx_t<U>;
+  constexpr int P = N * (N + 1) / 2;
+  if constexpr (SEP) {
+#pragma unroll
+    for (int q = 0; q < P; ++q) M[q] = cscale(t.E[(size_t)k * P + q], lam);
+#pragma unroll
+    for (int l = 0; l < N; ++l)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) B[l][m] = t.B[((size_t)k * N + l) * 3 + m];
+  } else {
+    const C* Kk = t.K + (size_t)k * N * 3;
+    const U S[3][3] = {{s[0], s[3], s[4]}, {s[3], s[1], s[5]}, {s[4], s[5], s[2]}};
+#pragma unroll
+    for (int l = 0; l < N; ++l)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        C x = cmake<C>((U)0, (U)0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) x = cadd(x, cscale(Kk[l * 3 + q], S[q][m]));
+        B[l][m] = x;
+      }
+#pragma unroll
+    for (int l = 0; l < N; ++l)
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        C x = cscale(t.E[(size_t)k * P + l * (l + 1) / 2 + m], lam);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) x = cadd(x, cmulc(B[l][q], Kk[m * 3 + q]));
+        M[l * (l + 1) / 2 + m] = x;
+      }
+  }
+#pragma unroll
+  for (int l = 0; l < N; ++l) M[l * (l + 1) / 2 + l].x += t.D[(size_t)k * N + l];
+}
+
+// One reduction per block for the statistics.
+__device__ __forceinline__ void block_stats(unsigned long long* st, unsigned long long nfail, double cmax) {
+  __shared__ unsigned long long s_f[kThreads / 64];
+  __shared__ double s_c[kThreads / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    nfail += __shfl_down(nfail, off, 64);
+    cmax = fmax(cmax, __shfl_down(cmax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) { s_f[threadIdx.x >> 6] = nfail; s_c[threadIdx.x >> 6] = cmax; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long f = 0;
+    double c = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) { f += s_f[w]; c = fmax(c, s_c[w]); }
+    if (f) atomicAdd(&st[0], f);
+    if (c > 0.0) atomicMax(&st[1], (unsigned long long)__double_as_longlong(c));
+  }
+}
+
+// Main pass: arithmetic in the grids' precision T.  LIST (FP32 refine mode)
+// queues rejected, ill-conditioned and low-|k| wavenumbers for the FP64 pass
+// instead of finishing them.
+template <typename T, int N, bool SEP, bool LIST>
 __global__ __launch_bounds__(kThreads) void k_posterior(PostArgs<T> a) {
+  using C = cplx_t<T>;
+  constexpr int P = N * (N + 1) / 2;
+  unsigned long long nfail = 0;
+  double cmax = 0.0;
   const long long stride = (long long)gridDim.x * kThreads;
-  for (long long base = (long long)blockIdx.x * kThreads; base < a.nc; base += stride) {
-    const long long idx = base + threadIdx.x;
-    unsigned long long ninv = 0, nfail = 0;
-    double cond = 0.0;
-    if (idx < a.nc) {
-      int i, j, k;
-      decode(idx, a.cnxp, a.nyp, i, j, k);
-      double s[6], lam;
-      load_prior(a, idx, i, j, k, s, lam);
-      bool done = false;
-      if (a.inv[k]) {
-        ninv = 1;
-        done = solve_one<T, T, N>(a, idx, k, s, lam, a.Kt, a.Et, a.Dt, cond);
-        if (!done) nfail = 1;
-      }
-      if (!done) store_cov_real(a, idx, s);
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < a.nc; idx += stride) {
+    int i, j, k;
+    decode(idx, a.cnxp, a.nyp, i, j, k);
+    T lam, s[6];
+    if constexpr (SEP) {
+      lam = a.t.lam[(size_t)j * a.cnxp + i];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) s[q] = lam * a.t.S[(size_t)k * 6 + q];
+    } else {
+      gather_general<T, T>(a, idx, a.sig0, s, lam);
     }
-    stat_add(&a.stats[0], ninv);
-    stat_add(&a.stats[1], nfail);
-    stat_max(&a.stats[3], cond);
+    if (!a.inv[k]) { store_prior(a, idx, s); continue; }
+    C A[P];
+    C B[N][4];  // columns 0..2: K S (separable: |ct| K Sigma0), column 3: d - K mu
+    build_system<T, N, SEP>(a.t, k, lam, s, A, B);
+    T rd[N], pmin = (T)0, pmax = (T)0;
+    const bool ok = chol_rcp<T, N>(A, rd, pmin, pmax);
+    const T cond = ok ? pmax / pmin : (T)0;
+    if constexpr (LIST) {
+      const int ky = min(j, a.nyp - j), kz = min(k, a.nzp - k);
+      const double fx = (double)i / (0.5 * a.nxp), fy = (double)ky / (0.5 * a.nyp), fz = (double)kz / (0.5 * a.nzp);
+      const bool lowk = (fx * fx + fy * fy + fz * fz) < a.lowk2;
+      if (!ok || lowk || cond > a.cond_thr) {
+        const unsigned int slot = atomicAdd(a.list_count, 1u);
+        a.list[slot] = (unsigned int)idx;
+        continue;
+      }
+    } else {
+      if (!ok) { ++nfail; store_prior(a, idx, s); continue; }
+    }
+    cmax = fmax(cmax, (double)cond);
+    const C* Kk = a.t.K + (size_t)k * N * 3;
+    C mu[3], d[N];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      d[l] = a.data[l][idx];
+      C x = d[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mu[m]));
+      B[l][3] = x;
+    }
+    lsolve_r<T, N, 4>(A, rd, B);
+    const T sb = SEP ? lam : (T)1;  // K S = sb * B
+    C mp[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      C x = cmake<C>((T)0, (T)0);
+#pragma unroll
+      for (int l = 0; l < N; ++l) x = cadd(x, cmulc(B[l][3], B[l][m]));
+      mp[m] = cadd(mu[m], cscale(x, sb));
+      a.mean[m][idx] = mp[m];
+    }
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      C x = d[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mp[m]));
+      a.data[l][idx] = x;
+    }
+    const T sb2 = sb * sb;
+    constexpr int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      C x = cmake<C>((T)0, (T)0);
+#pragma unroll
+      for (int l = 0; l < N; ++l) x = cadd(x, cmulc(B[l][pb[q]], B[l][pa[q]]));
+      a.cov[q][idx] = cmake<C>(s[q] - sb2 * x.x, -sb2 * x.y);
+    }
   }
+  block_stats(a.stats, nfail, cmax);
 }
 
-// FP32 storage, FP32 factorisation + one step of iterative refinement with
-// FP64 residuals; wavenumbers that are ill-conditioned (cond > cond_thr),
-// near |k| = 0, or rejected in FP32 are queued for an FP64 solve.
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_posterior_refine(PostArgs<float> a) {
-  using C = float2;
-  using D = double2;
-  const long long stride = (long long)gridDim.x * kThreads;
-  for (long long base = (long long)blockIdx.x * kThreads; base < a.nc; base += stride) {
-    const long long idx = base + threadIdx.x;
-    unsigned long long ninv = 0;
-    double cond = 0.0;
-    if (idx < a.nc) {
-      int i, j, k;
-      decode(idx, a.cnxp, a.nyp, i, j, k);
-      double s[6], lam;
-      load_prior(a, idx, i, j, k, s, lam);
-      bool done = false;
-      if (a.inv[k]) {
-        ninv = 1;
-        const int ky = min(j, a.nyp - j), kz = min(k, a.nzp - k);
-        const double fx = (double)i / (0.5 * a.nxp), fy = (double)ky / (0.5 * a.nyp), fz = (double)kz / (0.5 * a.nzp);
-        const bool lowk = (fx * fx + fy * fy + fz * fz) < a.lowk2;
-        C K[N][3];
-#pragma unroll
-        for (int l = 0; l < N; ++l)
-#pragma unroll
-          for (int m = 0; m < 3; ++m) K[l][m] = a.Kt[((size_t)k * N + l) * 3 + m];
-        const float S[3][3] = {{(float)s[0], (float)s[3], (float)s[4]},
-                               {(float)s[3], (float)s[1], (float)s[5]},
-                               {(float)s[4], (float)s[5], (float)s[2]}};
-        C B[N][4];
-#pragma unroll
-        for (int l = 0; l < N; ++l)
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            C x = cmake<C>(0.f, 0.f);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) x = cadd(x, cscale(K[l][q], S[q][m]));
-            B[l][m] = x;
-          }
-        C A[N * (N + 1) / 2];
-#pragma unroll
-        for (int l = 0; l < N; ++l)
-#pragma unroll
-          for (int m = 0; m <= l; ++m) {
-            C x = cscale(a.Et[((size_t)k * N + l) * N + m], (float)lam);
-            if (l == m) x.x += a.Dt[(size_t)k * N + l];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) x = cadd(x, cmulc(B[l][q], K[m][q]));
-            A[l * (l + 1) / 2 + m] = x;
-          }
-        C M[N * (N + 1) / 2];
-#pragma unroll
-        for (int t = 0; t < N * (N + 1) / 2; ++t) M[t] = A[t];
-        float dmin, dmax;
-        const bool ok = chol_ref<float, N>(A, dmin, dmax);
-        if (ok) cond = (double)(dmax / dmin) * (double)(dmax / dmin);
-        if (ok && !lowk && cond <= a.cond_thr) {
-          C mu[3];
-#pragma unroll
-          for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
-          C d[N];
-#pragma unroll
-          for (int l = 0; l < N; ++l) {
-            d[l] = a.data[l][idx];
-            C x = d[l];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) x = csub(x, cmul(K[l][m], mu[m]));
-            B[l][3] = x;
-          }
-          // One right-hand side at a time (columns of K Sm, then d - K mu):
-          // x = M^-1 b in FP32, residual b - M x in FP64, x += M^-1 residual.
-          // Each refined column is folded into the outputs at once, so only
-          // one solution vector is live.
-          C cv[6];
-          C mp[3];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            C x[N][1], y[N][1];
-#pragma unroll
-            for (int l = 0; l < N; ++l) x[l][0] = B[l][c];
-            lsolve<float, N, 1>(A, x);
-            lhsolve<float, N, 1>(A, x);
-#pragma unroll
-            for (int l = 0; l < N; ++l) {
-              D r = cmake<D>((double)B[l][c].x, (double)B[l][c].y);
-#pragma unroll
-              for (int m = 0; m < N; ++m) {
-                const C mm = (m <= l) ? M[l * (l + 1) / 2 + m] : cconj(M[m * (m + 1) / 2 + l]);
-                r = csub(r, cmul(cmake<D>((double)mm.x, (double)mm.y), cmake<D>((double)x[m][0].x, (double)x[m][0].y)));
-              }
-              y[l][0] = cmake<C>((float)r.x, (float)r.y);
-            }
-            lsolve<float, N, 1>(A, y);
-            lhsolve<float, N, 1>(A, y);
-#pragma unroll
-            for (int l = 0; l < N; ++l) x[l][0] = cadd(x[l][0], y[l][0]);
-            if (c < 3) {
-              // S_post[a][c] = Sm[a][c] - sum_l conj(KS[l][a]) x[l], a <= c
-#pragma unroll
-              for (int aa = 0; aa <= c; ++aa) {
-                C acc = cmake<C>(0.f, 0.f);
-#pragma unroll
-                for (int l = 0; l < N; ++l) acc = cadd(acc, cmulc(x[l][0], B[l][aa]));
-                const int q = (aa == c) ? c : (c == 1 ? 3 : (aa == 0 ? 4 : 5));
-                cv[q] = acc;
-              }
-            } else {
-#pragma unroll
-              for (int m = 0; m < 3; ++m) {
-                C acc = mu[m];
-#pragma unroll
-                for (int l = 0; l < N; ++l) acc = cadd(acc, cmulc(x[l][0], B[l][m]));
-                mp[m] = acc;
-              }
-            }
-          }
-#pragma unroll
-          for (int m = 0; m < 3; ++m) a.mean[m][idx] = mp[m];
-#pragma unroll
-          for (int l = 0; l < N; ++l) {
-            C x = d[l];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) x = csub(x, cmul(K[l][m], mp[m]));
-            a.data[l][idx] = x;
-          }
-#pragma unroll
-          for (int q = 0; q < 6; ++q)
-            a.cov[q][idx] = cmake<C>((float)(s[q] - (double)cv[q].x), (float)(-(double)cv[q].y));
-          done = true;
-        } else {
-          const unsigned int slot = atomicAdd(a.list_count, 1u);
-          a.list[slot] = (unsigned int)idx;
-          done = true;  // finished by k_posterior_list64
-        }
-      }
-      if (!done) store_cov_real(a, idx, s);
-    }
-    stat_add(&a.stats[0], ninv);
-    stat_max(&a.stats[3], cond);
-  }
-}
-
-// FP64 solve for the queued wavenumbers (FP32 storage).
-template <int N>
+// FP64 solve with one step of iterative refinement for the queued
+// wavenumbers (FP32 storage): per right-hand side b (columns of K S, then
+// d - K mu), x = M^-1 b, r = b - M x, x += M^-1 r, with M built in FP64 from
+// the FP64 tables.
+template <int N, bool SEP>
 __global__ __launch_bounds__(kThreads) void k_posterior_list64(PostArgs<float> a) {
+  using D = double2;
+  constexpr int P = N * (N + 1) / 2;
+  unsigned long long nfail = 0;
+  double cmax = 0.0;
   const unsigned int n = *a.list_count;
   for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < n; t += gridDim.x * kThreads) {
     const long long idx = a.list[t];
     int i, j, k;
     decode(idx, a.cnxp, a.nyp, i, j, k);
-    double s[6], lam;
-    load_prior(a, idx, i, j, k, s, lam);
-    double cond = 0.0;
-    const bool ok = solve_one<float, double, N>(a, idx, k, s, lam, a.Kt64, a.Et64, a.Dt64, cond);
-    if (!ok) {
-      store_cov_real(a, idx, s);
-      atomicAdd(&a.stats[1], 1ull);
+    double lam, s[6];
+    if constexpr (SEP) {
+      lam = a.t64.lam[(size_t)j * a.cnxp + i];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) s[q] = lam * a.t64.S[(size_t)k * 6 + q];
+    } else {
+      gather_general<float, double>(a, idx, a.sig0d, s, lam);
     }
-    atomicAdd(&a.stats[2], 1ull);
+    D M[P], B[N][4];
+    build_system<double, N, SEP>(a.t64, k, lam, s, M, B);
+    if constexpr (SEP) {
+#pragma unroll
+      for (int l = 0; l < N; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) B[l][m] = cscale(B[l][m], lam);
+    }
+    D A[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) A[q] = M[q];
+    double rd[N], pmin = 0.0, pmax = 0.0;
+    if (!chol_rcp<double, N>(A, rd, pmin, pmax)) { ++nfail; store_prior(a, idx, s); continue; }
+    cmax = fmax(cmax, pmax / pmin);
+    const D* Kk = a.t64.K + (size_t)k * N * 3;
+    D mu[3], d[N];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) { const float2 v = a.mean[m][idx]; mu[m] = cmake<D>(v.x, v.y); }
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      const float2 v = a.data[l][idx];
+      d[l] = cmake<D>(v.x, v.y);
+      D x = d[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mu[m]));
+      B[l][3] = x;
+    }
+    D cv[6], mp[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      D x[N][1], y[N][1];
+#pragma unroll
+      for (int l = 0; l < N; ++l) x[l][0] = B[l][c];
+      lsolve_r<double, N, 1>(A, rd, x);
+      lhsolve_r<double, N, 1>(A, rd, x);
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        D r = B[l][c];
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+          const D mm = (m <= l) ? M[l * (l + 1) / 2 + m] : cconj(M[m * (m + 1) / 2 + l]);
+          r = csub(r, cmul(mm, x[m][0]));
+        }
+        y[l][0] = r;
+      }
+      lsolve_r<double, N, 1>(A, rd, y);
+      lhsolve_r<double, N, 1>(A, rd, y);
+#pragma unroll
+      for (int l = 0; l < N; ++l) x[l][0] = cadd(x[l][0], y[l][0]);
+      if (c < 3) {
+        // (B^H M^-1 B)[aa][c] for aa <= c
+#pragma unroll
+        for (int aa = 0; aa <= c; ++aa) {
+          D acc = cmake<D>(0.0, 0.0);
+#pragma unroll
+          for (int l = 0; l < N; ++l) acc = cadd(acc, cmulc(x[l][0], B[l][aa]));
+          const int q = (aa == c) ? c : (c == 1 ? 3 : (aa == 0 ? 4 : 5));
+          cv[q] = acc;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          D acc = mu[m];
+#pragma unroll
+          for (int l = 0; l < N; ++l) acc = cadd(acc, cmulc(x[l][0], B[l][m]));
+          mp[m] = acc;
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) a.mean[m][idx] = make_float2((float)mp[m].x, (float)mp[m].y);
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      D x = d[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mp[m]));
+      a.data[l][idx] = make_float2((float)x.x, (float)x.y);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.cov[q][idx] = make_float2((float)(s[q] - cv[q].x), (float)(-cv[q].y));
   }
+  block_stats(a.stats, nfail, cmax);
 }
 
-template <typename T>
-int launch_plain(crv_ctx* ctx, int n, const PostArgs<T>& a, int nblocks) {
-  ProfScope ps(ctx, "posterior");
-  switch (n) {
-#define CASE(NN) case NN: hipLaunchKernelGGL((k_posterior<T, NN>), dim3(nblocks), dim3(kThreads), 0, ctx->stream, a); break;
-    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-    default: return crv_set_error(ctx, CRV_E_UNSUPPORTED, "ntheta must be 1..8");
-  }
+// Persistent grid: as many blocks as fit on the device at once.
+template <typename F>
+int persistent_blocks(crv_ctx* ctx, F kernel, long long work) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu < 1)
+    per_cu = 2;
+  const long long want = (work + kThreads - 1) / kThreads;
+  return (int)std::max<long long>(1, std::min<long long>(want, (long long)per_cu * ctx->num_cu));
+}
+
+template <typename T, int N, bool SEP, bool LIST>
+int launch_main(crv_ctx* ctx, const PostArgs<T>& a) {
+  const int nb = persistent_blocks(ctx, k_posterior<T, N, SEP, LIST>, a.nc);
+  ProfScope ps(ctx, LIST ? "posterior_fp32" : "posterior");
+  hipLaunchKernelGGL((k_posterior<T, N, SEP, LIST>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;
 }
 
-int launch_refine(crv_ctx* ctx, int n, const PostArgs<float>& a, int nblocks) {
+template <int N, bool SEP>
+int launch_list(crv_ctx* ctx, const PostArgs<float>& a) {
+  ProfScope ps(ctx, "posterior_fp64_list");
+  hipLaunchKernelGGL((k_posterior_list64<N, SEP>), dim3(256), dim3(kThreads), 0, ctx->stream, a);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
+template <typename T, bool SEP, bool LIST>
+int dispatch(crv_ctx* ctx, int n, const PostArgs<T>& a) {
+  int rc = CRV_OK;
   switch (n) {
-#define CASE(NN)                                                                                   \
-  case NN:                                                                                         \
-    {                                                                                              \
-      ProfScope p1(ctx, "posterior_refine");                                                       \
-      hipLaunchKernelGGL((k_posterior_refine<NN>), dim3(nblocks), dim3(kThreads), 0, ctx->stream, a); \
-    }                                                                                              \
-    {                                                                                              \
-      ProfScope p2(ctx, "posterior_fp64_list");                                                    \
-      hipLaunchKernelGGL((k_posterior_list64<NN>), dim3(1024), dim3(kThreads), 0, ctx->stream, a);  \
-    }                                                                                              \
+#define CASE(NN)                                                                          \
+  case NN:                                                                                \
+    rc = launch_main<T, NN, SEP, LIST>(ctx, a);                                           \
+    if constexpr (LIST && std::is_same<T, float>::value)                                  \
+      if (rc == CRV_OK) rc = launch_list<NN, SEP>(ctx, a);                                \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
     default: return crv_set_error(ctx, CRV_E_UNSUPPORTED, "ntheta must be 1..8");
   }
-  CRV_HIP_CHECK(ctx, hipGetLastError());
-  return CRV_OK;
+  return rc;
 }
 
 int check_grid(crv_ctx* ctx, const crv_grid* g, const crv_grid* ref, const char* what) {
@@ -422,6 +407,38 @@ int check_grid(crv_ctx* ctx, const crv_grid* g, const crv_grid* ref, const char*
 int ensure_transformed(crv_grid* g) {
   if (g->transformed) return CRV_OK;
   return crv_fft_r2c(g);
+}
+
+// Host-side FP64 tables, interleaved re/im for complex entries.
+struct HostTables {
+  std::vector<double> K, E, B, D, S, lam;
+  std::vector<int> inv;
+};
+
+// Serialises the tables in precision U into blob, recording offsets.
+struct TabOffsets { size_t K = 0, E = 0, B = 0, D = 0, S = 0, lam = 0; };
+
+template <typename U>
+TabOffsets put_tables(std::vector<unsigned char>& blob, const HostTables& h) {
+  TabOffsets o;
+  auto put = [&](const std::vector<double>& v) {
+    const size_t at = blob.size();
+    const size_t bytes = (v.size() * sizeof(U) + 255) & ~(size_t)255;
+    blob.resize(at + std::max<size_t>(bytes, 256), 0);
+    U* dst = (U*)(blob.data() + at);
+    for (size_t q = 0; q < v.size(); ++q) dst[q] = (U)v[q];
+    return at;
+  };
+  o.K = put(h.K); o.E = put(h.E); o.B = put(h.B); o.D = put(h.D); o.S = put(h.S); o.lam = put(h.lam);
+  return o;
+}
+
+template <typename U>
+KzTab<U> tab_at(unsigned char* base, const TabOffsets& o) {
+  KzTab<U> t;
+  t.K = (const cplx_t<U>*)(base + o.K); t.E = (const cplx_t<U>*)(base + o.E); t.B = (const cplx_t<U>*)(base + o.B);
+  t.D = (const U*)(base + o.D); t.S = (const U*)(base + o.S); t.lam = (const U*)(base + o.lam);
+  return t;
 }
 
 }  // namespace
@@ -445,8 +462,10 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
   const bool refine = (p->mode == CRV_MODE_FP32_REFINE);
   if (refine && g0->precision != CRV_FP32)
     return crv_set_error(ctx, CRV_E_INVALID, "CRV_MODE_FP32_REFINE needs FP32 grids");
-
   const int nxp = g0->nxp, nyp = g0->nyp, nzp = g0->nzp, cnxp = g0->cnxp;
+  const long long nc = (long long)cnxp * nyp * nzp;
+  if (refine && nc > 0xffffffffLL) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "refine mode: grid above 2^32 wavenumbers");
+
   // Forward transforms that the reference performs inside the call or just
   // before it (crava.cpp:177-184, 1039, 1062).
   for (int m = 0; m < 3; ++m) if ((rc = ensure_transformed(mean[m])) != CRV_OK) return rc;
@@ -457,25 +476,38 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
   }
 
   // ---- kz tables (crava.cpp:1104-1121, 1336-1364, 1441-1469) -------------
-  std::vector<double> Kt((size_t)nzp * n * 3 * 2), Et((size_t)nzp * n * n * 2), Dt((size_t)nzp * n);
-  std::vector<int> inv(nzp);
+  const int P = n * (n + 1) / 2;
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  HostTables h;
+  h.K.assign((size_t)nzp * n * 3 * 2, 0.0);
+  h.E.assign((size_t)nzp * P * 2, 0.0);
+  h.D.assign((size_t)nzp * n, 0.0);
+  h.inv.assign(nzp, 0);
+  if (p->separable) {
+    h.B.assign((size_t)nzp * n * 3 * 2, 0.0);
+    h.S.assign((size_t)nzp * 6, 0.0);
+    h.lam.resize((size_t)cnxp * nyp);
+    for (size_t q = 0; q < h.lam.size(); ++q) h.lam[q] = std::fabs(p->cxy[q]);
+  }
   const double wnc = p->wnc;
+  long long ninv_planes = 0;
   for (int k = 0; k < nzp; ++k) {
     // crava.cpp:1104-1105: float numerator over a double product, compared
     // with lowCut * minRelThick in double and with highCut in float
     const float fr = (float)((double)(p->nz * 1000.0f) / (p->lz * nzp) * (double)std::min(k, nzp - k));
-    inv[k] = ((double)fr > p->low_cut * p->min_rel_thick && fr < (float)p->high_cut) ? 1 : 0;
+    h.inv[k] = ((double)fr > p->low_cut * p->min_rel_thick && fr < (float)p->high_cut) ? 1 : 0;
+    ninv_planes += h.inv[k];
     for (int l = 0; l < n; ++l) {
       const double wr = p->kwd[((size_t)k * n + l) * 2], wi = p->kwd[((size_t)k * n + l) * 2 + 1];
       for (int m = 0; m < 3; ++m) {
         const double am = p->A[l * 3 + m];
-        Kt[(((size_t)k * n + l) * 3 + m) * 2] = wr * am;
-        Kt[(((size_t)k * n + l) * 3 + m) * 2 + 1] = wi * am;
+        h.K[(((size_t)k * n + l) * 3 + m) * 2] = wr * am;
+        h.K[(((size_t)k * n + l) * 3 + m) * 2 + 1] = wi * am;
       }
       const double* e1 = p->em1 + ((size_t)k * n) * 2;
       const double* e2 = p->em2 + ((size_t)k * n) * 2;
       const double* e3 = p->em3 + ((size_t)k * n) * 2;
-      for (int m = 0; m < n; ++m) {
+      for (int m = 0; m <= l; ++m) {
         const double th = p->err_theta_cov[l * n + m];
         const double c = 0.5 * (1.0 - wnc) * th;
         // e_l * conj(e_m), summed over the two error shapes
@@ -483,101 +515,94 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
                           e2[2 * l + 1] * e2[2 * m + 1];
         const double im = -e1[2 * l] * e1[2 * m + 1] + e1[2 * l + 1] * e1[2 * m] - e2[2 * l] * e2[2 * m + 1] +
                           e2[2 * l + 1] * e2[2 * m];
-        Et[(((size_t)k * n + l) * n + m) * 2] = c * re;
-        Et[(((size_t)k * n + l) * n + m) * 2 + 1] = (l == m) ? 0.0 : c * im;
+        const size_t o = ((size_t)k * P + l * (l + 1) / 2 + m) * 2;
+        h.E[o] = c * re;
+        h.E[o + 1] = (l == m) ? 0.0 : c * im;
       }
-      Dt[(size_t)k * n + l] = wnc * p->err_theta_cov[l * n + l] * e3[2 * l] * e3[2 * l];
+      h.D[(size_t)k * n + l] = wnc * p->err_theta_cov[l * n + l] * e3[2 * l] * e3[2 * l];
+    }
+    if (p->separable) {
+      // |ct| Sigma0, |ct| K Sigma0 and |ct| K Sigma0 K^H + E
+      const double act = std::fabs(p->ct[k]);
+      for (int q = 0; q < 6; ++q) h.S[(size_t)k * 6 + q] = act * p->sigma0[pa[q] * 3 + pb[q]];
+      const double* Kk = h.K.data() + (size_t)k * n * 3 * 2;
+      double* Bk = h.B.data() + (size_t)k * n * 3 * 2;
+      for (int l = 0; l < n; ++l)
+        for (int m = 0; m < 3; ++m) {
+          double xr = 0.0, xi = 0.0;
+          for (int q = 0; q < 3; ++q) {
+            const double sv = act * p->sigma0[q * 3 + m];
+            xr += Kk[(l * 3 + q) * 2] * sv;
+            xi += Kk[(l * 3 + q) * 2 + 1] * sv;
+          }
+          Bk[(l * 3 + m) * 2] = xr;
+          Bk[(l * 3 + m) * 2 + 1] = xi;
+        }
+      for (int l = 0; l < n; ++l)
+        for (int m = 0; m <= l; ++m) {
+          double xr = 0.0, xi = 0.0;  // sum_q B[l][q] conj(K[m][q])
+          for (int q = 0; q < 3; ++q) {
+            const double br = Bk[(l * 3 + q) * 2], bi = Bk[(l * 3 + q) * 2 + 1];
+            const double kr = Kk[(m * 3 + q) * 2], ki = Kk[(m * 3 + q) * 2 + 1];
+            xr += br * kr + bi * ki;
+            xi += bi * kr - br * ki;
+          }
+          const size_t o = ((size_t)k * P + l * (l + 1) / 2 + m) * 2;
+          h.E[o] += xr;
+          h.E[o + 1] = (l == m) ? 0.0 : h.E[o + 1] + xi;
+        }
     }
   }
   const bool f64 = (g0->precision == CRV_FP64);
-  const size_t esz = f64 ? 8 : 4;
-  // device table layout: [Kt T][Et T][Dt T][inv int][Kt64][Et64][Dt64][cxy][ct][stats][list count]
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t bK = al(Kt.size() * esz), bE = al(Et.size() * esz), bD = al(Dt.size() * esz), bI = al(nzp * 4);
-  const size_t bK64 = refine ? al(Kt.size() * 8) : 0, bE64 = refine ? al(Et.size() * 8) : 0,
-               bD64 = refine ? al(Dt.size() * 8) : 0;
-  const size_t bX = p->separable ? al((size_t)cnxp * nyp * 8) : 0, bT = p->separable ? al((size_t)nzp * 8) : 0;
-  const size_t bS = 256;
-  const size_t total = bK + bE + bD + bI + bK64 + bE64 + bD64 + bX + bT + bS;
-  std::vector<unsigned char> h(total, 0);
-  size_t off = 0;
-  auto put = [&](const void* src, size_t bytes, size_t slot) {
-    if (bytes) memcpy(h.data() + off, src, bytes);
-    const size_t o = off;
-    off += slot;
-    return o;
-  };
-  size_t oK, oE, oD;
-  if (f64) {
-    oK = put(Kt.data(), Kt.size() * 8, bK);
-    oE = put(Et.data(), Et.size() * 8, bE);
-    oD = put(Dt.data(), Dt.size() * 8, bD);
-  } else {
-    std::vector<float> fK(Kt.begin(), Kt.end()), fE(Et.begin(), Et.end()), fD(Dt.begin(), Dt.end());
-    oK = put(fK.data(), fK.size() * 4, bK);
-    oE = put(fE.data(), fE.size() * 4, bE);
-    oD = put(fD.data(), fD.size() * 4, bD);
-  }
-  const size_t oI = put(inv.data(), (size_t)nzp * 4, bI);
-  size_t oK64 = 0, oE64 = 0, oD64 = 0;
-  if (refine) {
-    oK64 = put(Kt.data(), Kt.size() * 8, bK64);
-    oE64 = put(Et.data(), Et.size() * 8, bE64);
-    oD64 = put(Dt.data(), Dt.size() * 8, bD64);
-  }
-  size_t oX = 0, oT = 0;
-  if (p->separable) {
-    oX = put(p->cxy, (size_t)cnxp * nyp * 8, bX);
-    oT = put(p->ct, (size_t)nzp * 8, bT);
-  }
-  const size_t oS = put(nullptr, 0, bS);
+  // device blob: [tables T][tables FP64 (refine)][inv][stats 256 B]
+  std::vector<unsigned char> blob;
+  const TabOffsets oT = f64 ? put_tables<double>(blob, h) : put_tables<float>(blob, h);
+  TabOffsets o64;
+  if (refine) o64 = put_tables<double>(blob, h);
+  const size_t oI = blob.size();
+  blob.resize(oI + (((size_t)nzp * 4 + 255) & ~(size_t)255), 0);
+  memcpy(blob.data() + oI, h.inv.data(), (size_t)nzp * 4);
+  const size_t oS = blob.size();
+  blob.resize(oS + 256, 0);
   unsigned char* dtab = nullptr;
-  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dtab, total, ctx->stream));
-  CRV_HIP_CHECK(ctx, hipMemcpyAsync(dtab, h.data(), total, hipMemcpyHostToDevice, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dtab, blob.size(), ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(dtab, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
   unsigned int* dlist = nullptr;
-  const long long nc = (long long)cnxp * nyp * nzp;
   if (refine) CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dlist, (size_t)nc * 4, ctx->stream));
+  unsigned long long* dstats = (unsigned long long*)(dtab + oS);
 
-  const int nblocks = (int)std::min<long long>((nc + kThreads - 1) / kThreads, 256LL * 64);
   if (f64) {
     PostArgs<double> a{};
     for (int m = 0; m < 3; ++m) a.mean[m] = (double2*)mean[m]->d;
     for (int l = 0; l < n; ++l) a.data[l] = (double2*)data[l]->d;
     a.err = p->separable ? nullptr : (const double2*)err_corr->d;
     for (int q = 0; q < 6; ++q) a.cov[q] = (double2*)cov[q]->d;
-    a.Kt = (const double2*)(dtab + oK); a.Et = (const double2*)(dtab + oE); a.Dt = (const double*)(dtab + oD);
+    a.t = tab_at<double>(dtab, oT);
     a.inv = (const int*)(dtab + oI);
-    a.cxy = p->separable ? (const double*)(dtab + oX) : nullptr;
-    a.ct = p->separable ? (const double*)(dtab + oT) : nullptr;
-    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
-    for (int q = 0; q < 6; ++q) a.sig0[q] = p->sigma0[pa[q] * 3 + pb[q]];
-    a.separable = p->separable; a.nxp = nxp; a.cnxp = cnxp; a.nyp = nyp; a.nzp = nzp; a.nc = nc;
-    a.stats = (unsigned long long*)(dtab + oS);
-    rc = launch_plain<double>(ctx, n, a, nblocks);
+    for (int q = 0; q < 6; ++q) a.sig0[q] = a.sig0d[q] = p->sigma0[pa[q] * 3 + pb[q]];
+    a.nxp = nxp; a.cnxp = cnxp; a.nyp = nyp; a.nzp = nzp; a.nc = nc;
+    a.stats = dstats;
+    rc = p->separable ? dispatch<double, true, false>(ctx, n, a) : dispatch<double, false, false>(ctx, n, a);
   } else {
     PostArgs<float> a{};
     for (int m = 0; m < 3; ++m) a.mean[m] = (float2*)mean[m]->d;
     for (int l = 0; l < n; ++l) a.data[l] = (float2*)data[l]->d;
     a.err = p->separable ? nullptr : (const float2*)err_corr->d;
     for (int q = 0; q < 6; ++q) a.cov[q] = (float2*)cov[q]->d;
-    a.Kt = (const float2*)(dtab + oK); a.Et = (const float2*)(dtab + oE); a.Dt = (const float*)(dtab + oD);
+    a.t = tab_at<float>(dtab, oT);
     a.inv = (const int*)(dtab + oI);
-    a.cxy = p->separable ? (const double*)(dtab + oX) : nullptr;
-    a.ct = p->separable ? (const double*)(dtab + oT) : nullptr;
-    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
-    for (int q = 0; q < 6; ++q) a.sig0[q] = p->sigma0[pa[q] * 3 + pb[q]];
-    a.separable = p->separable; a.nxp = nxp; a.cnxp = cnxp; a.nyp = nyp; a.nzp = nzp; a.nc = nc;
-    a.stats = (unsigned long long*)(dtab + oS);
+    for (int q = 0; q < 6; ++q) { a.sig0d[q] = p->sigma0[pa[q] * 3 + pb[q]]; a.sig0[q] = (float)a.sig0d[q]; }
+    a.nxp = nxp; a.cnxp = cnxp; a.nyp = nyp; a.nzp = nzp; a.nc = nc;
+    a.stats = dstats;
     if (refine) {
-      a.Kt64 = (const double2*)(dtab + oK64); a.Et64 = (const double2*)(dtab + oE64);
-      a.Dt64 = (const double*)(dtab + oD64);
+      a.t64 = tab_at<double>(dtab, o64);
       a.list = dlist;
       a.list_count = (unsigned int*)(dtab + oS + 64);
-      a.cond_thr = p->cond_threshold > 0 ? p->cond_threshold : 1e5;
+      a.cond_thr = (float)(p->cond_threshold > 0 ? p->cond_threshold : 1e5);
       a.lowk2 = p->lowk_fraction * p->lowk_fraction;
-      rc = launch_refine(ctx, n, a, nblocks);
+      rc = p->separable ? dispatch<float, true, true>(ctx, n, a) : dispatch<float, false, true>(ctx, n, a);
     } else {
-      rc = launch_plain<float>(ctx, n, a, nblocks);
+      rc = p->separable ? dispatch<float, true, false>(ctx, n, a) : dispatch<float, false, false>(ctx, n, a);
     }
   }
   if (rc != CRV_OK) { (void)hipFreeAsync(dtab, ctx->stream); if (dlist) (void)hipFreeAsync(dlist, ctx->stream); return rc; }
@@ -585,14 +610,15 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
   for (int q = 0; q < 6; ++q) { cov[q]->transformed = true; cov[q]->cubetype = CRV_COVARIANCE; }
 
   if (stats) {
-    unsigned long long hs[4];
+    unsigned long long hs[16];
     CRV_HIP_CHECK(ctx, hipMemcpyAsync(hs, dtab + oS, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     stats->n_wavenumbers = nc;
-    stats->n_inverted = (long long)hs[0];
-    stats->n_chol_fail = (long long)hs[1];
-    stats->n_fp64 = (long long)hs[2];
-    double mc; memcpy(&mc, &hs[3], 8);
+    stats->n_inverted = ninv_planes * cnxp * nyp;
+    stats->n_chol_fail = (long long)hs[0];
+    stats->n_fp64 = refine ? (long long)(unsigned int)(hs[8] & 0xffffffffull) : 0;
+    double mc;
+    memcpy(&mc, &hs[1], 8);
     stats->max_cond = mc;
   }
   CRV_HIP_CHECK(ctx, hipFreeAsync(dtab, ctx->stream));

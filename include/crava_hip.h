@@ -175,9 +175,12 @@ int crv_fft_c2r_add(crv_grid* dst, crv_grid* src, const crv_grid* add_real);
  * em3[k][l] = 1 -- all interleaved re/im doubles, nzp*n complex each.       */
 
 #define CRV_MODE_PLAIN        0   /* arithmetic in the grids' precision      */
-#define CRV_MODE_FP32_REFINE  1   /* FP32 grids; FP32 solve + 1 refinement
-                                     step, FP64 for ill-conditioned and
-                                     low-|k| wavenumbers                     */
+#define CRV_MODE_FP32_REFINE  1   /* FP32 grids; FP32 solve with FP32
+                                     accumulation; wavenumbers rejected in
+                                     FP32, with condition estimate above
+                                     cond_threshold or |k| below
+                                     lowk_fraction of Nyquist are re-solved
+                                     in FP64 with one refinement step        */
 
 typedef struct crv_post_params {
   int           ntheta;        /* n, 1..8                                    */
