@@ -28,6 +28,8 @@ EXPORTS = [
     "crv_fill_complex_noise", "crv_noise_draws", "crv_find_closest_factorable", "crv_padding_size",
     "crv_ctx_profile", "crv_ctx_profile_report", "crv_fill_prior_spectra", "crv_grid_fill",
     "crv_forward_model", "crv_add_white_noise", "crv_grid_moments", "crv_ctx_tune", "crv_rock_forward",
+    "crv_mlp_param_count", "crv_mlp_create", "crv_mlp_destroy", "crv_mlp_set_params", "crv_mlp_get_params",
+    "crv_mlp_gradients", "crv_mlp_train_step", "crv_mlp_predict", "crv_mlp_flops", "crv_rock_refine",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS = 1, 2, 3, 4
 
@@ -63,6 +65,11 @@ class RockParams(C.Structure):
                                            "temperature", "salinity", "gas_gravity")]
 
 
+class MlpNorm(C.Structure):
+    _fields_ = [("in_lo", C.c_float * 5), ("in_scale", C.c_float * 5), ("in_log_mask", C.c_int),
+                ("out_mean", C.c_float * 3), ("out_std", C.c_float * 3)]
+
+
 class NoiseSpec(C.Structure):
     _fields_ = [("mode", C.c_int), ("lcg_state", C.POINTER(C.c_uint)), ("seed", C.c_ulonglong),
                 ("realization", C.c_ulonglong), ("host_noise", C.c_void_p * 3)]
@@ -72,6 +79,13 @@ def load(path: Path | None = None) -> C.CDLL:
     p = Path(path) if path else LIBPATH
     if not p.exists():
         raise RuntimeError(f"libcrava_hip.so not found at {p}: run `python -m crava_amd.build` first")
+    # torch (device memory for the host-side drivers) brings its own HIP
+    # runtime; loading it first lets this library bind to the same one, so
+    # both see the device whichever is used first
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     vp = C.c_void_p
     i = C.c_int
@@ -114,6 +128,17 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_grid_moments": (i, [vp, _dp, _dp, C.POINTER(C.c_longlong)]),
         "crv_ctx_tune": (i, [vp, i, i]),
         "crv_rock_forward": (i, [vp, C.POINTER(RockParams), C.c_longlong, vp, vp, i, vp]),
+        "crv_mlp_param_count": (C.c_longlong, []),
+        "crv_mlp_create": (i, [vp, C.POINTER(MlpNorm), C.c_longlong, C.POINTER(vp)]),
+        "crv_mlp_destroy": (i, [vp]),
+        "crv_mlp_set_params": (i, [vp, C.POINTER(C.c_float)]),
+        "crv_mlp_get_params": (i, [vp, C.POINTER(C.c_float)]),
+        "crv_mlp_gradients": (i, [vp, vp, vp, C.c_longlong, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "crv_mlp_train_step": (i, [vp, vp, vp, C.c_longlong, C.c_float, i, C.POINTER(C.c_float)]),
+        "crv_mlp_predict": (i, [vp, vp, C.c_longlong, vp]),
+        "crv_mlp_flops": (i, [vp, _dp, _dp]),
+        "crv_rock_refine": (i, [vp, C.POINTER(RockParams), C.c_longlong, vp, vp, C.POINTER(MlpNorm), vp, i,
+                                C.c_float, C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

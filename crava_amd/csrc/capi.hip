@@ -114,9 +114,56 @@ int crv_grid_create(crv_ctx* ctx, int nx, int ny, int nz, int nxp, int nyp, int 
   return CRV_OK;
 }
 
+int crv_grid_create_batch(crv_ctx* ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int precision,
+                          int cubetype, int count, crv_grid** out) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!out || count < 1) return crv_set_error(ctx, CRV_E_INVALID, "grid batch: bad count or out");
+  if (nxp < 1 || nyp < 1 || nzp < 1 || nx < 0 || ny < 0 || nz < 0 || nx > nxp || ny > nyp || nz > nzp)
+    return crv_set_error(ctx, CRV_E_INVALID, "bad grid dimensions");
+  if (precision != CRV_FP32 && precision != CRV_FP64) return crv_set_error(ctx, CRV_E_INVALID, "bad precision");
+  if (cubetype < CRV_CTMISSING || cubetype > CRV_OPERATOR) return crv_set_error(ctx, CRV_E_INVALID, "bad cubetype");
+  FftPlan* plan = nullptr;
+  int rc = crv_get_plan(ctx, nxp, nyp, nzp, precision, &plan);
+  if (rc != CRV_OK) return rc;
+  const int cnxp = nxp / 2 + 1;
+  const size_t rsize = (size_t)2 * cnxp * nyp * nzp;
+  const size_t eb = precision == CRV_FP64 ? 8 : 4;
+  void* base = nullptr;
+  const hipError_t e = hipMalloc(&base, rsize * eb * (size_t)count);
+  if (e != hipSuccess) return crv_set_error(ctx, CRV_E_NOMEM, std::string("grid batch allocation: ") + hipGetErrorString(e));
+  int* shared = new (std::nothrow) int(count);
+  if (!shared) { (void)hipFree(base); return CRV_E_NOMEM; }
+  for (int i = 0; i < count; ++i) {
+    crv_grid* g = new (std::nothrow) crv_grid();
+    if (!g) {
+      for (int q = 0; q < i; ++q) delete out[q];
+      delete shared;
+      (void)hipFree(base);
+      return crv_set_error(ctx, CRV_E_NOMEM, "host allocation");
+    }
+    g->ctx = ctx;
+    g->nx = nx; g->ny = ny; g->nz = nz; g->nxp = nxp; g->nyp = nyp; g->nzp = nzp;
+    g->cnxp = cnxp; g->rnxp = 2 * cnxp;
+    g->precision = precision; g->cubetype = cubetype; g->transformed = false;
+    g->rsize = rsize;
+    g->csize = (size_t)cnxp * nyp * nzp;
+    g->d = (char*)base + (size_t)i * rsize * eb;
+    g->shared = shared;
+    g->base = base;
+    out[i] = g;
+  }
+  return CRV_OK;
+}
+
 int crv_grid_destroy(crv_grid* g) {
   if (!g) return CRV_E_INVALID;
-  if (g->d) {
+  if (g->shared) {
+    if (--*g->shared == 0) {
+      (void)hipStreamSynchronize(g->ctx->stream);
+      (void)hipFree(g->base);
+      delete g->shared;
+    }
+  } else if (g->d) {
     (void)hipStreamSynchronize(g->ctx->stream);
     (void)hipFree(g->d);
   }
@@ -331,9 +378,13 @@ extern "C" int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len) {
 
 extern "C" int crv_ctx_tune(crv_ctx* ctx, int key, int value) {
   if (!ctx) return CRV_E_INVALID;
-  if (key < CRV_TUNE_CHUNK_MB || key > CRV_TUNE_ROWS || value < 0)
+  if (key < CRV_TUNE_CHUNK_MB || key > CRV_TUNE_POST_BLOCKS || value < 0)
     return crv_set_error(ctx, CRV_E_INVALID, "crv_ctx_tune: bad key or value");
   CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (key == CRV_TUNE_POST_BLOCKS) {
+    ctx->post_blocks_per_cu = value;
+    return CRV_OK;
+  }
   const int old = ctx->tune[key];
   ctx->tune[key] = value;
   for (auto& kv : ctx->plans) {
@@ -344,5 +395,52 @@ extern "C" int crv_ctx_tune(crv_ctx* ctx, int key, int value) {
       return rc;
     }
   }
+  return CRV_OK;
+}
+
+// ---- HIP graphs ---------------------------------------------------------------
+struct crv_graph {
+  crv_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int crv_graph_begin(crv_ctx* ctx) {
+  if (!ctx) return CRV_E_INVALID;
+  CRV_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  return CRV_OK;
+}
+
+extern "C" int crv_graph_end(crv_ctx* ctx, crv_graph** out) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!out) return crv_set_error(ctx, CRV_E_INVALID, "graph_end: out is NULL");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  CRV_HIP_CHECK(ctx, hipStreamEndCapture(ctx->stream, &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return crv_set_error(ctx, CRV_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  }
+  crv_graph* gr = new (std::nothrow) crv_graph();
+  if (!gr) { (void)hipGraphExecDestroy(e); (void)hipGraphDestroy(g); return CRV_E_NOMEM; }
+  gr->ctx = ctx; gr->graph = g; gr->exec = e;
+  *out = gr;
+  return CRV_OK;
+}
+
+extern "C" int crv_graph_launch(crv_graph* gr) {
+  if (!gr) return CRV_E_INVALID;
+  CRV_HIP_CHECK(gr->ctx, hipGraphLaunch(gr->exec, gr->ctx->stream));
+  return CRV_OK;
+}
+
+extern "C" int crv_graph_destroy(crv_graph* gr) {
+  if (!gr) return CRV_E_INVALID;
+  (void)hipStreamSynchronize(gr->ctx->stream);
+  if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+  if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  delete gr;
   return CRV_OK;
 }

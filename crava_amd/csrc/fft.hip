@@ -8,10 +8,22 @@
 //          (via a complex transform of length nxp/2 when nxp is even);
 //   cols : y, pencils of stride cnxp complex, tiles of TX consecutive x;
 //   cols : z, pencils of stride cnxp*nyp complex, tiles of TX consecutive x.
-// Each pencil tile is staged in LDS and transformed with self-sorting
-// (Stockham) mixed-radix stages of radix 2, 3, 4, 5, 7 and 8; twiddles come
-// from a per-size table computed on the host in long double.
+//
+// Each tile goes through self-sorting (Stockham) mixed-radix stages.  The
+// first stage reads its butterflies straight from HBM into registers and
+// writes LDS; middle stages go LDS -> registers -> LDS; the last stage
+// writes HBM straight from registers (rows of an R2C go through LDS once
+// more for the real-to-complex post-processing).  Lengths that are a product
+// of two radices from a fixed list of pairs up to 25 x 18 (composite radices
+// done in registers, dft.hpp) -- the production grid's 225, 450 and 320
+// among them -- get kernels compiled for their pair: two stages, i.e. one
+// LDS exchange and one barrier per tile.  Other lengths run through radices
+// 2..8 with the stage radix dispatched at run time.
+// Twiddles of all stages come from a per-size table computed on the host in
+// long double and copied to LDS by each block; the index arithmetic uses
+// multiply-shift division, no runtime modulo.
 #include "internal.hpp"
+#include "dft.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -19,9 +31,10 @@
 
 namespace {
 
+using dft::Dft;
+
 // 256-thread blocks with tiles of at most 4096 (FP32) / 2048 (FP64) complex
-// values (32 KB of LDS): three blocks share a CU, so one block's global
-// loads overlap another's LDS stages.
+// values (32 KB of LDS).
 constexpr int kThreads = 256;
 
 template <typename T> struct Ept;
@@ -43,6 +56,21 @@ __device__ __forceinline__ int fdiv(int n, FastDiv f) {
   return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
 }
 
+// per-stage divisors: L / R_s (butterflies per pencil) and Ns
+struct StageDiv {
+  FastDiv lr[kMaxStages];
+  FastDiv ns[kMaxStages];
+};
+
+StageDiv make_stage_div(const Fft1D& f) {
+  StageDiv d{};
+  for (int s = 0; s < f.nst; ++s) {
+    d.lr[s] = make_fastdiv(f.L / f.R[s]);
+    d.ns[s] = make_fastdiv(f.Ns[s]);
+  }
+  return d;
+}
+
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
 // range of tiles so that neighbouring tiles (which share cache lines at
 // their edges) meet in the same L2.
@@ -52,158 +80,93 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return x * q + min(x, r) + i;
 }
 
-// multiply by -i (forward) or +i (inverse)
-template <typename C, bool INV> __device__ __forceinline__ C rot(C a) {
-  return INV ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);
+template <typename T>
+__device__ __forceinline__ cplx_t<T> twiddle(cplx_t<T> a, cplx_t<T> w, bool inv) {
+  return inv ? cmulc(a, w) : cmul(a, w);
 }
 
-// ---- small DFTs (sign -1 forward, +1 inverse), in place ------------------
-template <typename T, int R, bool INV> struct Dft;
-
-template <typename T, bool INV> struct Dft<T, 2, INV> {
-  using C = cplx_t<T>;
-  __device__ __forceinline__ static void run(C* v) {
-    C a = v[0];
-    v[0] = cadd(a, v[1]);
-    v[1] = csub(a, v[1]);
+// Butterfly b of a stage -> (pencil c, butterfly index j).  Pencil-fast
+// (c = b mod C) keeps LDS accesses consecutive across lanes and, for the
+// column passes, HBM accesses coalesced over the tile's x columns;
+// index-fast (j = b mod Lr) makes the HBM accesses of a row pass run along
+// the row.
+template <bool JFAST>
+__device__ __forceinline__ void bmap(int b, int cshift, FastDiv dlr, int& c, int& j) {
+  if (JFAST) {
+    c = fdiv(b, dlr);
+    j = b - c * dlr.d;
+  } else {
+    c = b & ((1 << cshift) - 1);
+    j = b >> cshift;
   }
-};
+}
 
-template <typename T, bool INV> struct Dft<T, 4, INV> {
-  using C = cplx_t<T>;
-  __device__ __forceinline__ static void run(C* v) {
-    C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-    C t2 = cadd(v[1], v[3]), t3 = rot<C, INV>(csub(v[1], v[3]));
-    v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
-    v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
-  }
-};
-
-template <typename T, bool INV> struct Dft<T, 8, INV> {
-  using C = cplx_t<T>;
-  __device__ __forceinline__ static void run(C* v) {
-    const T h = (T)0.70710678118654752440;  // sqrt(1/2)
-    C e[4] = {v[0], v[2], v[4], v[6]};
-    C o[4] = {v[1], v[3], v[5], v[7]};
-    Dft<T, 4, INV>::run(e);
-    Dft<T, 4, INV>::run(o);
-    // o[m] *= exp(-+ 2 pi i m / 8)
-    C o1, o3;
-    if (!INV) {
-      o1 = cmake<C>(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
-      o3 = cmake<C>(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
-    } else {
-      o1 = cmake<C>(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y));
-      o3 = cmake<C>(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y));
-    }
-    C o2 = rot<C, INV>(o[2]);
-    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
-    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
-    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
-  }
-};
-
-template <typename T, bool INV> struct Dft<T, 3, INV> {
-  using C = cplx_t<T>;
-  __device__ __forceinline__ static void run(C* v) {
-    const T s3 = (T)0.86602540378443864676;  // sin(2 pi / 3)
-    C t = cadd(v[1], v[2]);
-    C m = cmake<C>(v[0].x - (T)0.5 * t.x, v[0].y - (T)0.5 * t.y);
-    C d = rot<C, INV>(csub(v[1], v[2]));
-    v[0] = cadd(v[0], t);
-    v[1] = cmake<C>(m.x + s3 * d.x, m.y + s3 * d.y);
-    v[2] = cmake<C>(m.x - s3 * d.x, m.y - s3 * d.y);
-  }
-};
-
-// cos(2 pi j / R), sin(2 pi j / R) for j = 1 .. (R-1)/2
-template <int R> struct OddTab;
-template <> struct OddTab<5> {
-  __device__ static constexpr double c(int j) { return j == 1 ? 0.30901699437494742410 : -0.80901699437494742410; }
-  __device__ static constexpr double s(int j) { return j == 1 ? 0.95105651629515357212 : 0.58778525229247312917; }
-};
-template <> struct OddTab<7> {
-  __device__ static constexpr double c(int j) {
-    return j == 1 ? 0.62348980185873353053 : (j == 2 ? -0.22252093395631440429 : -0.90096886790241912624);
-  }
-  __device__ static constexpr double s(int j) {
-    return j == 1 ? 0.78183148246802980871 : (j == 2 ? 0.97492791218182360702 : 0.43388373911755812048);
-  }
-};
-
-// Odd radix by the symmetric split: a_q = v_q + v_{R-q}, b_q = v_q - v_{R-q};
-// X_m = v0 + sum cos(2 pi q m/R) a_q + sum sin(2 pi q m/R) rot(b_q),
-// X_{R-m} = v0 + sum cos(...) a_q - sum sin(...) rot(b_q).
-template <typename T, int R, bool INV> struct DftOdd {
-  using C = cplx_t<T>;
-  __device__ __forceinline__ static void run(C* v) {
-    constexpr int H = (R - 1) / 2;
-    C a[H], b[H];
-#pragma unroll
-    for (int q = 1; q <= H; ++q) {
-      a[q - 1] = cadd(v[q], v[R - q]);
-      b[q - 1] = rot<C, INV>(csub(v[q], v[R - q]));
-    }
-    C x0 = v[0];
-    C s0 = x0;
-#pragma unroll
-    for (int q = 0; q < H; ++q) s0 = cadd(s0, a[q]);
-#pragma unroll
-    for (int m = 1; m <= H; ++m) {
-      C cs = x0, sn = cmake<C>((T)0, (T)0);
-#pragma unroll
-      for (int q = 1; q <= H; ++q) {
-        const int j = (q * m) % R;
-        // cos/sin of 2 pi j / R with j possibly > R/2
-        const T cj = (T)(j <= H ? OddTab<R>::c(j) : OddTab<R>::c(R - j));
-        const T sj = (T)(j <= H ? OddTab<R>::s(j) : -OddTab<R>::s(R - j));
-        cs = cmake<C>(cs.x + cj * a[q - 1].x, cs.y + cj * a[q - 1].y);
-        sn = cmake<C>(sn.x + sj * b[q - 1].x, sn.y + sj * b[q - 1].y);
-      }
-      v[m] = cadd(cs, sn);
-      v[R - m] = csub(cs, sn);
-    }
-    v[0] = s0;
-  }
-};
-template <typename T, bool INV> struct Dft<T, 5, INV> : DftOdd<T, 5, INV> {};
-template <typename T, bool INV> struct Dft<T, 7, INV> : DftOdd<T, 7, INV> {};
-
-// ---- one Stockham stage over an LDS tile ---------------------------------
+// ---- stage pieces -----------------------------------------------------------
 // Tile element (e, c) lives at lds[e*S + c]; C = 1 << cshift pencils.
-// Butterfly b -> pencil c = b % C, index j = b / C; inputs j + r*L/R;
-// outputs (j - k)*R + k + r*Ns with k = j % Ns.
+// Stage s: butterfly (c, j) reads e = j + r*Lr, twiddles by tw[k*(R-1) + r-1]
+// with k = j mod Ns, and writes e' = (j - k)*R + k + r*Ns.
+
+// first stage (Ns = 1, no twiddles): inputs from load(c, e)
+template <typename T, int R, bool INV, bool JFAST, typename Load>
+__device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, int L, FastDiv dlr, Load load) {
+  using C = cplx_t<T>;
+  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  const int Lr = L / R;
+  const int total = Lr << cshift;
+  C v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      int c, j;
+      bmap<JFAST>(b, cshift, dlr, c, j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = load(c, j + r * Lr);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      int c, j;
+      bmap<JFAST>(b, cshift, dlr, c, j);
+      Dft<T, R, INV>::run(v[q]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[(j * R + r) * S + c] = v[q][r];
+    }
+  }
+}
+
+// middle stage: LDS -> LDS (pencil-fast); ends with the tile written
 template <typename T, int R, bool INV>
-__device__ __forceinline__ void stage(cplx_t<T>* lds, int S, int cshift, int L, int Ns,
-                                      const cplx_t<T>* __restrict__ tw) {
+__device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int L, int Ns, FastDiv dns,
+                                          const cplx_t<T>* tw) {
   using C = cplx_t<T>;
   constexpr int MAXB = (Ept<T>::value + R - 1) / R;
   const int Lr = L / R;
   const int total = Lr << cshift;
   const int cm = (1 << cshift) - 1;
-  const int tid = threadIdx.x;
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = tid + q * kThreads;
+    const int b = threadIdx.x + q * kThreads;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
+      const int k = j - fdiv(j, dns) * Ns;
+      const C* w = tw + k * (R - 1);
+      v[q][0] = lds[j * S + c];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = lds[(j + r * Lr) * S + c];
+      for (int r = 1; r < R; ++r) v[q][r] = twiddle<T>(lds[(j + r * Lr) * S + c], w[r - 1], INV);
+      Dft<T, R, INV>::run(v[q]);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = tid + q * kThreads;
+    const int b = threadIdx.x + q * kThreads;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
-      const int k = j % Ns;
-      const C* w = tw + k * (R - 1);
-#pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = INV ? cmulc(v[q][r], w[r - 1]) : cmul(v[q][r], w[r - 1]);
-      Dft<T, R, INV>::run(v[q]);
+      const int k = j - fdiv(j, dns) * Ns;
       const int base = (j - k) * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) lds[(base + r * Ns) * S + c] = v[q][r];
@@ -212,20 +175,114 @@ __device__ __forceinline__ void stage(cplx_t<T>* lds, int S, int cshift, int L, 
   __syncthreads();
 }
 
-template <typename T, bool INV>
-__device__ __forceinline__ void run_stages(cplx_t<T>* lds, int S, int cshift, const Fft1D& f,
-                                           const cplx_t<T>* __restrict__ tw) {
-  for (int s = 0; s < f.nst; ++s) {
-    const cplx_t<T>* t = tw + f.tw[s];
-    switch (f.R[s]) {
-      case 2: stage<T, 2, INV>(lds, S, cshift, f.L, f.Ns[s], t); break;
-      case 3: stage<T, 3, INV>(lds, S, cshift, f.L, f.Ns[s], t); break;
-      case 4: stage<T, 4, INV>(lds, S, cshift, f.L, f.Ns[s], t); break;
-      case 5: stage<T, 5, INV>(lds, S, cshift, f.L, f.Ns[s], t); break;
-      case 7: stage<T, 7, INV>(lds, S, cshift, f.L, f.Ns[s], t); break;
-      default: stage<T, 8, INV>(lds, S, cshift, f.L, f.Ns[s], t); break;
+// last stage (Ns = L/R, so k = j): LDS -> store(c, e)
+template <typename T, int R, bool INV, bool JFAST, typename Store>
+__device__ __forceinline__ void stage_last(const cplx_t<T>* lds, int S, int cshift, int L, FastDiv dlr,
+                                           const cplx_t<T>* tw, Store store) {
+  using C = cplx_t<T>;
+  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  const int Lr = L / R;
+  const int total = Lr << cshift;
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      int c, j;
+      bmap<JFAST>(b, cshift, dlr, c, j);
+      const C* w = tw + j * (R - 1);
+      C v[R];
+      v[0] = lds[j * S + c];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = twiddle<T>(lds[(j + r * Lr) * S + c], w[r - 1], INV);
+      Dft<T, R, INV>::run(v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) store(c, j + r * Lr, v[r]);
     }
   }
+}
+
+// a one-stage transform: load -> DFT -> store, no LDS
+template <typename T, int R, bool INV, typename Load, typename Store>
+__device__ __forceinline__ void stage_single(int cshift, Load load, Store store) {
+  using C = cplx_t<T>;
+  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  const int total = 1 << cshift;  // one butterfly per pencil
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int c = threadIdx.x + q * kThreads;
+    if (c < total) {
+      C v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = load(c, r);
+      Dft<T, R, INV>::run(v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) store(c, r, v[r]);
+    }
+  }
+}
+
+#define CRV_RADIX_SWITCH(RR, CALL)                     \
+  switch (RR) {                                        \
+    case 2: { constexpr int R_ = 2; CALL; } break;     \
+    case 3: { constexpr int R_ = 3; CALL; } break;     \
+    case 4: { constexpr int R_ = 4; CALL; } break;     \
+    case 5: { constexpr int R_ = 5; CALL; } break;     \
+    case 7: { constexpr int R_ = 7; CALL; } break;     \
+    default: { constexpr int R_ = 8; CALL; } break;    \
+  }
+
+// Radix pairs of the two-stage kernels (first stage, last stage), and the
+// radices of the general kernel, whose stage loop dispatches at run time.
+#define CRV_FFT_PAIRS(X) X(25, 18) X(15, 15) X(20, 16) X(16, 16) X(16, 8) X(20, 20) X(25, 16) X(18, 16) X(16, 12)
+constexpr int kSmallRadices[] = {8, 4, 2, 5, 3, 7};
+
+// Whole transform of a tile.  load/store move HBM data; with LAST_TO_LDS the
+// last stage also writes LDS (natural order) and store is not used.
+template <typename T, bool INV, bool JFAST, bool LAST_TO_LDS, int RF, int RL, typename Load, typename Store>
+__device__ __forceinline__ void transform_tile(cplx_t<T>* lds, int S, int cshift, const Fft1D& f, const StageDiv& sd,
+                                               const cplx_t<T>* twl, Load load, Store store) {
+  const int L = f.L, nst = f.nst;
+  if constexpr (RF > 0) {
+    // two stages of radices RF, RL known at compile time
+    stage_first<T, RF, INV, JFAST>(lds, S, cshift, L, sd.lr[0], load);
+    __syncthreads();
+    const cplx_t<T>* t = twl + (f.tw[1] - f.tw0);
+    if constexpr (LAST_TO_LDS)
+      stage_mid<T, RL, INV>(lds, S, cshift, L, f.Ns[1], sd.ns[1], t);
+    else
+      stage_last<T, RL, INV, JFAST>(lds, S, cshift, L, sd.lr[1], t, store);
+    return;
+  }
+  if (nst == 0) {
+    // length 1: the tile is its own transform
+    const int total = 1 << cshift;
+    for (int b = threadIdx.x; b < total; b += kThreads) {
+      if (LAST_TO_LDS) lds[b] = load(b, 0);  // element 0 of pencil b
+      else store(b, 0, load(b, 0));
+    }
+    __syncthreads();
+    return;
+  }
+  if (nst == 1 && !LAST_TO_LDS) {
+    CRV_RADIX_SWITCH(f.R[0], (stage_single<T, R_, INV>(cshift, load, store)));
+    return;
+  }
+  CRV_RADIX_SWITCH(f.R[0], (stage_first<T, R_, INV, JFAST>(lds, S, cshift, L, sd.lr[0], load)));
+  __syncthreads();
+  const int nmid_end = LAST_TO_LDS ? nst : nst - 1;
+  for (int s = 1; s < nmid_end; ++s) {
+    const cplx_t<T>* t = twl + (f.tw[s] - f.tw0);
+    CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
+  }
+  if (!LAST_TO_LDS) {
+    const cplx_t<T>* t = twl + (f.tw[nst - 1] - f.tw0);
+    CRV_RADIX_SWITCH(f.R[nst - 1], (stage_last<T, R_, INV, JFAST>(lds, S, cshift, L, sd.lr[nst - 1], t, store)));
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void copy_to_lds(cplx_t<T>* dst, const cplx_t<T>* src, int n) {
+  for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
 }
 
 // ---- x pass: rows ----------------------------------------------------------
@@ -233,58 +290,47 @@ template <typename T>
 struct RowArgs {
   T* dst;              // real view (output of C2R / row base of R2C output)
   const T* src;        // real view input (R2C) or complex input buffer (C2R)
-  const T* add;        // C2R: real grid added to the output (may be null)
+  const T* add[3];     // C2R: real grid added to the output of batch entry b: add[b % nadd]
+  int nadd;
+  long long bstride;   // elements between the grids of a batch (blockIdx.y)
   Fft1D f;             // length M = nxp/2 (even) or nxp (odd)
+  StageDiv sd;
   const cplx_t<T>* tw;
   const cplx_t<T>* wpost;  // exp(-2 pi i k / nxp), k = 0..nxp/2 (even only)
   int nrows, rnxp, nxp, cnxp;
   int xeven;
   int pshift;          // rows per block = 1 << pshift
-  FastDiv divL;        // by f.L
-  FastDiv divL1;       // by f.L + 1 (even) or cnxp (odd)
+  FastDiv divK;        // by cnxp
   T scale;
 };
 
-template <typename T>
+template <typename T, int RF, int RL>
 __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
+  const int L = a.f.L, K = a.cnxp;
+  C* twl = lds + S * L;
+  C* wpl = twl + a.f.twn;
+  copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
+  if (a.xeven) copy_to_lds<T>(wpl, a.wpost, K);
   const int row0 = xcd_remap(blockIdx.x, gridDim.x) << a.pshift;
   const int nr = min(P, a.nrows - row0);
-  const int L = a.f.L;
-  const int tid = threadIdx.x;
-  // load: every load of the thread is issued before the first LDS store, so
-  // a wave keeps Ept loads in flight
-  constexpr int EPT = Ept<T>::value;
-  C buf[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = tid + q * kThreads;
-    buf[q] = cmake<C>((T)0, (T)0);
-    if (idx < P * L) {
-      const int c = fdiv(idx, a.divL), e = idx - c * L;
-      if (c < nr) {
-        const T* row = a.src + (size_t)(row0 + c) * a.rnxp;
-        buf[q] = a.xeven ? *reinterpret_cast<const C*>(row + 2 * e) : cmake<C>(row[e], (T)0);
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = tid + q * kThreads;
-    if (idx < P * L) {
-      const int c = fdiv(idx, a.divL), e = idx - c * L;
-      lds[e * S + c] = cscale(buf[q], a.scale);
-    }
-  }
-  __syncthreads();
-  run_stages<T, false>(lds, S, a.pshift, a.f, a.tw);
+  const size_t boff = (size_t)blockIdx.y * a.bstride;
+  const T* src0 = a.src + boff + (size_t)row0 * a.rnxp;
+  T* dst0 = a.dst + boff;
+  auto load = [&](int c, int e) -> C {
+    if (c >= nr) return cmake<C>((T)0, (T)0);
+    const T* row = src0 + (size_t)c * a.rnxp;
+    const C v = a.xeven ? *reinterpret_cast<const C*>(row + 2 * e) : cmake<C>(row[e], (T)0);
+    return cscale(v, a.scale);
+  };
+  auto nostore = [&](int, int, C) {};
+  transform_tile<T, false, true, true, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl, load, nostore);
   // post-process and store cnxp complex per row
-  const int K = a.cnxp;
-  for (int idx = tid; idx < P * K; idx += kThreads) {
-    const int c = fdiv(idx, a.divL1), k = idx - c * K;
+  for (int idx = threadIdx.x; idx < P * K; idx += kThreads) {
+    const int c = fdiv(idx, a.divK), k = idx - c * K;
     if (c >= nr) continue;
     C X;
     if (a.xeven) {
@@ -292,88 +338,66 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
       C za = lds[(k == M ? 0 : k) * S + c];
       C zb = cconj(lds[(k == 0 ? 0 : M - k) * S + c]);
       C d = csub(za, zb);
-      C t = cmul(a.wpost[k], d);
+      C t = cmul(wpl[k], d);
       // X = 0.5 (za + zb) - 0.5 i t
       X = cmake<C>((T)0.5 * (za.x + zb.x + t.y), (T)0.5 * (za.y + zb.y - t.x));
     } else {
       X = lds[k * S + c];
     }
-    C* out = reinterpret_cast<C*>(a.dst + (size_t)(row0 + c) * a.rnxp);
+    C* out = reinterpret_cast<C*>(dst0 + (size_t)(row0 + c) * a.rnxp);
     out[k] = X;
   }
 }
 
-template <typename T>
+template <typename T, int RF, int RL>
 __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
+  const int L = a.f.L;
+  C* twl = lds + S * L;
+  C* wpl = twl + a.f.twn;
+  copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
+  if (a.xeven) copy_to_lds<T>(wpl, a.wpost, a.cnxp);
+  __syncthreads();  // the first stage's pre-processing reads wpl
   const int row0 = xcd_remap(blockIdx.x, gridDim.x) << a.pshift;
   const int nr = min(P, a.nrows - row0);
-  const int L = a.f.L;
-  const int tid = threadIdx.x;
-  constexpr int EPT = Ept<T>::value;
-  C ba[EPT], bb[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = tid + q * kThreads;
-    ba[q] = cmake<C>((T)0, (T)0);
-    bb[q] = ba[q];
-    if (idx < P * L) {
-      const int c = fdiv(idx, a.divL), k = idx - c * L;
-      if (c < nr) {
-        const C* X = reinterpret_cast<const C*>(a.src + (size_t)(row0 + c) * a.rnxp);
-        if (a.xeven) {
-          ba[q] = X[k];
-          bb[q] = X[L - k];
-        } else {
-          ba[q] = (k < a.cnxp) ? X[k] : cconj(X[L - k]);
-        }
-      }
+  const size_t boff = (size_t)blockIdx.y * a.bstride;
+  const T* src0 = a.src + boff;
+  T* dst0 = a.dst + boff;
+  const T* add = a.nadd ? a.add[blockIdx.y % a.nadd] : nullptr;
+  // Z[k] of the half-length complex transform from the half spectrum
+  auto load = [&](int c, int k) -> C {
+    if (c >= nr) return cmake<C>((T)0, (T)0);
+    const C* X = reinterpret_cast<const C*>(src0 + (size_t)(row0 + c) * a.rnxp);
+    if (a.xeven) {
+      C xa = X[k], xb = X[L - k];
+      if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
+      xb = cconj(xb);
+      const C d = csub(xa, xb);
+      const C t = cmulc(d, wpl[k]);  // d * exp(+2 pi i k / nxp)
+      // Z = (xa + xb) + i t
+      return cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
     }
-  }
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = tid + q * kThreads;
-    if (idx < P * L) {
-      const int c = fdiv(idx, a.divL), k = idx - c * L;
-      C z;
-      if (a.xeven) {
-        C xa = ba[q], xb = bb[q];
-        if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
-        xb = cconj(xb);
-        C d = csub(xa, xb);
-        C t = cmulc(d, a.wpost[k]);  // d * exp(+2 pi i k / nxp)
-        // Z = (xa + xb) + i t
-        z = cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
-      } else {
-        z = ba[q];
-        if (k == 0) z.y = (T)0;
-      }
-      lds[k * S + c] = z;
-    }
-  }
-  __syncthreads();
-  run_stages<T, true>(lds, S, a.pshift, a.f, a.tw);
-  for (int idx = tid; idx < P * L; idx += kThreads) {
-    const int c = fdiv(idx, a.divL), n = idx - c * L;
-    if (c >= nr) continue;
+    C z = (k < a.cnxp) ? X[k] : cconj(X[L - k]);
+    if (k == 0) z.y = (T)0;
+    return z;
+  };
+  auto store = [&](int c, int n, C z) {
+    if (c >= nr) return;
     const size_t rbase = (size_t)(row0 + c) * a.rnxp;
-    C z = lds[n * S + c];
     if (a.xeven) {
       C o = cscale(z, a.scale);
-      if (a.add) {
-        C ad = *reinterpret_cast<const C*>(a.add + rbase + 2 * n);
-        o = cadd(o, ad);
-      }
-      *reinterpret_cast<C*>(a.dst + rbase + 2 * n) = o;
+      if (add) o = cadd(o, *reinterpret_cast<const C*>(add + rbase + 2 * n));
+      *reinterpret_cast<C*>(dst0 + rbase + 2 * n) = o;
     } else {
       T o = z.x * a.scale;
-      if (a.add) o += a.add[rbase + n];
-      a.dst[rbase + n] = o;
+      if (add) o += add[rbase + n];
+      dst0[rbase + n] = o;
     }
-  }
+  };
+  transform_tile<T, true, true, false, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl, load, store);
 }
 
 // ---- y / z passes: strided complex pencils ---------------------------------
@@ -381,49 +405,45 @@ template <typename T>
 struct ColArgs {
   cplx_t<T>* data;
   Fft1D f;
+  StageDiv sd;
   const cplx_t<T>* tw;
   long long estride;   // element stride of a pencil (complex units)
   long long ostride;   // stride of the outer index
   int ncols;           // number of x columns (cnxp)
   int ntx;             // x tiles per outer index
   int cshift;          // TX = 1 << cshift
+  long long bstride;   // elements between the grids of a batch (blockIdx.y)
 };
 
-template <typename T, bool INV>
+template <typename T, bool INV, int RF, int RL>
 __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
+  const int TX = 1 << a.cshift;
+  C* twl = lds + a.f.L * TX;
+  copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int o = tile / a.ntx, t = tile - o * a.ntx;
-  const int TX = 1 << a.cshift, cm = TX - 1;
   const int x0 = t * TX;
   const int nc = min(TX, a.ncols - x0);
-  C* base = a.data + (size_t)o * a.ostride + x0;
-  const int n = a.f.L << a.cshift;
-  constexpr int EPT = Ept<T>::value;
-  C buf[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = threadIdx.x + q * kThreads;
-    const int c = idx & cm, e = idx >> a.cshift;
-    buf[q] = (idx < n && c < nc) ? base[(size_t)e * a.estride + c] : cmake<C>((T)0, (T)0);
-  }
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = threadIdx.x + q * kThreads;
-    if (idx < n) lds[idx] = buf[q];
-  }
-  __syncthreads();
-  run_stages<T, INV>(lds, TX, a.cshift, a.f, a.tw);
-  for (int idx = threadIdx.x; idx < n; idx += kThreads) {
-    const int c = idx & cm, e = idx >> a.cshift;
-    if (c < nc) base[(size_t)e * a.estride + c] = lds[idx];
-  }
+  C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
+  auto load = [&](int c, int e) -> C {
+    return (c < nc) ? base[(size_t)e * a.estride + c] : cmake<C>((T)0, (T)0);
+  };
+  auto store = [&](int c, int e, C v) {
+    if (c < nc) base[(size_t)e * a.estride + c] = v;
+  };
+  transform_tile<T, INV, false, false, RF, RL>(lds, TX, a.cshift, a.f, a.sd, twl, load, store);
 }
 
 // ---- planning ----------------------------------------------------------------
+// A length that is one of the two-stage radix pairs uses it; any other
+// 2^a 3^b 5^c 7^d length is split into radices 8, 4, 2, 5, 3, 7.
 std::vector<int> factorize(int L) {
+#define CRV_PAIR_MATCH(A, B) if (L == (A) * (B)) return {A, B};
+  CRV_FFT_PAIRS(CRV_PAIR_MATCH)
+#undef CRV_PAIR_MATCH
   std::vector<int> r;
   int m = L;
   while (m % 8 == 0) { r.push_back(8); m /= 8; }
@@ -434,10 +454,21 @@ std::vector<int> factorize(int L) {
   return r;
 }
 
-bool plan_1d(int L, Fft1D& f, std::vector<long double>& twr, std::vector<long double>& twi) {
+bool is_pair(const std::vector<int>& rs) {
+#define CRV_PAIR_IS(A, B) if (rs.size() == 2 && rs[0] == (A) && rs[1] == (B)) return true;
+  CRV_FFT_PAIRS(CRV_PAIR_IS)
+#undef CRV_PAIR_IS
+  return false;
+}
+
+// reverse: the stages of a length without a two-stage kernel run in the
+// opposite order (an autotuning choice)
+bool plan_1d(int L, Fft1D& f, std::vector<long double>& twr, std::vector<long double>& twi, bool reverse = false) {
   f.L = L;
+  f.tw0 = (int)twr.size();
   std::vector<int> rs = (L == 1) ? std::vector<int>{} : factorize(L);
   if (L != 1 && rs.empty()) return false;
+  if (reverse && !is_pair(rs)) std::reverse(rs.begin(), rs.end());
   if ((int)rs.size() > kMaxStages) return false;
   f.nst = (int)rs.size();
   int Ns = 1;
@@ -455,6 +486,7 @@ bool plan_1d(int L, Fft1D& f, std::vector<long double>& twr, std::vector<long do
       }
     Ns *= R;
   }
+  f.twn = (int)twr.size() - f.tw0;
   return true;
 }
 
@@ -462,39 +494,75 @@ int pow2_floor(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
 int ilog2(int v) { int s = 0; while ((1 << s) < v) ++s; return s; }
 
 template <typename T>
-size_t row_lds_bytes(const FftPlan* p) { return (size_t)((p->rows_per_block + 1) * p->fx.L) * sizeof(cplx_t<T>); }
+size_t row_lds_bytes(const FftPlan* p) {
+  const int cnxp = p->nxp / 2 + 1;
+  return (size_t)((p->rows_per_block + 1) * p->fx.L + p->fx.twn + cnxp) * sizeof(cplx_t<T>);
+}
+
+// kernel for an axis plan: the two-stage kernel of its radix pair, or the
+// general one
+template <typename T>
+const void* rows_kernel(const Fft1D& f, bool inv) {
+  if (f.nst == 2) {
+#define CRV_PAIR_ROWS(A, B) \
+    if (f.R[0] == (A) && f.R[1] == (B)) \
+      return inv ? (const void*)k_rows_c2r<T, A, B> : (const void*)k_rows_r2c<T, A, B>;
+    CRV_FFT_PAIRS(CRV_PAIR_ROWS)
+#undef CRV_PAIR_ROWS
+  }
+  return inv ? (const void*)k_rows_c2r<T, 0, 0> : (const void*)k_rows_r2c<T, 0, 0>;
+}
+
+template <typename T, bool INV>
+const void* cols_kernel(const Fft1D& f) {
+  if (f.nst == 2) {
+#define CRV_PAIR_COLS(A, B) \
+    if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<T, INV, A, B>;
+    CRV_FFT_PAIRS(CRV_PAIR_COLS)
+#undef CRV_PAIR_COLS
+  }
+  return (const void*)k_cols<T, INV, 0, 0>;
+}
+
+// Batches: nbatch grids bstride elements apart (real units), one launch.
+struct Batch {
+  int n = 1;
+  long long stride = 0;
+  int nadd = 0;
+  const void* add[3] = {nullptr, nullptr, nullptr};
+};
 
 template <typename T>
-int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const T* add, int row_begin,
+int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const Batch& bt, int row_begin,
                 int row_end, double scale) {
   const int cnxp = p->nxp / 2 + 1, rnxp = 2 * cnxp;
   const size_t off = (size_t)row_begin * rnxp;
   RowArgs<T> a{};
-  a.dst = dst + off; a.src = src + off; a.add = add ? add + off : nullptr;
-  a.f = p->fx; a.tw = reinterpret_cast<const cplx_t<T>*>(p->tw);
+  a.dst = dst + off; a.src = src + off;
+  a.nadd = bt.nadd;
+  for (int q = 0; q < bt.nadd; ++q) a.add[q] = (const T*)bt.add[q] + off;
+  a.bstride = bt.stride;
+  a.f = p->fx; a.sd = make_stage_div(p->fx); a.tw = reinterpret_cast<const cplx_t<T>*>(p->tw);
   a.wpost = a.tw + p->post_off;
   a.nrows = row_end - row_begin; a.rnxp = rnxp; a.nxp = p->nxp; a.cnxp = cnxp; a.xeven = p->xeven ? 1 : 0;
   a.pshift = ilog2(p->rows_per_block);
-  a.divL = make_fastdiv(p->fx.L); a.divL1 = make_fastdiv(cnxp);
+  a.divK = make_fastdiv(cnxp);
   a.scale = (T)scale;
   const int nb = (a.nrows + p->rows_per_block - 1) / p->rows_per_block;
-  if (inv) {
-    ProfScope ps(ctx, "fft_x_c2r");
-    hipLaunchKernelGGL(k_rows_c2r<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
-  } else {
-    ProfScope ps(ctx, "fft_x_r2c");
-    hipLaunchKernelGGL(k_rows_r2c<T>, dim3(nb), dim3(kThreads), row_lds_bytes<T>(p), ctx->stream, a);
-  }
-  CRV_HIP_CHECK(ctx, hipGetLastError());
+  void* args[] = {&a};
+  ProfScope ps(ctx, inv ? "fft_x_c2r" : "fft_x_r2c");
+  CRV_HIP_CHECK(ctx, hipLaunchKernel(rows_kernel<T>(p->fx, inv), dim3(nb, bt.n), dim3(kThreads), args,
+                                     row_lds_bytes<T>(p), ctx->stream));
   return CRV_OK;
 }
 
 // y pass over z planes [k0, k1) or z pass over y rows [j0, j1)
 template <typename T>
-int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data, int o0, int o1) {
+int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data, const Batch& bt, int o0, int o1) {
   using C = cplx_t<T>;
   const int cnxp = p->nxp / 2 + 1;
   ColArgs<T> a{};
+  a.bstride = bt.stride / 2;  // complex units
   a.tw = reinterpret_cast<const C*>(p->tw);
   a.ncols = cnxp;
   int tx;
@@ -503,18 +571,17 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
   } else {
     a.f = p->fy; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp; tx = p->tx_y;
   }
+  a.sd = make_stage_div(a.f);
   a.data = data + (size_t)o0 * a.ostride;
   a.cshift = ilog2(tx);
   a.ntx = (cnxp + tx - 1) / tx;
   const int nb = a.ntx * (o1 - o0);
-  const size_t lds = (size_t)a.f.L * tx * sizeof(C);
+  const size_t lds = ((size_t)a.f.L * tx + a.f.twn) * sizeof(C);
   static const char* names[4] = {"fft_y_fwd", "fft_y_inv", "fft_z_fwd", "fft_z_inv"};
   ProfScope ps(ctx, names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
-  if (inv)
-    hipLaunchKernelGGL((k_cols<T, true>), dim3(nb), dim3(kThreads), lds, ctx->stream, a);
-  else
-    hipLaunchKernelGGL((k_cols<T, false>), dim3(nb), dim3(kThreads), lds, ctx->stream, a);
-  CRV_HIP_CHECK(ctx, hipGetLastError());
+  void* args[] = {&a};
+  const void* kern = inv ? cols_kernel<T, true>(a.f) : cols_kernel<T, false>(a.f);
+  CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(kThreads), args, lds, ctx->stream));
   return CRV_OK;
 }
 
@@ -529,31 +596,31 @@ int chunk_planes(const FftPlan* p, size_t elem) {
 }
 
 template <typename T>
-int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale) {
+int launch_r2c(crv_ctx* ctx, FftPlan* p, T* dst, const T* src, double scale, const Batch& bt) {
   using C = cplx_t<T>;
   C* cd = reinterpret_cast<C*>(dst);
   const int cp = chunk_planes(p, sizeof(T));
   int rc;
   for (int k0 = 0; k0 < p->nzp; k0 += cp) {
     const int k1 = std::min(p->nzp, k0 + cp);
-    if ((rc = launch_rows<T>(ctx, p, false, dst, src, nullptr, k0 * p->nyp, k1 * p->nyp, scale)) != CRV_OK) return rc;
-    if (p->nyp > 1 && (rc = launch_cols<T>(ctx, p, false, false, cd, k0, k1)) != CRV_OK) return rc;
+    if ((rc = launch_rows<T>(ctx, p, false, dst, src, bt, k0 * p->nyp, k1 * p->nyp, scale)) != CRV_OK) return rc;
+    if (p->nyp > 1 && (rc = launch_cols<T>(ctx, p, false, false, cd, bt, k0, k1)) != CRV_OK) return rc;
   }
-  if (p->nzp > 1 && (rc = launch_cols<T>(ctx, p, false, true, cd, 0, p->nyp)) != CRV_OK) return rc;
+  if (p->nzp > 1 && (rc = launch_cols<T>(ctx, p, false, true, cd, bt, 0, p->nyp)) != CRV_OK) return rc;
   return CRV_OK;
 }
 
 template <typename T>
-int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const T* add) {
+int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const Batch& bt) {
   using C = cplx_t<T>;
   C* cd = reinterpret_cast<C*>(buf);
   int rc;
-  if (p->nzp > 1 && (rc = launch_cols<T>(ctx, p, true, true, cd, 0, p->nyp)) != CRV_OK) return rc;
+  if (p->nzp > 1 && (rc = launch_cols<T>(ctx, p, true, true, cd, bt, 0, p->nyp)) != CRV_OK) return rc;
   const int cp = chunk_planes(p, sizeof(T));
   for (int k0 = 0; k0 < p->nzp; k0 += cp) {
     const int k1 = std::min(p->nzp, k0 + cp);
-    if (p->nyp > 1 && (rc = launch_cols<T>(ctx, p, true, false, cd, k0, k1)) != CRV_OK) return rc;
-    if ((rc = launch_rows<T>(ctx, p, true, dst, buf, add, k0 * p->nyp, k1 * p->nyp, scale)) != CRV_OK) return rc;
+    if (p->nyp > 1 && (rc = launch_cols<T>(ctx, p, true, false, cd, bt, k0, k1)) != CRV_OK) return rc;
+    if ((rc = launch_rows<T>(ctx, p, true, dst, buf, bt, k0 * p->nyp, k1 * p->nyp, scale)) != CRV_OK) return rc;
   }
   return CRV_OK;
 }
@@ -561,6 +628,52 @@ int launch_c2r(crv_ctx* ctx, FftPlan* p, T* buf, T* dst, double scale, const T* 
 }  // namespace
 
 // ---------------------------------------------------------------------------
+namespace {
+
+// the three axis plans and their twiddle table (replaces an existing one)
+int build_twiddles(crv_ctx* ctx, FftPlan* p, int rev_x, int rev_y, int rev_z) {
+  std::vector<long double> tr, ti;
+  const int nxp = p->nxp;
+  const int Lx = p->xeven ? nxp / 2 : nxp;
+  if (!plan_1d(Lx, p->fx, tr, ti, rev_x) || !plan_1d(p->nyp, p->fy, tr, ti, rev_y) ||
+      !plan_1d(p->nzp, p->fz, tr, ti, rev_z))
+    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "grid size is not a product of 2, 3, 5, 7");
+  p->post_off = (int)tr.size();
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int k = 0; k <= nxp / 2; ++k) {
+    const long double ang = -twopi * (long double)k / (long double)nxp;
+    tr.push_back(cosl(ang));
+    ti.push_back(sinl(ang));
+  }
+  const size_t n = tr.size();
+  void* tw = nullptr;
+  hipError_t e;
+  if (p->prec == CRV_FP64) {
+    std::vector<double2> h(n);
+    for (size_t i = 0; i < n; ++i) { h[i].x = (double)tr[i]; h[i].y = (double)ti[i]; }
+    e = hipMalloc(&tw, n * sizeof(double2));
+    if (e == hipSuccess) e = hipMemcpy(tw, h.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+  } else {
+    std::vector<float2> h(n);
+    for (size_t i = 0; i < n; ++i) { h[i].x = (float)tr[i]; h[i].y = (float)ti[i]; }
+    e = hipMalloc(&tw, n * sizeof(float2));
+    if (e == hipSuccess) e = hipMemcpy(tw, h.data(), n * sizeof(float2), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    if (tw) (void)hipFree(tw);
+    return crv_set_error(ctx, CRV_E_HIP, std::string("plan twiddles: ") + hipGetErrorString(e));
+  }
+  if (p->tw) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p->tw);
+  }
+  p->tw = tw;
+  p->rev[0] = rev_x; p->rev[1] = rev_y; p->rev[2] = rev_z;
+  return CRV_OK;
+}
+
+}  // namespace
+
 void crv_free_plan(FftPlan* p) {
   if (!p) return;
   if (p->tw) (void)hipFree(p->tw);
@@ -575,39 +688,11 @@ int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** ou
   FftPlan* p = new FftPlan();
   p->nxp = nxp; p->nyp = nyp; p->nzp = nzp; p->prec = prec;
   p->xeven = (nxp % 2 == 0);
-  std::vector<long double> tr, ti;
-  const int Lx = p->xeven ? nxp / 2 : nxp;
-  if (!plan_1d(Lx, p->fx, tr, ti) || !plan_1d(nyp, p->fy, tr, ti) || !plan_1d(nzp, p->fz, tr, ti)) {
-    delete p;
-    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "grid size is not a product of 2, 3, 5, 7");
-  }
-  p->post_off = (int)tr.size();
-  const long double twopi = 6.283185307179586476925286766559L;
-  for (int k = 0; k <= nxp / 2; ++k) {
-    const long double ang = -twopi * (long double)k / (long double)nxp;
-    tr.push_back(cosl(ang));
-    ti.push_back(sinl(ang));
-  }
-  {
-    const int rc = crv_plan_shapes(ctx, p);
-    if (rc != CRV_OK) { delete p; return rc; }
-  }
-  const size_t n = tr.size();
-  hipError_t e;
-  if (prec == CRV_FP64) {
-    std::vector<double2> h(n);
-    for (size_t i = 0; i < n; ++i) { h[i].x = (double)tr[i]; h[i].y = (double)ti[i]; }
-    e = hipMalloc(&p->tw, n * sizeof(double2));
-    if (e == hipSuccess) e = hipMemcpy(p->tw, h.data(), n * sizeof(double2), hipMemcpyHostToDevice);
-  } else {
-    std::vector<float2> h(n);
-    for (size_t i = 0; i < n; ++i) { h[i].x = (float)tr[i]; h[i].y = (float)ti[i]; }
-    e = hipMalloc(&p->tw, n * sizeof(float2));
-    if (e == hipSuccess) e = hipMemcpy(p->tw, h.data(), n * sizeof(float2), hipMemcpyHostToDevice);
-  }
-  if (e != hipSuccess) {
+  int rc = build_twiddles(ctx, p, 0, 0, 0);
+  if (rc == CRV_OK) rc = crv_plan_shapes(ctx, p);
+  if (rc != CRV_OK) {
     crv_free_plan(p);
-    return crv_set_error(ctx, CRV_E_HIP, std::string("plan twiddles: ") + hipGetErrorString(e));
+    return rc;
   }
   ctx->plans[key] = p;
   *out = p;
@@ -640,13 +725,167 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
 }
 
 int crv_launch_r2c(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, double scale) {
+  Batch bt;
   if (plan->prec == CRV_FP64)
-    return launch_r2c<double>(ctx, plan, (double*)dst, (const double*)src, scale);
-  return launch_r2c<float>(ctx, plan, (float*)dst, (const float*)src, scale);
+    return launch_r2c<double>(ctx, plan, (double*)dst, (const double*)src, scale, bt);
+  return launch_r2c<float>(ctx, plan, (float*)dst, (const float*)src, scale, bt);
 }
 
 int crv_launch_c2r(crv_ctx* ctx, FftPlan* plan, void* buf, void* dst, double scale, const void* add) {
+  Batch bt;
+  if (add) { bt.nadd = 1; bt.add[0] = add; }
   if (plan->prec == CRV_FP64)
-    return launch_c2r<double>(ctx, plan, (double*)buf, (double*)dst, scale, (const double*)add);
-  return launch_c2r<float>(ctx, plan, (float*)buf, (float*)dst, scale, (const float*)add);
+    return launch_c2r<double>(ctx, plan, (double*)buf, (double*)dst, scale, bt);
+  return launch_c2r<float>(ctx, plan, (float*)buf, (float*)dst, scale, bt);
+}
+
+// nbatch grids of this plan, bstride reals apart, transformed in place with
+// one launch per pass; the output of batch entry b gets add[b % nadd] added
+int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale,
+                         const void* const* add, int nadd) {
+  if (nbatch < 1 || nbatch > 65535 || nadd < 0 || nadd > 3) return crv_set_error(ctx, CRV_E_INVALID, "bad batch");
+  Batch bt;
+  bt.n = nbatch; bt.stride = bstride; bt.nadd = nadd;
+  for (int q = 0; q < nadd; ++q) bt.add[q] = add[q];
+  if (plan->prec == CRV_FP64)
+    return launch_c2r<double>(ctx, plan, (double*)buf, (double*)buf, scale, bt);
+  return launch_c2r<float>(ctx, plan, (float*)buf, (float*)buf, scale, bt);
+}
+
+int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale) {
+  if (nbatch < 1 || nbatch > 65535) return crv_set_error(ctx, CRV_E_INVALID, "bad batch");
+  Batch bt;
+  bt.n = nbatch; bt.stride = bstride;
+  if (plan->prec == CRV_FP64) return launch_r2c<double>(ctx, plan, (double*)buf, (double*)buf, scale, bt);
+  return launch_r2c<float>(ctx, plan, (float*)buf, (float*)buf, scale, bt);
+}
+
+// ---- autotuner ---------------------------------------------------------------
+extern "C" int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int precision, int reps,
+                                crv_fft_shape* shape) {
+  if (!ctx) return CRV_E_INVALID;
+  if (precision != CRV_FP32 && precision != CRV_FP64) return crv_set_error(ctx, CRV_E_INVALID, "bad precision");
+  if (reps < 1) reps = 3;
+  FftPlan* p = nullptr;
+  int rc = crv_get_plan(ctx, nxp, nyp, nzp, precision, &p);
+  if (rc != CRV_OK) return rc;
+  const size_t eb = precision == CRV_FP64 ? 8 : 4;
+  const size_t rsize = (size_t)2 * (nxp / 2 + 1) * nyp * nzp;
+  void* buf = nullptr;
+  CRV_HIP_CHECK(ctx, hipMalloc(&buf, rsize * eb));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+  };
+  if (hipMemset(buf, 0, rsize * eb) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    return crv_set_error(ctx, CRV_E_HIP, "autotune: scratch set-up failed");
+  }
+  const bool prof = ctx->prof_on;
+  ctx->prof_on = false;
+  Batch bt;
+  const bool f64 = precision == CRV_FP64;
+  // milliseconds per call of fn, median of reps after a warm-up call
+  auto timed = [&](auto fn, float& ms) -> int {
+    int r = fn();
+    if (r != CRV_OK) return r;
+    std::vector<float> t;
+    for (int i = 0; i < reps; ++i) {
+      CRV_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+      if ((r = fn()) != CRV_OK) return r;
+      CRV_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+      CRV_HIP_CHECK(ctx, hipEventSynchronize(e1));
+      float x = 0.0f;
+      CRV_HIP_CHECK(ctx, hipEventElapsedTime(&x, e0, e1));
+      t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[t.size() / 2];
+    return CRV_OK;
+  };
+  auto rows = [&]() -> int {
+    const int nr = nyp * nzp;
+    int r = f64 ? launch_rows<double>(ctx, p, false, (double*)buf, (const double*)buf, bt, 0, nr, 1.0)
+                : launch_rows<float>(ctx, p, false, (float*)buf, (const float*)buf, bt, 0, nr, 1.0);
+    if (r != CRV_OK) return r;
+    return f64 ? launch_rows<double>(ctx, p, true, (double*)buf, (const double*)buf, bt, 0, nr, 1.0)
+               : launch_rows<float>(ctx, p, true, (float*)buf, (const float*)buf, bt, 0, nr, 1.0);
+  };
+  auto cols = [&](bool z) -> int {
+    const int o1 = z ? nyp : nzp;
+    int r = f64 ? launch_cols<double>(ctx, p, false, z, (double2*)buf, bt, 0, o1)
+                : launch_cols<float>(ctx, p, false, z, (float2*)buf, bt, 0, o1);
+    if (r != CRV_OK) return r;
+    return f64 ? launch_cols<double>(ctx, p, true, z, (double2*)buf, bt, 0, o1)
+               : launch_cols<float>(ctx, p, true, z, (float2*)buf, bt, 0, o1);
+  };
+  const int maxe = kThreads * (f64 ? Ept<double>::value : Ept<float>::value);
+  rc = CRV_OK;
+  // radix order of each axis (only lengths without a two-stage kernel)
+  int rev[3] = {0, 0, 0};
+  for (int ax = 0; ax < 3 && rc == CRV_OK; ++ax) {
+    const Fft1D& f = ax == 0 ? p->fx : (ax == 1 ? p->fy : p->fz);
+    if (f.nst < 2 || is_pair(std::vector<int>(f.R, f.R + f.nst))) continue;
+    float best = 1e30f;
+    for (int r = 0; r < 2 && rc == CRV_OK; ++r) {
+      int tr[3] = {rev[0], rev[1], rev[2]};
+      tr[ax] = r;
+      if ((rc = build_twiddles(ctx, p, tr[0], tr[1], tr[2])) != CRV_OK) break;
+      float ms = 0.0f;
+      rc = ax == 0 ? timed(rows, ms) : timed([&]() { return cols(ax == 2); }, ms);
+      if (rc == CRV_OK && ms < best) { best = ms; rev[ax] = r; }
+    }
+  }
+  if (rc == CRV_OK) rc = build_twiddles(ctx, p, rev[0], rev[1], rev[2]);
+  // x rows per tile
+  if (rc == CRV_OK) {
+    float best = 1e30f;
+    int bestP = p->rows_per_block;
+    for (int P = 1; P <= 64 && rc == CRV_OK; P *= 2) {
+      if ((P + 1) * p->fx.L > maxe) break;
+      p->rows_per_block = P;
+      float ms = 0.0f;
+      if ((rc = timed(rows, ms)) == CRV_OK && ms < best) { best = ms; bestP = P; }
+    }
+    p->rows_per_block = bestP;
+  }
+  // pencil tile widths
+  for (int ax = 1; ax <= 2 && rc == CRV_OK; ++ax) {
+    const int L = ax == 1 ? nyp : nzp;
+    if (L < 2) continue;
+    int& tx = ax == 1 ? p->tx_y : p->tx_z;
+    float best = 1e30f;
+    int bestT = tx;
+    for (int t = 1; t <= 64 && rc == CRV_OK; t *= 2) {
+      if (L * t > maxe) break;
+      tx = t;
+      float ms = 0.0f;
+      if ((rc = timed([&]() { return cols(ax == 2); }, ms)) == CRV_OK && ms < best) { best = ms; bestT = t; }
+    }
+    tx = bestT;
+  }
+  if (rc == CRV_OK && shape) {
+    shape->rows_per_tile = p->rows_per_block;
+    shape->tx_y = p->tx_y;
+    shape->tx_z = p->tx_z;
+    for (int ax = 0; ax < 3; ++ax) shape->radix_order[ax] = p->rev[ax];
+    float a = 0.0f, b = 0.0f;
+    rc = timed([&]() {
+      return f64 ? launch_r2c<double>(ctx, p, (double*)buf, (const double*)buf, 1.0, bt)
+                 : launch_r2c<float>(ctx, p, (float*)buf, (const float*)buf, 1.0, bt);
+    }, a);
+    if (rc == CRV_OK)
+      rc = timed([&]() {
+        return f64 ? launch_c2r<double>(ctx, p, (double*)buf, (double*)buf, 1.0, bt)
+                   : launch_c2r<float>(ctx, p, (float*)buf, (float*)buf, 1.0, bt);
+      }, b);
+    shape->ms_r2c = a;
+    shape->ms_c2r = b;
+  }
+  ctx->prof_on = prof;
+  cleanup();
+  return rc;
 }

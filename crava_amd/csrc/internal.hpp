@@ -27,7 +27,9 @@ template <typename C> __host__ __device__ inline C cconj(C a) { return cmake<C>(
 template <typename C> __host__ __device__ inline C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
 
 // ---------------------------------------------------------------------------
-// 1D transform plan: Stockham stages over LDS, twiddles in device memory.
+// 1D transform plan: Stockham stages over LDS; the twiddles of all stages
+// of one axis are a contiguous range [tw0, tw0 + twn) of the plan's table,
+// copied to LDS by each block.
 constexpr int kMaxStages = 16;
 struct Fft1D {
   int L;                 // transform length
@@ -35,6 +37,7 @@ struct Fft1D {
   int R[kMaxStages];     // radix of each stage
   int Ns[kMaxStages];    // product of the radices before the stage
   int tw[kMaxStages];    // offset of the stage's twiddles in the table
+  int tw0, twn;          // the axis's range of the table
 };
 
 struct FftPlan {
@@ -47,6 +50,7 @@ struct FftPlan {
   int rows_per_block = 0;
   int tx_y = 0, tx_z = 0;
   int chunk_mb = 0;      // x/y pass interleave chunk (MB of grid), 0 = off
+  int rev[3] = {0, 0, 0};  // radix order per axis (0 default, 1 reversed)
   // reference noise: per-row prefix of draws (nyp*nzp+1 int64), lazily made
   long long* noise_prefix = nullptr;
   long long noise_draws = 0;
@@ -63,6 +67,7 @@ struct crv_ctx {
   std::vector<ProfEvent> prof;
   int device = 0;
   int num_cu = 256;      // compute units of the device (persistent grids)
+  int post_blocks_per_cu = 0;  // posterior grid size (0: occupancy limit)
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string last_error;
@@ -84,6 +89,8 @@ struct crv_grid {
   bool transformed = false;
   size_t rsize = 0, csize = 0;
   void* d = nullptr;
+  int* shared = nullptr;   // batch grids: count of live grids in the allocation
+  void* base = nullptr;    // batch grids: start of the allocation
   size_t elem_bytes() const { return precision == CRV_FP64 ? 8 : 4; }
 };
 
@@ -126,6 +133,9 @@ int crv_launch_r2c(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, doub
 // Inverse C2R of buf in place (complex view in, real view out); result written
 // to dst (may alias buf) times scale, plus add (real grid) if non-null.
 int crv_launch_c2r(crv_ctx* ctx, FftPlan* plan, void* buf, void* dst, double scale, const void* add);
+int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale,
+                         const void* const* add, int nadd);
+int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale);
 
 // elementwise (simulate.hip) ---------------------------------------------------
 int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n);

@@ -108,25 +108,67 @@ __device__ __forceinline__ void counter_pair(unsigned long long seed, unsigned l
   n1 = r * sn;
 }
 
-__device__ __forceinline__ void noise_counter_value(const NoiseGeom& g, unsigned long long seed,
-                                                    unsigned long long real, int grid, long long i, double& re,
-                                                    double& im) {
+// Simulation noise of the three grids at one element.  One Philox call per
+// (element, realization, call tag) gives four words; grid g takes words
+// 2(g&1), 2(g&1)+1 of call g>>1 and turns them into 23-bit uniforms
+// (2 (w >> 9) + 1) 2^-24 in (0, 1), exact in FP32, and a Box-Muller pair
+// r cos(2 pi u2), r sin(2 pi u2) with r = sqrt(-2 ln u1), evaluated in the
+// precision U of the grids.  An element that copies its Hermitian partner
+// draws with the partner's index.
+__device__ __forceinline__ void sim_words(unsigned long long seed, unsigned long long real, unsigned tag,
+                                          long long s, unsigned w[4]) {
+  w[0] = (unsigned)s;
+  w[1] = (unsigned)((unsigned long long)s >> 32);
+  w[2] = (unsigned)real;
+  w[3] = ((unsigned)(real >> 32) & 0x00FFFFFFu) | (tag << 24);
+  philox10(w, (unsigned)seed, (unsigned)(seed >> 32));
+}
+
+__device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, float& n0, float& n1) {
+  const float u1 = (float)(((a >> 9) << 1) | 1u) * 0x1p-24f;
+  const float u2 = (float)(((b >> 9) << 1) | 1u) * 0x1p-24f;
+  const float r = sqrtf(-2.0f * logf(u1));
+  float sn, cs;
+  sincospif(2.0f * u2, &sn, &cs);
+  n0 = r * cs;
+  n1 = r * sn;
+}
+__device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, double& n0, double& n1) {
+  const double u1 = (double)(((a >> 9) << 1) | 1u) * 0x1p-24;
+  const double u2 = (double)(((b >> 9) << 1) | 1u) * 0x1p-24;
+  const double r = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
+  n0 = r * cs;
+  n1 = r * sn;
+}
+
+// (re, im) of grids 0..2 at element i, Hermitian rules of fillInComplexNoise,
+// values rounded to float as the reference's float grids hold them
+template <typename U>
+__device__ __forceinline__ void sim_noise3(const NoiseGeom& g, unsigned long long seed, unsigned long long real,
+                                           long long i, U re[3], U im[3]) {
   long long partner = 0, jk;
   int x;
   const int kind = noise_kind(g, i, partner, jk, x);
-  const double sd = noise_std();
-  double n0, n1;
-  if (kind == 1) {
-    counter_pair(seed, real, grid, i, n0, n1);
-    re = (double)(float)n0;
-    im = 0.0;
-  } else if (kind == 3) {
-    counter_pair(seed, real, grid, partner, n0, n1);
-    re = (double)(float)(sd * n0);
-    im = -(double)(float)(sd * n1);
-  } else {
-    counter_pair(seed, real, grid, i, n0, n1);
-    re = (double)(float)(sd * n0);
-    im = (double)(float)(sd * n1);
+  const long long s = (kind == 3) ? partner : i;
+  unsigned w0[4], w1[4];
+  sim_words(seed, real, 0u, s, w0);
+  sim_words(seed, real, 1u, s, w1);
+  U n[3][2];
+  sim_box_muller(w0[0], w0[1], n[0][0], n[0][1]);
+  sim_box_muller(w0[2], w0[3], n[1][0], n[1][1]);
+  sim_box_muller(w1[0], w1[1], n[2][0], n[2][1]);
+  const U sd = (U)noise_std();
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (kind == 1) {
+      re[q] = (U)(float)n[q][0];
+      im[q] = (U)0;
+    } else {
+      re[q] = (U)(float)(sd * n[q][0]);
+      im[q] = (U)(float)(sd * n[q][1]);
+      if (kind == 3) im[q] = -im[q];
+    }
   }
 }

@@ -358,6 +358,7 @@ int persistent_blocks(crv_ctx* ctx, F kernel, long long work) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu < 1)
     per_cu = 2;
+  if (ctx->post_blocks_per_cu > 0) per_cu = ctx->post_blocks_per_cu;
   const long long want = (work + kThreads - 1) / kThreads;
   return (int)std::max<long long>(1, std::min<long long>(want, (long long)per_cu * ctx->num_cu));
 }
@@ -441,41 +442,22 @@ KzTab<U> tab_at(unsigned char* base, const TabOffsets& o) {
   return t;
 }
 
-}  // namespace
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
 
-extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* mean[3], crv_grid** data,
-                             crv_grid* err_corr, crv_grid* cov[6], crv_post_stats* stats) {
-  if (!ctx) return CRV_E_INVALID;
-  if (!p || !mean || !data || !cov) return crv_set_error(ctx, CRV_E_INVALID, "NULL argument");
-  const int n = p->ntheta;
-  if (n < 1 || n > 8) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "ntheta must be 1..8");
-  crv_grid* g0 = mean[0];
-  if (!g0) return crv_set_error(ctx, CRV_E_INVALID, "mean[0] is NULL");
-  int rc;
-  for (int m = 0; m < 3; ++m) if ((rc = check_grid(ctx, mean[m], g0, "mean")) != CRV_OK) return rc;
-  for (int l = 0; l < n; ++l) if ((rc = check_grid(ctx, data[l], g0, "data")) != CRV_OK) return rc;
-  for (int q = 0; q < 6; ++q) if ((rc = check_grid(ctx, cov[q], g0, "cov")) != CRV_OK) return rc;
-  if (!p->separable && (rc = check_grid(ctx, err_corr, g0, "err_corr")) != CRV_OK) return rc;
+int validate_params(crv_ctx* ctx, const crv_post_params* p) {
+  if (!p) return crv_set_error(ctx, CRV_E_INVALID, "NULL parameters");
+  if (p->ntheta < 1 || p->ntheta > 8) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "ntheta must be 1..8");
   if (!p->A || !p->err_theta_cov || !p->kwd || !p->em1 || !p->em2 || !p->em3 || !p->sigma0)
     return crv_set_error(ctx, CRV_E_INVALID, "missing table");
   if (p->separable && (!p->cxy || !p->ct)) return crv_set_error(ctx, CRV_E_INVALID, "separable prior without cxy/ct");
-  const bool refine = (p->mode == CRV_MODE_FP32_REFINE);
-  if (refine && g0->precision != CRV_FP32)
-    return crv_set_error(ctx, CRV_E_INVALID, "CRV_MODE_FP32_REFINE needs FP32 grids");
-  const int nxp = g0->nxp, nyp = g0->nyp, nzp = g0->nzp, cnxp = g0->cnxp;
-  const long long nc = (long long)cnxp * nyp * nzp;
-  if (refine && nc > 0xffffffffLL) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "refine mode: grid above 2^32 wavenumbers");
+  return CRV_OK;
+}
 
-  // Forward transforms that the reference performs inside the call or just
-  // before it (crava.cpp:177-184, 1039, 1062).
-  for (int m = 0; m < 3; ++m) if ((rc = ensure_transformed(mean[m])) != CRV_OK) return rc;
-  for (int l = 0; l < n; ++l) if ((rc = ensure_transformed(data[l])) != CRV_OK) return rc;
-  if (!p->separable) {
-    for (int q = 0; q < 6; ++q) if ((rc = ensure_transformed(cov[q])) != CRV_OK) return rc;
-    if ((rc = ensure_transformed(err_corr)) != CRV_OK) return rc;
-  }
-
-  // ---- kz tables (crava.cpp:1104-1121, 1336-1364, 1441-1469) -------------
+// kz tables (crava.cpp:1104-1121, 1336-1364, 1441-1469) in FP64
+HostTables build_tables(const crv_post_params* p, int nyp, int nzp, int cnxp, long long& ninv_planes) {
+  const int n = p->ntheta;
   const int P = n * (n + 1) / 2;
   const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
   HostTables h;
@@ -490,7 +472,7 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
     for (size_t q = 0; q < h.lam.size(); ++q) h.lam[q] = std::fabs(p->cxy[q]);
   }
   const double wnc = p->wnc;
-  long long ninv_planes = 0;
+  ninv_planes = 0;
   for (int k = 0; k < nzp; ++k) {
     // crava.cpp:1104-1105: float numerator over a double product, compared
     // with lowCut * minRelThick in double and with highCut in float
@@ -553,77 +535,206 @@ extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* m
         }
     }
   }
-  const bool f64 = (g0->precision == CRV_FP64);
-  // device blob: [tables T][tables FP64 (refine)][inv][stats 256 B]
-  std::vector<unsigned char> blob;
-  const TabOffsets oT = f64 ? put_tables<double>(blob, h) : put_tables<float>(blob, h);
-  TabOffsets o64;
-  if (refine) o64 = put_tables<double>(blob, h);
-  const size_t oI = blob.size();
-  blob.resize(oI + (((size_t)nzp * 4 + 255) & ~(size_t)255), 0);
-  memcpy(blob.data() + oI, h.inv.data(), (size_t)nzp * 4);
-  const size_t oS = blob.size();
-  blob.resize(oS + 256, 0);
-  unsigned char* dtab = nullptr;
-  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dtab, blob.size(), ctx->stream));
-  CRV_HIP_CHECK(ctx, hipMemcpyAsync(dtab, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
-  unsigned int* dlist = nullptr;
-  if (refine) CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dlist, (size_t)nc * 4, ctx->stream));
-  unsigned long long* dstats = (unsigned long long*)(dtab + oS);
+  return h;
+}
 
-  if (f64) {
+}  // namespace
+
+struct crv_post_plan {
+  crv_ctx* ctx = nullptr;
+  int n = 0, nxp = 0, nyp = 0, nzp = 0, cnxp = 0, precision = 0;
+  bool separable = false, refine = false, async = false;
+  long long nc = 0, ninv_planes = 0;
+  unsigned char* dtab = nullptr;  // [tables T][tables FP64 (refine)][inv][stats 256 B]
+  TabOffsets oT, o64;
+  size_t oI = 0, oS = 0;
+  unsigned int* dlist = nullptr;
+  double sig0[6] = {0, 0, 0, 0, 0, 0};
+  float cond_thr = 1e5f;
+  double lowk2 = 0.0;
+};
+
+namespace {
+
+int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp, int precision, bool async,
+              crv_post_plan** out) {
+  int rc = validate_params(ctx, p);
+  if (rc != CRV_OK) return rc;
+  if (precision != CRV_FP32 && precision != CRV_FP64) return crv_set_error(ctx, CRV_E_INVALID, "bad precision");
+  const bool refine = (p->mode == CRV_MODE_FP32_REFINE);
+  if (refine && precision != CRV_FP32) return crv_set_error(ctx, CRV_E_INVALID, "CRV_MODE_FP32_REFINE needs FP32 grids");
+  crv_post_plan* pl = new (std::nothrow) crv_post_plan();
+  if (!pl) return CRV_E_NOMEM;
+  pl->ctx = ctx; pl->n = p->ntheta; pl->nxp = nxp; pl->nyp = nyp; pl->nzp = nzp; pl->cnxp = nxp / 2 + 1;
+  pl->precision = precision; pl->separable = p->separable != 0; pl->refine = refine; pl->async = async;
+  pl->nc = (long long)pl->cnxp * nyp * nzp;
+  if (refine && pl->nc > 0xffffffffLL) {
+    delete pl;
+    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "refine mode: grid above 2^32 wavenumbers");
+  }
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) pl->sig0[q] = p->sigma0[pa[q] * 3 + pb[q]];
+  pl->cond_thr = (float)(p->cond_threshold > 0 ? p->cond_threshold : 1e5);
+  pl->lowk2 = p->lowk_fraction * p->lowk_fraction;
+  const HostTables h = build_tables(p, nyp, nzp, pl->cnxp, pl->ninv_planes);
+  std::vector<unsigned char> blob;
+  pl->oT = (precision == CRV_FP64) ? put_tables<double>(blob, h) : put_tables<float>(blob, h);
+  if (refine) pl->o64 = put_tables<double>(blob, h);
+  pl->oI = blob.size();
+  blob.resize(pl->oI + (((size_t)nzp * 4 + 255) & ~(size_t)255), 0);
+  memcpy(blob.data() + pl->oI, h.inv.data(), (size_t)nzp * 4);
+  pl->oS = blob.size();
+  blob.resize(pl->oS + 256, 0);
+  hipError_t e;
+  if (async) {
+    e = hipMallocAsync((void**)&pl->dtab, blob.size(), ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->dtab, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && refine) e = hipMallocAsync((void**)&pl->dlist, (size_t)pl->nc * 4, ctx->stream);
+  } else {
+    e = hipMalloc((void**)&pl->dtab, blob.size());
+    if (e == hipSuccess) e = hipMemcpy(pl->dtab, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && refine) e = hipMalloc((void**)&pl->dlist, (size_t)pl->nc * 4);
+  }
+  if (e != hipSuccess) {
+    crv_post_plan_destroy(pl);
+    return crv_set_error(ctx, CRV_E_HIP, std::string("posterior plan: ") + hipGetErrorString(e));
+  }
+  *out = pl;
+  return CRV_OK;
+}
+
+}  // namespace
+
+extern "C" int crv_post_plan_create(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
+                                    int precision, crv_post_plan** out) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!out || nxp < 1 || nyp < 1 || nzp < 1) return crv_set_error(ctx, CRV_E_INVALID, "post_plan_create: bad args");
+  *out = nullptr;
+  return plan_make(ctx, p, nxp, nyp, nzp, precision, false, out);
+}
+
+extern "C" int crv_post_plan_destroy(crv_post_plan* pl) {
+  if (!pl) return CRV_E_INVALID;
+  crv_ctx* ctx = pl->ctx;
+  if (pl->async) {
+    if (pl->dtab) (void)hipFreeAsync(pl->dtab, ctx->stream);
+    if (pl->dlist) (void)hipFreeAsync(pl->dlist, ctx->stream);
+  } else {
+    (void)hipStreamSynchronize(ctx->stream);
+    if (pl->dtab) (void)hipFree(pl->dtab);
+    if (pl->dlist) (void)hipFree(pl->dlist);
+  }
+  delete pl;
+  return CRV_OK;
+}
+
+extern "C" int crv_posterior_exec(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data, crv_grid* err_corr,
+                                  crv_grid* cov[6]) {
+  if (!pl) return CRV_E_INVALID;
+  crv_ctx* ctx = pl->ctx;
+  if (!mean || !data || !cov) return crv_set_error(ctx, CRV_E_INVALID, "NULL argument");
+  const int n = pl->n;
+  auto fits = [&](const crv_grid* g, bool need_transformed) {
+    return g && g->nxp == pl->nxp && g->nyp == pl->nyp && g->nzp == pl->nzp && g->precision == pl->precision &&
+           (!need_transformed || g->transformed);
+  };
+  for (int m = 0; m < 3; ++m)
+    if (!fits(mean[m], true)) return crv_set_error(ctx, CRV_E_STATE, "mean: wrong size/precision or not transformed");
+  for (int l = 0; l < n; ++l)
+    if (!fits(data[l], true)) return crv_set_error(ctx, CRV_E_STATE, "data: wrong size/precision or not transformed");
+  for (int q = 0; q < 6; ++q)
+    if (!fits(cov[q], !pl->separable)) return crv_set_error(ctx, CRV_E_STATE, "cov: wrong size/precision or state");
+  if (!pl->separable && !fits(err_corr, true))
+    return crv_set_error(ctx, CRV_E_STATE, "err_corr: wrong size/precision or not transformed");
+  unsigned long long* dstats = (unsigned long long*)(pl->dtab + pl->oS);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, dstats, 32);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  int rc;
+  if (pl->precision == CRV_FP64) {
     PostArgs<double> a{};
     for (int m = 0; m < 3; ++m) a.mean[m] = (double2*)mean[m]->d;
     for (int l = 0; l < n; ++l) a.data[l] = (double2*)data[l]->d;
-    a.err = p->separable ? nullptr : (const double2*)err_corr->d;
+    a.err = pl->separable ? nullptr : (const double2*)err_corr->d;
     for (int q = 0; q < 6; ++q) a.cov[q] = (double2*)cov[q]->d;
-    a.t = tab_at<double>(dtab, oT);
-    a.inv = (const int*)(dtab + oI);
-    for (int q = 0; q < 6; ++q) a.sig0[q] = a.sig0d[q] = p->sigma0[pa[q] * 3 + pb[q]];
-    a.nxp = nxp; a.cnxp = cnxp; a.nyp = nyp; a.nzp = nzp; a.nc = nc;
+    a.t = tab_at<double>(pl->dtab, pl->oT);
+    a.inv = (const int*)(pl->dtab + pl->oI);
+    for (int q = 0; q < 6; ++q) a.sig0[q] = a.sig0d[q] = pl->sig0[q];
+    a.nxp = pl->nxp; a.cnxp = pl->cnxp; a.nyp = pl->nyp; a.nzp = pl->nzp; a.nc = pl->nc;
     a.stats = dstats;
-    rc = p->separable ? dispatch<double, true, false>(ctx, n, a) : dispatch<double, false, false>(ctx, n, a);
+    rc = pl->separable ? dispatch<double, true, false>(ctx, n, a) : dispatch<double, false, false>(ctx, n, a);
   } else {
     PostArgs<float> a{};
     for (int m = 0; m < 3; ++m) a.mean[m] = (float2*)mean[m]->d;
     for (int l = 0; l < n; ++l) a.data[l] = (float2*)data[l]->d;
-    a.err = p->separable ? nullptr : (const float2*)err_corr->d;
+    a.err = pl->separable ? nullptr : (const float2*)err_corr->d;
     for (int q = 0; q < 6; ++q) a.cov[q] = (float2*)cov[q]->d;
-    a.t = tab_at<float>(dtab, oT);
-    a.inv = (const int*)(dtab + oI);
-    for (int q = 0; q < 6; ++q) { a.sig0d[q] = p->sigma0[pa[q] * 3 + pb[q]]; a.sig0[q] = (float)a.sig0d[q]; }
-    a.nxp = nxp; a.cnxp = cnxp; a.nyp = nyp; a.nzp = nzp; a.nc = nc;
+    a.t = tab_at<float>(pl->dtab, pl->oT);
+    a.inv = (const int*)(pl->dtab + pl->oI);
+    for (int q = 0; q < 6; ++q) { a.sig0d[q] = pl->sig0[q]; a.sig0[q] = (float)pl->sig0[q]; }
+    a.nxp = pl->nxp; a.cnxp = pl->cnxp; a.nyp = pl->nyp; a.nzp = pl->nzp; a.nc = pl->nc;
     a.stats = dstats;
-    if (refine) {
-      a.t64 = tab_at<double>(dtab, o64);
-      a.list = dlist;
-      a.list_count = (unsigned int*)(dtab + oS + 64);
-      a.cond_thr = (float)(p->cond_threshold > 0 ? p->cond_threshold : 1e5);
-      a.lowk2 = p->lowk_fraction * p->lowk_fraction;
-      rc = p->separable ? dispatch<float, true, true>(ctx, n, a) : dispatch<float, false, true>(ctx, n, a);
+    if (pl->refine) {
+      a.t64 = tab_at<double>(pl->dtab, pl->o64);
+      a.list = pl->dlist;
+      a.list_count = (unsigned int*)(pl->dtab + pl->oS + 64);
+      a.cond_thr = pl->cond_thr;
+      a.lowk2 = pl->lowk2;
+      rc = pl->separable ? dispatch<float, true, true>(ctx, n, a) : dispatch<float, false, true>(ctx, n, a);
     } else {
-      rc = p->separable ? dispatch<float, true, false>(ctx, n, a) : dispatch<float, false, false>(ctx, n, a);
+      rc = pl->separable ? dispatch<float, true, false>(ctx, n, a) : dispatch<float, false, false>(ctx, n, a);
     }
   }
-  if (rc != CRV_OK) { (void)hipFreeAsync(dtab, ctx->stream); if (dlist) (void)hipFreeAsync(dlist, ctx->stream); return rc; }
+  if (rc != CRV_OK) return rc;
   for (int l = 0; l < n; ++l) data[l]->transformed = true;
   for (int q = 0; q < 6; ++q) { cov[q]->transformed = true; cov[q]->cubetype = CRV_COVARIANCE; }
+  return CRV_OK;
+}
 
-  if (stats) {
-    unsigned long long hs[16];
-    CRV_HIP_CHECK(ctx, hipMemcpyAsync(hs, dtab + oS, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    stats->n_wavenumbers = nc;
-    stats->n_inverted = ninv_planes * cnxp * nyp;
-    stats->n_chol_fail = (long long)hs[0];
-    stats->n_fp64 = refine ? (long long)(unsigned int)(hs[8] & 0xffffffffull) : 0;
-    double mc;
-    memcpy(&mc, &hs[1], 8);
-    stats->max_cond = mc;
+extern "C" int crv_post_plan_stats(crv_post_plan* pl, crv_post_stats* stats) {
+  if (!pl || !stats) return CRV_E_INVALID;
+  crv_ctx* ctx = pl->ctx;
+  unsigned long long hs[16];
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(hs, pl->dtab + pl->oS, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  stats->n_wavenumbers = pl->nc;
+  stats->n_inverted = pl->ninv_planes * pl->cnxp * pl->nyp;
+  stats->n_chol_fail = (long long)hs[0];
+  stats->n_fp64 = pl->refine ? (long long)(unsigned int)(hs[8] & 0xffffffffull) : 0;
+  double mc;
+  memcpy(&mc, &hs[1], 8);
+  stats->max_cond = mc;
+  return CRV_OK;
+}
+
+extern "C" int crv_posterior(crv_ctx* ctx, const crv_post_params* p, crv_grid* mean[3], crv_grid** data,
+                             crv_grid* err_corr, crv_grid* cov[6], crv_post_stats* stats) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!mean || !data || !cov) return crv_set_error(ctx, CRV_E_INVALID, "NULL argument");
+  int rc = validate_params(ctx, p);
+  if (rc != CRV_OK) return rc;
+  const int n = p->ntheta;
+  crv_grid* g0 = mean[0];
+  if (!g0) return crv_set_error(ctx, CRV_E_INVALID, "mean[0] is NULL");
+  for (int m = 0; m < 3; ++m) if ((rc = check_grid(ctx, mean[m], g0, "mean")) != CRV_OK) return rc;
+  for (int l = 0; l < n; ++l) if ((rc = check_grid(ctx, data[l], g0, "data")) != CRV_OK) return rc;
+  for (int q = 0; q < 6; ++q) if ((rc = check_grid(ctx, cov[q], g0, "cov")) != CRV_OK) return rc;
+  if (!p->separable && (rc = check_grid(ctx, err_corr, g0, "err_corr")) != CRV_OK) return rc;
+  if (p->mode == CRV_MODE_FP32_REFINE && g0->precision != CRV_FP32)
+    return crv_set_error(ctx, CRV_E_INVALID, "CRV_MODE_FP32_REFINE needs FP32 grids");
+  // Forward transforms that the reference performs inside the call or just
+  // before it (crava.cpp:177-184, 1039, 1062).
+  for (int m = 0; m < 3; ++m) if ((rc = ensure_transformed(mean[m])) != CRV_OK) return rc;
+  for (int l = 0; l < n; ++l) if ((rc = ensure_transformed(data[l])) != CRV_OK) return rc;
+  if (!p->separable) {
+    for (int q = 0; q < 6; ++q) if ((rc = ensure_transformed(cov[q])) != CRV_OK) return rc;
+    if ((rc = ensure_transformed(err_corr)) != CRV_OK) return rc;
   }
-  CRV_HIP_CHECK(ctx, hipFreeAsync(dtab, ctx->stream));
-  if (dlist) CRV_HIP_CHECK(ctx, hipFreeAsync(dlist, ctx->stream));
-
+  crv_post_plan* pl = nullptr;
+  if ((rc = plan_make(ctx, p, g0->nxp, g0->nyp, g0->nzp, g0->precision, true, &pl)) != CRV_OK) return rc;
+  rc = crv_posterior_exec(pl, mean, data, err_corr, cov);
+  if (rc == CRV_OK && stats) rc = crv_post_plan_stats(pl, stats);
+  crv_post_plan_destroy(pl);
+  if (rc != CRV_OK) return rc;
   if (!p->skip_mean_inverse) {
     for (int m = 0; m < 3; ++m) if ((rc = crv_fft_c2r(mean[m])) != CRV_OK) return rc;
   }

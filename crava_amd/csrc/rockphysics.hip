@@ -141,31 +141,87 @@ __device__ void gas(double T, double P, double G, double& k_kpa, double& rho) {
   k_kpa = P * g0 / (1.0 - Ppr / Z * dZ) * 1E3;
 }
 
+// one sample: in = (porosity, clay, Sw, aspect, Pe MPa) -> ln vp, ln vs, ln rho
+__device__ __forceinline__ int rock_sample(const crv_rock_params& p, double phi, double clay, double sw, double asp,
+                                           double pe, int max_steps, double out[3]) {
+  const double vq = 1.0 - clay, vc = clay;
+  const double kmin = 0.5 * ((vq * p.k_quartz + vc * p.k_clay) + 1.0 / (vq / p.k_quartz + vc / p.k_clay));
+  const double gmin = 0.5 * ((vq * p.g_quartz + vc * p.g_clay) + 1.0 / (vq / p.g_quartz + vc / p.g_clay));
+  const double rhomin = vq * p.rho_quartz + vc * p.rho_clay;
+  double y[2] = {kmin, gmin};
+  int st = 0;
+  if (phi != 1.0) st = dem_rkf45(asp, phi, 0.0, 0.0, y, max_steps);
+  const double kd = y[0], gd = y[1];
+  double kb, rb, kg, rg;
+  brine(p.temperature, pe, p.salinity, kb, rb);
+  gas(p.temperature, pe, p.gas_gravity, kg, rg);
+  const double kfl = 1.0 / (sw / kb + (1.0 - sw) / kg);
+  const double rfl = sw * rb + (1.0 - sw) * rg;
+  const double t1 = (1 - kd / kmin) * (1 - kd / kmin);
+  const double k = kd + t1 / (phi / kfl + (1 - phi) / kmin + kd / (kmin * kmin));
+  const double rho = (1 - phi) * rhomin + phi * rfl;
+  out[0] = log(sqrt((k + 4.0 / 3.0 * gd) / rho));
+  out[1] = log(sqrt(gd / rho));
+  out[2] = log(rho * 1000.0);
+  return st;
+}
+
 template <typename TO>
 __global__ __launch_bounds__(kThreads) void k_rock(crv_rock_params p, long long n, const double* __restrict__ in,
                                                    TO* __restrict__ out, int* __restrict__ status, int max_steps) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double phi = in[5 * i], clay = in[5 * i + 1], sw = in[5 * i + 2], asp = in[5 * i + 3], pe = in[5 * i + 4];
-    const double vq = 1.0 - clay, vc = clay;
-    const double kmin = 0.5 * ((vq * p.k_quartz + vc * p.k_clay) + 1.0 / (vq / p.k_quartz + vc / p.k_clay));
-    const double gmin = 0.5 * ((vq * p.g_quartz + vc * p.g_clay) + 1.0 / (vq / p.g_quartz + vc / p.g_clay));
-    const double rhomin = vq * p.rho_quartz + vc * p.rho_clay;
-    double y[2] = {kmin, gmin};
-    int st = 0;
-    if (phi != 1.0) st = dem_rkf45(asp, phi, 0.0, 0.0, y, max_steps);
-    const double kd = y[0], gd = y[1];
-    double kb, rb, kg, rg;
-    brine(p.temperature, pe, p.salinity, kb, rb);
-    gas(p.temperature, pe, p.gas_gravity, kg, rg);
-    const double kfl = 1.0 / (sw / kb + (1.0 - sw) / kg);
-    const double rfl = sw * rb + (1.0 - sw) * rg;
-    const double t1 = (1 - kd / kmin) * (1 - kd / kmin);
-    const double k = kd + t1 / (phi / kfl + (1 - phi) / kmin + kd / (kmin * kmin));
-    const double rho = (1 - phi) * rhomin + phi * rfl;
-    out[3 * i] = (TO)log(sqrt((k + 4.0 / 3.0 * gd) / rho));
-    out[3 * i + 1] = (TO)log(sqrt(gd / rho));
-    out[3 * i + 2] = (TO)log(rho * 1000.0);
+    double o[3];
+    const int st = rock_sample(p, in[5 * i], in[5 * i + 1], in[5 * i + 2], in[5 * i + 3], in[5 * i + 4], max_steps, o);
+    out[3 * i] = (TO)o[0];
+    out[3 * i + 1] = (TO)o[1];
+    out[3 * i + 2] = (TO)o[2];
     if (status) status[i] = st;
+  }
+}
+
+// Error gate: list the samples whose error-table cell predicts more than
+// thresh (one atomic per wavefront for the list slots).
+__global__ __launch_bounds__(kThreads) void k_gate(long long n, const float* __restrict__ x, crv_mlp_norm nm,
+                                                   const float* __restrict__ table, int nb, float thresh,
+                                                   unsigned int* __restrict__ list, unsigned int* __restrict__ count) {
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += (long long)gridDim.x * blockDim.x) {
+    const long long i = base + threadIdx.x;
+    bool flag = false;
+    if (i < n) {
+      int cell = 0;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        float v = x[5 * i + f];
+        if (nm.in_log_mask & (1 << f)) v = log10f(v);
+        const float t = ((v - nm.in_lo[f]) * nm.in_scale[f]) * 0.5f;  // [0, 1] inside the box
+        const int b = min(nb - 1, max(0, (int)floorf(t * (float)nb)));
+        cell = cell * nb + b;
+      }
+      flag = !(table[cell] <= thresh);
+    }
+    const unsigned long long mask = __ballot(flag);
+    const int lane = threadIdx.x & 63;
+    unsigned int slot0 = 0;
+    if (lane == 0 && mask) slot0 = atomicAdd(count, (unsigned int)__popcll(mask));
+    slot0 = __shfl(slot0, 0, 64);
+    if (flag) list[slot0 + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned int)i;
+  }
+}
+
+// rplib chain for the listed samples (float in/out)
+__global__ __launch_bounds__(kThreads) void k_rock_list(crv_rock_params p, const unsigned int* __restrict__ count,
+                                                        const unsigned int* __restrict__ list,
+                                                        const float* __restrict__ x, float* __restrict__ y,
+                                                        int max_steps) {
+  const unsigned int m = *count;
+  for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < m; t += gridDim.x * blockDim.x) {
+    const long long i = list[t];
+    double o[3];
+    const int st = rock_sample(p, x[5 * i], x[5 * i + 1], x[5 * i + 2], x[5 * i + 3], x[5 * i + 4], max_steps, o);
+    const float nan = __builtin_nanf("");
+    y[3 * i] = st ? nan : (float)o[0];
+    y[3 * i + 1] = st ? nan : (float)o[1];
+    y[3 * i + 2] = st ? nan : (float)o[2];
   }
 }
 
@@ -185,5 +241,41 @@ extern "C" int crv_rock_forward(crv_ctx* ctx, const crv_rock_params* p, long lon
     hipLaunchKernelGGL(k_rock<double>, dim3(nb), dim3(kThreads), 0, ctx->stream, *p, n, d_in, (double*)d_out,
                        d_status, 4096);
   CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
+extern "C" int crv_rock_refine(crv_ctx* ctx, const crv_rock_params* p, long long n, const float* d_x, float* d_y,
+                               const crv_mlp_norm* norm, const float* d_table, int nb, float thresh,
+                               long long* n_refined) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!p || !norm || n < 0 || nb < 1 || (n > 0 && (!d_x || !d_y || !d_table)))
+    return crv_set_error(ctx, CRV_E_INVALID, "rock_refine: bad args");
+  if (n > 0xffffffffLL) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "rock_refine: more than 2^32 samples");
+  if (n_refined) *n_refined = 0;
+  if (n == 0) return CRV_OK;
+  unsigned int* buf = nullptr;
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&buf, ((size_t)n + 64) * sizeof(unsigned int), ctx->stream));
+  unsigned int* count = buf;
+  unsigned int* list = buf + 64;
+  CRV_HIP_CHECK(ctx, hipMemsetAsync(count, 0, sizeof(unsigned int), ctx->stream));
+  {
+    ProfScope ps(ctx, "rock_gate");
+    const int nb_g = (int)std::min<long long>((n + kThreads - 1) / kThreads, 1 << 16);
+    hipLaunchKernelGGL(k_gate, dim3(nb_g), dim3(kThreads), 0, ctx->stream, n, d_x, *norm, d_table, nb, thresh, list,
+                       count);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  {
+    ProfScope ps(ctx, "rock_refine");
+    const int nb_r = (int)std::min<long long>((n + kThreads - 1) / kThreads, 4096);
+    hipLaunchKernelGGL(k_rock_list, dim3(nb_r), dim3(kThreads), 0, ctx->stream, *p, (const unsigned int*)count,
+                       (const unsigned int*)list, d_x, d_y, 4096);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  unsigned int hc = 0;
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(&hc, count, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipFreeAsync(buf, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (n_refined) *n_refined = hc;
   return CRV_OK;
 }

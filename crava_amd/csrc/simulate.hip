@@ -5,7 +5,9 @@
 // stream) or read from the decoded reference stream; per wavenumber the
 // 3 x 3 posterior covariance is factored with lib_matrCholCpx's rules and the
 // noise vector multiplied by L (lib_matrProdCholVec, lib_matr.c:453-470), or
-// zeroed when the factorisation is rejected (crava.cpp:1639-1651); the three
+// zeroed when the factorisation is rejected (crava.cpp:1639-1651).  With
+// fp64_chol the factor of FP32 covariances is computed in FP64 and refined
+// by one Newton step before it scales the noise.  The three
 // inverse transforms then add the posterior means in their last pass
 // (FFTGrid::add, fftgrid.cpp:1870-1891).
 #include "internal.hpp"
@@ -13,6 +15,7 @@
 #include "noise.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 namespace {
@@ -30,54 +33,163 @@ struct SimArgs {
   const cplx_t<T>* host[3];  // host-supplied noise (uploaded)
   unsigned long long seed, real;
   long long nc;
+  long long rstride;         // batches: complex elements between realizations (blockIdx.y)
 };
 
-template <typename T, typename U>
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  // hardware estimate and two Newton steps
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+
+// 3 x 3 Cholesky with lib_matrCholCpx's acceptance rule (factor A00 > 0,
+// scaled pivots > 1e-20) without divisions; P and L packed lower
+// (00, 10, 11, 20, 21, 22).
+__device__ __forceinline__ bool chol3_f64(const double2* P, double2* L) {
+  using D = double2;
+  const double factor = P[0].x;
+  if (factor <= 0.0) return false;
+  const double tol = kCholTol * factor;
+  const double p0 = P[0].x;
+  if (p0 <= tol) return false;
+  const double r0 = rsqrt_f64(p0);
+  L[0] = cmake<D>(p0 * r0, 0.0);
+  L[1] = cscale(P[1], r0);
+  const double p1 = P[2].x - (L[1].x * L[1].x + L[1].y * L[1].y);
+  if (p1 <= tol) return false;
+  const double r1 = rsqrt_f64(p1);
+  L[2] = cmake<D>(p1 * r1, 0.0);
+  L[3] = cscale(P[3], r0);
+  L[4] = cscale(csub(P[4], cmulc(L[3], L[1])), r1);
+  const double p2 = P[5].x - (L[3].x * L[3].x + L[3].y * L[3].y) - (L[4].x * L[4].x + L[4].y * L[4].y);
+  if (p2 <= tol) return false;
+  L[5] = cmake<D>(p2 * rsqrt_f64(p2), 0.0);
+  return true;
+}
+
+// One step of iterative refinement of a Cholesky factor (a Newton step for
+// L L^H = P): E = P - L L^H, X = L^-1 E L^-H, L <- L + L Phi(X), where
+// Phi keeps the strict lower triangle of X and half its diagonal.
+__device__ __forceinline__ void chol3_refine(const double2* P, double2* L) {
+  using D = double2;
+  auto at = [&](int i, int j) -> D { return L[i * (i + 1) / 2 + j]; };
+  // E, full Hermitian 3 x 3 from the lower part
+  D E[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      D x = P[i * (i + 1) / 2 + j];
+#pragma unroll
+      for (int k = 0; k <= j; ++k) x = csub(x, cmulc(at(i, k), at(j, k)));
+      if (i == j) x.y = 0.0;
+      E[i][j] = x;
+      E[j][i] = cconj(x);
+    }
+  const double r[3] = {1.0 / L[0].x, 1.0 / L[2].x, 1.0 / L[5].x};
+  // Y = L^-1 E (columns), then X = L^-1 Y^H
+  D Y[3][3], X[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      D x = E[i][c];
+#pragma unroll
+      for (int k = 0; k < i; ++k) x = csub(x, cmul(at(i, k), Y[k][c]));
+      Y[i][c] = cscale(x, r[i]);
+    }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      D x = cconj(Y[c][i]);
+#pragma unroll
+      for (int k = 0; k < i; ++k) x = csub(x, cmul(at(i, k), X[k][c]));
+      X[i][c] = cscale(x, r[i]);
+    }
+  D dL[6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      D x = cmake<D>(0.0, 0.0);
+#pragma unroll
+      for (int k = j; k <= i; ++k) {
+        const D phi = (k == j) ? cmake<D>(0.5 * X[j][j].x, 0.0) : X[k][j];
+        x = cadd(x, cmul(at(i, k), phi));
+      }
+      dL[i * (i + 1) / 2 + j] = x;
+    }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) L[q] = cadd(L[q], dL[q]);
+  L[0].y = 0.0; L[2].y = 0.0; L[5].y = 0.0;
+}
+
+// F64: factor in FP64 with one refinement step (FP32 grids, refine mode);
+// otherwise the reference's factorisation in the grids' precision (without
+// divisions for FP64 grids).
+template <typename T, bool F64>
 __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
   using C = cplx_t<T>;
+  using U = typename std::conditional<F64, double, T>::type;
   using CU = cplx_t<U>;
   const long long stride = (long long)gridDim.x * kThreads;
+  const long long roff = (long long)blockIdx.y * a.rstride;
+  const unsigned long long real = a.real + blockIdx.y;
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < a.nc; i += stride) {
-    CU P[6];
     const C c0 = a.cov[0][i], c1 = a.cov[1][i], c2 = a.cov[2][i];
     const C c3 = a.cov[3][i], c4 = a.cov[4][i], c5 = a.cov[5][i];
     // lower triangle packed: P00, P10, P11, P20, P21, P22 (lower = conj(upper))
+    CU P[6];
     P[0] = cmake<CU>((U)c0.x, (U)c0.y);
     P[1] = cmake<CU>((U)c3.x, -(U)c3.y);
     P[2] = cmake<CU>((U)c1.x, (U)c1.y);
     P[3] = cmake<CU>((U)c4.x, -(U)c4.y);
     P[4] = cmake<CU>((U)c5.x, -(U)c5.y);
     P[5] = cmake<CU>((U)c2.x, (U)c2.y);
-    U dmin, dmax;
-    const bool ok = chol_ref<U, 3>(P, dmin, dmax);
-    CU z[3];
+    CU Lf[6];
+    bool ok;
+    if constexpr (std::is_same<U, double>::value) {
+      ok = chol3_f64(P, Lf);
+      if (F64 && ok) chol3_refine(P, Lf);
+    } else {
+      U dmin, dmax;
 #pragma unroll
-    for (int g = 0; g < 3; ++g) {
-      double re = 0.0, im = 0.0;
-      if (ok) {
-        if (a.mode == CRV_NOISE_COUNTER) {
-          noise_counter_value(a.geo, a.seed, a.real, g, i, re, im);
-        } else if (a.mode == CRV_NOISE_REFERENCE) {
-          noise_ref_value(a.geo, a.normals + g * a.draws, i, re, im);
-        } else {
-          const C h = a.host[g][i];
-          re = (double)h.x; im = (double)h.y;
+      for (int q = 0; q < 6; ++q) Lf[q] = P[q];
+      ok = chol_ref<U, 3>(Lf, dmin, dmax);
+    }
+    C o[3] = {cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0)};
+    if (ok) {
+      T re[3], im[3];
+      if (a.mode == CRV_NOISE_COUNTER) {
+        sim_noise3<T>(a.geo, a.seed, real, i, re, im);
+      } else {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          if (a.mode == CRV_NOISE_REFERENCE) {
+            double r_, i_;
+            noise_ref_value(a.geo, a.normals + g * a.draws, i, r_, i_);
+            re[g] = (T)r_; im[g] = (T)i_;
+          } else {
+            const C h = a.host[g][i];
+            re[g] = h.x; im[g] = h.y;
+          }
         }
       }
-      z[g] = cmake<CU>((U)re, (U)im);
-    }
-    CU o[3];
+      CU z[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      CU x = cmake<CU>((U)0, (U)0);
-      if (ok) {
+      for (int g = 0; g < 3; ++g) z[g] = cmake<CU>((U)re[g], (U)im[g]);
 #pragma unroll
-        for (int c = 0; c <= r; ++c) x = cadd(x, cmul(P[r * (r + 1) / 2 + c], z[c]));
+      for (int r = 0; r < 3; ++r) {
+        CU x = cmake<CU>((U)0, (U)0);
+#pragma unroll
+        for (int c = 0; c <= r; ++c) x = cadd(x, cmul(Lf[r * (r + 1) / 2 + c], z[c]));
+        o[r] = cmake<C>((T)x.x, (T)x.y);
       }
-      o[r] = x;
     }
 #pragma unroll
-    for (int g = 0; g < 3; ++g) a.out[g][i] = cmake<C>((T)o[g].x, (T)o[g].y);
+    for (int g = 0; g < 3; ++g) a.out[g][roff + i] = o[g];
   }
 }
 
@@ -189,7 +301,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     a.geo = make_noise_geom(plan); a.mode = noise->mode; a.normals = ctx->normals; a.draws = draws;
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const double2*)((char*)hostbuf + g * gb) : nullptr;
     a.seed = noise->seed; a.real = noise->realization; a.nc = nc;
-    hipLaunchKernelGGL((k_simulate<double, double>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   } else {
     SimArgs<float> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q]->d;
@@ -198,9 +310,9 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const float2*)((char*)hostbuf + g * gb) : nullptr;
     a.seed = noise->seed; a.real = noise->realization; a.nc = nc;
     if (fp64_chol)
-      hipLaunchKernelGGL((k_simulate<float, double>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
     else
-      hipLaunchKernelGGL((k_simulate<float, float>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   }
   }
   CRV_HIP_CHECK(ctx, hipGetLastError());
@@ -227,5 +339,72 @@ extern "C" int crv_fill_complex_noise(crv_grid* g, unsigned int* lcg_state) {
   if ((rc = crv_launch_fill_noise(ctx, plan, g, 0)) != CRV_OK) return rc;
   g->transformed = true;
   g->cubetype = CRV_PARAMETER;
+  return CRV_OK;
+}
+
+extern "C" int crv_simulate_batch(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[3], unsigned long long seed,
+                                  unsigned long long realization0, int nreal, int fp64_chol, crv_grid** out) {
+  if (!ctx) return CRV_E_INVALID;
+  if (!cov || !out || nreal < 1 || nreal > 65535) return crv_set_error(ctx, CRV_E_INVALID, "simulate_batch: bad args");
+  crv_grid* g0 = cov[0];
+  if (!g0) return crv_set_error(ctx, CRV_E_INVALID, "cov[0] is NULL");
+  auto same = [&](const crv_grid* g) {
+    return g && g->nxp == g0->nxp && g->nyp == g0->nyp && g->nzp == g0->nzp && g->precision == g0->precision;
+  };
+  for (int q = 0; q < 6; ++q) {
+    if (!same(cov[q])) return crv_set_error(ctx, CRV_E_INVALID, "cov grids inconsistent");
+    if (!cov[q]->transformed) return crv_set_error(ctx, CRV_E_STATE, "cov grid not transformed");
+  }
+  const size_t eb = g0->elem_bytes();
+  const int nout = 3 * nreal;
+  for (int b = 0; b < nout; ++b) {
+    if (!same(out[b])) return crv_set_error(ctx, CRV_E_INVALID, "out grids inconsistent");
+    if ((char*)out[b]->d != (char*)out[0]->d + (size_t)b * out[0]->rsize * eb)
+      return crv_set_error(ctx, CRV_E_INVALID, "simulate_batch: out must be one contiguous grid batch");
+  }
+  const void* add[3] = {nullptr, nullptr, nullptr};
+  int nadd = 0;
+  if (post_mean) {
+    for (int g = 0; g < 3; ++g) {
+      if (!post_mean[g]) { nadd = 0; break; }
+      if (!same(post_mean[g])) return crv_set_error(ctx, CRV_E_INVALID, "post_mean grids inconsistent");
+      if (post_mean[g]->transformed) return crv_set_error(ctx, CRV_E_STATE, "post_mean must be real (untransformed)");
+      add[g] = post_mean[g]->d;
+      nadd = 3;
+    }
+  }
+  FftPlan* plan = nullptr;
+  int rc = crv_get_plan(ctx, g0->nxp, g0->nyp, g0->nzp, g0->precision, &plan);
+  if (rc != CRV_OK) return rc;
+  const long long nc = (long long)g0->csize;
+  const long long gstride = (long long)out[0]->rsize / 2;  // complex elements per grid
+  const int nb = (int)std::min<long long>((nc + kThreads - 1) / kThreads, 2048);
+  {
+    ProfScope ps(ctx, "simulate_scale_batch");
+    if (g0->precision == CRV_FP64) {
+      SimArgs<double> a{};
+      for (int q = 0; q < 6; ++q) a.cov[q] = (const double2*)cov[q]->d;
+      for (int g = 0; g < 3; ++g) a.out[g] = (double2*)out[0]->d + g * gstride;
+      a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
+      a.seed = seed; a.real = realization0; a.nc = nc; a.rstride = 3 * gstride;
+      hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+    } else {
+      SimArgs<float> a{};
+      for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q]->d;
+      for (int g = 0; g < 3; ++g) a.out[g] = (float2*)out[0]->d + g * gstride;
+      a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
+      a.seed = seed; a.real = realization0; a.nc = nc; a.rstride = 3 * gstride;
+      if (fp64_chol)
+        hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+      else
+        hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+    }
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  for (int b = 0; b < nout; ++b) { out[b]->cubetype = CRV_PARAMETER; out[b]->transformed = true; }
+  const double sc = crv_scale_inverse(out[0]);
+  if ((rc = crv_launch_c2r_batch(ctx, plan, out[0]->d, nout, (long long)out[0]->rsize, sc, add, nadd)) != CRV_OK)
+    return rc;
+  for (int b = 0; b < nout; ++b) out[b]->transformed = false;
   return CRV_OK;
 }

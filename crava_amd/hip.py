@@ -282,3 +282,35 @@ def find_closest_factorable(n: int) -> int:
 
 def padding_size(n: int, frac: float) -> int:
     return int(L.lib().crv_padding_size(n, frac))
+
+
+# rock physics constants of the learned-surrogate workload: quartz and clay
+# moduli in kPa, densities in g/cm3, fluids at 80 C
+ROCK_DEFAULTS = dict(k_quartz=37.0e6, g_quartz=44.0e6, rho_quartz=2.65, k_clay=21.0e6, g_clay=7.0e6,
+                     rho_clay=2.58, temperature=80.0, salinity=0.035, gas_gravity=0.6)
+
+
+def rock_forward_device(ctx: Context, params: dict, d_in: int, d_out: int, n: int, out_fp32: bool = False,
+                        d_status: int = 0):
+    """crv_rock_forward on device buffers (addresses as ints)."""
+    p = L.RockParams(**params)
+    ctx.check(ctx.lib.crv_rock_forward(ctx.h, C.byref(p), int(n), C.c_void_p(d_in), C.c_void_p(d_out),
+                                       int(out_fp32), C.c_void_p(d_status) if d_status else None),
+              "rock_forward")
+
+
+def rock_forward(ctx: Context, x: np.ndarray, params: dict | None = None, out_fp32: bool = False):
+    """(n, 5) inputs (porosity, clay, Sw, aspect ratio, effective pressure MPa)
+    -> ((n, 3) ln vp, ln vs, ln rho; (n,) status).  Device memory via torch."""
+    import torch
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.shape[0]
+    dev = torch.device("cuda", 0)
+    d_in = torch.from_numpy(x.ravel()).to(dev)
+    d_out = torch.empty(n * 3, dtype=torch.float32 if out_fp32 else torch.float64, device=dev)
+    d_st = torch.zeros(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    rock_forward_device(ctx, params or ROCK_DEFAULTS, d_in.data_ptr(), d_out.data_ptr(), n, out_fp32,
+                        d_st.data_ptr())
+    ctx.synchronize()
+    return d_out.cpu().numpy().reshape(n, 3), d_st.cpu().numpy()

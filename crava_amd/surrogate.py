@@ -1,0 +1,306 @@
+"""Rock-physics surrogate: training on the GPU, error-gated inference with
+the rplib kernel as fallback, and facies-conditional sample generation.
+
+The workload (5 -> 128 -> 128 -> 128 -> 3 tanh MLP of porosity, clay
+fraction, water saturation, pore aspect ratio and effective pressure ->
+ln vp, ln vs, ln rho) stands in for the per-sample rplib chain that
+DistributionsRockDEM runs when faciesprob generates synthetic elastic
+samples (reference rplib/distributionsrockdem.cpp:125-156,
+src/faciesprob.cpp:2494-2559).  crv_rock_forward, the GPU port of that chain,
+produces the training targets, judges the network's error and re-evaluates
+every sample the error gate does not trust.
+
+Device memory is held in torch tensors (plumbing); every computation is a
+crv_* kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .hip import ROCK_DEFAULTS, Context, rock_forward_device
+
+# input box of the workload: porosity U(0.05, 0.35), clay Beta(2, 5),
+# Sw U(0, 1), aspect ratio log-uniform (0.01, 1), effective pressure U(10, 40) MPa
+IN_LO = (0.05, 0.0, 0.0, -2.0, 10.0)
+IN_HI = (0.35, 1.0, 1.0, 0.0, 40.0)
+IN_LOG_MASK = 1 << 3
+
+
+def sample_inputs(n: int, rng: np.random.Generator, facies: str | None = None) -> np.ndarray:
+    """(n, 5) float32 inputs.  facies None draws the training box; 'shale',
+    'brine_sand' and 'gas_sand' narrow clay and saturation to the facies."""
+    phi = rng.uniform(0.05, 0.35, n)
+    clay = rng.beta(2, 5, n)
+    sw = rng.uniform(0, 1, n)
+    asp = 10.0 ** rng.uniform(-2, 0, n)
+    pe = rng.uniform(10, 40, n)
+    if facies == "shale":
+        clay = 0.5 + 0.5 * rng.beta(4, 2, n)
+        sw = np.ones(n)
+        phi = rng.uniform(0.05, 0.2, n)
+        asp = 10.0 ** rng.uniform(-2, -1, n)
+    elif facies == "brine_sand":
+        clay = 0.25 * rng.beta(2, 5, n)
+        sw = np.ones(n)
+        phi = rng.uniform(0.15, 0.35, n)
+        asp = 10.0 ** rng.uniform(-0.7, 0, n)
+    elif facies == "gas_sand":
+        clay = 0.25 * rng.beta(2, 5, n)
+        sw = rng.uniform(0.05, 0.4, n)
+        phi = rng.uniform(0.15, 0.35, n)
+        asp = 10.0 ** rng.uniform(-0.7, 0, n)
+    return np.column_stack([phi, clay, sw, asp, pe]).astype(np.float32)
+
+
+def make_norm(out_mean, out_std) -> L.MlpNorm:
+    nm = L.MlpNorm()
+    for f in range(5):
+        nm.in_lo[f] = IN_LO[f]
+        nm.in_scale[f] = 2.0 / (IN_HI[f] - IN_LO[f])
+    nm.in_log_mask = IN_LOG_MASK
+    for c in range(3):
+        nm.out_mean[c] = float(out_mean[c])
+        nm.out_std[c] = float(out_std[c])
+    return nm
+
+
+def norm_dict(nm: L.MlpNorm) -> dict:
+    return dict(in_lo=list(nm.in_lo), in_scale=list(nm.in_scale), in_log_mask=nm.in_log_mask,
+                out_mean=list(nm.out_mean), out_std=list(nm.out_std))
+
+
+def _dev():
+    import torch
+    return torch.device("cuda", 0)
+
+
+class MLP:
+    """crv_mlp handle."""
+
+    def __init__(self, ctx: Context, norm: L.MlpNorm, max_batch: int):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.norm = norm
+        self.max_batch = int(max_batch)
+        h = C.c_void_p()
+        ctx.check(self.lib.crv_mlp_create(ctx.h, C.byref(norm), self.max_batch, C.byref(h)), "mlp_create")
+        self.h = h
+        self.nparams = int(self.lib.crv_mlp_param_count())
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        self.ctx.check(self.lib.crv_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))), "mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.nparams, np.float32)
+        self.ctx.check(self.lib.crv_mlp_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))), "mlp_get_params")
+        return p
+
+    def gradients(self, d_x, d_t, n: int):
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.lib.crv_mlp_gradients(self.h, C.c_void_p(d_x), C.c_void_p(d_t), int(n),
+                                                  g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)),
+                       "mlp_gradients")
+        return float(loss.value), g
+
+    def train_step(self, d_x, d_t, n: int, lr: float, step: int, want_loss: bool = False):
+        loss = C.c_float()
+        self.ctx.check(self.lib.crv_mlp_train_step(self.h, C.c_void_p(d_x), C.c_void_p(d_t), int(n), float(lr),
+                                                   int(step), C.byref(loss) if want_loss else None), "mlp_train_step")
+        return float(loss.value) if want_loss else None
+
+    def predict(self, d_x, n: int, d_y):
+        self.ctx.check(self.lib.crv_mlp_predict(self.h, C.c_void_p(d_x), int(n), C.c_void_p(d_y)), "mlp_predict")
+
+    def flops(self):
+        f, b = C.c_double(), C.c_double()
+        self.ctx.check(self.lib.crv_mlp_flops(self.h, C.byref(f), C.byref(b)), "mlp_flops")
+        return f.value, b.value
+
+    def close(self):
+        if self.h:
+            self.lib.crv_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rplib_targets(ctx: Context, d_x, n: int, params: dict | None = None):
+    """ln vp, ln vs, ln rho of device inputs through crv_rock_forward (float
+    outputs) and the per-sample status."""
+    import torch
+    x64 = d_x.to(torch.float64)
+    y = torch.empty((n, 3), dtype=torch.float32, device=_dev())
+    st = torch.zeros(n, dtype=torch.int32, device=_dev())
+    torch.cuda.synchronize()
+    rock_forward_device(ctx, params or ROCK_DEFAULTS, x64.data_ptr(), y.data_ptr(), n, True, st.data_ptr())
+    ctx.synchronize()
+    return y, st
+
+
+@dataclass
+class TrainResult:
+    mlp: MLP
+    history: list = field(default_factory=list)
+    seconds: float = 0.0
+    steps: int = 0
+    train_rows: int = 0
+    mfma_tflops: float = 0.0
+    mfma_util: float = 0.0
+
+
+def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int = 16384, lr: float = 2e-3,
+          seed: int = 7, n_val: int = 200_000, params: dict | None = None, log=None) -> TrainResult:
+    """Generate n_train + n_val rplib samples on the GPU (crv_rock_forward)
+    and fit the MLP with Adam, cosine learning-rate decay and reshuffled
+    mini-batches.  Returns the trained network with loss history, wall
+    time and the matrix-core rate of the training kernels."""
+    import torch
+    dev = _dev()
+    rng = np.random.default_rng(seed)
+    x = sample_inputs(n_train + n_val, rng)
+    d_x = torch.from_numpy(x).to(dev)
+    d_y, st = rplib_targets(ctx, d_x, x.shape[0], params)
+    keep = (st == 0)
+    d_x, d_y = d_x[keep].contiguous(), d_y[keep].contiguous()
+    n_all = d_x.shape[0]
+    n_tr = n_all - n_val
+    xtr, ytr = d_x[:n_tr].contiguous(), d_y[:n_tr].contiguous()
+    yh = ytr.double().cpu().numpy()
+    nm = make_norm(yh.mean(0), yh.std(0))
+    mlp = MLP(ctx, nm, batch)
+    mlp.set_params(_init_params(seed))
+    per_epoch = n_tr // batch
+    total = max(1, epochs * per_epoch)
+    res = TrainResult(mlp=mlp)
+    step = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = torch.randperm(n_tr, device=dev)
+        xs, ys = xtr[perm].contiguous(), ytr[perm].contiguous()
+        torch.cuda.synchronize()
+        loss = None
+        for b in range(per_epoch):
+            step += 1
+            lr_t = 1e-5 + (lr - 1e-5) * 0.5 * (1.0 + math.cos(math.pi * step / total))
+            last = b == per_epoch - 1
+            loss = mlp.train_step(xs[b * batch:].data_ptr(), ys[b * batch:].data_ptr(), batch, lr_t, step,
+                                  want_loss=last)
+        res.history.append((ep, loss))
+        if log:
+            log(f"[surrogate] epoch {ep:3d} loss {loss:.3e}")
+    ctx.synchronize()
+    res.seconds = time.perf_counter() - t0
+    res.steps = step
+    res.train_rows = step * batch
+    # matrix-core rate of one profiled step
+    ctx.profile(True)
+    mlp.train_step(xtr.data_ptr(), ytr.data_ptr(), batch, 1e-5, step + 1)
+    ctx.profile(False)
+    rep = ctx.profile_report()
+    ms = sum(v[1] for k, v in rep.items() if k.startswith("mlp_") and k not in ("mlp_reduce", "mlp_adam"))
+    f, b_ = mlp.flops()
+    if ms > 0:
+        res.mfma_tflops = (f + b_) / (ms * 1e-3) / 1e12
+        res.mfma_util = res.mfma_tflops / 2500.0
+    res.val_x, res.val_y = d_x[n_tr:].contiguous(), d_y[n_tr:].contiguous()
+    return res
+
+
+def _init_params(seed: int) -> np.ndarray:
+    """Glorot-uniform weights, zero biases and padding, in the augmented layout."""
+    rng = np.random.default_rng(seed)
+    HID, KA, K1, NO = 128, 144, 16, 32
+    W1, W2, W3, W4 = (np.zeros((HID, K1)), np.zeros((HID, KA)), np.zeros((HID, KA)), np.zeros((NO, KA)))
+    W1[:, :5] = rng.uniform(-1, 1, (HID, 5)) * math.sqrt(6.0 / (5 + HID))
+    W2[:, :HID] = rng.uniform(-1, 1, (HID, HID)) * math.sqrt(6.0 / (2 * HID))
+    W3[:, :HID] = rng.uniform(-1, 1, (HID, HID)) * math.sqrt(6.0 / (2 * HID))
+    W4[:3, :HID] = rng.uniform(-1, 1, (3, HID)) * math.sqrt(6.0 / (HID + 3))
+    return np.concatenate([W1.ravel(), W2.ravel(), W3.ravel(), W4.ravel()]).astype(np.float32)
+
+
+def predict(ctx: Context, mlp: MLP, d_x):
+    """Network outputs (device (n, 3) float) for device inputs, in chunks of
+    the handle's batch size."""
+    import torch
+    n = d_x.shape[0]
+    d_y = torch.empty((n, 3), dtype=torch.float32, device=d_x.device)
+    for a in range(0, n, mlp.max_batch):
+        b = min(n, a + mlp.max_batch)
+        mlp.predict(d_x[a:].data_ptr(), b - a, d_y[a:].data_ptr())
+    return d_y
+
+
+def validation_errors(ctx: Context, mlp: MLP, d_x, d_y) -> np.ndarray:
+    """(n, 3) network minus rplib on held-out samples (host)."""
+    p = predict(ctx, mlp, d_x)
+    ctx.synchronize()
+    return (p - d_y).cpu().numpy().astype(np.float64)
+
+
+def input_cells(x: np.ndarray, nb: int) -> np.ndarray:
+    """Error-table cell of each (n, 5) input, as the gate kernel forms it."""
+    x = np.asarray(x, np.float32).copy()
+    cell = np.zeros(x.shape[0], np.int64)
+    for f in range(5):
+        v = x[:, f]
+        if IN_LOG_MASK & (1 << f):
+            v = np.log10(v)
+        lo, sc = np.float32(IN_LO[f]), np.float32(2.0 / (IN_HI[f] - IN_LO[f]))
+        t = ((v - lo) * sc) * np.float32(0.5)
+        b = np.clip(np.floor(t * np.float32(nb)).astype(np.int64), 0, nb - 1)
+        cell = cell * nb + b
+    return cell
+
+
+class ErrorTable:
+    """Largest |ln vp| error of the network seen on held-out samples, per
+    cell of an nb^5 grid over the normalised inputs; +inf where fewer than
+    min_count samples fell."""
+
+    def __init__(self, values: np.ndarray, nb: int):
+        import torch
+        self.nb = nb
+        self.values = np.asarray(values, np.float32)
+        self.d = torch.from_numpy(self.values).to(_dev())
+
+    @classmethod
+    def build(cls, ctx: Context, mlp: MLP, d_x, d_y, nb: int = 8, min_count: int = 3) -> "ErrorTable":
+        err = np.abs(validation_errors(ctx, mlp, d_x, d_y)[:, 0])
+        cell = input_cells(d_x.cpu().numpy(), nb)
+        vmax = np.zeros(nb ** 5)
+        cnt = np.zeros(nb ** 5, np.int64)
+        np.maximum.at(vmax, cell, err)
+        np.add.at(cnt, cell, 1)
+        vmax[cnt < min_count] = np.inf
+        return cls(vmax, nb)
+
+
+def predict_checked(ctx: Context, mlp: MLP, table: ErrorTable, d_x, thresh: float = 0.005,
+                    params: dict | None = None):
+    """Network outputs with every sample whose table cell predicts an ln vp
+    error above thresh recomputed by the rplib kernel.  Returns the device
+    outputs and the number of recomputed samples."""
+    import torch
+    n = d_x.shape[0]
+    d_y = predict(ctx, mlp, d_x)
+    torch.cuda.synchronize()
+    p = L.RockParams(**(params or ROCK_DEFAULTS))
+    nref = C.c_longlong()
+    ctx.check(ctx.lib.crv_rock_refine(ctx.h, C.byref(p), n, C.c_void_p(d_x.data_ptr()), C.c_void_p(d_y.data_ptr()),
+                                      C.byref(mlp.norm), C.c_void_p(table.d.data_ptr()), table.nb, float(thresh),
+                                      C.byref(nref)), "rock_refine")
+    return d_y, int(nref.value)

@@ -96,7 +96,47 @@ int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
 #define CRV_TUNE_TX_Y     2
 #define CRV_TUNE_TX_Z     3
 #define CRV_TUNE_ROWS     4
+/*   CRV_TUNE_POST_BLOCKS  posterior blocks (256 wavenumber-threads) per CU,
+ *                      0 = as many as fit                                  */
+#define CRV_TUNE_POST_BLOCKS 5
 int crv_ctx_tune(crv_ctx* ctx, int key, int value);
+
+/* Autotuner for one grid size: times the candidate launch shapes of each
+ * pass (x rows per tile, y and z pencil tile widths, and the order of the
+ * radix stages where a length has no two-stage kernel) with the transforms
+ * themselves on scratch memory, and keeps the fastest in the size's plan.
+ * shape (may be NULL) receives the choice.  A later crv_ctx_tune call
+ * resets every plan to the defaults.                                        */
+typedef struct crv_fft_shape {
+  int rows_per_tile, tx_y, tx_z;
+  int radix_order[3];            /* x, y, z: 0 default, 1 reversed           */
+  double ms_r2c, ms_c2r;         /* one transform with the chosen shapes    */
+} crv_fft_shape;
+int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int precision, int reps, crv_fft_shape* shape);
+
+/* ---- batches and graphs --------------------------------------------------
+ * count grids of one size in one allocation, grids[i] i rsize elements
+ * after grids[0] (each handle is destroyed on its own; the memory goes with
+ * the last one).  Batched entry points run one launch per pass over the
+ * whole batch.                                                              */
+int crv_grid_create_batch(crv_ctx* ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int precision,
+                          int cubetype, int count, crv_grid** grids);
+/* HIP-graph capture of the context's stream: calls between begin and end
+ * are recorded (kernel launches only: plans must exist, see
+ * crv_posterior_exec and the batched simulation), then replayed with
+ * crv_graph_launch.                                                         */
+/* nreal realizations of one posterior with the counter noise stream (seed,
+ * realization0 + r): out is a contiguous batch of 3 nreal grids, out[3r+g]
+ * the realization r of parameter g (posterior means added when post_mean is
+ * given); one launch for the noise and scaling of all realizations, one per
+ * inverse-transform pass.  Launches only: capturable.                      */
+int crv_simulate_batch(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[3], unsigned long long seed,
+                       unsigned long long realization0, int nreal, int fp64_chol, crv_grid** out);
+typedef struct crv_graph crv_graph;
+int crv_graph_begin(crv_ctx* ctx);
+int crv_graph_end(crv_ctx* ctx, crv_graph** out);
+int crv_graph_launch(crv_graph* graph);
+int crv_graph_destroy(crv_graph* graph);
 
 /* ---- grids --------------------------------------------------------------
  * crv_grid_create replaces FFTGrid::FFTGrid + createRealGrid
@@ -224,6 +264,20 @@ int crv_posterior(crv_ctx* ctx, const crv_post_params* p,
                   crv_grid* mean[3], crv_grid** data, crv_grid* err_corr,
                   crv_grid* cov[6], crv_post_stats* stats);
 
+/* The same update split for batches and HIP-graph capture: a plan holds the
+ * per-kz tables on the device for one grid size and precision;
+ * crv_posterior_exec only launches kernels (no allocation, no host
+ * synchronisation) on inputs that are already transformed and leaves the
+ * posterior means as spectra; crv_post_plan_stats reads the counters of the
+ * last exec (synchronises).  Execs of one plan must not overlap.          */
+typedef struct crv_post_plan crv_post_plan;
+int crv_post_plan_create(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp, int precision,
+                         crv_post_plan** out);
+int crv_post_plan_destroy(crv_post_plan* plan);
+int crv_posterior_exec(crv_post_plan* plan, crv_grid* mean[3], crv_grid** data, crv_grid* err_corr,
+                       crv_grid* cov[6]);
+int crv_post_plan_stats(crv_post_plan* plan, crv_post_stats* stats);
+
 /* ---- simulation ---------------------------------------------------------
  * crv_simulate replaces one realization of Crava::simulate
  * (src/crava.cpp:1550-1734, loop body 1599-1708): complex white noise as
@@ -321,6 +375,52 @@ typedef struct crv_rock_params {
 
 int crv_rock_forward(crv_ctx* ctx, const crv_rock_params* p, long long n, const double* d_in, void* d_out,
                      int out_fp32, int* d_status);
+
+/* ---- rock-physics surrogate ---------------------------------------------
+ * A 5 -> 128 -> 128 -> 128 -> 3 tanh MLP standing in for crv_rock_forward
+ * when facies-conditional elastic samples are generated in bulk (the role
+ * rplib plays for faciesprob: rplib/distributionsrockdem.cpp:125-156,
+ * src/faciesprob.cpp:2494-2559).  Training and inference run on BF16 matrix
+ * cores with FP32 accumulation; FP32 master weights and Adam moments.
+ * Inputs are crv_rock_forward's (porosity, clay fraction, Sw, aspect ratio,
+ * effective pressure MPa) as float; features with bit f of in_log_mask set
+ * are log10'd; then x~ = (x - in_lo) * in_scale - 1.  Outputs are the
+ * normalised (ln vp, ln vs, ln rho): y = y~ * out_std + out_mean.  The loss
+ * is the mean squared error in normalised units.
+ * Parameters (crv_mlp_param_count() floats): W1a [128][16] (bias in column
+ * 5), W2a and W3a [128][144] (bias in column 128), W4a [32][144] (rows 0..2
+ * used, bias in column 128), row-major, concatenated; padding stays zero.
+ * Device pointers: d_x n x 5, d_t / d_y n x 3 floats.                      */
+typedef struct crv_mlp crv_mlp;
+typedef struct crv_mlp_norm {
+  float in_lo[5], in_scale[5];
+  int   in_log_mask;
+  float out_mean[3], out_std[3];
+} crv_mlp_norm;
+long long crv_mlp_param_count(void);
+int crv_mlp_create(crv_ctx* ctx, const crv_mlp_norm* norm, long long max_batch, crv_mlp** out);
+int crv_mlp_destroy(crv_mlp* h);
+/* host params -> device (Adam moments reset) / device -> host */
+int crv_mlp_set_params(crv_mlp* h, const float* params);
+int crv_mlp_get_params(crv_mlp* h, float* params);
+/* gradient of the loss at the current parameters (host copy; either
+ * output may be NULL); no update */
+int crv_mlp_gradients(crv_mlp* h, const float* d_x, const float* d_t, long long n, float* grad, float* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on a batch; step counts from 1 */
+int crv_mlp_train_step(crv_mlp* h, const float* d_x, const float* d_t, long long n, float lr, int step,
+                       float* loss);
+int crv_mlp_predict(crv_mlp* h, const float* d_x, long long n, float* d_y);
+/* matrix-core flops of the last forward and backward launches (padded shapes) */
+int crv_mlp_flops(crv_mlp* h, double* fwd, double* bwd);
+/* Error gate of the surrogate with rplib as fallback: every sample whose
+ * cell of the error table predicts an error above thresh is recomputed with
+ * the crv_rock_forward chain and its outputs overwritten (NaN when the DEM
+ * integration fails).  The table has nb^5 cells over the normalised inputs
+ * (cell index sum_f b_f nb^(4-f), b_f = floor(nb (x~_f + 1) / 2) clamped),
+ * a cell's value being the largest error seen there (+inf when never
+ * seen).  d_x n x 5, d_y n x 3 floats, d_table nb^5 floats, device memory. */
+int crv_rock_refine(crv_ctx* ctx, const crv_rock_params* p, long long n, const float* d_x, float* d_y,
+                    const crv_mlp_norm* norm, const float* d_table, int nb, float thresh, long long* n_refined);
 
 /* ---- padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622) -- */
 /* Smallest 2^a 3^b 5^c 7^d >= leastint (FFTGrid::findClosestFactorableNumber). */

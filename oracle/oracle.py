@@ -275,7 +275,9 @@ def philox10(c0, c1, c2, c3, k0, k1):
 
 
 def counter_noise(nxp, nyp, nzp, seed, realization, grid):
-    """Counter-stream noise laid out with the reference Hermitian rules."""
+    """Counter-stream simulation noise of one grid (0..2) laid out with the
+    reference Hermitian rules (FFTGrid::fillInComplexNoise,
+    src/fftgrid.cpp:1191-1240)."""
     cnxp = nxp // 2 + 1
     n = cnxp * nyp * nzp
     i = np.arange(n, dtype=np.int64)
@@ -290,12 +292,15 @@ def counter_noise(nxp, nyp, nzp, seed, realization, grid):
     src = np.where(special & (jkcc < jk), jkcc * cnxp + x, i)
 
     def pair(idx):
-        hi = (np.uint64(realization >> 32) & np.uint64(0xFFFFFF)) | np.uint64(grid << 24)
+        # call tag grid >> 1, words 2 (grid & 1) and 2 (grid & 1) + 1,
+        # 23-bit uniforms (2 (w >> 9) + 1) 2^-24
+        hi = (np.uint64(realization >> 32) & np.uint64(0xFFFFFF)) | np.uint64((grid >> 1) << 24)
         c = philox10(idx.astype(np.uint64) & np.uint64(M32), idx.astype(np.uint64) >> np.uint64(32),
                      np.full(idx.shape, realization & M32, np.uint64), np.full(idx.shape, hi, np.uint64),
                      seed & M32, (seed >> 32) & M32)
-        u1 = (c[0].astype(np.float64) + 0.5) / 4294967296.0
-        u2 = (c[1].astype(np.float64) + 0.5) / 4294967296.0
+        wa, wb = c[2 * (grid & 1)], c[2 * (grid & 1) + 1]
+        u1 = ((wa >> np.uint64(9)) * np.uint64(2) + np.uint64(1)).astype(np.float64) / 16777216.0
+        u2 = ((wb >> np.uint64(9)) * np.uint64(2) + np.uint64(1)).astype(np.float64) / 16777216.0
         r = np.sqrt(-2.0 * np.log(u1))
         return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
 
@@ -309,3 +314,28 @@ def counter_noise(nxp, nyp, nzp, seed, realization, grid):
     copy = special & (jkcc < jk)
     im = np.where(copy, -im, im)
     return (re + 1j * im).reshape(nzp, nyp, cnxp)
+
+
+# ---------------------------------------------------------------- rplib chain
+class _RockParams(C.Structure):
+    _fields_ = [("kq", C.c_double), ("gq", C.c_double), ("rhoq", C.c_double), ("kc", C.c_double),
+                ("gc", C.c_double), ("rhoc", C.c_double), ("temp_c", C.c_double), ("salinity", C.c_double),
+                ("gas_gravity", C.c_double), ("gas", C.c_int)]
+
+
+def rock_forward(x, params):
+    """or_rock_forward_batch (rplib_oracle.c): (n, 5) inputs -> (n, 3) ln vp,
+    ln vs, ln rho and (n,) status; params use the crv_rock_params names."""
+    o = olib()
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.shape[0]
+    out = np.zeros((n, 3), np.float64)
+    st = np.zeros(n, np.int32)
+    p = _RockParams(params["k_quartz"], params["g_quartz"], params["rho_quartz"], params["k_clay"],
+                    params["g_clay"], params["rho_clay"], params["temperature"], params["salinity"],
+                    params["gas_gravity"], 0)
+    dp = C.POINTER(C.c_double)
+    o.or_rock_forward_batch.argtypes = [C.POINTER(_RockParams), C.c_long, dp, dp, C.POINTER(C.c_int)]
+    o.or_rock_forward_batch(C.byref(p), n, x.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                            st.ctypes.data_as(C.POINTER(C.c_int)))
+    return out, st

@@ -354,3 +354,67 @@ def test_gassmann_and_gas_limits():
     r2, k2 = C.c_double(), C.c_double()
     o.or_gas_props(80.0, 40.0, 0.6, C.byref(r2), C.byref(k2))
     assert r2.value > rho.value and k2.value > k.value
+
+
+def test_philox_known_answers():
+    """Philox-4x32-10 against the known-answer vectors published with
+    Random123 (Salmon et al., SC 2011)."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for c, k, want in kat:
+        assert tuple(int(x) for x in O.philox10(*c, *k)) == want
+
+
+def test_counter_noise_statistics():
+    """The simulation noise is N(0,1)-distributed per component (1/2 per
+    component of a complex (N, N)/sqrt(2) value), Hermitian-consistent, and
+    the three grids are independent."""
+    nx, ny, nz = 64, 48, 40
+    z = [O.counter_noise(nx, ny, nz, 42, 0, g) for g in range(3)]
+    for a in z:
+        # ~1.2e5 samples: 5 sigma of the mean and variance is 0.01
+        v = np.concatenate([a[:, :, 1:-1].real.ravel(), a[:, :, 1:-1].imag.ravel()])
+        assert abs(v.mean()) < 0.01 and abs(v.var() - 0.5) < 0.01
+        # x = 0 plane is Hermitian in (y, z): z[-k, -j, 0] == conj(z[k, j, 0])
+        h = a[:, :, 0]
+        hc = np.conj(h[(-np.arange(nz)) % nz][:, (-np.arange(ny)) % ny])
+        assert np.allclose(h, hc)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs(np.corrcoef(z[a][:, :, 1:-1].real.ravel(), z[b][:, :, 1:-1].real.ravel())[0, 1]) < 0.02
+
+
+def test_mlp_oracle_gradient_finite_differences():
+    """The float64 backward of the surrogate MLP oracle against central
+    differences of its forward, on a handful of parameters of every layer."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(3)
+    norm = dict(in_lo=[0.05, 0.0, 0.0, -2.0, 10.0], in_scale=[2 / 0.3, 2.0, 2.0, 1.0, 2 / 30.0], in_log_mask=8,
+                out_mean=[8.0, 7.2, 7.7], out_std=[0.2, 0.5, 0.05])
+    p = np.zeros(M.NPARAMS)
+    mask = M.trainable_mask().astype(bool)
+    p[mask] = rng.normal(0, 0.3, mask.sum())
+    x = np.column_stack([rng.uniform(0.05, 0.35, 64), rng.beta(2, 5, 64), rng.uniform(0, 1, 64),
+                         10 ** rng.uniform(-2, 0, 64), rng.uniform(10, 40, 64)])
+    t = np.column_stack([rng.normal(8.0, 0.2, 64), rng.normal(7.2, 0.5, 64), rng.normal(7.7, 0.05, 64)])
+    _, g = M.loss_grad(p, x, t, norm, mode="exact")
+    idx = np.nonzero(mask)[0]
+    for q in np.concatenate([idx[:3], rng.choice(idx, 20, replace=False), idx[-3:]]):
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[q] += h
+        pm[q] -= h
+        fd = (M.loss_grad(pp, x, t, norm, mode="exact")[0] - M.loss_grad(pm, x, t, norm, mode="exact")[0]) / (2 * h)
+        assert fd == pytest.approx(g[q], rel=1e-5, abs=1e-9)
+    # padding receives no gradient
+    assert np.all(g[~mask] == 0)
+
+
+def test_bf16_rounding():
+    from oracle import mlp_oracle as M
+    # bf16 keeps 7 mantissa bits: ulp(1) = 2^-7.  1 + 2^-8 and 1 + 3 2^-8
+    # are ties and go to the even neighbour (1 and 1 + 2^-6); 1 + 3 2^-9 is
+    # above the midpoint and rounds up
+    v = np.array([1.0, 1.00390625, 1.01171875, 1.001953125, 1.005859375, -3.14159265, 0.0], np.float32)
+    assert list(M.bf16(v)) == [1.0, 1.0, 1.015625, 1.0, 1.0078125, -3.140625, 0.0]
