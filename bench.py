@@ -37,16 +37,18 @@ def log(msg):
 
 
 def make_grids(ctx, s, prec):
+    """Each family of grids is one contiguous batch, so its transforms run
+    as one launch per pass."""
     from crava_amd import _lib as L
-    from crava_amd.hip import FFTGrid
-    mk = lambda ct: FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=prec, cubetype=ct, nx=s.nx, ny=s.ny, nz=s.nz)
+    from crava_amd.hip import grid_batch
+    mk = lambda n, ct: grid_batch(ctx, n, s.nxp, s.nyp, s.nzp, prec, ct, s.nx, s.ny, s.nz)
     return dict(
-        mean_in=[mk(L.CRV_PARAMETER) for _ in range(3)],
-        data_in=[mk(L.CRV_DATA) for _ in range(s.n)],
-        mean=[mk(L.CRV_PARAMETER) for _ in range(3)],
-        data=[mk(L.CRV_DATA) for _ in range(s.n)],
-        cov=[mk(L.CRV_COVARIANCE) for _ in range(6)],
-        sim=[mk(L.CRV_PARAMETER) for _ in range(3)],
+        mean_in=mk(3, L.CRV_PARAMETER),
+        data_in=mk(s.n, L.CRV_DATA),
+        mean=mk(3, L.CRV_PARAMETER),
+        data=mk(s.n, L.CRV_DATA),
+        cov=mk(6, L.CRV_COVARIANCE),
+        sim=mk(3, L.CRV_PARAMETER),
     )
 
 
@@ -54,16 +56,14 @@ def step(ctx, G, kw, mode, realization, seed):
     """One pass of the hot path (see module docstring)."""
     from crava_amd import _lib as L
     from crava_amd import hip
-    for w, src in zip(G["data"], G["data_in"]):
-        w.fftFrom(src)
-    for w, src in zip(G["mean"], G["mean_in"]):
-        w.fftFrom(src)
+    hip.fft_r2c_batch(G["data"], src=G["data_in"])
+    hip.fft_r2c_batch(G["mean"], src=G["mean_in"])
     hip.computePostMeanResidAndFFTCov(ctx, G["mean"], G["data"], None, G["cov"], mode=mode, separable=True,
-                                      want_stats=False, **kw)
-    hip.simulate(ctx, G["cov"], G["mean"], G["sim"], noise_mode=L.CRV_NOISE_COUNTER, seed=seed,
-                 realization=realization, fp64_chol=(mode == L.CRV_MODE_FP32_REFINE))
-    for c in G["cov"]:
-        c.invFFTInPlace()
+                                      want_stats=False, skip_mean_inverse=True, **kw)
+    hip.fft_c2r_batch(G["mean"])
+    hip.simulate_batch(ctx, G["cov"], G["mean"], G["sim"], seed=seed, realization0=realization,
+                       fp64_chol=(mode == L.CRV_MODE_FP32_REFINE))
+    hip.fft_c2r_batch(G["cov"])
 
 
 def timed(ctx, G, kw, mode, steps, warmup, seed):
@@ -117,7 +117,7 @@ def kernel_bytes(name, s, prec, ninv):
         return 2 * e * Nc + e * N            # (+ e*N when the mean is added)
     if name in ("posterior", "posterior_fp32"):
         return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e
-    if name == "simulate_scale":
+    if name in ("simulate_scale", "simulate_scale_batch"):
         return Nc * (6 + 3) * 2 * e
     return None
 
@@ -220,6 +220,7 @@ KERNEL_SYMBOLS = {
     "fft_y_fwd": "k_cols<float, false>", "fft_z_fwd": "k_cols<float, false>",
     "fft_y_inv": "k_cols<float, true>", "fft_z_inv": "k_cols<float, true>",
     "posterior_fp32": "k_posterior<float, ", "simulate_scale": "k_simulate<float",
+    "simulate_scale_batch": "k_simulate<float",
 }
 
 
@@ -240,12 +241,138 @@ def read_traffic(kernel):
     return None
 
 
+def bench_c3(a):
+    """Batched and graph-captured runs: 64 realizations of the 256x256x200
+    5-angle inversion (one launch per stage for all realizations), and 12
+    uneven volumes in one captured graph; the FFT launch shapes of every
+    grid size and the posterior's blocks per CU come from the autotuner."""
+    import torch
+    from crava_amd import _lib as L
+    from crava_amd import hip, model
+    from crava_amd.survey import GpuSurvey
+    ctx = hip.Context(0)
+    refine = L.CRV_MODE_FP32_REFINE
+
+    def grids(s, nreal):
+        mk = lambda ct: hip.FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=32, cubetype=ct, nx=s.nx, ny=s.ny, nz=s.nz)
+        G = dict(mean_in=[mk(L.CRV_PARAMETER) for _ in range(3)], data_in=[mk(L.CRV_DATA) for _ in range(s.n)],
+                 mean=[mk(L.CRV_PARAMETER) for _ in range(3)], data=[mk(L.CRV_DATA) for _ in range(s.n)],
+                 cov=[mk(L.CRV_COVARIANCE) for _ in range(6)],
+                 sims=hip.grid_batch(ctx, 3 * nreal, s.nxp, s.nyp, s.nzp, 32, L.CRV_PARAMETER, s.nx, s.ny, s.nz))
+        return G
+
+    def load(G, sv):
+        for g, src in zip(G["mean_in"], sv.mean):
+            g.copyFrom(src)
+        for g, src in zip(G["data_in"], sv.data):
+            g.copyFrom(src)
+
+    def step(G, plan, s, nreal):
+        for w, src in zip(G["data"], G["data_in"]):
+            w.fftFrom(src)
+        for w, src in zip(G["mean"], G["mean_in"]):
+            w.fftFrom(src)
+        plan.exec(G["mean"], G["data"], None, G["cov"])
+        for m in G["mean"]:
+            m.invFFTInPlace()
+        hip.simulate_batch(ctx, G["cov"], G["mean"], G["sims"], seed=s.seed_sim, realization0=0, fp64_chol=True)
+        for c in G["cov"]:
+            c.invFFTInPlace()
+
+    def timeit(fn, reps):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    out = {"metric": "batched inversion: realizations/s (256x256x200, 5 angles, 64 realizations) and uneven "
+                     "volumes/s (12 volumes), FP32-refined, hipGraph-captured", "unit": "realizations/s",
+           "n_gpus": 1, "data": "synthetic (GPU-generated surveys, counter-stream truth)"}
+    # ---- part 1: 64 realizations -----------------------------------------
+    s = model.config3_realizations()
+    nreal = 64
+    log(f"[bench c3] part 1: {s.nxp}x{s.nyp}x{s.nzp}, {s.n} angles, {nreal} realizations")
+    sv = GpuSurvey(ctx, s, truth_noise="counter")
+    G = grids(s, nreal)
+    load(G, sv)
+    plan = hip.PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=refine, separable=True, skip_mean_inverse=True,
+                        **sv.posterior_kwargs())
+    # posterior blocks per CU: time the whole step for each candidate
+    best = (1e30, 0)
+    for v in (0, 1, 2, 3, 4):
+        ctx.tune(L.CRV_TUNE_POST_BLOCKS, v)
+        t = timeit(lambda: step(G, plan, s, nreal), 2)
+        best = min(best, (t, v))
+    ctx.tune(L.CRV_TUNE_POST_BLOCKS, best[1])
+    # a tune call resets the plans' shapes: tune the transforms last
+    t0 = time.perf_counter()
+    shape = hip.autotune(ctx, s.nxp, s.nyp, s.nzp, 32, reps=3)
+    t_tune = time.perf_counter() - t0
+    eager = timeit(lambda: step(G, plan, s, nreal), a.steps)
+    graph = hip.Graph.begin(ctx)
+    step(G, plan, s, nreal)
+    graph.end()
+    graphed = timeit(graph.launch, a.steps)
+    graph.close()
+    out["value"] = nreal / graphed
+    out["part1"] = {"grid": [s.nxp, s.nyp, s.nzp], "angles": s.n, "realizations": nreal,
+                    "ms_per_batch_graph": graphed * 1e3, "ms_per_batch_eager": eager * 1e3,
+                    "realizations_per_s": nreal / graphed,
+                    "voxels_per_s": nreal * s.voxels / graphed, "autotune": shape, "autotune_s": t_tune,
+                    "posterior_blocks_per_cu": best[1]}
+    log(f"[bench c3] part 1: graph {graphed * 1e3:.2f} ms, eager {eager * 1e3:.2f} ms per batch; "
+        f"{nreal / graphed:.1f} realizations/s")
+    for g in sum((v for v in G.values()), []):
+        g.close()
+    sv.close()
+    plan.close()
+    # ---- part 2: uneven volumes --------------------------------------------
+    vols = model.config3_volumes()
+    log(f"[bench c3] part 2: {len(vols)} volumes " + ", ".join(f"{v.nxp}x{v.nyp}x{v.nzp}" for v in vols))
+    shapes = {}
+    for v in vols:
+        key = (v.nxp, v.nyp, v.nzp)
+        if key not in shapes:
+            shapes[key] = hip.autotune(ctx, *key, 32, reps=2)
+    work = []
+    for v in vols:
+        sv = GpuSurvey(ctx, v, truth_noise="counter")
+        Gv = grids(v, 1)
+        load(Gv, sv)
+        pl = hip.PostPlan(ctx, v.nxp, v.nyp, v.nzp, 32, v.n, mode=refine, separable=True, skip_mean_inverse=True,
+                          **sv.posterior_kwargs())
+        sv.close()
+        work.append((v, Gv, pl))
+
+    def all_volumes():
+        for v, Gv, pl in work:
+            step(Gv, pl, v, 1)
+
+    eager2 = timeit(all_volumes, a.steps)
+    graph = hip.Graph.begin(ctx)
+    all_volumes()
+    graph.end()
+    graphed2 = timeit(graph.launch, a.steps)
+    graph.close()
+    vox = sum(v.voxels for v in vols)
+    out["part2"] = {"volumes": [[v.nxp, v.nyp, v.nzp] for v in vols], "ms_per_batch_graph": graphed2 * 1e3,
+                    "ms_per_batch_eager": eager2 * 1e3, "volumes_per_s": len(vols) / graphed2,
+                    "voxels_per_s": vox / graphed2,
+                    "autotune": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in shapes.items()}}
+    log(f"[bench c3] part 2: graph {graphed2 * 1e3:.2f} ms, eager {eager2 * 1e3:.2f} ms for {len(vols)} volumes")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c3"],
+                    help="c1: headline 450x450x320 inversion; c0: 64x64x128; c3: batched realizations and volumes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp32-mode", choices=["refine", "plain"], default="refine",
@@ -262,6 +389,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if a.config == "c3":
+        bench_c3(a)
+        return
     if a.copy_calibration:
         n = 1 << 28  # 1 GiB of float32
         x = torch.ones(n, device="cuda:0")

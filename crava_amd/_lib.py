@@ -30,8 +30,12 @@ EXPORTS = [
     "crv_forward_model", "crv_add_white_noise", "crv_grid_moments", "crv_ctx_tune", "crv_rock_forward",
     "crv_mlp_param_count", "crv_mlp_create", "crv_mlp_destroy", "crv_mlp_set_params", "crv_mlp_get_params",
     "crv_mlp_gradients", "crv_mlp_train_step", "crv_mlp_predict", "crv_mlp_flops", "crv_rock_refine",
+    "crv_post_plan_create", "crv_post_plan_destroy", "crv_posterior_exec", "crv_post_plan_stats",
+    "crv_fft_autotune", "crv_grid_create_batch", "crv_simulate_batch", "crv_graph_begin", "crv_graph_end",
+    "crv_graph_launch", "crv_graph_destroy", "crv_facies_density", "crv_facies_prob",
+    "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
 ]
-CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS = 1, 2, 3, 4
+CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
 
 
 class GridInfo(C.Structure):
@@ -63,6 +67,15 @@ class PostStats(C.Structure):
 class RockParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("k_quartz", "g_quartz", "rho_quartz", "k_clay", "g_clay", "rho_clay",
                                            "temperature", "salinity", "gas_gravity")]
+
+
+class FaciesBox(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3)]
+
+
+class FftShape(C.Structure):
+    _fields_ = [("rows_per_tile", C.c_int), ("tx_y", C.c_int), ("tx_z", C.c_int), ("radix_order", C.c_int * 3),
+                ("ms_r2c", C.c_double), ("ms_c2r", C.c_double)]
 
 
 class MlpNorm(C.Structure):
@@ -139,6 +152,23 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_mlp_flops": (i, [vp, _dp, _dp]),
         "crv_rock_refine": (i, [vp, C.POINTER(RockParams), C.c_longlong, vp, vp, C.POINTER(MlpNorm), vp, i,
                                 C.c_float, C.POINTER(C.c_longlong)]),
+        "crv_post_plan_create": (i, [vp, C.POINTER(PostParams), i, i, i, i, C.POINTER(vp)]),
+        "crv_post_plan_destroy": (i, [vp]),
+        "crv_posterior_exec": (i, [vp, C.POINTER(vp), C.POINTER(vp), vp, C.POINTER(vp)]),
+        "crv_post_plan_stats": (i, [vp, C.POINTER(PostStats)]),
+        "crv_fft_autotune": (i, [vp, i, i, i, i, i, C.POINTER(FftShape)]),
+        "crv_grid_create_batch": (i, [vp, i, i, i, i, i, i, i, i, i, C.POINTER(vp)]),
+        "crv_simulate_batch": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.c_ulonglong, C.c_ulonglong, i, i,
+                                   C.POINTER(vp)]),
+        "crv_graph_begin": (i, [vp]),
+        "crv_graph_end": (i, [vp, C.POINTER(vp)]),
+        "crv_graph_launch": (i, [vp]),
+        "crv_graph_destroy": (i, [vp]),
+        "crv_facies_density": (i, [vp, vp, C.c_longlong, C.POINTER(FaciesBox), _dp, vp]),
+        "crv_fft_r2c_batch": (i, [C.POINTER(vp), i]),
+        "crv_fft_r2c_batch_from": (i, [C.POINTER(vp), C.POINTER(vp), i]),
+        "crv_fft_c2r_batch": (i, [C.POINTER(vp), i]),
+        "crv_facies_prob": (i, [vp, i, C.POINTER(vp), _dp, C.POINTER(FaciesBox), C.POINTER(vp), C.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

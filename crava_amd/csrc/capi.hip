@@ -286,6 +286,63 @@ int crv_fft_c2r(crv_grid* g) {
   return crv_fft_c2r_add(g, g, nullptr);
 }
 
+// grids[0..count) one contiguous batch of one size, type and state
+static int check_batch(crv_grid** g, int count, bool transformed, const char* what) {
+  if (!g || count < 1 || !g[0]) return CRV_E_INVALID;
+  crv_ctx* ctx = g[0]->ctx;
+  const size_t bytes = g[0]->rsize * g[0]->elem_bytes();
+  for (int i = 0; i < count; ++i) {
+    const crv_grid* x = g[i];
+    if (!x || x->nxp != g[0]->nxp || x->nyp != g[0]->nyp || x->nzp != g[0]->nzp || x->precision != g[0]->precision ||
+        x->cubetype != g[0]->cubetype)
+      return crv_set_error(ctx, CRV_E_INVALID, std::string(what) + ": grids differ in size, precision or type");
+    if (x->cubetype == CRV_CTMISSING) return crv_set_error(ctx, CRV_E_STATE, std::string(what) + ": cubetype missing");
+    if (x->transformed != transformed)
+      return crv_set_error(ctx, CRV_E_STATE, std::string(what) + ": wrong transformed state");
+    if ((const char*)x->d != (const char*)g[0]->d + (size_t)i * bytes)
+      return crv_set_error(ctx, CRV_E_INVALID, std::string(what) + ": grids are not one contiguous batch");
+  }
+  return CRV_OK;
+}
+
+int crv_fft_r2c_batch_from(crv_grid** dst, crv_grid** src, int count) {
+  int rc = check_batch(src, count, false, "fft_r2c_batch");
+  if (rc != CRV_OK) return rc;
+  crv_ctx* ctx = src[0]->ctx;
+  if (dst != src) {
+    for (int i = 0; i < count; ++i) {
+      if (!dst || !dst[i] || dst[i]->nxp != src[0]->nxp || dst[i]->nyp != src[0]->nyp ||
+          dst[i]->nzp != src[0]->nzp || dst[i]->precision != src[0]->precision)
+        return crv_set_error(ctx, CRV_E_INVALID, "fft_r2c_batch: destination grids differ");
+      if ((const char*)dst[i]->d != (const char*)dst[0]->d + (size_t)i * src[0]->rsize * src[0]->elem_bytes())
+        return crv_set_error(ctx, CRV_E_INVALID, "fft_r2c_batch: destination is not one contiguous batch");
+    }
+  }
+  FftPlan* plan = nullptr;
+  if ((rc = crv_get_plan(ctx, src[0]->nxp, src[0]->nyp, src[0]->nzp, src[0]->precision, &plan)) != CRV_OK) return rc;
+  rc = crv_launch_r2c_batch(ctx, plan, dst[0]->d, src[0]->d, count, (long long)src[0]->rsize,
+                            crv_scale_forward(src[0]));
+  if (rc != CRV_OK) return rc;
+  for (int i = 0; i < count; ++i) { dst[i]->cubetype = src[0]->cubetype; dst[i]->transformed = true; }
+  return CRV_OK;
+}
+
+int crv_fft_r2c_batch(crv_grid** grids, int count) { return crv_fft_r2c_batch_from(grids, grids, count); }
+
+int crv_fft_c2r_batch(crv_grid** grids, int count) {
+  int rc = check_batch(grids, count, true, "fft_c2r_batch");
+  if (rc != CRV_OK) return rc;
+  crv_ctx* ctx = grids[0]->ctx;
+  FftPlan* plan = nullptr;
+  if ((rc = crv_get_plan(ctx, grids[0]->nxp, grids[0]->nyp, grids[0]->nzp, grids[0]->precision, &plan)) != CRV_OK)
+    return rc;
+  rc = crv_launch_c2r_batch(ctx, plan, grids[0]->d, count, (long long)grids[0]->rsize, crv_scale_inverse(grids[0]),
+                            nullptr, 0);
+  if (rc != CRV_OK) return rc;
+  for (int i = 0; i < count; ++i) grids[i]->transformed = false;
+  return CRV_OK;
+}
+
 int crv_fft_c2r_add(crv_grid* dst, crv_grid* src, const crv_grid* add) {
   if (!dst || !src) return CRV_E_INVALID;
   crv_ctx* ctx = dst->ctx;

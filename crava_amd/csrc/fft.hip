@@ -752,12 +752,13 @@ int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, lon
   return launch_c2r<float>(ctx, plan, (float*)buf, (float*)buf, scale, bt);
 }
 
-int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale) {
+int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, int nbatch, long long bstride,
+                         double scale) {
   if (nbatch < 1 || nbatch > 65535) return crv_set_error(ctx, CRV_E_INVALID, "bad batch");
   Batch bt;
   bt.n = nbatch; bt.stride = bstride;
-  if (plan->prec == CRV_FP64) return launch_r2c<double>(ctx, plan, (double*)buf, (double*)buf, scale, bt);
-  return launch_r2c<float>(ctx, plan, (float*)buf, (float*)buf, scale, bt);
+  if (plan->prec == CRV_FP64) return launch_r2c<double>(ctx, plan, (double*)dst, (const double*)src, scale, bt);
+  return launch_r2c<float>(ctx, plan, (float*)dst, (const float*)src, scale, bt);
 }
 
 // ---- autotuner ---------------------------------------------------------------

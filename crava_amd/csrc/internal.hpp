@@ -135,7 +135,8 @@ int crv_launch_r2c(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, doub
 int crv_launch_c2r(crv_ctx* ctx, FftPlan* plan, void* buf, void* dst, double scale, const void* add);
 int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale,
                          const void* const* add, int nadd);
-int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale);
+int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, int nbatch, long long bstride,
+                         double scale);
 
 // elementwise (simulate.hip) ---------------------------------------------------
 int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n);

@@ -75,11 +75,15 @@ class Context:
 class FFTGrid:
     DATA, PARAMETER, COVARIANCE, OPERATOR = L.CRV_DATA, L.CRV_PARAMETER, L.CRV_COVARIANCE, L.CRV_OPERATOR
 
-    def __init__(self, ctx: Context, nxp, nyp, nzp, precision=32, cubetype=L.CRV_PARAMETER, nx=None, ny=None, nz=None):
+    def __init__(self, ctx: Context, nxp, nyp, nzp, precision=32, cubetype=L.CRV_PARAMETER, nx=None, ny=None, nz=None,
+                 handle=None):
         self.ctx = ctx
-        h = C.c_void_p()
-        ctx.check(ctx.lib.crv_grid_create(ctx.h, nx or nxp, ny or nyp, nz or nzp, nxp, nyp, nzp, precision,
-                                          cubetype, C.byref(h)), "crv_grid_create")
+        if handle is None:
+            h = C.c_void_p()
+            ctx.check(ctx.lib.crv_grid_create(ctx.h, nx or nxp, ny or nyp, nz or nzp, nxp, nyp, nzp, precision,
+                                              cubetype, C.byref(h)), "crv_grid_create")
+        else:
+            h = C.c_void_p(handle)
         self.h = h
         self.nxp, self.nyp, self.nzp = nxp, nyp, nzp
         self.cnxp = nxp // 2 + 1
@@ -179,13 +183,10 @@ class PostResult:
     max_cond: float
 
 
-def computePostMeanResidAndFFTCov(ctx: Context, mean, data, err_corr, cov, *, A, theta, tables, nz, lz,
-                                  low_cut=5.0, high_cut=55.0, wnc=0.1, min_rel_thick=1.0, sigma0,
-                                  mode=L.CRV_MODE_PLAIN, cond_threshold=1e5, lowk_fraction=0.02,
-                                  separable=False, cxy=None, ct=None, skip_mean_inverse=False,
-                                  want_stats=True) -> PostResult | None:
-    """Crava::computePostMeanResidAndFFTCov over FFTGrid objects (crava.h:43-64)."""
-    n = len(data)
+def _post_params(n, *, A, theta, tables, nz, lz, low_cut=5.0, high_cut=55.0, wnc=0.1, min_rel_thick=1.0, sigma0,
+                 mode=L.CRV_MODE_PLAIN, cond_threshold=1e5, lowk_fraction=0.02, separable=False, cxy=None, ct=None,
+                 skip_mean_inverse=False):
+    """crv_post_params and the arrays it points into (kept alive by the caller)."""
     keep = []
     A_ = np.ascontiguousarray(A, dtype=np.float32); keep.append(A_)
     th = np.ascontiguousarray(theta, dtype=np.float64); keep.append(th)
@@ -205,6 +206,17 @@ def computePostMeanResidAndFFTCov(ctx: Context, mean, data, err_corr, cov, *, A,
         keep += [cx, c_t]
         p.cxy = _arr(cx, C.c_double); p.ct = _arr(c_t, C.c_double)
     p.skip_mean_inverse = int(skip_mean_inverse)
+    return p, keep
+
+
+def computePostMeanResidAndFFTCov(ctx: Context, mean, data, err_corr, cov, *, want_stats=True,
+                                  **kw) -> PostResult | None:
+    """Crava::computePostMeanResidAndFFTCov over FFTGrid objects (crava.h:43-64).
+    Keywords: A, theta, tables, nz, lz, low_cut, high_cut, wnc, min_rel_thick,
+    sigma0, mode, cond_threshold, lowk_fraction, separable, cxy, ct,
+    skip_mean_inverse."""
+    n = len(data)
+    p, keep = _post_params(n, **kw)
     means = (C.c_void_p * 3)(*[g.h.value for g in mean])
     datas = (C.c_void_p * n)(*[g.h.value for g in data])
     covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
@@ -314,3 +326,124 @@ def rock_forward(ctx: Context, x: np.ndarray, params: dict | None = None, out_fp
                         d_st.data_ptr())
     ctx.synchronize()
     return d_out.cpu().numpy().reshape(n, 3), d_st.cpu().numpy()
+
+
+class PostPlan:
+    """crv_post_plan: the posterior's per-kz tables on the device for one grid
+    size; exec launches kernels only (capturable in a Graph)."""
+
+    def __init__(self, ctx: Context, nxp, nyp, nzp, precision, n, **kw):
+        self.ctx = ctx
+        self.n = n
+        p, keep = _post_params(n, **kw)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.crv_post_plan_create(ctx.h, C.byref(p), nxp, nyp, nzp, precision, C.byref(h)),
+                  "post_plan_create")
+        self.h = h
+
+    def exec(self, mean, data, err_corr, cov):
+        means = (C.c_void_p * 3)(*[g.h.value for g in mean])
+        datas = (C.c_void_p * len(data))(*[g.h.value for g in data])
+        covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+        self.ctx.check(self.ctx.lib.crv_posterior_exec(self.h, means, datas,
+                                                       err_corr.h if err_corr is not None else None, covs),
+                       "posterior_exec")
+
+    def stats(self) -> PostResult:
+        st = L.PostStats()
+        self.ctx.check(self.ctx.lib.crv_post_plan_stats(self.h, C.byref(st)), "post_plan_stats")
+        return PostResult(st.n_wavenumbers, st.n_inverted, st.n_chol_fail, st.n_fp64, st.max_cond)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.crv_post_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx: Context, nxp, nyp, nzp, precision=32, reps=3) -> dict:
+    """crv_fft_autotune: the launch shapes chosen for this grid size."""
+    sh = L.FftShape()
+    ctx.check(ctx.lib.crv_fft_autotune(ctx.h, nxp, nyp, nzp, precision, reps, C.byref(sh)), "fft_autotune")
+    return dict(rows_per_tile=sh.rows_per_tile, tx_y=sh.tx_y, tx_z=sh.tx_z, radix_order=list(sh.radix_order),
+                ms_r2c=sh.ms_r2c, ms_c2r=sh.ms_c2r)
+
+
+def grid_batch(ctx: Context, count, nxp, nyp, nzp, precision=32, cubetype=L.CRV_PARAMETER, nx=None, ny=None,
+               nz=None) -> list:
+    """count FFTGrids in one contiguous allocation (crv_grid_create_batch)."""
+    hs = (C.c_void_p * count)()
+    ctx.check(ctx.lib.crv_grid_create_batch(ctx.h, nx or nxp, ny or nyp, nz or nzp, nxp, nyp, nzp, precision,
+                                            cubetype, count, hs), "grid_create_batch")
+    return [FFTGrid(ctx, nxp, nyp, nzp, precision, cubetype, handle=hs[i]) for i in range(count)]
+
+
+def simulate_batch(ctx: Context, cov, post_mean, out, *, seed=0, realization0=0, fp64_chol=False):
+    """len(out) // 3 realizations with the counter stream; out from grid_batch."""
+    nreal = len(out) // 3
+    covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+    pm = (C.c_void_p * 3)(*[g.h.value for g in post_mean]) if post_mean else None
+    outs = (C.c_void_p * len(out))(*[g.h.value for g in out])
+    ctx.check(ctx.lib.crv_simulate_batch(ctx.h, covs, pm, seed, realization0, nreal, int(fp64_chol), outs),
+              "simulate_batch")
+
+
+class Graph:
+    """A HIP graph captured from the context's stream:
+
+        g = Graph.begin(ctx); ...launches...; g.end(); g.launch()
+    """
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+
+    @classmethod
+    def begin(cls, ctx: Context) -> "Graph":
+        g = cls(ctx)
+        ctx.check(ctx.lib.crv_graph_begin(ctx.h), "graph_begin")
+        return g
+
+    def end(self) -> "Graph":
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.crv_graph_end(self.ctx.h, C.byref(h)), "graph_end")
+        self.h = h
+        return self
+
+    def launch(self):
+        self.ctx.check(self.ctx.lib.crv_graph_launch(self.h), "graph_launch")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.crv_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fft_r2c_batch(grids, src=None):
+    """FFTCovGrids over one contiguous batch (grid_batch); with src, the
+    spectra of the src batch are written into grids."""
+    ctx = grids[0].ctx
+    d = (C.c_void_p * len(grids))(*[g.h.value for g in grids])
+    if src is None:
+        ctx.check(ctx.lib.crv_fft_r2c_batch(d, len(grids)), "fft_r2c_batch")
+    else:
+        s_ = (C.c_void_p * len(src))(*[g.h.value for g in src])
+        ctx.check(ctx.lib.crv_fft_r2c_batch_from(d, s_, len(grids)), "fft_r2c_batch_from")
+
+
+def fft_c2r_batch(grids):
+    """invFFTCovGrids over one contiguous batch (grid_batch)."""
+    ctx = grids[0].ctx
+    d = (C.c_void_p * len(grids))(*[g.h.value for g in grids])
+    ctx.check(ctx.lib.crv_fft_c2r_batch(d, len(grids)), "fft_c2r_batch")

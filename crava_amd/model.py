@@ -117,6 +117,32 @@ def config1() -> Survey:
     return Survey(450, 450, 320, angles_deg=ang, peak_hz=peaks, sn=(4.0,) * 5)
 
 
+def config3_realizations() -> Survey:
+    """Batched simulation: the config-1 survey setup on a 256x256x200 padded
+    grid (64 realizations, counter-stream seeds 0..63)."""
+    ang = (5.0, 12.0, 19.0, 26.0, 33.0)
+    peaks = tuple(35.0 - 7.0 * i / 4.0 for i in range(5))
+    return Survey(256, 256, 200, angles_deg=ang, peak_hz=peaks, sn=(4.0,) * 5)
+
+
+def config3_volumes(count: int = 12, seed: int = 3) -> list:
+    """Uneven volumes: nx, ny from {64, 96, 128, 160, 192, 240, 320}, nz from
+    {100, 128, 160, 200, 256, 320, 400}, padded with the reference's default
+    pad fractions, prior stds scaled by U(0.7, 1.3) around config 0."""
+    rng = np.random.default_rng(seed)
+    xs = (64, 96, 128, 160, 192, 240, 320)
+    zs = (100, 128, 160, 200, 256, 320, 400)
+    base = config0()
+    out = []
+    for v in range(count):
+        nx, ny, nz = int(rng.choice(xs)), int(rng.choice(xs)), int(rng.choice(zs))
+        nxp, nyp, nzp = estimate_padding(nx, ny, nz, base.dx, base.dy, base.dt, base.lateral_range)
+        f = rng.uniform(0.7, 1.3, 3)
+        stds = tuple(float(a * b) for a, b in zip(base.stds, f))
+        out.append(Survey(nxp, nyp, nzp, nx=nx, ny=ny, nz=nz, stds=stds, seed_truth=100 + v, seed_noise=200 + v))
+    return out
+
+
 def config_small(nxp=12, nyp=10, nzp=16, n=3) -> Survey:
     ang = tuple(np.linspace(0.0, 30.0, n))
     return Survey(nxp, nyp, nzp, angles_deg=ang, peak_hz=(30.0,) * n, sn=(5.0,) * n)

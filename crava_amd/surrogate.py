@@ -162,11 +162,14 @@ class TrainResult:
 
 
 def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int = 16384, lr: float = 2e-3,
-          seed: int = 7, n_val: int = 200_000, params: dict | None = None, log=None) -> TrainResult:
+          seed: int = 7, n_val: int = 200_000, params: dict | None = None, log=None,
+          vp_weight: float = 1.0) -> TrainResult:
     """Generate n_train + n_val rplib samples on the GPU (crv_rock_forward)
     and fit the MLP with Adam, cosine learning-rate decay and reshuffled
-    mini-batches.  Returns the trained network with loss history, wall
-    time and the matrix-core rate of the training kernels."""
+    mini-batches.  vp_weight > 1 shrinks the normalisation scale of ln vp,
+    which weights its errors by vp_weight^2 in the loss (the error gate is
+    on vp).  Returns the trained network with loss history, wall time and
+    the matrix-core rate of the training kernels."""
     import torch
     dev = _dev()
     rng = np.random.default_rng(seed)
@@ -179,7 +182,9 @@ def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int =
     n_tr = n_all - n_val
     xtr, ytr = d_x[:n_tr].contiguous(), d_y[:n_tr].contiguous()
     yh = ytr.double().cpu().numpy()
-    nm = make_norm(yh.mean(0), yh.std(0))
+    sd = yh.std(0)
+    sd[0] /= vp_weight
+    nm = make_norm(yh.mean(0), sd)
     mlp = MLP(ctx, nm, batch)
     mlp.set_params(_init_params(seed))
     per_epoch = n_tr // batch
@@ -233,14 +238,12 @@ def _init_params(seed: int) -> np.ndarray:
 
 
 def predict(ctx: Context, mlp: MLP, d_x):
-    """Network outputs (device (n, 3) float) for device inputs, in chunks of
-    the handle's batch size."""
+    """Network outputs (device (n, 3) float) for device inputs (inference
+    needs no per-batch scratch: one launch for any n)."""
     import torch
     n = d_x.shape[0]
     d_y = torch.empty((n, 3), dtype=torch.float32, device=d_x.device)
-    for a in range(0, n, mlp.max_batch):
-        b = min(n, a + mlp.max_batch)
-        mlp.predict(d_x[a:].data_ptr(), b - a, d_y[a:].data_ptr())
+    mlp.predict(d_x.data_ptr(), n, d_y.data_ptr())
     return d_y
 
 
@@ -278,15 +281,24 @@ class ErrorTable:
         self.d = torch.from_numpy(self.values).to(_dev())
 
     @classmethod
-    def build(cls, ctx: Context, mlp: MLP, d_x, d_y, nb: int = 8, min_count: int = 3) -> "ErrorTable":
+    def build(cls, ctx: Context, mlp: MLP, d_x, d_y, nb: int = 6, min_count: int = 20,
+              quantile: float = 0.999) -> "ErrorTable":
+        """Per cell, the `quantile` of the held-out |ln vp| errors (the
+        largest error when quantile >= 1); +inf below min_count samples."""
         err = np.abs(validation_errors(ctx, mlp, d_x, d_y)[:, 0])
         cell = input_cells(d_x.cpu().numpy(), nb)
-        vmax = np.zeros(nb ** 5)
-        cnt = np.zeros(nb ** 5, np.int64)
-        np.maximum.at(vmax, cell, err)
-        np.add.at(cnt, cell, 1)
-        vmax[cnt < min_count] = np.inf
-        return cls(vmax, nb)
+        order = np.lexsort((err, cell))
+        cs, es = cell[order], err[order]
+        ncell = nb ** 5
+        start = np.searchsorted(cs, np.arange(ncell), side="left")
+        stop = np.searchsorted(cs, np.arange(ncell), side="right")
+        cnt = stop - start
+        vals = np.full(ncell, np.inf)
+        ok = cnt >= max(1, min_count)
+        # index of the quantile inside each cell's sorted errors
+        q = np.minimum(stop - 1, start + np.floor(quantile * (cnt - 1)).astype(np.int64))
+        vals[ok] = es[q[ok]]
+        return cls(vals, nb)
 
 
 def predict_checked(ctx: Context, mlp: MLP, table: ErrorTable, d_x, thresh: float = 0.005,

@@ -121,6 +121,14 @@ int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int precision, int
  * whole batch.                                                              */
 int crv_grid_create_batch(crv_ctx* ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, int precision,
                           int cubetype, int count, crv_grid** grids);
+/* SeismicParametersHolder::FFTCovGrids / invFFTCovGrids
+ * (seismicparametersholder.cpp:273-322) and the means'/data's transforms
+ * over a batch: every grid of one contiguous batch (same size, type and
+ * state) transformed with one launch per pass.  _from writes the spectra of
+ * the src batch into the dst batch (src unchanged).                         */
+int crv_fft_r2c_batch(crv_grid** grids, int count);
+int crv_fft_r2c_batch_from(crv_grid** dst, crv_grid** src, int count);
+int crv_fft_c2r_batch(crv_grid** grids, int count);
 /* HIP-graph capture of the context's stream: calls between begin and end
  * are recorded (kernel launches only: plans must exist, see
  * crv_posterior_exec and the batched simulation), then replayed with
@@ -421,6 +429,26 @@ int crv_mlp_flops(crv_mlp* h, double* fwd, double* bwd);
  * seen).  d_x n x 5, d_y n x 3 floats, d_table nb^5 floats, device memory. */
 int crv_rock_refine(crv_ctx* ctx, const crv_rock_params* p, long long n, const float* d_x, float* d_y,
                     const crv_mlp_norm* norm, const float* d_table, int nb, float thresh, long long* n_refined);
+
+/* ---- facies probabilities -------------------------------------------------
+ * FaciesProb's density estimate and classification (src/faciesprob.cpp:
+ * 544-930): crv_facies_density bins n facies-conditional samples (d_samples
+ * n x 3 floats: ln vp, ln vs, ln rho) on the box, smooths the histogram by
+ * a Gaussian of covariance smooth_cov (3 x 3, log units: posterior
+ * uncertainty plus kernel bandwidth) through the library's transforms, and
+ * normalises it to a density on `density` (an FP32 grid; x bins = nxp along
+ * ln vp, y along ln vs, z along ln rho; the transform is circular, so the
+ * box should extend beyond the samples by a few smoothing widths).
+ * crv_facies_prob gives every voxel of the real posterior means (FP32)
+ * P(f | m) = prior_f p_f(m) / sum_g prior_g p_g(m), p_f by trilinear
+ * interpolation of the nf (<= 8) densities.                                */
+typedef struct crv_facies_box {
+  double lo[3], hi[3];
+} crv_facies_box;
+int crv_facies_density(crv_ctx* ctx, const float* d_samples, long long n, const crv_facies_box* box,
+                       const double* smooth_cov, crv_grid* density);
+int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
+                    const crv_facies_box* box, crv_grid* mean[3], crv_grid** prob);
 
 /* ---- padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622) -- */
 /* Smallest 2^a 3^b 5^c 7^d >= leastint (FFTGrid::findClosestFactorableNumber). */

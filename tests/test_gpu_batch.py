@@ -1,0 +1,166 @@
+"""Batched and graph-captured execution (the many-realization / many-volume
+workload): the plan/exec posterior, the batched simulation and a captured
+graph reproduce the one-call entry points exactly, and the autotuner's
+shapes give the same transforms."""
+import numpy as np
+import pytest
+
+from conftest import rel_rms
+from crava_amd import model
+from pipeline import Case
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(ctx, case, prec=32):
+    """Transformed means/data of a small case plus separable prior keywords."""
+    from crava_amd.hip import FFTGrid
+    from oracle import oracle as O
+    s = case.s
+    mk = lambda ct: FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=prec, cubetype=ct)
+    mean = [mk(2) for _ in range(3)]
+    data = [mk(1) for _ in range(s.n)]
+    for g, a in zip(mean, case.syn.mean):
+        g.upload_complex(O.fft_r2c(a, s.nxp, 2))
+    for g, a in zip(data, case.syn.data):
+        g.upload_complex(O.fft_r2c(a, s.nxp, 1))
+    p = case.pin
+    cxy, ct = model.prior_spectra(s)
+    kw = dict(A=p.A, theta=p.theta, tables=p.tables, nz=s.nz, lz=s.lz, sigma0=p.sigma0, separable=True, cxy=cxy,
+              ct=ct, skip_mean_inverse=True)
+    return mean, data, kw
+
+
+def _copy(ctx, grids):
+    from crava_amd.hip import FFTGrid
+    out = []
+    for g in grids:
+        c = FFTGrid(ctx, g.nxp, g.nyp, g.nzp, precision=g.precision, cubetype=g.getType())
+        c.copyFrom(g)
+        out.append(c)
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_posterior_exec_matches_one_call(ctx, mode):
+    from crava_amd.hip import FFTGrid, PostPlan, computePostMeanResidAndFFTCov
+    case = Case(model.config_small(20, 18, 30, 5))
+    s = case.s
+    mean, data, kw = _inputs(ctx, case)
+    mean2, data2 = _copy(ctx, mean), _copy(ctx, data)
+    cov = [FFTGrid(ctx, s.nxp, s.nyp, s.nzp, cubetype=3) for _ in range(6)]
+    cov2 = [FFTGrid(ctx, s.nxp, s.nyp, s.nzp, cubetype=3) for _ in range(6)]
+    st = computePostMeanResidAndFFTCov(ctx, mean, data, None, cov, mode=mode, **kw)
+    plan = PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=mode, **kw)
+    plan.exec(mean2, data2, None, cov2)
+    st2 = plan.stats()
+    for a, b in zip(mean + data + cov, mean2 + data2 + cov2):
+        assert np.array_equal(a.download_complex(), b.download_complex())
+    assert (st.n_inverted, st.n_chol_fail, st.n_fp64) == (st2.n_inverted, st2.n_chol_fail, st2.n_fp64)
+
+
+def _posterior_cov(ctx, case):
+    from crava_amd.hip import FFTGrid, computePostMeanResidAndFFTCov
+    s = case.s
+    mean, data, kw = _inputs(ctx, case)
+    cov = [FFTGrid(ctx, s.nxp, s.nyp, s.nzp, cubetype=3) for _ in range(6)]
+    kw["skip_mean_inverse"] = False
+    computePostMeanResidAndFFTCov(ctx, mean, data, None, cov, mode=1, **kw)
+    return mean, cov
+
+
+@pytest.mark.parametrize("fp64_chol", [False, True])
+def test_simulate_batch_matches_single_calls(ctx, fp64_chol):
+    from crava_amd import _lib as L
+    from crava_amd.hip import FFTGrid, grid_batch, simulate, simulate_batch
+    case = Case(model.config_small(20, 18, 30, 5))
+    s = case.s
+    mean, cov = _posterior_cov(ctx, case)
+    nreal = 4
+    out = grid_batch(ctx, 3 * nreal, s.nxp, s.nyp, s.nzp)
+    simulate_batch(ctx, cov, mean, out, seed=99, realization0=10, fp64_chol=fp64_chol)
+    for r in range(nreal):
+        one = [FFTGrid(ctx, s.nxp, s.nyp, s.nzp) for _ in range(3)]
+        simulate(ctx, cov, mean, one, noise_mode=L.CRV_NOISE_COUNTER, seed=99, realization=10 + r,
+                 fp64_chol=fp64_chol)
+        for g in range(3):
+            assert np.array_equal(out[3 * r + g].download_real()[:, :, :s.nxp], one[g].download_real()[:, :, :s.nxp])
+    # different realizations differ
+    assert not np.array_equal(out[0].download_real(), out[3].download_real())
+
+
+def test_graph_replays_match_eager(ctx):
+    """Capture forward transforms, the posterior exec, a batch of
+    realizations and the covariance inverse transforms; every replay (with
+    eager work and synchronisations between replays) equals the eager run."""
+    from crava_amd.hip import FFTGrid, Graph, PostPlan, grid_batch, simulate_batch
+    case = Case(model.config_small(20, 18, 30, 5))
+    s = case.s
+    mk = lambda ct: FFTGrid(ctx, s.nxp, s.nyp, s.nzp, cubetype=ct)
+    mean_in = [mk(2) for _ in range(3)]
+    data_in = [mk(1) for _ in range(s.n)]
+    for g, a in zip(mean_in, case.syn.mean):
+        g.upload_real(a)
+    for g, a in zip(data_in, case.syn.data):
+        g.upload_real(a)
+    _, _, kw = _inputs(ctx, case)
+
+    def pipeline_grids():
+        return ([mk(2) for _ in range(3)], [mk(1) for _ in range(s.n)], [mk(3) for _ in range(6)],
+                grid_batch(ctx, 6, s.nxp, s.nyp, s.nzp))
+
+    plan = PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=1, **kw)
+
+    def run(mean, data, cov, sims):
+        for w, src in zip(data, data_in):
+            w.fftFrom(src)
+        for w, src in zip(mean, mean_in):
+            w.fftFrom(src)
+        plan.exec(mean, data, None, cov)
+        simulate_batch(ctx, cov, None, sims, seed=5, realization0=0, fp64_chol=True)
+        for c in cov:
+            c.invFFTInPlace()
+
+    eager = pipeline_grids()
+    run(*eager)
+    ctx.synchronize()
+    want = [g.download_complex() if i < 3 + s.n else g.download_real()
+            for i, g in enumerate(eager[0] + eager[1] + eager[2] + eager[3])]
+    cap = pipeline_grids()
+    graph = Graph.begin(ctx)
+    run(*cap)
+    graph.end()
+    for rep in range(3):
+        # eager work on other grids between replays
+        junk = mk(2)
+        junk.upload_real(np.full((s.nzp, s.nyp, s.rnxp), float(rep), np.float32))
+        junk.fftInPlace()
+        ctx.synchronize()
+        graph.launch()
+        ctx.synchronize()
+        got = [g.download_complex() if i < 3 + s.n else g.download_real()
+               for i, g in enumerate(cap[0] + cap[1] + cap[2] + cap[3])]
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+    graph.close()
+
+
+@pytest.mark.parametrize("dims", [(80, 60, 50), (450, 450, 320)])
+def test_autotuned_shapes_keep_the_transform(ctx, dims):
+    from crava_amd.hip import FFTGrid, autotune
+    nx, ny, nz = dims
+    rng = np.random.default_rng(1)
+    a = FFTGrid(ctx, nx, ny, nz)
+    x = rng.standard_normal((nz, ny, 2 * (nx // 2 + 1))).astype(np.float32)
+    a.upload_real(x)
+    a.fftInPlace()
+    before = a.download_complex()
+    shape = autotune(ctx, nx, ny, nz, 32, reps=2)
+    assert shape["ms_r2c"] > 0 and shape["ms_c2r"] > 0
+    assert shape["rows_per_tile"] >= 1 and shape["tx_y"] >= 1 and shape["tx_z"] >= 1
+    b = FFTGrid(ctx, nx, ny, nz)
+    b.upload_real(x)
+    b.fftInPlace()
+    assert rel_rms(b.download_complex(), before) < 1e-6
+    b.invFFTInPlace()
+    assert rel_rms(b.download_real()[:, :, :nx], x[:, :, :nx]) < 1e-6

@@ -418,3 +418,55 @@ def test_bf16_rounding():
     # above the midpoint and rounds up
     v = np.array([1.0, 1.00390625, 1.01171875, 1.001953125, 1.005859375, -3.14159265, 0.0], np.float32)
     assert list(M.bf16(v)) == [1.0, 1.0, 1.015625, 1.0, 1.0078125, -3.140625, 0.0]
+
+
+def test_facies_density_gaussian_closed_form():
+    """A Gaussian cloud N(mu, C) binned with width w and smoothed by S is the
+    Gaussian of covariance C + S + diag(w^2 / 12), up to sampling noise."""
+    from oracle import facies_oracle as F
+    rng = np.random.default_rng(5)
+    mu = np.array([8.0, 7.3, 7.7])
+    C = np.array([[0.010, 0.006, 0.001], [0.006, 0.020, 0.0015], [0.001, 0.0015, 0.002]])
+    S = 0.5 * C
+    x = rng.multivariate_normal(mu, C, 2_000_000)
+    lo, hi, bins = mu - 8 * np.sqrt(np.diag(C)), mu + 8 * np.sqrt(np.diag(C)), (64, 60, 50)
+    d = F.density(x, lo, hi, S, bins)
+    w = (hi - lo) / np.array(bins)
+    T = C + S + np.diag(w ** 2 / 12)
+    Ti = np.linalg.inv(T)
+    # bin centres within 1.5 sigma of the mean
+    cz, cy, cx = np.meshgrid(*[lo[k] + (np.arange(bins[k]) + 0.5) * w[k] for k in (2, 1, 0)], indexing="ij")
+    pts = np.stack([cx, cy, cz], -1) - mu
+    quad = np.einsum("...i,ij,...j->...", pts, Ti, pts)
+    pdf = np.exp(-0.5 * quad) / np.sqrt((2 * np.pi) ** 3 * np.linalg.det(T))
+    core = quad < 1.5 ** 2
+    assert np.max(np.abs(d[core] - pdf[core]) / pdf[core]) < 0.03
+    assert d.sum() * np.prod(w) == pytest.approx(1.0, rel=1e-12)
+
+
+def test_facies_probabilities_logistic():
+    """Two Gaussian facies densities with one covariance T: P(1 | m) is the
+    logistic function of the linear discriminant."""
+    from oracle import facies_oracle as F
+    lo, hi, bins = np.array([7.0, 6.0, 7.0]), np.array([9.0, 8.5, 8.4]), (80, 100, 70)
+    w = (hi - lo) / np.array(bins)
+    T = np.array([[0.02, 0.01, 0.002], [0.01, 0.05, 0.003], [0.002, 0.003, 0.01]])
+    Ti = np.linalg.inv(T)
+    mus = [np.array([7.9, 7.1, 7.65]), np.array([8.1, 7.4, 7.75])]
+    cz, cy, cx = np.meshgrid(*[lo[k] + (np.arange(bins[k]) + 0.5) * w[k] for k in (2, 1, 0)], indexing="ij")
+    grid = np.stack([cx, cy, cz], -1)
+    dens = []
+    for mu in mus:
+        p = grid - mu
+        dens.append(np.exp(-0.5 * np.einsum("...i,ij,...j->...", p, Ti, p)) / np.sqrt((2 * np.pi) ** 3 * np.linalg.det(T)))
+    prior = [0.3, 0.7]
+    rng = np.random.default_rng(2)
+    m = np.column_stack([rng.uniform(7.8, 8.2, 2000), rng.uniform(7.0, 7.5, 2000), rng.uniform(7.6, 7.8, 2000)])
+    P = F.probabilities(dens, prior, lo, hi, bins, m)
+    a = Ti @ (mus[0] - mus[1])
+    b = -0.5 * (mus[0] @ Ti @ mus[0] - mus[1] @ Ti @ mus[1]) + np.log(prior[0] / prior[1])
+    want = 1.0 / (1.0 + np.exp(-(m @ a + b)))
+    # trilinear interpolation of the bin-centre densities costs ~(w/sigma)^2/8
+    assert np.max(np.abs(P[0] - want)) < 1e-2
+    assert np.mean(np.abs(P[0] - want)) < 2e-3
+    assert np.allclose(P[0] + P[1], 1.0)
