@@ -38,7 +38,7 @@ def log(msg):
 
 def make_grids(ctx, s, prec):
     """Each family of grids is one contiguous batch, so its transforms run
-    as one launch per pass."""
+    as batched launches (sub-batches sized to the memory-side cache)."""
     from crava_amd import _lib as L
     from crava_amd.hip import grid_batch
     mk = lambda n, ct: grid_batch(ctx, n, s.nxp, s.nyp, s.nzp, prec, ct, s.nx, s.ny, s.nz)
@@ -307,6 +307,14 @@ def bench_c3(a):
         t = timeit(lambda: step(G, plan, s, nreal), 2)
         best = min(best, (t, v))
     ctx.tune(L.CRV_TUNE_POST_BLOCKS, best[1])
+    # sub-batch size of the batched transforms (MB; 1 << 20 = whole batch)
+    best_b = (1e30, 0)
+    for v in (0, 128, 1 << 20):
+        ctx.tune(L.CRV_TUNE_BATCH_MB, v)
+        t = timeit(lambda: step(G, plan, s, nreal), 2)
+        log(f"[bench c3] batch_mb {v}: {t * 1e3:.2f} ms per batch")
+        best_b = min(best_b, (t, v))
+    ctx.tune(L.CRV_TUNE_BATCH_MB, best_b[1])
     # a tune call resets the plans' shapes: tune the transforms last
     t0 = time.perf_counter()
     shape = hip.autotune(ctx, s.nxp, s.nyp, s.nzp, 32, reps=3)
@@ -322,7 +330,7 @@ def bench_c3(a):
                     "ms_per_batch_graph": graphed * 1e3, "ms_per_batch_eager": eager * 1e3,
                     "realizations_per_s": nreal / graphed,
                     "voxels_per_s": nreal * s.voxels / graphed, "autotune": shape, "autotune_s": t_tune,
-                    "posterior_blocks_per_cu": best[1]}
+                    "posterior_blocks_per_cu": best[1], "batch_mb": best_b[1]}
     log(f"[bench c3] part 1: graph {graphed * 1e3:.2f} ms, eager {eager * 1e3:.2f} ms per batch; "
         f"{nreal / graphed:.1f} realizations/s")
     for g in sum((v for v in G.values()), []):

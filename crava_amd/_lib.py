@@ -36,6 +36,7 @@ EXPORTS = [
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
+CRV_TUNE_BATCH_MB = 6
 
 
 class GridInfo(C.Structure):

@@ -721,6 +721,7 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   p->tx_z = pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z]);
   if (!p->tx_y || !p->tx_z) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nyp or nzp too large");
   p->chunk_mb = ctx->tune[CRV_TUNE_CHUNK_MB];
+  p->batch_mb = ctx->tune[CRV_TUNE_BATCH_MB];
   return CRV_OK;
 }
 
@@ -739,26 +740,56 @@ int crv_launch_c2r(crv_ctx* ctx, FftPlan* plan, void* buf, void* dst, double sca
   return launch_c2r<float>(ctx, plan, (float*)buf, (float*)dst, scale, bt);
 }
 
-// nbatch grids of this plan, bstride reals apart, transformed in place with
-// one launch per pass; the output of batch entry b gets add[b % nadd] added
+// Grids per launch of a batched transform.  A pass over a grid that fits
+// the memory-side cache leaves its output there for the next pass, so a
+// batch of such grids goes in sub-batches of about batch_mb (default 256 MB,
+// the cache's size); grids more than twice that size get no reuse anyway and
+// the whole batch goes in one launch per pass.
+int sub_batch(const FftPlan* p, int nbatch) {
+  const size_t eb = p->prec == CRV_FP64 ? 8 : 4;
+  const size_t grid = (size_t)2 * (p->nxp / 2 + 1) * p->nyp * p->nzp * eb;
+  const size_t mall = (size_t)(p->batch_mb > 0 ? p->batch_mb : 256) << 20;
+  if (grid > 2 * mall) return nbatch;
+  return (int)std::max<size_t>(1, std::min<size_t>((size_t)nbatch, mall / grid));
+}
+
+// nbatch grids of this plan, bstride reals apart, transformed in place; the
+// output of batch entry b gets add[b % nadd] added
 int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, double scale,
                          const void* const* add, int nadd) {
   if (nbatch < 1 || nbatch > 65535 || nadd < 0 || nadd > 3) return crv_set_error(ctx, CRV_E_INVALID, "bad batch");
-  Batch bt;
-  bt.n = nbatch; bt.stride = bstride; bt.nadd = nadd;
-  for (int q = 0; q < nadd; ++q) bt.add[q] = add[q];
-  if (plan->prec == CRV_FP64)
-    return launch_c2r<double>(ctx, plan, (double*)buf, (double*)buf, scale, bt);
-  return launch_c2r<float>(ctx, plan, (float*)buf, (float*)buf, scale, bt);
+  const int sb = sub_batch(plan, nbatch);
+  const size_t eb = plan->prec == CRV_FP64 ? 8 : 4;
+  for (int b0 = 0; b0 < nbatch; b0 += sb) {
+    Batch bt;
+    bt.n = std::min(sb, nbatch - b0); bt.stride = bstride; bt.nadd = nadd;
+    for (int q = 0; q < nadd; ++q) bt.add[q] = add[(b0 + q) % nadd];
+    char* g = (char*)buf + (size_t)b0 * bstride * eb;
+    const int rc = plan->prec == CRV_FP64
+                       ? launch_c2r<double>(ctx, plan, (double*)g, (double*)g, scale, bt)
+                       : launch_c2r<float>(ctx, plan, (float*)g, (float*)g, scale, bt);
+    if (rc != CRV_OK) return rc;
+  }
+  return CRV_OK;
 }
 
 int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, int nbatch, long long bstride,
                          double scale) {
   if (nbatch < 1 || nbatch > 65535) return crv_set_error(ctx, CRV_E_INVALID, "bad batch");
-  Batch bt;
-  bt.n = nbatch; bt.stride = bstride;
-  if (plan->prec == CRV_FP64) return launch_r2c<double>(ctx, plan, (double*)dst, (const double*)src, scale, bt);
-  return launch_r2c<float>(ctx, plan, (float*)dst, (const float*)src, scale, bt);
+  const int sb = sub_batch(plan, nbatch);
+  const size_t eb = plan->prec == CRV_FP64 ? 8 : 4;
+  for (int b0 = 0; b0 < nbatch; b0 += sb) {
+    Batch bt;
+    bt.n = std::min(sb, nbatch - b0); bt.stride = bstride;
+    const size_t off = (size_t)b0 * bstride * eb;
+    char* d = (char*)dst + off;
+    const char* sp = (const char*)src + off;
+    const int rc = plan->prec == CRV_FP64
+                       ? launch_r2c<double>(ctx, plan, (double*)d, (const double*)sp, scale, bt)
+                       : launch_r2c<float>(ctx, plan, (float*)d, (const float*)sp, scale, bt);
+    if (rc != CRV_OK) return rc;
+  }
+  return CRV_OK;
 }
 
 // ---- autotuner ---------------------------------------------------------------

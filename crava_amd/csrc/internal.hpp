@@ -50,6 +50,7 @@ struct FftPlan {
   int rows_per_block = 0;
   int tx_y = 0, tx_z = 0;
   int chunk_mb = 0;      // x/y pass interleave chunk (MB of grid), 0 = off
+  int batch_mb = 0;      // batched transforms: MB of grids per launch, 0 = 256
   int rev[3] = {0, 0, 0};  // radix order per axis (0 default, 1 reversed)
   // reference noise: per-row prefix of draws (nyp*nzp+1 int64), lazily made
   long long* noise_prefix = nullptr;

@@ -58,6 +58,23 @@ def sample_inputs(n: int, rng: np.random.Generator, facies: str | None = None) -
     return np.column_stack([phi, clay, sw, asp, pe]).astype(np.float32)
 
 
+FACIES_INPUTS = ("shale", "brine_sand", "gas_sand")
+
+
+def training_inputs(n: int, rng: np.random.Generator) -> np.ndarray:
+    """(n, 5) training inputs: half from the broad box, the other half split
+    evenly over the facies distributions that the sampling step queries,
+    shuffled (a surrogate is trained on the distribution it will see)."""
+    nb = n // 2
+    nf = (n - nb) // len(FACIES_INPUTS)
+    parts = [sample_inputs(nb, rng)]
+    for i, f in enumerate(FACIES_INPUTS):
+        m = nf if i < len(FACIES_INPUTS) - 1 else n - nb - nf * (len(FACIES_INPUTS) - 1)
+        parts.append(sample_inputs(m, rng, facies=f))
+    x = np.concatenate(parts)
+    return x[rng.permutation(n)]
+
+
 def make_norm(out_mean, out_std) -> L.MlpNorm:
     nm = L.MlpNorm()
     for f in range(5):
@@ -163,9 +180,9 @@ class TrainResult:
 
 def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int = 16384, lr: float = 2e-3,
           seed: int = 7, n_val: int = 200_000, params: dict | None = None, log=None,
-          vp_weight: float = 1.0) -> TrainResult:
+          vp_weight: float = 1.0, mix_facies: bool = True) -> TrainResult:
     """Generate n_train + n_val rplib samples on the GPU (crv_rock_forward)
-    and fit the MLP with Adam, cosine learning-rate decay and reshuffled
+    (training_inputs, or the broad box only without mix_facies) and fit the MLP with Adam, cosine learning-rate decay and reshuffled
     mini-batches.  vp_weight > 1 shrinks the normalisation scale of ln vp,
     which weights its errors by vp_weight^2 in the loss (the error gate is
     on vp).  Returns the trained network with loss history, wall time and
@@ -173,7 +190,7 @@ def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int =
     import torch
     dev = _dev()
     rng = np.random.default_rng(seed)
-    x = sample_inputs(n_train + n_val, rng)
+    x = training_inputs(n_train + n_val, rng) if mix_facies else sample_inputs(n_train + n_val, rng)
     d_x = torch.from_numpy(x).to(dev)
     d_y, st = rplib_targets(ctx, d_x, x.shape[0], params)
     keep = (st == 0)
@@ -270,9 +287,9 @@ def input_cells(x: np.ndarray, nb: int) -> np.ndarray:
 
 
 class ErrorTable:
-    """Largest |ln vp| error of the network seen on held-out samples, per
-    cell of an nb^5 grid over the normalised inputs; +inf where fewer than
-    min_count samples fell."""
+    """A high quantile of the network's error on held-out samples (ln vp by
+    default), per cell of an nb^5 grid over the normalised inputs; +inf
+    where fewer than min_count samples fell."""
 
     def __init__(self, values: np.ndarray, nb: int):
         import torch
@@ -282,10 +299,11 @@ class ErrorTable:
 
     @classmethod
     def build(cls, ctx: Context, mlp: MLP, d_x, d_y, nb: int = 6, min_count: int = 20,
-              quantile: float = 0.999) -> "ErrorTable":
-        """Per cell, the `quantile` of the held-out |ln vp| errors (the
-        largest error when quantile >= 1); +inf below min_count samples."""
-        err = np.abs(validation_errors(ctx, mlp, d_x, d_y)[:, 0])
+              quantile: float = 0.999, outputs=(0,)) -> "ErrorTable":
+        """Per cell, the `quantile` of the held-out errors (the largest error
+        when quantile >= 1); +inf below min_count samples.  A sample's error
+        is the largest |error| over `outputs` (0 ln vp, 1 ln vs, 2 ln rho)."""
+        err = np.abs(validation_errors(ctx, mlp, d_x, d_y)[:, list(outputs)]).max(axis=1)
         cell = input_cells(d_x.cpu().numpy(), nb)
         order = np.lexsort((err, cell))
         cs, es = cell[order], err[order]

@@ -99,6 +99,12 @@ int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
 /*   CRV_TUNE_POST_BLOCKS  posterior blocks (256 wavenumber-threads) per CU,
  *                      0 = as many as fit                                  */
 #define CRV_TUNE_POST_BLOCKS 5
+/*   CRV_TUNE_BATCH_MB  batched transforms (crv_fft_*_batch) launch sub-batches
+ *                      of about this many MB of grids, so that each pass's
+ *                      output is still in the memory-side cache when the
+ *                      next pass reads it; grids larger than twice this go
+ *                      in one launch per pass (0 = 256, the cache's size) */
+#define CRV_TUNE_BATCH_MB 6
 int crv_ctx_tune(crv_ctx* ctx, int key, int value);
 
 /* Autotuner for one grid size: times the candidate launch shapes of each
@@ -124,7 +130,7 @@ int crv_grid_create_batch(crv_ctx* ctx, int nx, int ny, int nz, int nxp, int nyp
 /* SeismicParametersHolder::FFTCovGrids / invFFTCovGrids
  * (seismicparametersholder.cpp:273-322) and the means'/data's transforms
  * over a batch: every grid of one contiguous batch (same size, type and
- * state) transformed with one launch per pass.  _from writes the spectra of
+ * state) transformed by sub-batch launches.  _from writes the spectra of
  * the src batch into the dst batch (src unchanged).                         */
 int crv_fft_r2c_batch(crv_grid** grids, int count);
 int crv_fft_r2c_batch_from(crv_grid** dst, crv_grid** src, int count);
@@ -136,8 +142,8 @@ int crv_fft_c2r_batch(crv_grid** grids, int count);
 /* nreal realizations of one posterior with the counter noise stream (seed,
  * realization0 + r): out is a contiguous batch of 3 nreal grids, out[3r+g]
  * the realization r of parameter g (posterior means added when post_mean is
- * given); one launch for the noise and scaling of all realizations, one per
- * inverse-transform pass.  Launches only: capturable.                      */
+ * given); one launch for the noise and scaling of all realizations, then
+ * the batched inverse transforms.  Launches only: capturable.              */
 int crv_simulate_batch(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[3], unsigned long long seed,
                        unsigned long long realization0, int nreal, int fp64_chol, crv_grid** out);
 typedef struct crv_graph crv_graph;

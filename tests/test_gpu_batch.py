@@ -69,11 +69,54 @@ def _posterior_cov(ctx, case):
     return mean, cov
 
 
+@pytest.fixture
+def batch_mb(ctx, request):
+    """CRV_TUNE_BATCH_MB for one test, restored afterwards (1 MB splits the
+    batches of these small grids into sub-batches)."""
+    from crava_amd import _lib as L
+    ctx.tune(L.CRV_TUNE_BATCH_MB, request.param)
+    yield request.param
+    ctx.tune(L.CRV_TUNE_BATCH_MB, 0)
+
+
+@pytest.mark.parametrize("batch_mb", [0, 1], indirect=True)
+def test_batched_transforms_match_single_grids(ctx, batch_mb):
+    from crava_amd import _lib as L
+    from crava_amd.hip import FFTGrid, fft_c2r_batch, fft_r2c_batch, grid_batch
+    nx, ny, nz, cnt = 64, 48, 40, 7
+    rng = np.random.default_rng(2)
+    xs = [rng.standard_normal((nz, ny, 2 * (nx // 2 + 1))).astype(np.float32) for _ in range(cnt)]
+    src = grid_batch(ctx, cnt, nx, ny, nz, 32, L.CRV_DATA)
+    dst = grid_batch(ctx, cnt, nx, ny, nz, 32, L.CRV_DATA)
+    inp = grid_batch(ctx, cnt, nx, ny, nz, 32, L.CRV_DATA)
+    for g, h, x in zip(src, inp, xs):
+        g.upload_real(x)
+        h.upload_real(x)
+    fft_r2c_batch(dst, src=src)
+    fft_r2c_batch(inp)
+    ctx.synchronize()
+    one = []
+    for x, g, d, h in zip(xs, src, dst, inp):
+        a = FFTGrid(ctx, nx, ny, nz, cubetype=L.CRV_DATA)
+        a.upload_real(x)
+        a.fftInPlace()
+        want = a.download_complex()
+        assert np.array_equal(d.download_complex(), want)
+        assert np.array_equal(h.download_complex(), want)
+        assert np.array_equal(g.download_real()[:, :, :nx], x[:, :, :nx])   # source untouched
+        one.append(a)
+    fft_c2r_batch(dst)
+    for a, d in zip(one, dst):
+        a.invFFTInPlace()
+        assert np.array_equal(d.download_real()[:, :, :nx], a.download_real()[:, :, :nx])
+
+
+@pytest.mark.parametrize("batch_mb", [0, 1], indirect=True)
 @pytest.mark.parametrize("fp64_chol", [False, True])
-def test_simulate_batch_matches_single_calls(ctx, fp64_chol):
+def test_simulate_batch_matches_single_calls(ctx, fp64_chol, batch_mb):
     from crava_amd import _lib as L
     from crava_amd.hip import FFTGrid, grid_batch, simulate, simulate_batch
-    case = Case(model.config_small(20, 18, 30, 5))
+    case = Case(model.config_small(64, 48, 40, 5))
     s = case.s
     mean, cov = _posterior_cov(ctx, case)
     nreal = 4
