@@ -53,13 +53,20 @@ def make_grids(ctx, s, prec):
 
 
 def step(ctx, G, kw, mode, realization, seed):
-    """One pass of the hot path (see module docstring)."""
+    """One pass of the hot path (see module docstring): crv_invert, which
+    runs the z transforms, the posterior and the noise by chunks of y rows
+    while their data are in the memory-side cache, or ("sequence") the
+    one-call entry points one after the other."""
     from crava_amd import _lib as L
     from crava_amd import hip
+    plan = G["plan"]
+    if G["path"] == "pipeline":
+        plan.invert(G["mean"], G["data"], G["cov"], G["sim"], mean_src=G["mean_in"], data_src=G["data_in"],
+                    seed=seed, realization0=realization)
+        return
     hip.fft_r2c_batch(G["data"], src=G["data_in"])
     hip.fft_r2c_batch(G["mean"], src=G["mean_in"])
-    hip.computePostMeanResidAndFFTCov(ctx, G["mean"], G["data"], None, G["cov"], mode=mode, separable=True,
-                                      want_stats=False, skip_mean_inverse=True, **kw)
+    plan.exec(G["mean"], G["data"], None, G["cov"])
     hip.fft_c2r_batch(G["mean"])
     hip.simulate_batch(ctx, G["cov"], G["mean"], G["sim"], seed=seed, realization0=realization,
                        fp64_chol=(mode == L.CRV_MODE_FP32_REFINE))
@@ -92,12 +99,13 @@ def transforms_per_step(name, s):
 
 
 def launch_bytes(name, s, prec, ninv, launches_per_step):
-    """Algorithmic bytes of one launch: a transform pass split into chunks
-    moves its share of the whole-grid figure."""
+    """Algorithmic bytes of one launch: a kernel whose work per step is split
+    over several launches (chunks of rows, sub-batches of grids) moves its
+    share of the step's figure."""
     b = kernel_bytes(name, s, prec, ninv)
-    t = transforms_per_step(name, s)
-    if b is None or t is None:
-        return b
+    if b is None:
+        return None
+    t = transforms_per_step(name, s) or 1
     return b * t / launches_per_step
 
 
@@ -216,11 +224,12 @@ def cpu_baseline(s, nthreads):
 
 # profiled launch name -> kernel symbol prefix in the rocprofv3 summaries
 KERNEL_SYMBOLS = {
-    "fft_x_r2c": "k_rows_r2c<float>", "fft_x_c2r": "k_rows_c2r<float>",
-    "fft_y_fwd": "k_cols<float, false>", "fft_z_fwd": "k_cols<float, false>",
-    "fft_y_inv": "k_cols<float, true>", "fft_z_inv": "k_cols<float, true>",
-    "posterior_fp32": "k_posterior<float, ", "simulate_scale": "k_simulate<float",
-    "simulate_scale_batch": "k_simulate<float",
+    # kernels of the FP32-refine step at 450x450x320 (x 225 = 15 x 15 as a
+    # complex half-length transform, y 450 = 25 x 18, z 320 = 20 x 16)
+    "fft_x_r2c": "k_rows_r2c<float, 15, 15>", "fft_x_c2r": "k_rows_c2r<float, 15, 15>",
+    "fft_y_fwd": "k_cols<float, false, 25, 18>", "fft_z_fwd": "k_cols<float, false, 20, 16>",
+    "fft_y_inv": "k_cols<float, true, 25, 18>", "fft_z_inv": "k_cols<float, true, 20, 16>",
+    "posterior_fp32": "k_posterior<float, 5, true, true>", "simulate_scale_batch": "k_simulate<float, true>",
 }
 
 
@@ -235,10 +244,8 @@ def read_traffic(kernel):
         tab = json.loads(p.read_text())
     except (OSError, ValueError):
         return None
-    for k, v in tab.items():
-        if k.startswith(sym) and isinstance(v, dict):
-            return v.get("bytes")
-    return None
+    v = tab.get(sym)
+    return v.get("bytes") if isinstance(v, dict) else None
 
 
 def bench_c3(a):
@@ -386,6 +393,8 @@ def main():
     ap.add_argument("--fp32-mode", choices=["refine", "plain"], default="refine",
                     help="FP32 path: refinement + FP64 fallback (the metric) or the reference's plain FP32")
     ap.add_argument("--chunk-mb", type=int, default=0, help="CRV_TUNE_CHUNK_MB for the transforms")
+    ap.add_argument("--path", choices=["pipeline", "sequence"], default="pipeline",
+                    help="crv_invert, or the one-call entry points in sequence")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain 1 GiB device copies (the reference point for HBM counter readings)")
@@ -427,6 +436,11 @@ def main():
         if prec == 64 and a.no_fp64:
             continue
         G = make_grids(ctx, s, prec)
+        # per-kz tables on the device once (crv_post_plan), as a caller that
+        # inverts many surveys of one geometry keeps them
+        G["plan"] = hip.PostPlan(ctx, s.nxp, s.nyp, s.nzp, prec, s.n, mode=mode, separable=True,
+                                 skip_mean_inverse=True, **kw)
+        G["path"] = a.path
         for g, src in zip(G["mean_in"], sv.mean):
             g.copyFrom(src)
         for g, src in zip(G["data_in"], sv.data):
@@ -496,7 +510,7 @@ def main():
         "config": {"workload": f"AVO inversion {s.nxp}x{s.nyp}x{s.nzp} padded, {s.n} angles "
                                f"({'/'.join(str(int(x)) for x in s.angles_deg)} deg), separable prior, "
                                "posterior mean + 6 cov + 1 realization",
-                   "voxels": s.voxels, "angles": s.n},
+                   "voxels": s.voxels, "angles": s.n, "path": a.path},
         "roofline": roof,
         "pipeline": {"bytes_per_voxel": bpv, "GB/s": bpv * value / 1e9, "frac": bpv * value / PEAK_HBM},
         "fp64": ({"value": s.voxels / res[64]["sec"], "ms_per_step": res[64]["sec"] * 1e3,

@@ -30,13 +30,14 @@ EXPORTS = [
     "crv_forward_model", "crv_add_white_noise", "crv_grid_moments", "crv_ctx_tune", "crv_rock_forward",
     "crv_mlp_param_count", "crv_mlp_create", "crv_mlp_destroy", "crv_mlp_set_params", "crv_mlp_get_params",
     "crv_mlp_gradients", "crv_mlp_train_step", "crv_mlp_predict", "crv_mlp_flops", "crv_rock_refine",
-    "crv_post_plan_create", "crv_post_plan_destroy", "crv_posterior_exec", "crv_post_plan_stats",
+    "crv_post_plan_create", "crv_post_plan_destroy", "crv_posterior_exec", "crv_post_plan_stats", "crv_invert",
     "crv_fft_autotune", "crv_grid_create_batch", "crv_simulate_batch", "crv_graph_begin", "crv_graph_end",
     "crv_graph_launch", "crv_graph_destroy", "crv_facies_density", "crv_facies_prob",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
 CRV_TUNE_BATCH_MB = 6
+CRV_TUNE_ORDER = 7
 
 
 class GridInfo(C.Structure):
@@ -157,6 +158,8 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_post_plan_destroy": (i, [vp]),
         "crv_posterior_exec": (i, [vp, C.POINTER(vp), C.POINTER(vp), vp, C.POINTER(vp)]),
         "crv_post_plan_stats": (i, [vp, C.POINTER(PostStats)]),
+        "crv_invert": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                           C.c_ulonglong, C.c_ulonglong, i, C.POINTER(vp), i]),
         "crv_fft_autotune": (i, [vp, i, i, i, i, i, C.POINTER(FftShape)]),
         "crv_grid_create_batch": (i, [vp, i, i, i, i, i, i, i, i, i, C.POINTER(vp)]),
         "crv_simulate_batch": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.c_ulonglong, C.c_ulonglong, i, i,

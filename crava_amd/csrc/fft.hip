@@ -37,24 +37,13 @@ using dft::Dft;
 // values (32 KB of LDS).
 constexpr int kThreads = 256;
 
+// passes that walk their tiles backwards by default (CRV_TUNE_ORDER bits)
+constexpr int kDefaultOrder = 0;
+
 template <typename T> struct Ept;
 template <> struct Ept<float>  { static constexpr int value = 16; };  // 4096 complex / block
 template <> struct Ept<double> { static constexpr int value = 8; };   // 2048 complex / block
 
-// Unsigned division by a runtime constant (n < 2^31): q = (umulhi(n,m)+n)>>s.
-struct FastDiv {
-  unsigned m; int s; int d;
-};
-FastDiv make_fastdiv(int d) {
-  FastDiv f; f.d = d; f.s = 0;
-  while ((1u << f.s) < (unsigned)d) ++f.s;
-  unsigned long long num = (1ull << 32) * (unsigned long long)((1ull << f.s) - d);
-  f.m = (unsigned)(num / (unsigned)d + 1);
-  return f;
-}
-__device__ __forceinline__ int fdiv(int n, FastDiv f) {
-  return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
-}
 
 // per-stage divisors: L / R_s (butterflies per pencil) and Ns
 struct StageDiv {
@@ -73,11 +62,14 @@ StageDiv make_stage_div(const Fft1D& f) {
 
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
 // range of tiles so that neighbouring tiles (which share cache lines at
-// their edges) meet in the same L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+// their edges) meet in the same L2.  rev walks each range backwards: a pass
+// that follows another over the same grid then starts on the tiles written
+// last, which the memory-side cache still holds.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks, int rev) {
   const int x = bid & 7, i = bid >> 3;
   const int q = nblocks >> 3, r = nblocks & 7;
-  return x * q + min(x, r) + i;
+  const int cnt = q + (x < r ? 1 : 0);
+  return x * q + min(x, r) + (rev ? cnt - 1 - i : i);
 }
 
 template <typename T>
@@ -137,8 +129,9 @@ __device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, i
   }
 }
 
-// middle stage: LDS -> LDS (pencil-fast); ends with the tile written
-template <typename T, int R, bool INV>
+// middle stage: LDS -> LDS (pencil-fast); ends with the tile written.
+// TW = false: a first stage (Ns = 1), whose twiddles are all 1.
+template <typename T, int R, bool INV, bool TW = true>
 __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int L, int Ns, FastDiv dns,
                                           const cplx_t<T>* tw) {
   using C = cplx_t<T>;
@@ -152,11 +145,16 @@ __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int
     const int b = threadIdx.x + q * kThreads;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
-      const int k = j - fdiv(j, dns) * Ns;
-      const C* w = tw + k * (R - 1);
       v[q][0] = lds[j * S + c];
+      if constexpr (TW) {
+        const int k = j - fdiv(j, dns) * Ns;
+        const C* w = tw + k * (R - 1);
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = twiddle<T>(lds[(j + r * Lr) * S + c], w[r - 1], INV);
+        for (int r = 1; r < R; ++r) v[q][r] = twiddle<T>(lds[(j + r * Lr) * S + c], w[r - 1], INV);
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = lds[(j + r * Lr) * S + c];
+      }
       Dft<T, R, INV>::run(v[q]);
     }
   }
@@ -166,7 +164,7 @@ __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int
     const int b = threadIdx.x + q * kThreads;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
-      const int k = j - fdiv(j, dns) * Ns;
+      const int k = TW ? j - fdiv(j, dns) * Ns : 0;
       const int base = (j - k) * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) lds[(base + r * Ns) * S + c] = v[q][r];
@@ -280,6 +278,28 @@ __device__ __forceinline__ void transform_tile(cplx_t<T>* lds, int S, int cshift
   }
 }
 
+// Whole transform of a tile already in LDS in natural order (lds[e*S + c]),
+// left there in natural order; every stage runs LDS -> LDS.  The caller
+// synchronises before; the tile is complete on return.
+template <typename T, bool INV, int RF, int RL>
+__device__ __forceinline__ void transform_tile_lds(cplx_t<T>* lds, int S, int cshift, const Fft1D& f,
+                                                   const StageDiv& sd, const cplx_t<T>* twl) {
+  const int L = f.L;
+  if constexpr (RF > 0) {
+    stage_mid<T, RF, INV, false>(lds, S, cshift, L, 1, sd.ns[0], twl);
+    stage_mid<T, RL, INV>(lds, S, cshift, L, f.Ns[1], sd.ns[1], twl + (f.tw[1] - f.tw0));
+    return;
+  }
+  for (int s = 0; s < f.nst; ++s) {
+    const cplx_t<T>* t = twl + (f.tw[s] - f.tw0);
+    if (s == 0) {
+      CRV_RADIX_SWITCH(f.R[0], (stage_mid<T, R_, INV, false>(lds, S, cshift, L, 1, sd.ns[0], t)));
+    } else {
+      CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void copy_to_lds(cplx_t<T>* dst, const cplx_t<T>* src, int n) {
   for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
@@ -300,34 +320,60 @@ struct RowArgs {
   int nrows, rnxp, nxp, cnxp;
   int xeven;
   int pshift;          // rows per block = 1 << pshift
+  int tile;            // complex elements of the tile region
+  int rev;             // walk the tiles backwards (xcd_remap)
   FastDiv divK;        // by cnxp
+  FastDiv divL;        // by f.L
   T scale;
 };
 
+// Row tiles move between HBM and LDS with lanes running along the rows
+// (fully coalesced), and every stage of the transform runs LDS -> LDS.  The
+// tile region holds max((P+1) L, P cnxp) complex: the transform's
+// lds[e*S + c] layout, or, for a C2R, the staged half-spectrum rows.
 template <typename T, int RF, int RL>
 __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   using C = cplx_t<T>;
+  constexpr int E = Ept<T>::value;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
   const int L = a.f.L, K = a.cnxp;
-  C* twl = lds + S * L;
+  C* twl = lds + a.tile;
   C* wpl = twl + a.f.twn;
   copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
   if (a.xeven) copy_to_lds<T>(wpl, a.wpost, K);
-  const int row0 = xcd_remap(blockIdx.x, gridDim.x) << a.pshift;
+  const int row0 = xcd_remap(blockIdx.x, gridDim.x, a.rev) << a.pshift;
   const int nr = min(P, a.nrows - row0);
   const size_t boff = (size_t)blockIdx.y * a.bstride;
   const T* src0 = a.src + boff + (size_t)row0 * a.rnxp;
   T* dst0 = a.dst + boff;
-  auto load = [&](int c, int e) -> C {
-    if (c >= nr) return cmake<C>((T)0, (T)0);
-    const T* row = src0 + (size_t)c * a.rnxp;
-    const C v = a.xeven ? *reinterpret_cast<const C*>(row + 2 * e) : cmake<C>(row[e], (T)0);
-    return cscale(v, a.scale);
-  };
-  auto nostore = [&](int, int, C) {};
-  transform_tile<T, false, true, true, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl, load, nostore);
+  // rows -> registers -> tile (natural order), scaled
+  const int PL = P * L;
+  C v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    v[q] = cmake<C>((T)0, (T)0);
+    if (idx < PL) {
+      const int c = fdiv(idx, a.divL), e = idx - c * L;
+      if (c < nr) {
+        const T* row = src0 + (size_t)c * a.rnxp;
+        v[q] = a.xeven ? *reinterpret_cast<const C*>(row + 2 * e) : cmake<C>(row[e], (T)0);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    if (idx < PL) {
+      const int c = fdiv(idx, a.divL), e = idx - c * L;
+      lds[e * S + c] = cscale(v[q], a.scale);
+    }
+  }
+  __syncthreads();
+  transform_tile_lds<T, false, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl);
+  __syncthreads();
   // post-process and store cnxp complex per row
   for (int idx = threadIdx.x; idx < P * K; idx += kThreads) {
     const int c = fdiv(idx, a.divK), k = idx - c * K;
@@ -352,52 +398,114 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
 template <typename T, int RF, int RL>
 __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   using C = cplx_t<T>;
+  constexpr int E = Ept<T>::value;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
-  const int L = a.f.L;
-  C* twl = lds + S * L;
+  const int L = a.f.L, K = a.cnxp;
+  C* twl = lds + a.tile;
   C* wpl = twl + a.f.twn;
   copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
-  if (a.xeven) copy_to_lds<T>(wpl, a.wpost, a.cnxp);
-  __syncthreads();  // the first stage's pre-processing reads wpl
-  const int row0 = xcd_remap(blockIdx.x, gridDim.x) << a.pshift;
+  if (a.xeven) copy_to_lds<T>(wpl, a.wpost, K);
+  const int row0 = xcd_remap(blockIdx.x, gridDim.x, a.rev) << a.pshift;
   const int nr = min(P, a.nrows - row0);
   const size_t boff = (size_t)blockIdx.y * a.bstride;
-  const T* src0 = a.src + boff;
-  T* dst0 = a.dst + boff;
-  const T* add = a.nadd ? a.add[blockIdx.y % a.nadd] : nullptr;
-  // Z[k] of the half-length complex transform from the half spectrum
-  auto load = [&](int c, int k) -> C {
-    if (c >= nr) return cmake<C>((T)0, (T)0);
-    const C* X = reinterpret_cast<const C*>(src0 + (size_t)(row0 + c) * a.rnxp);
-    if (a.xeven) {
-      C xa = X[k], xb = X[L - k];
-      if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
-      xb = cconj(xb);
-      const C d = csub(xa, xb);
-      const C t = cmulc(d, wpl[k]);  // d * exp(+2 pi i k / nxp)
-      // Z = (xa + xb) + i t
-      return cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
+  const T* src0 = a.src + boff + (size_t)row0 * a.rnxp;
+  T* dst0 = a.dst + boff + (size_t)row0 * a.rnxp;
+  // the added grids are separate allocations, not members of the batch
+  const T* add = a.nadd ? a.add[blockIdx.y % a.nadd] + (size_t)row0 * a.rnxp : nullptr;
+  // half-spectrum rows -> registers -> staging rows lds[c*K + k]
+  const int PK = P * K, PL = P * L;
+  C v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    if (idx < PK) {
+      const int c = fdiv(idx, a.divK), k = idx - c * K;
+      v[q] = (c < nr) ? reinterpret_cast<const C*>(src0 + (size_t)c * a.rnxp)[k] : cmake<C>((T)0, (T)0);
     }
-    C z = (k < a.cnxp) ? X[k] : cconj(X[L - k]);
-    if (k == 0) z.y = (T)0;
-    return z;
-  };
-  auto store = [&](int c, int n, C z) {
-    if (c >= nr) return;
-    const size_t rbase = (size_t)(row0 + c) * a.rnxp;
-    if (a.xeven) {
-      C o = cscale(z, a.scale);
-      if (add) o = cadd(o, *reinterpret_cast<const C*>(add + rbase + 2 * n));
-      *reinterpret_cast<C*>(dst0 + rbase + 2 * n) = o;
-    } else {
-      T o = z.x * a.scale;
-      if (add) o += add[rbase + n];
-      dst0[rbase + n] = o;
+  }
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    if (idx < PK) lds[idx] = v[q];
+  }
+  __syncthreads();
+  // Z[k] of the half-length complex transform (or the full Hermitian row)
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    if (idx < PL) {
+      const int c = fdiv(idx, a.divL), k = idx - c * L;
+      const C* X = lds + c * K;
+      if (a.xeven) {
+        C xa = X[k], xb = X[L - k];
+        if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
+        xb = cconj(xb);
+        const C d = csub(xa, xb);
+        const C t = cmulc(d, wpl[k]);  // d * exp(+2 pi i k / nxp)
+        // Z = (xa + xb) + i t
+        v[q] = cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
+      } else {
+        C z = (k < K) ? X[k] : cconj(X[L - k]);
+        if (k == 0) z.y = (T)0;
+        v[q] = z;
+      }
     }
-  };
-  transform_tile<T, true, true, false, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl, load, store);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    if (idx < PL) {
+      const int c = fdiv(idx, a.divL), k = idx - c * L;
+      lds[k * S + c] = v[q];
+    }
+  }
+  __syncthreads();
+  transform_tile_lds<T, true, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl);
+  __syncthreads();
+  // tile -> rows, scaled, plus the added grid
+  if (a.xeven) {
+    C w[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      w[q] = cmake<C>((T)0, (T)0);
+      if (add && idx < PL) {
+        const int c = fdiv(idx, a.divL), n = idx - c * L;
+        if (c < nr) w[q] = *reinterpret_cast<const C*>(add + (size_t)c * a.rnxp + 2 * n);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      if (idx < PL) {
+        const int c = fdiv(idx, a.divL), n = idx - c * L;
+        if (c < nr)
+          *reinterpret_cast<C*>(dst0 + (size_t)c * a.rnxp + 2 * n) = cadd(cscale(lds[n * S + c], a.scale), w[q]);
+      }
+    }
+  } else {
+    T w[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      w[q] = (T)0;
+      if (add && idx < PL) {
+        const int c = fdiv(idx, a.divL), n = idx - c * L;
+        if (c < nr) w[q] = add[(size_t)c * a.rnxp + n];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      if (idx < PL) {
+        const int c = fdiv(idx, a.divL), n = idx - c * L;
+        if (c < nr) dst0[(size_t)c * a.rnxp + n] = lds[n * S + c].x * a.scale + w[q];
+      }
+    }
+  }
 }
 
 // ---- y / z passes: strided complex pencils ---------------------------------
@@ -413,6 +521,7 @@ struct ColArgs {
   int ntx;             // x tiles per outer index
   int cshift;          // TX = 1 << cshift
   long long bstride;   // elements between the grids of a batch (blockIdx.y)
+  int rev;             // walk the tiles backwards (xcd_remap)
 };
 
 template <typename T, bool INV, int RF, int RL>
@@ -423,7 +532,7 @@ __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   const int TX = 1 << a.cshift;
   C* twl = lds + a.f.L * TX;
   copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = xcd_remap(blockIdx.x, gridDim.x, a.rev);
   const int o = tile / a.ntx, t = tile - o * a.ntx;
   const int x0 = t * TX;
   const int nc = min(TX, a.ncols - x0);
@@ -493,10 +602,17 @@ bool plan_1d(int L, Fft1D& f, std::vector<long double>& twr, std::vector<long do
 int pow2_floor(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
 int ilog2(int v) { int s = 0; while ((1 << s) < v) ++s; return s; }
 
+// complex elements of a row tile: the transform's (P+1) L, or the staged
+// half-spectrum rows P cnxp of a C2R
+int row_tile_elems(const FftPlan* p) {
+  const int cnxp = p->nxp / 2 + 1;
+  return std::max((p->rows_per_block + 1) * p->fx.L, p->rows_per_block * cnxp);
+}
+
 template <typename T>
 size_t row_lds_bytes(const FftPlan* p) {
   const int cnxp = p->nxp / 2 + 1;
-  return (size_t)((p->rows_per_block + 1) * p->fx.L + p->fx.twn + cnxp) * sizeof(cplx_t<T>);
+  return (size_t)(row_tile_elems(p) + p->fx.twn + cnxp) * sizeof(cplx_t<T>);
 }
 
 // kernel for an axis plan: the two-stage kernel of its radix pair, or the
@@ -546,7 +662,10 @@ int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const 
   a.wpost = a.tw + p->post_off;
   a.nrows = row_end - row_begin; a.rnxp = rnxp; a.nxp = p->nxp; a.cnxp = cnxp; a.xeven = p->xeven ? 1 : 0;
   a.pshift = ilog2(p->rows_per_block);
+  a.tile = row_tile_elems(p);
+  a.rev = (p->order >> (inv ? 5 : 0)) & 1;
   a.divK = make_fastdiv(cnxp);
+  a.divL = make_fastdiv(p->fx.L);
   a.scale = (T)scale;
   const int nb = (a.nrows + p->rows_per_block - 1) / p->rows_per_block;
   void* args[] = {&a};
@@ -572,6 +691,7 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
     a.f = p->fy; a.estride = cnxp; a.ostride = (long long)cnxp * p->nyp; tx = p->tx_y;
   }
   a.sd = make_stage_div(a.f);
+  a.rev = (p->order >> (zaxis ? (inv ? 3 : 2) : (inv ? 4 : 1))) & 1;
   a.data = data + (size_t)o0 * a.ostride;
   a.cshift = ilog2(tx);
   a.ntx = (cnxp + tx - 1) / tx;
@@ -700,7 +820,7 @@ int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** ou
 }
 
 // Launch shapes: a tile holds at most 4096 (FP32) / 2048 (FP64) complex
-// values (32 KB of LDS); x rows per block P with (P+1) L <= that; y/z tiles
+// values (32 KB of LDS); x rows per block P with (P+1) L and P cnxp <= that; y/z tiles
 // of TX consecutive x columns, 16 (FP32) or 8 (FP64) at most, halved until
 // the pencil tile fits.  CRV_TUNE_* values override, within those limits.
 int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
@@ -709,8 +829,9 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   const int Lx = p->fx.L;
   int P = 1;
   const int pmax = ctx->tune[CRV_TUNE_ROWS] > 0 ? ctx->tune[CRV_TUNE_ROWS] : 64;
-  while (2 * P <= pmax && (2 * P + 1) * Lx <= maxe) P *= 2;
-  if ((P + 1) * Lx > maxe) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nxp too large");
+  const int cnx = p->nxp / 2 + 1;
+  while (2 * P <= pmax && (2 * P + 1) * Lx <= maxe && 2 * P * cnx <= maxe) P *= 2;
+  if ((P + 1) * Lx > maxe || P * cnx > maxe) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nxp too large");
   p->rows_per_block = P;
   auto pick_tx = [&](int L, int want) {
     int t = want > 0 ? pow2_floor(want) : txdef;
@@ -722,6 +843,7 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   if (!p->tx_y || !p->tx_z) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nyp or nzp too large");
   p->chunk_mb = ctx->tune[CRV_TUNE_CHUNK_MB];
   p->batch_mb = ctx->tune[CRV_TUNE_BATCH_MB];
+  p->order = ctx->tune[CRV_TUNE_ORDER] > 0 ? ctx->tune[CRV_TUNE_ORDER] - 1 : kDefaultOrder;
   return CRV_OK;
 }
 
@@ -752,6 +874,8 @@ int sub_batch(const FftPlan* p, int nbatch) {
   if (grid > 2 * mall) return nbatch;
   return (int)std::max<size_t>(1, std::min<size_t>((size_t)nbatch, mall / grid));
 }
+
+int crv_sub_batch(const FftPlan* p, int nbatch) { return sub_batch(p, nbatch); }
 
 // nbatch grids of this plan, bstride reals apart, transformed in place; the
 // output of batch entry b gets add[b % nadd] added
@@ -790,6 +914,32 @@ int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src
     if (rc != CRV_OK) return rc;
   }
   return CRV_OK;
+}
+
+// One pass of a transform over part of a batch (see internal.hpp).
+int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
+                    int o0, int o1, double scale, const void* const* add, int nadd) {
+  if (nbatch < 1 || nbatch > 65535 || nadd < 0 || nadd > 3 || pass < 0 || pass > 5 || o0 < 0 || o1 < o0)
+    return crv_set_error(ctx, CRV_E_INVALID, "bad pass");
+  if (o1 == o0) return CRV_OK;
+  Batch bt;
+  bt.n = nbatch; bt.stride = bstride; bt.nadd = nadd;
+  for (int q = 0; q < nadd; ++q) bt.add[q] = add[q];
+  const bool f64 = plan->prec == CRV_FP64;
+  switch (pass) {
+    case 0:
+      return f64 ? launch_rows<double>(ctx, plan, false, (double*)buf, (const double*)src, bt, o0, o1, scale)
+                 : launch_rows<float>(ctx, plan, false, (float*)buf, (const float*)src, bt, o0, o1, scale);
+    case 5:
+      return f64 ? launch_rows<double>(ctx, plan, true, (double*)buf, (const double*)buf, bt, o0, o1, scale)
+                 : launch_rows<float>(ctx, plan, true, (float*)buf, (const float*)buf, bt, o0, o1, scale);
+    default: {
+      const bool inv = pass >= 3, zaxis = (pass == 2 || pass == 3);
+      if ((zaxis ? plan->nzp : plan->nyp) == 1) return CRV_OK;
+      return f64 ? launch_cols<double>(ctx, plan, inv, zaxis, (double2*)buf, bt, o0, o1)
+                 : launch_cols<float>(ctx, plan, inv, zaxis, (float2*)buf, bt, o0, o1);
+    }
+  }
 }
 
 // ---- autotuner ---------------------------------------------------------------
@@ -877,7 +1027,7 @@ extern "C" int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int pre
     float best = 1e30f;
     int bestP = p->rows_per_block;
     for (int P = 1; P <= 64 && rc == CRV_OK; P *= 2) {
-      if ((P + 1) * p->fx.L > maxe) break;
+      if ((P + 1) * p->fx.L > maxe || P * (p->nxp / 2 + 1) > maxe) break;
       p->rows_per_block = P;
       float ms = 0.0f;
       if ((rc = timed(rows, ms)) == CRV_OK && ms < best) { best = ms; bestP = P; }

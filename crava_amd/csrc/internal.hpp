@@ -27,6 +27,23 @@ template <typename C> __host__ __device__ inline C cconj(C a) { return cmake<C>(
 template <typename C> __host__ __device__ inline C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
 
 // ---------------------------------------------------------------------------
+// Unsigned division by a runtime constant d (numerators below 2^31):
+// q = (umulhi(n, m) + n) >> s with s = ceil(log2 d), m = 2^32 (2^s - d) / d + 1.
+struct FastDiv {
+  unsigned m; int s; int d;
+};
+inline FastDiv make_fastdiv(int d) {
+  FastDiv f; f.d = d; f.s = 0;
+  while ((1u << f.s) < (unsigned)d) ++f.s;
+  unsigned long long num = (1ull << 32) * (unsigned long long)((1ull << f.s) - d);
+  f.m = (unsigned)(num / (unsigned)d + 1);
+  return f;
+}
+__device__ __forceinline__ int fdiv(int n, FastDiv f) {
+  return (int)((__umulhi((unsigned)n, f.m) + (unsigned)n) >> f.s);
+}
+
+// ---------------------------------------------------------------------------
 // 1D transform plan: Stockham stages over LDS; the twiddles of all stages
 // of one axis are a contiguous range [tw0, tw0 + twn) of the plan's table,
 // copied to LDS by each block.
@@ -51,6 +68,7 @@ struct FftPlan {
   int tx_y = 0, tx_z = 0;
   int chunk_mb = 0;      // x/y pass interleave chunk (MB of grid), 0 = off
   int batch_mb = 0;      // batched transforms: MB of grids per launch, 0 = 256
+  int order = 0;         // CRV_TUNE_ORDER bits: passes that walk their tiles backwards
   int rev[3] = {0, 0, 0};  // radix order per axis (0 default, 1 reversed)
   // reference noise: per-row prefix of draws (nyp*nzp+1 int64), lazily made
   long long* noise_prefix = nullptr;
@@ -138,6 +156,19 @@ int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, lon
                          const void* const* add, int nadd);
 int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, int nbatch, long long bstride,
                          double scale);
+// One pass of a transform over part of a batch of nbatch grids bstride reals
+// apart: pass 0 x R2C (rows [o0, o1), row = j + nyp k; reads src, scales),
+// 1 y forward (z planes [o0, o1)), 2 z forward (y rows [o0, o1)), 3 z
+// inverse, 4 y inverse, 5 x C2R (rows, in place, scales, adds add[b % nadd]).
+// grids per launch of a batched transform of nbatch grids (CRV_TUNE_BATCH_MB)
+int crv_sub_batch(const FftPlan* p, int nbatch);
+int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
+                    int o0, int o1, double scale, const void* const* add, int nadd);
+// Noise and Cholesky scaling of nreal counter-stream realizations over y
+// rows [j0, j1): realization r, parameter g written at out0 + (3 r + g)
+// gstride complex elements (simulate.hip).
+int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], void* out0, long long gstride,
+                        int nreal, unsigned long long seed, unsigned long long real0, int fp64_chol, int j0, int j1);
 
 // elementwise (simulate.hip) ---------------------------------------------------
 int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n);

@@ -14,22 +14,39 @@
 struct NoiseGeom {
   int cnxp, nyp, nzp;
   const long long* prefix;  // draws before each row (reference stream only)
+  FastDiv dc, dy;           // by cnxp, nyp
+  int small;                // cnxp nyp nzp < 2^31: 32-bit index arithmetic
 };
 
 inline NoiseGeom make_noise_geom(const FftPlan* p) {
   NoiseGeom g;
   g.cnxp = p->nxp / 2 + 1; g.nyp = p->nyp; g.nzp = p->nzp; g.prefix = p->noise_prefix;
+  g.dc = make_fastdiv(g.cnxp); g.dy = make_fastdiv(g.nyp);
+  g.small = ((long long)g.cnxp * g.nyp * g.nzp) < (1LL << 31);
   return g;
 }
 
 // classification of element i: 0 ordinary, 1 self-conjugate, 2 owns draws
 // (partner later), 3 copies the partner (returned in partner)
 __device__ __forceinline__ int noise_kind(const NoiseGeom& g, long long i, long long& partner, long long& jk, int& x) {
-  x = (int)(i % g.cnxp);
-  jk = i / g.cnxp;
+  if (g.small) {
+    const int q = fdiv((int)i, g.dc);
+    x = (int)i - q * g.cnxp;
+    jk = q;
+  } else {
+    x = (int)(i % g.cnxp);
+    jk = i / g.cnxp;
+  }
   const bool special = (x == 0) || (x == g.cnxp - 1 && (i & 1));
   if (!special) return 0;
-  const int j = (int)(jk % g.nyp), k = (int)(jk / g.nyp);
+  int j, k;
+  if (g.small) {
+    k = fdiv((int)jk, g.dy);
+    j = (int)jk - k * g.nyp;
+  } else {
+    j = (int)(jk % g.nyp);
+    k = (int)(jk / g.nyp);
+  }
   const int jcc = (j == 0) ? 0 : g.nyp - j, kcc = (k == 0) ? 0 : g.nzp - k;
   const long long jkcc = jcc + (long long)kcc * g.nyp;
   if (jkcc == jk) return 1;

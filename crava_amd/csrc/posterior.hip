@@ -69,15 +69,33 @@ struct PostArgs {
   unsigned int* list;         // refine mode: wavenumbers for the FP64 solve
   unsigned int* list_count;
   float cond_thr;
-  double lowk2;
+  float lowk2;
+  float rx, ry, rz;           // 2 / nxp, 2 / nyp, 2 / nzp
+  FastDiv dplane, dcx;        // by cnxp nyp, cnxp (indices below 2^31)
+  // the main pass covers y rows [j0, j0 + nyr) of every kz plane:
+  // ncr = cnxp nyr nzp wavenumbers, dplr divides by cnxp nyr
+  int j0, nyr;
+  long long ncr;
+  FastDiv dplr;
 };
 
-__device__ __forceinline__ void decode(long long idx, int cnxp, int nyp, int& i, int& j, int& k) {
-  const long long plane = (long long)cnxp * nyp;
-  k = (int)(idx / plane);
-  const int rem = (int)(idx - (long long)k * plane);
-  j = rem / cnxp;
-  i = rem - j * cnxp;
+// wavenumber t of the main pass's row range -> (i, j, k) and its index
+template <typename T>
+__device__ __forceinline__ long long decode_rows(const PostArgs<T>& a, long long t, int& i, int& j, int& k) {
+  k = fdiv((int)t, a.dplr);
+  const int rem = (int)t - k * a.dplr.d;
+  const int jl = fdiv(rem, a.dcx);
+  i = rem - jl * a.cnxp;
+  j = a.j0 + jl;
+  return (long long)i + (long long)a.cnxp * ((long long)j + (long long)a.nyp * k);
+}
+
+template <typename T>
+__device__ __forceinline__ void decode(const PostArgs<T>& a, long long idx, int& i, int& j, int& k) {
+  k = fdiv((int)idx, a.dplane);
+  const int rem = (int)idx - k * a.dplane.d;
+  j = fdiv(rem, a.dcx);
+  i = rem - j * a.cnxp;
 }
 
 // Cleaned prior from the covariance grids (getParameterCovariance keeps
@@ -171,9 +189,9 @@ __global__ __launch_bounds__(kThreads) void k_posterior(PostArgs<T> a) {
   unsigned long long nfail = 0;
   double cmax = 0.0;
   const long long stride = (long long)gridDim.x * kThreads;
-  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < a.nc; idx += stride) {
+  for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < a.ncr; t += stride) {
     int i, j, k;
-    decode(idx, a.cnxp, a.nyp, i, j, k);
+    const long long idx = decode_rows(a, t, i, j, k);
     T lam, s[6];
     if constexpr (SEP) {
       lam = a.t.lam[(size_t)j * a.cnxp + i];
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior(PostArgs<T> a) {
     const T cond = ok ? pmax / pmin : (T)0;
     if constexpr (LIST) {
       const int ky = min(j, a.nyp - j), kz = min(k, a.nzp - k);
-      const double fx = (double)i / (0.5 * a.nxp), fy = (double)ky / (0.5 * a.nyp), fz = (double)kz / (0.5 * a.nzp);
+      const float fx = (float)i * a.rx, fy = (float)ky * a.ry, fz = (float)kz * a.rz;
       const bool lowk = (fx * fx + fy * fy + fz * fz) < a.lowk2;
       if (!ok || lowk || cond > a.cond_thr) {
         const unsigned int slot = atomicAdd(a.list_count, 1u);
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior_list64(PostArgs<float> a
   for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < n; t += gridDim.x * kThreads) {
     const long long idx = a.list[t];
     int i, j, k;
-    decode(idx, a.cnxp, a.nyp, i, j, k);
+    decode(a, idx, i, j, k);
     double lam, s[6];
     if constexpr (SEP) {
       lam = a.t64.lam[(size_t)j * a.cnxp + i];
@@ -365,7 +383,7 @@ int persistent_blocks(crv_ctx* ctx, F kernel, long long work) {
 
 template <typename T, int N, bool SEP, bool LIST>
 int launch_main(crv_ctx* ctx, const PostArgs<T>& a) {
-  const int nb = persistent_blocks(ctx, k_posterior<T, N, SEP, LIST>, a.nc);
+  const int nb = persistent_blocks(ctx, k_posterior<T, N, SEP, LIST>, a.ncr);
   ProfScope ps(ctx, LIST ? "posterior_fp32" : "posterior");
   hipLaunchKernelGGL((k_posterior<T, N, SEP, LIST>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   CRV_HIP_CHECK(ctx, hipGetLastError());
@@ -568,9 +586,9 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
   pl->ctx = ctx; pl->n = p->ntheta; pl->nxp = nxp; pl->nyp = nyp; pl->nzp = nzp; pl->cnxp = nxp / 2 + 1;
   pl->precision = precision; pl->separable = p->separable != 0; pl->refine = refine; pl->async = async;
   pl->nc = (long long)pl->cnxp * nyp * nzp;
-  if (refine && pl->nc > 0xffffffffLL) {
+  if (pl->nc >= (1LL << 31)) {
     delete pl;
-    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "refine mode: grid above 2^32 wavenumbers");
+    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "posterior: grid of 2^31 or more wavenumbers");
   }
   const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
   for (int q = 0; q < 6; ++q) pl->sig0[q] = p->sigma0[pa[q] * 3 + pb[q]];
@@ -628,6 +646,77 @@ extern "C" int crv_post_plan_destroy(crv_post_plan* pl) {
   return CRV_OK;
 }
 
+namespace {
+
+// after a row range's FP64 pass: count its queued wavenumbers into stats[9]
+// and empty the list for the next range
+__global__ void k_list_done(unsigned long long* stats) {
+  unsigned int* cnt = (unsigned int*)(stats + 8);
+  stats[9] += *cnt;
+  *cnt = 0u;
+}
+
+}  // namespace
+
+// The posterior over y rows [j0, j1) of every kz plane (launches only).
+// reset clears the statistics first (the first range of an exec).
+int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, crv_grid* err_corr,
+              crv_grid* const* cov, int j0, int j1, bool reset) {
+  crv_ctx* ctx = pl->ctx;
+  const int n = pl->n;
+  unsigned long long* dstats = (unsigned long long*)(pl->dtab + pl->oS);
+  if (reset) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, dstats, 32);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  const int nyr = j1 - j0;
+  int rc;
+  if (pl->precision == CRV_FP64) {
+    PostArgs<double> a{};
+    for (int m = 0; m < 3; ++m) a.mean[m] = (double2*)mean[m]->d;
+    for (int l = 0; l < n; ++l) a.data[l] = (double2*)data[l]->d;
+    a.err = pl->separable ? nullptr : (const double2*)err_corr->d;
+    for (int q = 0; q < 6; ++q) a.cov[q] = (double2*)cov[q]->d;
+    a.t = tab_at<double>(pl->dtab, pl->oT);
+    a.inv = (const int*)(pl->dtab + pl->oI);
+    for (int q = 0; q < 6; ++q) a.sig0[q] = a.sig0d[q] = pl->sig0[q];
+    a.nxp = pl->nxp; a.cnxp = pl->cnxp; a.nyp = pl->nyp; a.nzp = pl->nzp; a.nc = pl->nc;
+    a.dplane = make_fastdiv(pl->cnxp * pl->nyp); a.dcx = make_fastdiv(pl->cnxp);
+    a.j0 = j0; a.nyr = nyr; a.ncr = (long long)pl->cnxp * nyr * pl->nzp; a.dplr = make_fastdiv(pl->cnxp * nyr);
+    a.stats = dstats;
+    rc = pl->separable ? dispatch<double, true, false>(ctx, n, a) : dispatch<double, false, false>(ctx, n, a);
+  } else {
+    PostArgs<float> a{};
+    for (int m = 0; m < 3; ++m) a.mean[m] = (float2*)mean[m]->d;
+    for (int l = 0; l < n; ++l) a.data[l] = (float2*)data[l]->d;
+    a.err = pl->separable ? nullptr : (const float2*)err_corr->d;
+    for (int q = 0; q < 6; ++q) a.cov[q] = (float2*)cov[q]->d;
+    a.t = tab_at<float>(pl->dtab, pl->oT);
+    a.inv = (const int*)(pl->dtab + pl->oI);
+    for (int q = 0; q < 6; ++q) { a.sig0d[q] = pl->sig0[q]; a.sig0[q] = (float)pl->sig0[q]; }
+    a.nxp = pl->nxp; a.cnxp = pl->cnxp; a.nyp = pl->nyp; a.nzp = pl->nzp; a.nc = pl->nc;
+    a.dplane = make_fastdiv(pl->cnxp * pl->nyp); a.dcx = make_fastdiv(pl->cnxp);
+    a.j0 = j0; a.nyr = nyr; a.ncr = (long long)pl->cnxp * nyr * pl->nzp; a.dplr = make_fastdiv(pl->cnxp * nyr);
+    a.stats = dstats;
+    if (pl->refine) {
+      a.t64 = tab_at<double>(pl->dtab, pl->o64);
+      a.list = pl->dlist;
+      a.list_count = (unsigned int*)(pl->dtab + pl->oS + 64);
+      a.cond_thr = pl->cond_thr;
+      a.lowk2 = (float)pl->lowk2;
+      a.rx = 2.0f / (float)pl->nxp; a.ry = 2.0f / (float)pl->nyp; a.rz = 2.0f / (float)pl->nzp;
+      rc = pl->separable ? dispatch<float, true, true>(ctx, n, a) : dispatch<float, false, true>(ctx, n, a);
+      if (rc == CRV_OK) {
+        hipLaunchKernelGGL(k_list_done, dim3(1), dim3(1), 0, ctx->stream, dstats);
+        CRV_HIP_CHECK(ctx, hipGetLastError());
+      }
+    } else {
+      rc = pl->separable ? dispatch<float, true, false>(ctx, n, a) : dispatch<float, false, false>(ctx, n, a);
+    }
+  }
+  return rc;
+}
+
 extern "C" int crv_posterior_exec(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data, crv_grid* err_corr,
                                   crv_grid* cov[6]) {
   if (!pl) return CRV_E_INVALID;
@@ -646,47 +735,160 @@ extern "C" int crv_posterior_exec(crv_post_plan* pl, crv_grid* mean[3], crv_grid
     if (!fits(cov[q], !pl->separable)) return crv_set_error(ctx, CRV_E_STATE, "cov: wrong size/precision or state");
   if (!pl->separable && !fits(err_corr, true))
     return crv_set_error(ctx, CRV_E_STATE, "err_corr: wrong size/precision or not transformed");
-  unsigned long long* dstats = (unsigned long long*)(pl->dtab + pl->oS);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, dstats, 32);
-  CRV_HIP_CHECK(ctx, hipGetLastError());
-  int rc;
-  if (pl->precision == CRV_FP64) {
-    PostArgs<double> a{};
-    for (int m = 0; m < 3; ++m) a.mean[m] = (double2*)mean[m]->d;
-    for (int l = 0; l < n; ++l) a.data[l] = (double2*)data[l]->d;
-    a.err = pl->separable ? nullptr : (const double2*)err_corr->d;
-    for (int q = 0; q < 6; ++q) a.cov[q] = (double2*)cov[q]->d;
-    a.t = tab_at<double>(pl->dtab, pl->oT);
-    a.inv = (const int*)(pl->dtab + pl->oI);
-    for (int q = 0; q < 6; ++q) a.sig0[q] = a.sig0d[q] = pl->sig0[q];
-    a.nxp = pl->nxp; a.cnxp = pl->cnxp; a.nyp = pl->nyp; a.nzp = pl->nzp; a.nc = pl->nc;
-    a.stats = dstats;
-    rc = pl->separable ? dispatch<double, true, false>(ctx, n, a) : dispatch<double, false, false>(ctx, n, a);
-  } else {
-    PostArgs<float> a{};
-    for (int m = 0; m < 3; ++m) a.mean[m] = (float2*)mean[m]->d;
-    for (int l = 0; l < n; ++l) a.data[l] = (float2*)data[l]->d;
-    a.err = pl->separable ? nullptr : (const float2*)err_corr->d;
-    for (int q = 0; q < 6; ++q) a.cov[q] = (float2*)cov[q]->d;
-    a.t = tab_at<float>(pl->dtab, pl->oT);
-    a.inv = (const int*)(pl->dtab + pl->oI);
-    for (int q = 0; q < 6; ++q) { a.sig0d[q] = pl->sig0[q]; a.sig0[q] = (float)pl->sig0[q]; }
-    a.nxp = pl->nxp; a.cnxp = pl->cnxp; a.nyp = pl->nyp; a.nzp = pl->nzp; a.nc = pl->nc;
-    a.stats = dstats;
-    if (pl->refine) {
-      a.t64 = tab_at<double>(pl->dtab, pl->o64);
-      a.list = pl->dlist;
-      a.list_count = (unsigned int*)(pl->dtab + pl->oS + 64);
-      a.cond_thr = pl->cond_thr;
-      a.lowk2 = pl->lowk2;
-      rc = pl->separable ? dispatch<float, true, true>(ctx, n, a) : dispatch<float, false, true>(ctx, n, a);
-    } else {
-      rc = pl->separable ? dispatch<float, true, false>(ctx, n, a) : dispatch<float, false, false>(ctx, n, a);
-    }
-  }
+  int rc = exec_rows(pl, mean, data, err_corr, cov, 0, pl->nyp, true);
   if (rc != CRV_OK) return rc;
   for (int l = 0; l < n; ++l) data[l]->transformed = true;
   for (int q = 0; q < 6; ++q) { cov[q]->transformed = true; cov[q]->cubetype = CRV_COVARIANCE; }
+  return CRV_OK;
+}
+
+// ---- the whole inversion, pipelined through the memory-side cache ----------
+namespace {
+
+// grids of one role: one launch per pass when they form one contiguous batch
+struct Family {
+  crv_grid* const* g = nullptr;
+  int n = 0;
+  bool contiguous = false;
+};
+
+Family make_family(crv_grid* const* g, int n) {
+  Family f;
+  f.g = g; f.n = n; f.contiguous = true;
+  for (int b = 1; b < n; ++b)
+    if ((char*)g[b]->d != (char*)g[0]->d + (size_t)b * g[0]->rsize * g[0]->elem_bytes()) f.contiguous = false;
+  return f;
+}
+
+int family_pass(crv_ctx* ctx, FftPlan* fp, const Family& f, int pass, int o0, int o1, double scale,
+                const void* const* add, int nadd) {
+  if (f.n == 0) return CRV_OK;
+  if (f.contiguous)
+    return crv_launch_pass(ctx, fp, pass, f.g[0]->d, f.g[0]->d, f.n, (long long)f.g[0]->rsize, o0, o1, scale, add,
+                           nadd);
+  for (int b = 0; b < f.n; ++b) {
+    const void* a1[1] = {nadd ? add[b % nadd] : nullptr};
+    const int rc = crv_launch_pass(ctx, fp, pass, f.g[b]->d, f.g[b]->d, 1, 0, o0, o1, scale, a1, nadd ? 1 : 0);
+    if (rc != CRV_OK) return rc;
+  }
+  return CRV_OK;
+}
+
+// x and y passes (forward: x then y; inverse: y then x) over a family in
+// sub-batches that fit the memory-side cache, so the second pass of a
+// sub-batch reads the first pass's output from there
+int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double scale, const void* const* add,
+              int nadd, crv_grid* const* src = nullptr) {
+  const int nrows = fp->nyp * fp->nzp;
+  // a forward x pass may read other grids (src): one launch per sub-batch
+  // when they form a batch with the same stride, else one per grid
+  const bool src_batch = src && make_family(src, f.n).contiguous && src[0]->rsize == f.g[0]->rsize;
+  const int sb = (f.contiguous && (!src || src_batch)) ? crv_sub_batch(fp, f.n) : 1;
+  for (int b0 = 0; b0 < f.n; b0 += sb) {
+    Family s;
+    s.g = f.g + b0; s.n = std::min(sb, f.n - b0); s.contiguous = f.contiguous;
+    const void* ad[3] = {nullptr, nullptr, nullptr};
+    for (int q = 0; q < nadd; ++q) ad[q] = add[(b0 + q) % nadd];
+    int rc;
+    if (!inverse) {
+      if (src) {
+        if ((rc = crv_launch_pass(ctx, fp, 0, s.g[0]->d, src[b0]->d, s.n, (long long)s.g[0]->rsize, 0, nrows, scale,
+                                  nullptr, 0)) != CRV_OK)
+          return rc;
+      } else if ((rc = family_pass(ctx, fp, s, 0, 0, nrows, scale, nullptr, 0)) != CRV_OK) {
+        return rc;
+      }
+      if ((rc = family_pass(ctx, fp, s, 1, 0, fp->nzp, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    } else {
+      if ((rc = family_pass(ctx, fp, s, 4, 0, fp->nzp, 1.0, nullptr, 0)) != CRV_OK) return rc;
+      if ((rc = family_pass(ctx, fp, s, 5, 0, nrows, scale, ad, nadd)) != CRV_OK) return rc;
+    }
+  }
+  return CRV_OK;
+}
+
+}  // namespace
+
+extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
+                          crv_grid* const* data_src, crv_grid* cov[6], unsigned long long seed,
+                          unsigned long long realization0, int nreal, crv_grid** sims, int chunk_rows) {
+  if (!pl) return CRV_E_INVALID;
+  crv_ctx* ctx = pl->ctx;
+  if (!mean || !data || !cov || nreal < 0 || nreal > 21845 || (nreal > 0 && !sims))
+    return crv_set_error(ctx, CRV_E_INVALID, "crv_invert: bad arguments");
+  if (!pl->separable) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "crv_invert needs a separable-prior plan");
+  const int n = pl->n;
+  auto fits = [&](const crv_grid* g) {
+    return g && g->nxp == pl->nxp && g->nyp == pl->nyp && g->nzp == pl->nzp && g->precision == pl->precision;
+  };
+  for (int m = 0; m < 3; ++m) {
+    const crv_grid* in = mean_src ? mean_src[m] : mean[m];
+    if (!fits(mean[m]) || !fits(in) || in->transformed)
+      return crv_set_error(ctx, CRV_E_STATE, "mean: wrong size or not real");
+  }
+  for (int l = 0; l < n; ++l) {
+    const crv_grid* in = data_src ? data_src[l] : data[l];
+    if (!fits(data[l]) || !fits(in) || in->transformed)
+      return crv_set_error(ctx, CRV_E_STATE, "data: wrong size or not real");
+  }
+  if (mean_src)
+    for (int m = 0; m < 3; ++m) mean[m]->cubetype = mean_src[m]->cubetype;
+  if (data_src)
+    for (int l = 0; l < n; ++l) data[l]->cubetype = data_src[l]->cubetype;
+  for (int q = 0; q < 6; ++q)
+    if (!fits(cov[q])) return crv_set_error(ctx, CRV_E_INVALID, "cov: wrong size");
+  const int nsim = 3 * nreal;
+  for (int b = 0; b < nsim; ++b) {
+    if (!fits(sims[b])) return crv_set_error(ctx, CRV_E_INVALID, "sims: wrong size");
+    if ((char*)sims[b]->d != (char*)sims[0]->d + (size_t)b * sims[0]->rsize * sims[0]->elem_bytes())
+      return crv_set_error(ctx, CRV_E_INVALID, "sims must be one contiguous grid batch");
+  }
+  FftPlan* fp = nullptr;
+  int rc = crv_get_plan(ctx, pl->nxp, pl->nyp, pl->nzp, pl->precision, &fp);
+  if (rc != CRV_OK) return rc;
+  const int nyp = pl->nyp;
+  const Family fd = make_family(data, n), fm = make_family(mean, 3), fc = make_family(cov, 6);
+  const Family fs = nsim ? make_family(sims, nsim) : Family();
+  // rows per chunk: the chunk's share of the grids it touches (inputs,
+  // outputs and realizations) stays near 192 MB
+  if (chunk_rows <= 0) {
+    const double grid = (double)mean[0]->rsize * mean[0]->elem_bytes();
+    const double touched = grid * (n + 3 + 6 + nsim);
+    chunk_rows = (int)std::max(1.0, std::floor(192.0 * 1048576.0 * nyp / touched));
+  }
+  chunk_rows = std::min(chunk_rows, nyp);
+  // forward x and y passes of the data and the means
+  if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src)) != CRV_OK) return rc;
+  if ((rc = family_xy(ctx, fp, fm, false, crv_scale_forward(mean[0]), nullptr, 0, mean_src)) != CRV_OK) return rc;
+  for (int q = 0; q < 6; ++q) cov[q]->cubetype = CRV_COVARIANCE;
+  const void* cv[6];
+  for (int q = 0; q < 6; ++q) cv[q] = cov[q]->d;
+  // per chunk of y rows: z forward, posterior, simulation, z inverse
+  for (int j0 = 0; j0 < nyp; j0 += chunk_rows) {
+    const int j1 = std::min(nyp, j0 + chunk_rows);
+    if ((rc = family_pass(ctx, fp, fd, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    if ((rc = family_pass(ctx, fp, fm, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0)) != CRV_OK) return rc;
+    if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
+                                          realization0, pl->refine ? 1 : 0, j0, j1)) != CRV_OK)
+      return rc;
+    if ((rc = family_pass(ctx, fp, fm, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    if ((rc = family_pass(ctx, fp, fc, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    if ((rc = family_pass(ctx, fp, fs, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+  }
+  // inverse y and x passes: the means first, whose real values the
+  // realizations add in their last pass
+  if ((rc = family_xy(ctx, fp, fm, true, crv_scale_inverse(mean[0]), nullptr, 0)) != CRV_OK) return rc;
+  if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0)) != CRV_OK) return rc;
+  if (nsim) {
+    for (int b = 0; b < nsim; ++b) sims[b]->cubetype = CRV_PARAMETER;
+    const void* add[3] = {mean[0]->d, mean[1]->d, mean[2]->d};
+    if ((rc = family_xy(ctx, fp, fs, true, crv_scale_inverse(sims[0]), add, 3)) != CRV_OK) return rc;
+  }
+  for (int l = 0; l < n; ++l) data[l]->transformed = true;
+  for (int m = 0; m < 3; ++m) mean[m]->transformed = false;
+  for (int q = 0; q < 6; ++q) cov[q]->transformed = false;
+  for (int b = 0; b < nsim; ++b) sims[b]->transformed = false;
   return CRV_OK;
 }
 
@@ -699,7 +901,7 @@ extern "C" int crv_post_plan_stats(crv_post_plan* pl, crv_post_stats* stats) {
   stats->n_wavenumbers = pl->nc;
   stats->n_inverted = pl->ninv_planes * pl->cnxp * pl->nyp;
   stats->n_chol_fail = (long long)hs[0];
-  stats->n_fp64 = pl->refine ? (long long)(unsigned int)(hs[8] & 0xffffffffull) : 0;
+  stats->n_fp64 = pl->refine ? (long long)hs[9] : 0;
   double mc;
   memcpy(&mc, &hs[1], 8);
   stats->max_cond = mc;

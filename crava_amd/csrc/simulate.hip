@@ -34,7 +34,21 @@ struct SimArgs {
   unsigned long long seed, real;
   long long nc;
   long long rstride;         // batches: complex elements between realizations (blockIdx.y)
+  // wavenumbers covered: y rows [j0, j0 + nyr) of every kz plane, ncr of
+  // them (the whole grid: j0 = 0, nyr = nyp, ncr = nc); dplr divides by
+  // cnxp nyr, dcx by cnxp (indices below 2^31)
+  int cnxp, nyp, j0, nyr;
+  long long ncr;
+  FastDiv dplr, dcx;
 };
+
+template <typename T>
+void sim_rows(SimArgs<T>& a, int cnxp, int nyp, int nzp, int j0, int j1) {
+  a.cnxp = cnxp; a.nyp = nyp; a.j0 = j0; a.nyr = j1 - j0;
+  a.ncr = (long long)cnxp * a.nyr * nzp;
+  a.dplr = make_fastdiv(cnxp * a.nyr);
+  a.dcx = make_fastdiv(cnxp);
+}
 
 __device__ __forceinline__ double rsqrt_f64(double x) {
   // hardware estimate and two Newton steps
@@ -137,7 +151,11 @@ __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
   const long long stride = (long long)gridDim.x * kThreads;
   const long long roff = (long long)blockIdx.y * a.rstride;
   const unsigned long long real = a.real + blockIdx.y;
-  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < a.nc; i += stride) {
+  for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < a.ncr; t += stride) {
+    const int k = fdiv((int)t, a.dplr);
+    const int rem = (int)t - k * a.dplr.d;
+    const int jl = fdiv(rem, a.dcx);
+    const long long i = (long long)(rem - jl * a.cnxp) + (long long)a.cnxp * ((long long)(a.j0 + jl) + (long long)a.nyp * k);
     const C c0 = a.cov[0][i], c1 = a.cov[1][i], c2 = a.cov[2][i];
     const C c3 = a.cov[3][i], c4 = a.cov[4][i], c5 = a.cov[5][i];
     // lower triangle packed: P00, P10, P11, P20, P21, P22 (lower = conj(upper))
@@ -272,6 +290,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
   int rc = crv_get_plan(ctx, g0->nxp, g0->nyp, g0->nzp, g0->precision, &plan);
   if (rc != CRV_OK) return rc;
   const long long nc = (long long)g0->csize;
+  if (nc >= (1LL << 31)) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "simulate: grid of 2^31 or more wavenumbers");
   const bool f64 = g0->precision == CRV_FP64;
   void* hostbuf = nullptr;
   long long draws = 0;
@@ -301,6 +320,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     a.geo = make_noise_geom(plan); a.mode = noise->mode; a.normals = ctx->normals; a.draws = draws;
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const double2*)((char*)hostbuf + g * gb) : nullptr;
     a.seed = noise->seed; a.real = noise->realization; a.nc = nc;
+    sim_rows(a, plan->nxp / 2 + 1, plan->nyp, plan->nzp, 0, plan->nyp);
     hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   } else {
     SimArgs<float> a{};
@@ -309,6 +329,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     a.geo = make_noise_geom(plan); a.mode = noise->mode; a.normals = ctx->normals; a.draws = draws;
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const float2*)((char*)hostbuf + g * gb) : nullptr;
     a.seed = noise->seed; a.real = noise->realization; a.nc = nc;
+    sim_rows(a, plan->nxp / 2 + 1, plan->nyp, plan->nzp, 0, plan->nyp);
     if (fp64_chol)
       hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
     else
@@ -339,6 +360,43 @@ extern "C" int crv_fill_complex_noise(crv_grid* g, unsigned int* lcg_state) {
   if ((rc = crv_launch_fill_noise(ctx, plan, g, 0)) != CRV_OK) return rc;
   g->transformed = true;
   g->cubetype = CRV_PARAMETER;
+  return CRV_OK;
+}
+
+// Noise and Cholesky scaling of nreal counter-stream realizations over y
+// rows [j0, j1) of every kz plane: realization r, parameter g at
+// out0 + (3 r + g) gstride complex elements (see internal.hpp).
+int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], void* out0, long long gstride,
+                        int nreal, unsigned long long seed, unsigned long long real0, int fp64_chol, int j0, int j1) {
+  const int cnxp = plan->nxp / 2 + 1;
+  const long long nc = (long long)cnxp * plan->nyp * plan->nzp;
+  if (nc >= (1LL << 31)) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "simulate: grid of 2^31 or more wavenumbers");
+  if (j0 < 0 || j1 > plan->nyp || j1 <= j0 || nreal < 1 || nreal > 65535)
+    return crv_set_error(ctx, CRV_E_INVALID, "simulate: bad rows or realizations");
+  const long long ncr = (long long)cnxp * (j1 - j0) * plan->nzp;
+  const int nb = (int)std::min<long long>((ncr + kThreads - 1) / kThreads, 2048);
+  ProfScope ps(ctx, "simulate_scale_batch");
+  if (plan->prec == CRV_FP64) {
+    SimArgs<double> a{};
+    for (int q = 0; q < 6; ++q) a.cov[q] = (const double2*)cov[q];
+    for (int g = 0; g < 3; ++g) a.out[g] = (double2*)out0 + g * gstride;
+    a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
+    a.seed = seed; a.real = real0; a.nc = nc; a.rstride = 3 * gstride;
+    sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
+    hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+  } else {
+    SimArgs<float> a{};
+    for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q];
+    for (int g = 0; g < 3; ++g) a.out[g] = (float2*)out0 + g * gstride;
+    a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
+    a.seed = seed; a.real = real0; a.nc = nc; a.rstride = 3 * gstride;
+    sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
+    if (fp64_chol)
+      hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+  }
+  CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;
 }
 
@@ -376,31 +434,12 @@ extern "C" int crv_simulate_batch(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post
   FftPlan* plan = nullptr;
   int rc = crv_get_plan(ctx, g0->nxp, g0->nyp, g0->nzp, g0->precision, &plan);
   if (rc != CRV_OK) return rc;
-  const long long nc = (long long)g0->csize;
   const long long gstride = (long long)out[0]->rsize / 2;  // complex elements per grid
-  const int nb = (int)std::min<long long>((nc + kThreads - 1) / kThreads, 2048);
-  {
-    ProfScope ps(ctx, "simulate_scale_batch");
-    if (g0->precision == CRV_FP64) {
-      SimArgs<double> a{};
-      for (int q = 0; q < 6; ++q) a.cov[q] = (const double2*)cov[q]->d;
-      for (int g = 0; g < 3; ++g) a.out[g] = (double2*)out[0]->d + g * gstride;
-      a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
-      a.seed = seed; a.real = realization0; a.nc = nc; a.rstride = 3 * gstride;
-      hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
-    } else {
-      SimArgs<float> a{};
-      for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q]->d;
-      for (int g = 0; g < 3; ++g) a.out[g] = (float2*)out[0]->d + g * gstride;
-      a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
-      a.seed = seed; a.real = realization0; a.nc = nc; a.rstride = 3 * gstride;
-      if (fp64_chol)
-        hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
-      else
-        hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
-    }
-    CRV_HIP_CHECK(ctx, hipGetLastError());
-  }
+  const void* cv[6];
+  for (int q = 0; q < 6; ++q) cv[q] = cov[q]->d;
+  if ((rc = crv_launch_sim_rows(ctx, plan, cv, out[0]->d, gstride, nreal, seed, realization0, fp64_chol, 0,
+                                g0->nyp)) != CRV_OK)
+    return rc;
   for (int b = 0; b < nout; ++b) { out[b]->cubetype = CRV_PARAMETER; out[b]->transformed = true; }
   const double sc = crv_scale_inverse(out[0]);
   if ((rc = crv_launch_c2r_batch(ctx, plan, out[0]->d, nout, (long long)out[0]->rsize, sc, add, nadd)) != CRV_OK)

@@ -24,12 +24,17 @@ def make_box(lo, hi) -> L.FaciesBox:
 
 
 def density(ctx: Context, d_samples, box: L.FaciesBox, smooth_cov, bins=(100, 100, 50)) -> FFTGrid:
-    """Smoothed, normalised density of device samples (n, 3) float on the box."""
+    """Smoothed, normalised density of device samples (n, 3) float on the box
+    (a torch tensor: torch's work on it ends before the library's stream
+    reads it, and the library is done with it on return)."""
+    import torch
     g = FFTGrid(ctx, bins[0], bins[1], bins[2], precision=32, cubetype=L.CRV_OPERATOR)
     S = np.ascontiguousarray(smooth_cov, np.float64).reshape(3, 3)
+    torch.cuda.synchronize()
     ctx.check(ctx.lib.crv_facies_density(ctx.h, C.c_void_p(d_samples.data_ptr()), int(d_samples.shape[0]),
                                          C.byref(box), S.ctypes.data_as(C.POINTER(C.c_double)), g.h),
               "facies_density")
+    ctx.synchronize()
     return g
 
 

@@ -349,6 +349,17 @@ class PostPlan:
                                                        err_corr.h if err_corr is not None else None, covs),
                        "posterior_exec")
 
+    def invert(self, mean, data, cov, sims=None, *, mean_src=None, data_src=None, seed=0, realization0=0,
+               chunk_rows=0):
+        """crv_invert: forward transforms of the real means and data (or of
+        mean_src / data_src into them), the posterior, len(sims) // 3
+        realizations and the inverse transforms, pipelined by chunks of y
+        rows."""
+        arr = lambda gs: (C.c_void_p * len(gs))(*[g.h.value for g in gs]) if gs else None
+        sims = sims or []
+        self.ctx.check(self.ctx.lib.crv_invert(self.h, arr(mean), arr(data), arr(mean_src), arr(data_src), arr(cov),
+                                               seed, realization0, len(sims) // 3, arr(sims), chunk_rows), "invert")
+
     def stats(self) -> PostResult:
         st = L.PostStats()
         self.ctx.check(self.ctx.lib.crv_post_plan_stats(self.h, C.byref(st)), "post_plan_stats")

@@ -211,6 +211,9 @@ def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int =
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for ep in range(epochs):
+        # the previous epoch's steps read xs/ys on the library's stream:
+        # finish them before torch may reuse that memory
+        ctx.synchronize()
         perm = torch.randperm(n_tr, device=dev)
         xs, ys = xtr[perm].contiguous(), ytr[perm].contiguous()
         torch.cuda.synchronize()
@@ -260,7 +263,11 @@ def predict(ctx: Context, mlp: MLP, d_x):
     import torch
     n = d_x.shape[0]
     d_y = torch.empty((n, 3), dtype=torch.float32, device=d_x.device)
+    # the library's stream does not order itself after torch's: torch's work
+    # on these tensors ends first, and the outputs are complete on return
+    torch.cuda.synchronize()
     mlp.predict(d_x.data_ptr(), n, d_y.data_ptr())
+    ctx.synchronize()
     return d_y
 
 
@@ -327,10 +334,10 @@ def predict_checked(ctx: Context, mlp: MLP, table: ErrorTable, d_x, thresh: floa
     import torch
     n = d_x.shape[0]
     d_y = predict(ctx, mlp, d_x)
-    torch.cuda.synchronize()
     p = L.RockParams(**(params or ROCK_DEFAULTS))
     nref = C.c_longlong()
     ctx.check(ctx.lib.crv_rock_refine(ctx.h, C.byref(p), n, C.c_void_p(d_x.data_ptr()), C.c_void_p(d_y.data_ptr()),
                                       C.byref(mlp.norm), C.c_void_p(table.d.data_ptr()), table.nb, float(thresh),
                                       C.byref(nref)), "rock_refine")
+    ctx.synchronize()
     return d_y, int(nref.value)

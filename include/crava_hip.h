@@ -105,6 +105,12 @@ int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
  *                      next pass reads it; grids larger than twice this go
  *                      in one launch per pass (0 = 256, the cache's size) */
 #define CRV_TUNE_BATCH_MB 6
+/*   CRV_TUNE_ORDER     1 + a bit mask of the passes that walk their tiles
+ *                      backwards (1 x R2C, 2 y forward, 4 z forward, 8 z
+ *                      inverse, 16 y inverse, 32 x C2R): a pass that starts
+ *                      on the tiles its predecessor wrote last finds them in
+ *                      the memory-side cache (0 = the library's default)   */
+#define CRV_TUNE_ORDER    7
 int crv_ctx_tune(crv_ctx* ctx, int key, int value);
 
 /* Autotuner for one grid size: times the candidate launch shapes of each
@@ -291,6 +297,27 @@ int crv_post_plan_destroy(crv_post_plan* plan);
 int crv_posterior_exec(crv_post_plan* plan, crv_grid* mean[3], crv_grid** data, crv_grid* err_corr,
                        crv_grid* cov[6]);
 int crv_post_plan_stats(crv_post_plan* plan, crv_post_stats* stats);
+/* The inversion sequence of the Crava constructor (src/crava.cpp:205-300):
+ * the FFTs of the data and the prior means (fftInPlace, crava.cpp:263-271),
+ * computePostMeanResidAndFFTCov (crava.cpp:975-1285), nreal realizations of
+ * simulate with the counter noise stream (crava.cpp:1550-1734) and
+ * invFFTCovGrids (seismicparametersholder.cpp:310-322), in one call on a
+ * separable-prior plan.  In: mean[3] prior means and data[ntheta] cubes,
+ * real, or, when mean_src / data_src are given, those grids (left
+ * unchanged; mean and data then only receive the results).  Out: mean[3]
+ * posterior means (real), data the residual spectra,
+ * cov[6] the posterior covariance (real), sims (a contiguous batch of
+ * 3 nreal grids, may be NULL when nreal = 0) the realizations with the
+ * posterior means added.  The results equal crv_fft_r2c of the inputs,
+ * crv_posterior_exec, crv_fft_c2r of the means, crv_simulate_batch and
+ * crv_fft_c2r of the covariances, bit for bit; the work is ordered so that
+ * each chunk of chunk_rows y rows goes through the z transforms, the
+ * posterior and the noise while its data are in the memory-side cache
+ * (chunk_rows <= 0: chosen from the grid size).  Launches only
+ * (capturable); statistics through crv_post_plan_stats.                    */
+int crv_invert(crv_post_plan* plan, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
+               crv_grid* const* data_src, crv_grid* cov[6], unsigned long long seed,
+               unsigned long long realization0, int nreal, crv_grid** sims, int chunk_rows);
 
 /* ---- simulation ---------------------------------------------------------
  * crv_simulate replaces one realization of Crava::simulate

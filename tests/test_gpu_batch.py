@@ -207,3 +207,71 @@ def test_autotuned_shapes_keep_the_transform(ctx, dims):
     assert rel_rms(b.download_complex(), before) < 1e-6
     b.invFFTInPlace()
     assert rel_rms(b.download_real()[:, :, :nx], x[:, :, :nx]) < 1e-6
+
+
+@pytest.mark.parametrize("chunk_rows,batched,src", [(0, False, False), (7, False, False), (1, True, False),
+                                                    (0, True, True), (7, True, True), (5, False, True)])
+def test_invert_matches_sequence(ctx, chunk_rows, batched, src):
+    """crv_invert (transforms, posterior, realizations and inverse transforms
+    pipelined by chunks of y rows) equals the one-call sequence bit for bit."""
+    from crava_amd.hip import FFTGrid, PostPlan, fft_c2r_batch, grid_batch, simulate_batch
+    case = Case(model.config_small(64, 48, 40, 5))
+    s = case.s
+    _, _, kw = _inputs(ctx, case)
+    plan = PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=1, **kw)
+
+    def grids(ct, count):
+        if batched:
+            return grid_batch(ctx, count, s.nxp, s.nyp, s.nzp, 32, ct)
+        return [FFTGrid(ctx, s.nxp, s.nyp, s.nzp, cubetype=ct) for _ in range(count)]
+
+    def inputs():
+        mean, data = grids(2, 3), grids(1, s.n)
+        for g, a in zip(mean, case.syn.mean):
+            g.upload_real(a)
+        for g, a in zip(data, case.syn.data):
+            g.upload_real(a)
+        return mean, data
+
+    nreal = 2
+    # the sequence of one-call entry points
+    mean, data = inputs()
+    cov = grids(3, 6)
+    sims = grid_batch(ctx, 3 * nreal, s.nxp, s.nyp, s.nzp)
+    for g in data + mean:
+        g.fftInPlace()
+    plan.exec(mean, data, None, cov)
+    st_seq = plan.stats()
+    for g in mean:
+        g.invFFTInPlace()
+    simulate_batch(ctx, cov, mean, sims, seed=17, realization0=3, fp64_chol=True)
+    for g in cov:
+        g.invFFTInPlace()
+    ctx.synchronize()
+    want = ([g.download_real() for g in mean], [g.download_complex() for g in data],
+            [g.download_real() for g in cov], [g.download_real() for g in sims])
+    # the pipelined call, in place or from source grids that stay unchanged
+    if src:
+        msrc, dsrc = inputs()
+        mean2, data2 = grids(2, 3), grids(1, s.n)
+    else:
+        msrc = dsrc = None
+        mean2, data2 = inputs()
+    cov2 = grids(3, 6)
+    sims2 = grid_batch(ctx, 3 * nreal, s.nxp, s.nyp, s.nzp)
+    plan.invert(mean2, data2, cov2, sims2, mean_src=msrc, data_src=dsrc, seed=17, realization0=3,
+                chunk_rows=chunk_rows)
+    if src:
+        for g, a in zip(msrc + dsrc, list(case.syn.mean) + list(case.syn.data)):
+            assert np.array_equal(g.download_real()[:, :, :s.nxp], a[:, :, :s.nxp])
+    st = plan.stats()
+    got = ([g.download_real() for g in mean2], [g.download_complex() for g in data2],
+           [g.download_real() for g in cov2], [g.download_real() for g in sims2])
+    nx = s.nxp
+    for w, g in zip(want[0] + want[2] + want[3], got[0] + got[2] + got[3]):
+        assert np.array_equal(w[:, :, :nx], g[:, :, :nx])
+    for w, g in zip(want[1], got[1]):
+        assert np.array_equal(w, g)
+    assert st.n_fp64 == st_seq.n_fp64 and st.n_chol_fail == st_seq.n_chol_fail
+    assert all(not g.getIsTransformed() for g in mean2 + cov2 + sims2)
+    assert all(g.getIsTransformed() for g in data2)

@@ -126,7 +126,7 @@ def test_surrogate_gain_in_facies_probabilities(ctx):
     diff = max(np.max(np.abs(a.download_real()[:, :, :nx].ravel() - b.download_real()[:, :, :nx].ravel())[sig])
                for a, b in zip(P_rp, P_nn))
     refined = [o[1] / 10_000_000 for o in outs]
-    print(f"facies probabilities: max |surrogate - rplib| = {diff:.4f}; samples: rplib {t_rplib * 1e3:.1f} ms, "
+    print(f"facies probabilities: max |surrogate - rplib| = {diff:.2e}; samples: rplib {t_rplib * 1e3:.1f} ms, "
           f"surrogate + gate {t_nn * 1e3:.1f} ms (refined {', '.join(f'{f:.1%}' for f in refined)})")
     assert diff < 0.03
     assert t_nn < t_rplib
