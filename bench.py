@@ -248,6 +248,109 @@ def read_traffic(kernel):
     return v.get("bytes") if isinstance(v, dict) else None
 
 
+def bench_c4(a):
+    """The learned rock-physics surrogate: BF16 MFMA training on 1e6 rplib
+    samples, 1e7 samples per facies (shale, brine sand, gas sand) from the
+    surrogate with rplib re-evaluating every sample whose predicted ln vp
+    error exceeds 0.5 %, against rplib itself; the samples' densities feed
+    facies probabilities over the FP32 posterior mean of the 256x256x200
+    5-angle survey."""
+    import torch
+    from crava_amd import _lib as L
+    from crava_amd import facies, hip, model, surrogate as S
+    from crava_amd.survey import GpuSurvey
+    ctx = hip.Context(0)
+    dev = torch.device("cuda", 0)
+    out = {"metric": "facies samples/s (surrogate + rplib-backed error gate), 3 facies x 1e7",
+           "unit": "samples/s", "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 (MFMA) + f64 (rplib)",
+           "data": "synthetic (rplib targets generated on the GPU; GPU-generated survey)"}
+    log("[bench c4] training the surrogate: 1e6 rplib samples, 400 epochs of batch 16384")
+    t0 = time.perf_counter()
+    r = S.train(ctx, n_train=1_000_000, n_val=2_000_000, epochs=400, batch=16384, lr=3e-3, seed=5, vp_weight=3.0)
+    t_all = time.perf_counter() - t0
+    err = np.abs(S.validation_errors(ctx, r.mlp, r.val_x, r.val_y))
+    table = S.ErrorTable.build(ctx, r.mlp, r.val_x, r.val_y, nb=6)
+    f_fwd, f_bwd = r.mlp.flops()
+    out["training"] = {"samples": 1_000_000, "epochs": 400, "steps": r.steps, "train_s": r.seconds,
+                       "with_data_generation_s": t_all, "mfma_tflops": r.mfma_tflops,
+                       "mfma_util_bf16_dense": r.mfma_util,
+                       "val_abs_err": {nm: {"median": float(np.median(err[:, c])),
+                                            "p99": float(np.quantile(err[:, c], 0.99))}
+                                       for c, nm in enumerate(("ln_vp", "ln_vs", "ln_rho"))}}
+    log(f"[bench c4] trained in {r.seconds:.1f} s ({r.steps} steps), MFMA {r.mfma_tflops:.1f} TFLOP/s "
+        f"({100 * r.mfma_util:.2f} % of 2.5 PFLOP/s); val |ln vp| median {np.median(err[:, 0]):.2e}")
+    rng = np.random.default_rng(21)
+    xs = [torch.from_numpy(S.sample_inputs(10_000_000, rng, facies=f)).to(dev) for f in facies.FACIES]
+    # warm-up of every kernel involved
+    S.rplib_targets(ctx, xs[0][:65536], 65536)
+    S.predict_checked(ctx, r.mlp, table, xs[0][:65536], thresh=0.005)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ys_rp = [S.rplib_targets(ctx, x, x.shape[0])[0] for x in xs]
+    ctx.synchronize()
+    t_rp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gated = [S.predict_checked(ctx, r.mlp, table, x, thresh=0.005) for x in xs]
+    ctx.synchronize()
+    t_nn = time.perf_counter() - t0
+    # inference alone: matrix-core rate of the forward kernel
+    t0 = time.perf_counter()
+    for x in xs:
+        S.predict(ctx, r.mlp, x)
+    ctx.synchronize()
+    t_inf = time.perf_counter() - t0
+    n_all = sum(x.shape[0] for x in xs)
+    inf_tflops = f_fwd * n_all / t_inf / 1e12
+    out["sampling"] = {"samples": n_all, "rplib_s": t_rp, "surrogate_gate_s": t_nn, "speedup": t_rp / t_nn,
+                       "refined_fraction": {f: g[1] / x.shape[0] for f, g, x in zip(facies.FACIES, gated, xs)},
+                       "inference_only_s": t_inf, "inference_mfma_tflops": inf_tflops,
+                       "inference_mfma_util_bf16_dense": inf_tflops / 2500.0}
+    log(f"[bench c4] 3 x 1e7 samples: rplib {t_rp * 1e3:.1f} ms, surrogate + gate {t_nn * 1e3:.1f} ms "
+        f"(refined " + ", ".join(f"{g[1] / 1e7:.1%}" for g in gated) + f"); inference alone {t_inf * 1e3:.1f} ms "
+        f"= {inf_tflops:.1f} TFLOP/s")
+    # facies probabilities over the FP32 posterior of the 256x256x200 survey
+    sv_s = model.config3_realizations()
+    sv = GpuSurvey(ctx, sv_s, truth_noise="counter")
+    mk = lambda ct, n: [hip.FFTGrid(ctx, sv_s.nxp, sv_s.nyp, sv_s.nzp, cubetype=ct) for _ in range(n)]
+    mean, data, cov = mk(L.CRV_PARAMETER, 3), mk(L.CRV_DATA, sv_s.n), mk(L.CRV_COVARIANCE, 6)
+    plan = hip.PostPlan(ctx, sv_s.nxp, sv_s.nyp, sv_s.nzp, 32, sv_s.n, mode=L.CRV_MODE_FP32_REFINE,
+                        separable=True, skip_mean_inverse=True, **sv.posterior_kwargs())
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plan.invert(mean, data, cov, None, mean_src=sv.mean, data_src=sv.data)
+    ctx.synchronize()
+    t_post = time.perf_counter() - t0
+    # smoothing: the posterior variances (covariance grids at lag 0) plus a
+    # kernel bandwidth of one bin
+    var = np.array([cov[q].download_real()[0, 0, 0] for q in range(6)], np.float64)
+    Sm = np.array([[var[0], var[3], var[4]], [var[3], var[1], var[5]], [var[4], var[5], var[2]]])
+    # the box of (ln vp, ln vs, ln rho) that holds the facies samples
+    lo, hi = np.array([6.5, 4.5, 7.3]), np.array([8.9, 8.4, 8.0])
+    box = facies.make_box(lo, hi)
+    bins = (100, 100, 50)
+    Sm = Sm + np.diag(((hi - lo) / np.array(bins)) ** 2)
+    keep = lambda y: y[torch.isfinite(y).all(1)].contiguous()
+    prior = [0.4, 0.4, 0.2]
+    t0 = time.perf_counter()
+    d_nn = [facies.density(ctx, keep(g[0]), box, Sm, bins) for g in gated]
+    P_nn = facies.probabilities(ctx, d_nn, prior, box, mean)
+    ctx.synchronize()
+    t_fac = time.perf_counter() - t0
+    d_rp = [facies.density(ctx, keep(y), box, Sm, bins) for y in ys_rp]
+    P_rp = facies.probabilities(ctx, d_rp, prior, box, mean)
+    nx = sv_s.nx
+    dP = np.stack([np.abs(p.download_real()[:, :, :nx] - q.download_real()[:, :, :nx]) for p, q in zip(P_nn, P_rp)])
+    out["facies"] = {"cube": [sv_s.nxp, sv_s.nyp, sv_s.nzp], "posterior_s": t_post, "densities_and_probabilities_s":
+                     t_fac, "bins": list(bins), "max_abs_dP_vs_rplib": float(dP.max()),
+                     "mean_abs_dP_vs_rplib": float(dP.mean())}
+    log(f"[bench c4] posterior {t_post * 1e3:.1f} ms, densities + probabilities {t_fac * 1e3:.1f} ms; "
+        f"|P(surrogate) - P(rplib)| max {dP.max():.2e} mean {dP.mean():.2e}")
+    out["value"] = n_all / t_nn
+    out["config"] = {"workload": "rock-physics surrogate 5-128-128-128-3 tanh, 3 facies x 1e7 samples, "
+                                 "facies probabilities over a 256x256x200 FP32 posterior"}
+    print(json.dumps(out), flush=True)
+
+
 def bench_c3(a):
     """Batched and graph-captured runs: 64 realizations of the 256x256x200
     5-angle inversion (one launch per stage for all realizations), and 12
@@ -386,7 +489,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c3"],
+    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c3", "c4"],
                     help="c1: headline 450x450x320 inversion; c0: 64x64x128; c3: batched realizations and volumes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -408,6 +511,9 @@ def main():
     import torch
     if a.config == "c3":
         bench_c3(a)
+        return
+    if a.config == "c4":
+        bench_c4(a)
         return
     if a.copy_calibration:
         n = 1 << 28  # 1 GiB of float32
@@ -471,6 +577,30 @@ def main():
                 f"{v_[1] / 3:.3f} ms/step {gbs}")
         log(f"[bench] FP{prec}: {sec * 1e3:.2f} ms/step, {s.voxels / sec:.3e} voxels/s")
 
+    # host transfers, reported beside the resident-input figure: the 8 input
+    # cubes uploaded from pageable host arrays (as FFTGrid's rvalue_) and the
+    # step's outputs (3 means, 6 covariances, 1 realization, 5 residual
+    # spectra) downloaded
+    G32 = res[32]["grids"]
+    scratch = hip.FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=32)
+    host = np.zeros((s.nzp, s.nyp, s.rnxp), np.float32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(s.n + 3):
+        scratch.upload_real(host)
+    ctx.synchronize()
+    h2d = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for g in G32["mean"] + G32["cov"] + G32["sim"] + G32["data"]:
+        g.download_real()
+    d2h = time.perf_counter() - t0
+    scratch.close()
+    gb = 4.0 * s.nzp * s.nyp * s.rnxp / 1e9
+    transfers = {"h2d_ms": h2d * 1e3, "h2d_GB": gb * (s.n + 3), "d2h_ms": d2h * 1e3, "d2h_GB": gb * (s.n + 12),
+                 "voxels_per_s_with_transfers": s.voxels / (res[32]["sec"] + h2d + d2h), "host_memory": "pageable"}
+    log(f"[bench] transfers: H2D {h2d * 1e3:.1f} ms ({gb * (s.n + 3):.2f} GB), D2H {d2h * 1e3:.1f} ms "
+        f"({gb * (s.n + 12):.2f} GB)")
+
     # error table of FP32 + refinement against FP64
     errors = None
     if 64 in res:
@@ -516,6 +646,7 @@ def main():
         "fp64": ({"value": s.voxels / res[64]["sec"], "ms_per_step": res[64]["sec"] * 1e3,
                   "speedup_fp32_refine": res[64]["sec"] / sec} if 64 in res else None),
         "errors_vs_fp64": errors,
+        "transfers": transfers,
         "kernels": kernels,
         "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}
                          if 64 in res else None),

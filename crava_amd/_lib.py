@@ -33,6 +33,7 @@ EXPORTS = [
     "crv_post_plan_create", "crv_post_plan_destroy", "crv_posterior_exec", "crv_post_plan_stats", "crv_invert",
     "crv_fft_autotune", "crv_grid_create_batch", "crv_simulate_batch", "crv_graph_begin", "crv_graph_end",
     "crv_graph_launch", "crv_graph_destroy", "crv_facies_density", "crv_facies_prob",
+    "crv_grid_derived", "crv_crava_header", "crv_grid_write_crava", "crv_grid_read_crava",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
@@ -73,6 +74,17 @@ class RockParams(C.Structure):
 
 class FaciesBox(C.Structure):
     _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3)]
+
+
+class SimboxInfo(C.Structure):
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("nx", C.c_int), ("ny", C.c_int), ("il0", C.c_double), ("xl0", C.c_double),
+                ("il_step_x", C.c_double), ("il_step_y", C.c_double), ("xl_step_x", C.c_double),
+                ("xl_step_y", C.c_double), ("angle", C.c_double)]
+
+
+CRV_OUT_EXP, CRV_OUT_AI, CRV_OUT_SI, CRV_OUT_VPVS, CRV_OUT_POISSON = 0, 1, 2, 3, 4
+CRV_OUT_LAME_MU, CRV_OUT_LAME_LAMBDA, CRV_OUT_MU_RHO, CRV_OUT_LAMBDA_RHO = 5, 6, 7, 8
 
 
 class FftShape(C.Structure):
@@ -173,6 +185,10 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_fft_r2c_batch_from": (i, [C.POINTER(vp), C.POINTER(vp), i]),
         "crv_fft_c2r_batch": (i, [C.POINTER(vp), i]),
         "crv_facies_prob": (i, [vp, i, C.POINTER(vp), _dp, C.POINTER(FaciesBox), C.POINTER(vp), C.POINTER(vp)]),
+        "crv_grid_derived": (i, [vp, i, vp, vp, vp, vp]),
+        "crv_crava_header": (i, [C.POINTER(SimboxInfo), i, i, i, C.c_char_p, C.c_size_t]),
+        "crv_grid_write_crava": (i, [vp, C.c_char_p, C.POINTER(SimboxInfo)]),
+        "crv_grid_read_crava": (i, [vp, C.c_char_p, C.POINTER(SimboxInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

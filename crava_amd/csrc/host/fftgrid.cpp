@@ -65,6 +65,21 @@ void FFTGrid::add(FFTGrid* g) {
 
 void FFTGrid::multiplyByScalar(float scalar) { check(crv_grid_multiply_by_scalar(grid_, scalar), "multiplyByScalar"); }
 
+int FFTGrid::expTransf() {
+  check(crv_grid_derived(ctx_, CRV_OUT_EXP, grid_, nullptr, nullptr, grid_), "expTransf");
+  return 0;
+}
+
+// the reference appends IO::SuffixCrava() (".crava") to the name it is given
+void FFTGrid::writeCravaFile(const std::string& fileName, const crv_simbox_info& simbox) {
+  check(crv_grid_write_crava(grid_, (fileName + ".crava").c_str(), &simbox), "writeCravaFile");
+}
+
+void FFTGrid::readCravaFile(const std::string& fileName, std::string& errText) {
+  if (crv_grid_read_crava(grid_, fileName.c_str(), nullptr) != CRV_OK)
+    errText += std::string("Error: ") + crv_last_error(ctx_) + "\n";
+}
+
 void FFTGrid::setRealValues(const std::vector<float>& r) {
   if ((long long)r.size() != getrsize()) throw HipError("setRealValues: size != rsize");
   if (precision_ == CRV_FP32) {

@@ -49,6 +49,10 @@ class FFTGrid {
   virtual void invFFTInPlace();                            // fftgrid.cpp:1811-1841
   virtual void add(FFTGrid* fftGrid);                      // fftgrid.cpp:1870-1891
   virtual void multiplyByScalar(float scalar);             // fftgrid.cpp:1985-1995
+  int  expTransf();                                         // fftgrid.cpp:1726-1743
+  // fftgrid.cpp:2411-2498: CRAVA's binary grid format with the simbox geometry
+  void writeCravaFile(const std::string& fileName, const crv_simbox_info& simbox);
+  void readCravaFile(const std::string& fileName, std::string& errText);
 
   // host copies in the reference layout (rvalue_ with pads / cvalue_)
   void setRealValues(const std::vector<float>& r);

@@ -458,3 +458,37 @@ def fft_c2r_batch(grids):
     ctx = grids[0].ctx
     d = (C.c_void_p * len(grids))(*[g.h.value for g in grids])
     ctx.check(ctx.lib.crv_fft_c2r_batch(d, len(grids)), "fft_c2r_batch")
+
+
+# ---- output: derived parameters and CRAVA's binary grid format --------------
+def derived(ctx: Context, kind: int, alpha=None, beta=None, rho=None, out=None) -> FFTGrid:
+    """crv_grid_derived (ParameterOutput's per-voxel formulas) into out, or a
+    new real grid shaped like the first input."""
+    g0 = next(g for g in (alpha, beta, rho, out) if g is not None)
+    if out is None:
+        out = FFTGrid(ctx, g0.nxp, g0.nyp, g0.nzp, precision=g0.precision)
+    h = lambda g: g.h if g is not None else None
+    ctx.check(ctx.lib.crv_grid_derived(ctx.h, kind, h(alpha), h(beta), h(rho), out.h), "derived")
+    return out
+
+
+def crava_header(sb: "L.SimboxInfo", rnxp: int, nyp: int, nzp: int) -> bytes:
+    """The CRAVA binary header (crv_crava_header); needs no GPU."""
+    lib = L.lib()
+    n = lib.crv_crava_header(C.byref(sb), rnxp, nyp, nzp, None, 0)
+    buf = C.create_string_buffer(n)
+    if lib.crv_crava_header(C.byref(sb), rnxp, nyp, nzp, buf, n) != n:
+        raise CravaError("crava_header failed")
+    return buf.raw[:n]
+
+
+def write_crava(g: FFTGrid, path: str, sb: "L.SimboxInfo"):
+    """FFTGrid::writeCravaFile through crv_grid_write_crava."""
+    g.ctx.check(g.ctx.lib.crv_grid_write_crava(g.h, str(path).encode(), C.byref(sb)), "write_crava")
+
+
+def read_crava(g: FFTGrid, path: str) -> "L.SimboxInfo":
+    """FFTGrid::readCravaFile through crv_grid_read_crava; returns the geometry."""
+    sb = L.SimboxInfo()
+    g.ctx.check(g.ctx.lib.crv_grid_read_crava(g.h, str(path).encode(), C.byref(sb)), "read_crava")
+    return sb

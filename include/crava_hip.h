@@ -489,6 +489,47 @@ int crv_find_closest_factorable(int leastint);
 /* ModelGeneral::setPaddingSize: findClosestFactorable(ceil(n*(1+pad_fac))). */
 int crv_padding_size(int n, double pad_fac);
 
+/* ---- output (next to the path) ------------------------------------------
+ * Derived parameters of ParameterOutput::writeParameters
+ * (src/parameteroutput.cpp:14-420) from the real log-parameter grids
+ * alpha = ln vp, beta = ln vs, rho = ln rho, evaluated in double and stored
+ * in out's precision (out may alias an input):
+ *   CRV_OUT_EXP          exp(alpha), RMISSING kept (FFTGrid::expTransf,
+ *                        src/fftgrid.cpp:1726-1743; vp, vs or rho output)
+ *   CRV_OUT_AI / SI      exp(alpha + rho) / exp(beta + rho)
+ *   CRV_OUT_VPVS         exp(alpha - beta)
+ *   CRV_OUT_POISSON      0.5 (q - 2) / (q - 1), q = exp(2 (alpha - beta))
+ *   CRV_OUT_LAME_MU      exp(rho + 2 beta - 13.81551)
+ *   CRV_OUT_LAME_LAMBDA  exp(rho) (exp(2 alpha - 13.81551) - 2 exp(2 beta - 13.81551))
+ *   CRV_OUT_MU_RHO       exp(2 (beta + rho) - 13.81551)
+ *   CRV_OUT_LAMBDA_RHO   exp(2 (alpha + rho) - 13.81551) - 2 exp(2 (beta + rho) - 13.81551) */
+#define CRV_OUT_EXP         0
+#define CRV_OUT_AI          1
+#define CRV_OUT_SI          2
+#define CRV_OUT_VPVS        3
+#define CRV_OUT_POISSON     4
+#define CRV_OUT_LAME_MU     5
+#define CRV_OUT_LAME_LAMBDA 6
+#define CRV_OUT_MU_RHO      7
+#define CRV_OUT_LAMBDA_RHO  8
+int crv_grid_derived(crv_ctx* ctx, int kind, crv_grid* alpha, crv_grid* beta, crv_grid* rho, crv_grid* out);
+/* CRAVA's binary grid format (FFTGrid::writeCravaFile / readCravaFile,
+ * src/fftgrid.cpp:2411-2498): the line "crava_fftgrid_binary", the simbox
+ * geometry below in this order, rnxp, nyp, nzp, then the rsize floats of the
+ * padded real grid; numbers big-endian IEEE.  Write and read move the grid
+ * through pinned staging with the copies overlapping the file I/O.  FP32
+ * real grids; read checks the dimensions and returns the geometry in sb
+ * (may be NULL).  crv_crava_header encodes the header into buf (NULL: the
+ * length only) and returns its length.                                    */
+typedef struct {
+  double x0, y0, dx, dy;
+  int nx, ny;
+  double il0, xl0, il_step_x, il_step_y, xl_step_x, xl_step_y, angle;
+} crv_simbox_info;
+int crv_crava_header(const crv_simbox_info* sb, int rnxp, int nyp, int nzp, unsigned char* buf, size_t len);
+int crv_grid_write_crava(crv_grid* g, const char* path, const crv_simbox_info* sb);
+int crv_grid_read_crava(crv_grid* g, const char* path, crv_simbox_info* sb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,25 @@ def test_host_entry_points_without_gpu():
     assert lib.crv_padding_size(320, 0.0625) == 343
 
 
+def test_crava_header_layout():
+    """The CRAVA binary header (FFTGrid::writeCravaFile, reference
+    src/fftgrid.cpp:2411-2440): the type line, then 4 doubles, 2 ints, 7
+    doubles and rnxp, nyp, nzp, big-endian IEEE as NRLib writes by default
+    (libs/nrlib/iotools/fileio.hpp:266-268) -- checked against numpy's
+    big-endian encoders."""
+    import numpy as np
+    from crava_amd import _lib as L
+    from crava_amd.hip import crava_header
+    sb = L.SimboxInfo(x0=456000.5, y0=6.7e6, dx=25.0, dy=12.5, nx=450, ny=451, il0=1001.0, xl0=2002.0,
+                      il_step_x=0.5, il_step_y=-0.25, xl_step_x=1.0 / 3.0, xl_step_y=-2.0, angle=0.3)
+    got = crava_header(sb, 452, 450, 320)
+    want = (b"crava_fftgrid_binary\n" + np.array([456000.5, 6.7e6, 25.0, 12.5], ">f8").tobytes()
+            + np.array([450, 451], ">i4").tobytes()
+            + np.array([1001.0, 2002.0, 0.5, -0.25, 1.0 / 3.0, -2.0, 0.3], ">f8").tobytes()
+            + np.array([452, 450, 320], ">i4").tobytes())
+    assert got == want
+
+
 def test_context_without_gpu_fails_cleanly():
     import torch
     if torch.cuda.is_available():
