@@ -37,8 +37,12 @@ using dft::Dft;
 // values (32 KB of LDS).
 constexpr int kThreads = 256;
 
-// passes that walk their tiles backwards by default (CRV_TUNE_ORDER bits)
-constexpr int kDefaultOrder = 0;
+// passes that walk their tiles backwards by default (CRV_TUNE_ORDER bits):
+// the y forward pass (after an x pass that wrote the grid plane by plane)
+// and the x C2R pass (after the y inverse pass), which then start on the
+// planes still in the memory-side cache; measured at 450x450x320 FP32 in
+// the inversion step: 8.72 -> 8.54 ms
+constexpr int kDefaultOrder = (1 << 1) | (1 << 5);
 
 template <typename T> struct Ept;
 template <> struct Ept<float>  { static constexpr int value = 16; };  // 4096 complex / block

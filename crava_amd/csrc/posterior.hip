@@ -77,7 +77,14 @@ struct PostArgs {
   int j0, nyr;
   long long ncr;
   FastDiv dplr;
+  // FP32 separable plans with n <= kRecMaxN: the per-kz tables of plane k as
+  // one record of 128 floats at rec + 64 k (see k_posterior_rec); cpp =
+  // chunks of 64 wavenumbers per plane of the row range
+  const float2* rec;
+  int cpp;
 };
+
+constexpr int kRecMaxN = 6;
 
 // wavenumber t of the main pass's row range -> (i, j, k) and its index
 template <typename T>
@@ -263,6 +270,124 @@ __global__ __launch_bounds__(kThreads) void k_posterior(PostArgs<T> a) {
   block_stats(a.stats, nfail, cmax);
 }
 
+// Main pass of FP32 separable plans (n <= kRecMaxN), same arithmetic as
+// k_posterior<float, N, true, LIST>.  Each wave takes 64 consecutive
+// wavenumbers of one kz plane, so everything that depends on kz is uniform
+// across the wave: the plane's record (E P complex, B 3n complex, K 3n
+// complex, D n, S 6, the band flag at float 127) is read with one float2
+// per lane and broadcast with readlane instead of per-lane table loads.
+template <int N, bool LIST>
+__global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
+  using C = float2;
+  constexpr int P = N * (N + 1) / 2;
+  constexpr int OE = 0, OB = 2 * P, OK = OB + 6 * N, OD = OK + 6 * N, OS = OD + N;
+  static_assert(OS + 6 <= 127, "record overflow");
+  unsigned long long nfail = 0;
+  double cmax = 0.0;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
+  const int nwv = (int)gridDim.x * (kThreads / 64);
+  const int plane = a.dplr.d, cpp = a.cpp, nch = cpp * a.nzp;
+  for (int ch = wv; ch < nch; ch += nwv) {
+    const int k = ch / cpp;
+    const int p0 = (ch - k * cpp) * 64;
+    const float2 rv = a.rec[(size_t)k * 64 + lane];
+    auto R = [&](int m) -> float {
+      const float v = (m & 1) ? rv.y : rv.x;
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), m >> 1));
+    };
+    // lanes past the plane's end repeat its last wavenumber and store nothing
+    const bool live = p0 + lane < plane;
+    const int pp = live ? p0 + lane : plane - 1;
+    const int jl = fdiv(pp, a.dcx);
+    const int i = pp - jl * a.cnxp, j = a.j0 + jl;
+    const long long idx = (long long)i + (long long)a.cnxp * ((long long)j + (long long)a.nyp * k);
+    const float lam = a.t.lam[(size_t)j * a.cnxp + i];
+    float s[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s[q] = lam * R(OS + q);
+    if (R(127) == 0.0f) {
+      if (live) store_prior(a, idx, s);
+      continue;
+    }
+    C A[P];
+    C B[N][4];
+#pragma unroll
+    for (int q = 0; q < P; ++q) A[q] = cscale(cmake<C>(R(OE + 2 * q), R(OE + 2 * q + 1)), lam);
+#pragma unroll
+    for (int l = 0; l < N; ++l)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) B[l][m] = cmake<C>(R(OB + 2 * (l * 3 + m)), R(OB + 2 * (l * 3 + m) + 1));
+#pragma unroll
+    for (int l = 0; l < N; ++l) A[l * (l + 1) / 2 + l].x += R(OD + l);
+    // the forward matrix, taken while the whole wave is active
+    C Kk[N * 3];
+#pragma unroll
+    for (int q = 0; q < N * 3; ++q) Kk[q] = cmake<C>(R(OK + 2 * q), R(OK + 2 * q + 1));
+    float rd[N], pmin = 0.0f, pmax = 0.0f;
+    const bool ok = chol_rcp<float, N>(A, rd, pmin, pmax);
+    const float cond = ok ? pmax / pmin : 0.0f;
+    if constexpr (LIST) {
+      const int ky = min(j, a.nyp - j), kz = min(k, a.nzp - k);
+      const float fx = (float)i * a.rx, fy = (float)ky * a.ry, fz = (float)kz * a.rz;
+      const bool lowk = (fx * fx + fy * fy + fz * fz) < a.lowk2;
+      if (!ok || lowk || cond > a.cond_thr) {
+        if (live) {
+          const unsigned int slot = atomicAdd(a.list_count, 1u);
+          a.list[slot] = (unsigned int)idx;
+        }
+        continue;
+      }
+    } else {
+      if (!ok) {
+        if (live) { ++nfail; store_prior(a, idx, s); }
+        continue;
+      }
+    }
+    if (!live) continue;
+    cmax = fmax(cmax, (double)cond);
+    C mu[3], d[N];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      d[l] = a.data[l][idx];
+      C x = d[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mu[m]));
+      B[l][3] = x;
+    }
+    lsolve_r<float, N, 4>(A, rd, B);
+    const float sb = lam;  // K S = lam * B
+    C mp[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      C x = cmake<C>(0.0f, 0.0f);
+#pragma unroll
+      for (int l = 0; l < N; ++l) x = cadd(x, cmulc(B[l][3], B[l][m]));
+      mp[m] = cadd(mu[m], cscale(x, sb));
+      a.mean[m][idx] = mp[m];
+    }
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      C x = d[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mp[m]));
+      a.data[l][idx] = x;
+    }
+    const float sb2 = sb * sb;
+    constexpr int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      C x = cmake<C>(0.0f, 0.0f);
+#pragma unroll
+      for (int l = 0; l < N; ++l) x = cadd(x, cmulc(B[l][pb[q]], B[l][pa[q]]));
+      a.cov[q][idx] = cmake<C>(s[q] - sb2 * x.x, -sb2 * x.y);
+    }
+  }
+  block_stats(a.stats, nfail, cmax);
+}
+
 // FP64 solve with one step of iterative refinement for the queued
 // wavenumbers (FP32 storage): per right-hand side b (columns of K S, then
 // d - K mu), x = M^-1 b, r = b - M x, x += M^-1 r, with M built in FP64 from
@@ -383,6 +508,15 @@ int persistent_blocks(crv_ctx* ctx, F kernel, long long work) {
 
 template <typename T, int N, bool SEP, bool LIST>
 int launch_main(crv_ctx* ctx, const PostArgs<T>& a) {
+  if constexpr (std::is_same<T, float>::value && SEP && N <= kRecMaxN) {
+    if (a.rec) {
+      const int nb = persistent_blocks(ctx, k_posterior_rec<N, LIST>, a.ncr);
+      ProfScope ps(ctx, LIST ? "posterior_fp32" : "posterior");
+      hipLaunchKernelGGL((k_posterior_rec<N, LIST>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+      CRV_HIP_CHECK(ctx, hipGetLastError());
+      return CRV_OK;
+    }
+  }
   const int nb = persistent_blocks(ctx, k_posterior<T, N, SEP, LIST>, a.ncr);
   ProfScope ps(ctx, LIST ? "posterior_fp32" : "posterior");
   hipLaunchKernelGGL((k_posterior<T, N, SEP, LIST>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
@@ -565,7 +699,7 @@ struct crv_post_plan {
   long long nc = 0, ninv_planes = 0;
   unsigned char* dtab = nullptr;  // [tables T][tables FP64 (refine)][inv][stats 256 B]
   TabOffsets oT, o64;
-  size_t oI = 0, oS = 0;
+  size_t oI = 0, oS = 0, oR = 0;   // oR: FP32 separable records (0: none)
   unsigned int* dlist = nullptr;
   double sig0[6] = {0, 0, 0, 0, 0, 0};
   float cond_thr = 1e5f;
@@ -598,6 +732,24 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
   std::vector<unsigned char> blob;
   pl->oT = (precision == CRV_FP64) ? put_tables<double>(blob, h) : put_tables<float>(blob, h);
   if (refine) pl->o64 = put_tables<double>(blob, h);
+  if (precision == CRV_FP32 && pl->separable && p->ntheta <= kRecMaxN) {
+    // one 128-float record per kz: E, B, K, D, S (the FP32 tables' values)
+    // and the band flag at float 127
+    const int n = p->ntheta, P = n * (n + 1) / 2;
+    pl->oR = blob.size();
+    blob.resize(pl->oR + (size_t)nzp * 128 * sizeof(float), 0);
+    float* r = (float*)(blob.data() + pl->oR);
+    for (int k = 0; k < nzp; ++k) {
+      float* rk = r + (size_t)k * 128;
+      int o = 0;
+      for (int q = 0; q < 2 * P; ++q) rk[o++] = (float)h.E[(size_t)k * 2 * P + q];
+      for (int q = 0; q < 6 * n; ++q) rk[o++] = (float)h.B[(size_t)k * 6 * n + q];
+      for (int q = 0; q < 6 * n; ++q) rk[o++] = (float)h.K[(size_t)k * 6 * n + q];
+      for (int q = 0; q < n; ++q) rk[o++] = (float)h.D[(size_t)k * n + q];
+      for (int q = 0; q < 6; ++q) rk[o++] = (float)h.S[(size_t)k * 6 + q];
+      rk[127] = h.inv[k] ? 1.0f : 0.0f;
+    }
+  }
   pl->oI = blob.size();
   blob.resize(pl->oI + (((size_t)nzp * 4 + 255) & ~(size_t)255), 0);
   memcpy(blob.data() + pl->oI, h.inv.data(), (size_t)nzp * 4);
@@ -698,6 +850,10 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
     a.dplane = make_fastdiv(pl->cnxp * pl->nyp); a.dcx = make_fastdiv(pl->cnxp);
     a.j0 = j0; a.nyr = nyr; a.ncr = (long long)pl->cnxp * nyr * pl->nzp; a.dplr = make_fastdiv(pl->cnxp * nyr);
     a.stats = dstats;
+    if (pl->oR) {
+      a.rec = (const float2*)(pl->dtab + pl->oR);
+      a.cpp = (pl->cnxp * nyr + 63) / 64;
+    }
     if (pl->refine) {
       a.t64 = tab_at<double>(pl->dtab, pl->o64);
       a.list = pl->dlist;
@@ -774,11 +930,12 @@ int family_pass(crv_ctx* ctx, FftPlan* fp, const Family& f, int pass, int o0, in
   return CRV_OK;
 }
 
-// x and y passes (forward: x then y; inverse: y then x) over a family in
-// sub-batches that fit the memory-side cache, so the second pass of a
-// sub-batch reads the first pass's output from there
+// x and y passes (forward: x then y; inverse: y then x), and with with_z the
+// z pass too (forward last, inverse first), over a family in sub-batches
+// that fit the memory-side cache, so each pass of a sub-batch reads the
+// previous pass's output from there
 int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double scale, const void* const* add,
-              int nadd, crv_grid* const* src = nullptr) {
+              int nadd, crv_grid* const* src = nullptr, bool with_z = false) {
   const int nrows = fp->nyp * fp->nzp;
   // a forward x pass may read other grids (src): one launch per sub-batch
   // when they form a batch with the same stride, else one per grid
@@ -799,7 +956,9 @@ int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double s
         return rc;
       }
       if ((rc = family_pass(ctx, fp, s, 1, 0, fp->nzp, 1.0, nullptr, 0)) != CRV_OK) return rc;
+      if (with_z && (rc = family_pass(ctx, fp, s, 2, 0, fp->nyp, 1.0, nullptr, 0)) != CRV_OK) return rc;
     } else {
+      if (with_z && (rc = family_pass(ctx, fp, s, 3, 0, fp->nyp, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if ((rc = family_pass(ctx, fp, s, 4, 0, fp->nzp, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if ((rc = family_pass(ctx, fp, s, 5, 0, nrows, scale, ad, nadd)) != CRV_OK) return rc;
     }
@@ -849,42 +1008,53 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   const int nyp = pl->nyp;
   const Family fd = make_family(data, n), fm = make_family(mean, 3), fc = make_family(cov, 6);
   const Family fs = nsim ? make_family(sims, nsim) : Family();
-  // rows per chunk: the chunk's share of the grids it touches (inputs,
-  // outputs and realizations) stays near 192 MB
-  if (chunk_rows <= 0) {
-    const double grid = (double)mean[0]->rsize * mean[0]->elem_bytes();
-    const double touched = grid * (n + 3 + 6 + nsim);
-    chunk_rows = (int)std::max(1.0, std::floor(192.0 * 1048576.0 * nyp / touched));
-  }
-  chunk_rows = std::min(chunk_rows, nyp);
+  // chunk_rows <= 0 or >= nyp: whole transforms, each grid's passes back to
+  // back (the order of the one-call sequence); otherwise chunks of y rows
+  const bool chunked = chunk_rows > 0 && chunk_rows < nyp;
+  if (!chunked) chunk_rows = nyp;
   // forward x and y passes of the data and the means
-  if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src)) != CRV_OK) return rc;
-  if ((rc = family_xy(ctx, fp, fm, false, crv_scale_forward(mean[0]), nullptr, 0, mean_src)) != CRV_OK) return rc;
+  if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src, !chunked)) != CRV_OK)
+    return rc;
+  if ((rc = family_xy(ctx, fp, fm, false, crv_scale_forward(mean[0]), nullptr, 0, mean_src, !chunked)) != CRV_OK)
+    return rc;
   for (int q = 0; q < 6; ++q) cov[q]->cubetype = CRV_COVARIANCE;
   const void* cv[6];
   for (int q = 0; q < 6; ++q) cv[q] = cov[q]->d;
   // per chunk of y rows: z forward, posterior, simulation, z inverse
   for (int j0 = 0; j0 < nyp; j0 += chunk_rows) {
     const int j1 = std::min(nyp, j0 + chunk_rows);
-    if ((rc = family_pass(ctx, fp, fd, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
-    if ((rc = family_pass(ctx, fp, fm, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    if (chunked) {
+      if ((rc = family_pass(ctx, fp, fd, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+      if ((rc = family_pass(ctx, fp, fm, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    }
     if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0)) != CRV_OK) return rc;
-    if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
-                                          realization0, pl->refine ? 1 : 0, j0, j1)) != CRV_OK)
-      return rc;
-    if ((rc = family_pass(ctx, fp, fm, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
-    if ((rc = family_pass(ctx, fp, fc, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
-    if ((rc = family_pass(ctx, fp, fs, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    if (chunked) {
+      if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
+                                            realization0, pl->refine ? 1 : 0, j0, j1)) != CRV_OK)
+        return rc;
+      if ((rc = family_pass(ctx, fp, fm, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+      if ((rc = family_pass(ctx, fp, fc, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+      if ((rc = family_pass(ctx, fp, fs, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
+    }
   }
-  // inverse y and x passes: the means first, whose real values the
-  // realizations add in their last pass
-  if ((rc = family_xy(ctx, fp, fm, true, crv_scale_inverse(mean[0]), nullptr, 0)) != CRV_OK) return rc;
-  if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0)) != CRV_OK) return rc;
+  // inverse (y and x, after the z pass when not chunked): the means first,
+  // whose real values the realizations add in their last pass; the
+  // realizations' noise when not chunked after the covariances' spectra are
+  // complete and before their inverse transforms
+  if ((rc = family_xy(ctx, fp, fm, true, crv_scale_inverse(mean[0]), nullptr, 0, nullptr, !chunked)) != CRV_OK)
+    return rc;
+  if (!chunked && nsim &&
+      (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed, realization0,
+                                pl->refine ? 1 : 0, 0, nyp)) != CRV_OK)
+    return rc;
   if (nsim) {
     for (int b = 0; b < nsim; ++b) sims[b]->cubetype = CRV_PARAMETER;
     const void* add[3] = {mean[0]->d, mean[1]->d, mean[2]->d};
-    if ((rc = family_xy(ctx, fp, fs, true, crv_scale_inverse(sims[0]), add, 3)) != CRV_OK) return rc;
+    if ((rc = family_xy(ctx, fp, fs, true, crv_scale_inverse(sims[0]), add, 3, nullptr, !chunked)) != CRV_OK)
+      return rc;
   }
+  if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked)) != CRV_OK)
+    return rc;
   for (int l = 0; l < n; ++l) data[l]->transformed = true;
   for (int m = 0; m < 3; ++m) mean[m]->transformed = false;
   for (int q = 0; q < 6; ++q) cov[q]->transformed = false;

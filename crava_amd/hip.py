@@ -159,9 +159,11 @@ class FFTGrid:
         return st.value
 
     def close(self):
-        if self.h:
+        # a grid outliving its context (closed first) is left to the process
+        # exit: its memory went with the context's device
+        if self.h and self.ctx.h:
             self.ctx.lib.crv_grid_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -366,9 +368,9 @@ class PostPlan:
         return PostResult(st.n_wavenumbers, st.n_inverted, st.n_chol_fail, st.n_fp64, st.max_cond)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.lib.crv_post_plan_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -430,9 +432,9 @@ class Graph:
         self.ctx.check(self.ctx.lib.crv_graph_launch(self.h), "graph_launch")
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.lib.crv_graph_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -143,9 +143,9 @@ class MLP:
         return f.value, b.value
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.lib.crv_mlp_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -67,7 +67,9 @@ typedef struct crv_grid_info {
 /* ---- context ------------------------------------------------------------
  * Replaces the global FFTW plan creation of FFTGrid::fftInPlace
  * (src/fftgrid.cpp:1799-1804): a context owns the device, the stream and the
- * cached transform plans (twiddle tables) for every grid size it has seen.  */
+ * cached transform plans (twiddle tables) for every grid size it has seen.
+ * Grids, posterior plans, graphs and networks made on a context are
+ * destroyed before it; after crv_ctx_destroy their handles are invalid.   */
 int         crv_ctx_create(int device, crv_ctx** out);
 int         crv_ctx_destroy(crv_ctx* ctx);
 const char* crv_last_error(const crv_ctx* ctx);
@@ -310,11 +312,11 @@ int crv_post_plan_stats(crv_post_plan* plan, crv_post_stats* stats);
  * 3 nreal grids, may be NULL when nreal = 0) the realizations with the
  * posterior means added.  The results equal crv_fft_r2c of the inputs,
  * crv_posterior_exec, crv_fft_c2r of the means, crv_simulate_batch and
- * crv_fft_c2r of the covariances, bit for bit; the work is ordered so that
- * each chunk of chunk_rows y rows goes through the z transforms, the
- * posterior and the noise while its data are in the memory-side cache
- * (chunk_rows <= 0: chosen from the grid size).  Launches only
- * (capturable); statistics through crv_post_plan_stats.                    */
+ * crv_fft_c2r of the covariances, bit for bit, which is also the order of
+ * the work when chunk_rows <= 0; with 0 < chunk_rows < nyp each chunk of
+ * chunk_rows y rows goes through the z transforms, the posterior and the
+ * noise in turn (smaller working sets, but smaller launches).  Launches
+ * only (capturable); statistics through crv_post_plan_stats.               */
 int crv_invert(crv_post_plan* plan, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
                crv_grid* const* data_src, crv_grid* cov[6], unsigned long long seed,
                unsigned long long realization0, int nreal, crv_grid** sims, int chunk_rows);

@@ -263,7 +263,7 @@ def test_invert_matches_sequence(ctx, chunk_rows, batched, src):
                 chunk_rows=chunk_rows)
     if src:
         for g, a in zip(msrc + dsrc, list(case.syn.mean) + list(case.syn.data)):
-            assert np.array_equal(g.download_real()[:, :, :s.nxp], a[:, :, :s.nxp])
+            assert np.array_equal(g.download_real()[:, :, :s.nxp], a[:, :, :s.nxp].astype(np.float32))
     st = plan.stats()
     got = ([g.download_real() for g in mean2], [g.download_complex() for g in data2],
            [g.download_real() for g in cov2], [g.download_real() for g in sims2])

@@ -84,6 +84,6 @@ def test_crava_file_round_trip(ctx, tmp_path):
         assert np.array_equal(h.download_real(), X)
         assert (sb2.nx, sb2.ny, sb2.angle) == (400, 60, 0.1)
     # dimensions are checked
-    wrong = FFTGrid(ctx, 450, 64, 99)
+    wrong = FFTGrid(ctx, 450, 64, 98)
     with pytest.raises(Exception):
         read_crava(wrong, tmp_path / "grid.crava")
