@@ -127,6 +127,9 @@ def kernel_bytes(name, s, prec, ninv):
         return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e
     if name in ("simulate_scale", "simulate_scale_batch"):
         return Nc * (6 + 3) * 2 * e
+    if name == "posterior_sim_fp32":
+        # the posterior plus the realization's 3 spectra it writes
+        return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e + Nc * 3 * 2 * e
     return None
 
 
@@ -226,10 +229,11 @@ def cpu_baseline(s, nthreads):
 KERNEL_SYMBOLS = {
     # kernels of the FP32-refine step at 450x450x320 (x 225 = 15 x 15 as a
     # complex half-length transform, y 450 = 25 x 18, z 320 = 20 x 16)
-    "fft_x_r2c": "k_rows_r2c<float, 15, 15>", "fft_x_c2r": "k_rows_c2r<float, 15, 15>",
-    "fft_y_fwd": "k_cols<float, false, 25, 18>", "fft_z_fwd": "k_cols<float, false, 20, 16>",
-    "fft_y_inv": "k_cols<float, true, 25, 18>", "fft_z_inv": "k_cols<float, true, 20, 16>",
-    "posterior_fp32": "k_posterior<float, 5, true, true>", "simulate_scale_batch": "k_simulate<float, true>",
+    "fft_x_r2c": "k_rows_r2c<float, 15, 15, false>", "fft_x_c2r": "k_rows_c2r<float, 15, 15, false>",
+    "fft_y_fwd": "k_cols<float, false, 25, 18, false>", "fft_z_fwd": "k_cols<float, false, 20, 16, false>",
+    "fft_y_inv": "k_cols<float, true, 25, 18, false>", "fft_z_inv": "k_cols<float, true, 20, 16, false>",
+    "posterior_fp32": "k_posterior_rec<5, true, 0>", "posterior_sim_fp32": "k_posterior_rec<5, true, 2>",
+    "simulate_scale_batch": "k_simulate<float, true>",
 }
 
 
@@ -480,6 +484,11 @@ def bench_c3(a):
             step(Gv, pl, v, 1)
 
     eager2 = timeit(all_volumes, a.steps)
+    ctx.profile(True)
+    all_volumes()
+    ctx.profile(False)
+    for k_, v_ in sorted(ctx.profile_report().items(), key=lambda kv: -kv[1][1])[:8]:
+        log(f"[bench c3] part 2: {k_:22s} x{v_[0]:4d}  {v_[1]:.3f} ms")
     graph = hip.Graph.begin(ctx)
     all_volumes()
     graph.end()

@@ -48,6 +48,13 @@ template <typename T> struct Ept;
 template <> struct Ept<float>  { static constexpr int value = 16; };  // 4096 complex / block
 template <> struct Ept<double> { static constexpr int value = 8; };   // 2048 complex / block
 
+// Elements per thread of a kernel: the two-stage (radix-pair) kernels and
+// the general kernel for long pencils hold Ept; the general kernel whose
+// tile fits half of that (H) holds Ept / 2 -- its stage loop dispatches the
+// radix at run time and would otherwise need about twice the registers.
+template <typename T, int RF, bool H>
+constexpr int ept() { return (RF == 0 && H) ? Ept<T>::value / 2 : Ept<T>::value; }
+
 
 // per-stage divisors: L / R_s (butterflies per pencil) and Ns
 struct StageDiv {
@@ -103,10 +110,10 @@ __device__ __forceinline__ void bmap(int b, int cshift, FastDiv dlr, int& c, int
 // with k = j mod Ns, and writes e' = (j - k)*R + k + r*Ns.
 
 // first stage (Ns = 1, no twiddles): inputs from load(c, e)
-template <typename T, int R, bool INV, bool JFAST, typename Load>
+template <typename T, int R, bool INV, bool JFAST, int E, typename Load>
 __device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, int L, FastDiv dlr, Load load) {
   using C = cplx_t<T>;
-  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  constexpr int MAXB = (E + R - 1) / R;
   const int Lr = L / R;
   const int total = Lr << cshift;
   C v[MAXB][R];
@@ -135,11 +142,11 @@ __device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, i
 
 // middle stage: LDS -> LDS (pencil-fast); ends with the tile written.
 // TW = false: a first stage (Ns = 1), whose twiddles are all 1.
-template <typename T, int R, bool INV, bool TW = true>
+template <typename T, int R, bool INV, bool TW, int E>
 __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int L, int Ns, FastDiv dns,
                                           const cplx_t<T>* tw) {
   using C = cplx_t<T>;
-  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  constexpr int MAXB = (E + R - 1) / R;
   const int Lr = L / R;
   const int total = Lr << cshift;
   const int cm = (1 << cshift) - 1;
@@ -178,11 +185,11 @@ __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int
 }
 
 // last stage (Ns = L/R, so k = j): LDS -> store(c, e)
-template <typename T, int R, bool INV, bool JFAST, typename Store>
+template <typename T, int R, bool INV, bool JFAST, int E, typename Store>
 __device__ __forceinline__ void stage_last(const cplx_t<T>* lds, int S, int cshift, int L, FastDiv dlr,
                                            const cplx_t<T>* tw, Store store) {
   using C = cplx_t<T>;
-  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  constexpr int MAXB = (E + R - 1) / R;
   const int Lr = L / R;
   const int total = Lr << cshift;
 #pragma unroll
@@ -204,10 +211,10 @@ __device__ __forceinline__ void stage_last(const cplx_t<T>* lds, int S, int cshi
 }
 
 // a one-stage transform: load -> DFT -> store, no LDS
-template <typename T, int R, bool INV, typename Load, typename Store>
+template <typename T, int R, bool INV, int E, typename Load, typename Store>
 __device__ __forceinline__ void stage_single(int cshift, Load load, Store store) {
   using C = cplx_t<T>;
-  constexpr int MAXB = (Ept<T>::value + R - 1) / R;
+  constexpr int MAXB = (E + R - 1) / R;
   const int total = 1 << cshift;  // one butterfly per pencil
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
@@ -240,19 +247,20 @@ constexpr int kSmallRadices[] = {8, 4, 2, 5, 3, 7};
 
 // Whole transform of a tile.  load/store move HBM data; with LAST_TO_LDS the
 // last stage also writes LDS (natural order) and store is not used.
-template <typename T, bool INV, bool JFAST, bool LAST_TO_LDS, int RF, int RL, typename Load, typename Store>
+template <typename T, bool INV, bool JFAST, bool LAST_TO_LDS, int RF, int RL, bool H, typename Load, typename Store>
 __device__ __forceinline__ void transform_tile(cplx_t<T>* lds, int S, int cshift, const Fft1D& f, const StageDiv& sd,
                                                const cplx_t<T>* twl, Load load, Store store) {
+  constexpr int E = ept<T, RF, H>();
   const int L = f.L, nst = f.nst;
   if constexpr (RF > 0) {
     // two stages of radices RF, RL known at compile time
-    stage_first<T, RF, INV, JFAST>(lds, S, cshift, L, sd.lr[0], load);
+    stage_first<T, RF, INV, JFAST, E>(lds, S, cshift, L, sd.lr[0], load);
     __syncthreads();
     const cplx_t<T>* t = twl + (f.tw[1] - f.tw0);
     if constexpr (LAST_TO_LDS)
-      stage_mid<T, RL, INV>(lds, S, cshift, L, f.Ns[1], sd.ns[1], t);
+      stage_mid<T, RL, INV, true, E>(lds, S, cshift, L, f.Ns[1], sd.ns[1], t);
     else
-      stage_last<T, RL, INV, JFAST>(lds, S, cshift, L, sd.lr[1], t, store);
+      stage_last<T, RL, INV, JFAST, E>(lds, S, cshift, L, sd.lr[1], t, store);
     return;
   }
   if (nst == 0) {
@@ -266,41 +274,40 @@ __device__ __forceinline__ void transform_tile(cplx_t<T>* lds, int S, int cshift
     return;
   }
   if (nst == 1 && !LAST_TO_LDS) {
-    CRV_RADIX_SWITCH(f.R[0], (stage_single<T, R_, INV>(cshift, load, store)));
+    CRV_RADIX_SWITCH(f.R[0], (stage_single<T, R_, INV, E>(cshift, load, store)));
     return;
   }
-  CRV_RADIX_SWITCH(f.R[0], (stage_first<T, R_, INV, JFAST>(lds, S, cshift, L, sd.lr[0], load)));
+  CRV_RADIX_SWITCH(f.R[0], (stage_first<T, R_, INV, JFAST, E>(lds, S, cshift, L, sd.lr[0], load)));
   __syncthreads();
   const int nmid_end = LAST_TO_LDS ? nst : nst - 1;
   for (int s = 1; s < nmid_end; ++s) {
     const cplx_t<T>* t = twl + (f.tw[s] - f.tw0);
-    CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
+    CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV, true, E>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
   }
   if (!LAST_TO_LDS) {
     const cplx_t<T>* t = twl + (f.tw[nst - 1] - f.tw0);
-    CRV_RADIX_SWITCH(f.R[nst - 1], (stage_last<T, R_, INV, JFAST>(lds, S, cshift, L, sd.lr[nst - 1], t, store)));
+    CRV_RADIX_SWITCH(f.R[nst - 1], (stage_last<T, R_, INV, JFAST, E>(lds, S, cshift, L, sd.lr[nst - 1], t, store)));
   }
 }
 
 // Whole transform of a tile already in LDS in natural order (lds[e*S + c]),
 // left there in natural order; every stage runs LDS -> LDS.  The caller
 // synchronises before; the tile is complete on return.
-template <typename T, bool INV, int RF, int RL>
+template <typename T, bool INV, int RF, int RL, bool H>
 __device__ __forceinline__ void transform_tile_lds(cplx_t<T>* lds, int S, int cshift, const Fft1D& f,
                                                    const StageDiv& sd, const cplx_t<T>* twl) {
+  constexpr int E = ept<T, RF, H>();
   const int L = f.L;
   if constexpr (RF > 0) {
-    stage_mid<T, RF, INV, false>(lds, S, cshift, L, 1, sd.ns[0], twl);
-    stage_mid<T, RL, INV>(lds, S, cshift, L, f.Ns[1], sd.ns[1], twl + (f.tw[1] - f.tw0));
+    stage_mid<T, RF, INV, false, E>(lds, S, cshift, L, 1, sd.ns[0], twl);
+    stage_mid<T, RL, INV, true, E>(lds, S, cshift, L, f.Ns[1], sd.ns[1], twl + (f.tw[1] - f.tw0));
     return;
   }
+  // general lengths: every stage with its twiddles (those of the first stage
+  // are all 1), one radix dispatch per stage
   for (int s = 0; s < f.nst; ++s) {
     const cplx_t<T>* t = twl + (f.tw[s] - f.tw0);
-    if (s == 0) {
-      CRV_RADIX_SWITCH(f.R[0], (stage_mid<T, R_, INV, false>(lds, S, cshift, L, 1, sd.ns[0], t)));
-    } else {
-      CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
-    }
+    CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV, true, E>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
   }
 }
 
@@ -335,10 +342,10 @@ struct RowArgs {
 // (fully coalesced), and every stage of the transform runs LDS -> LDS.  The
 // tile region holds max((P+1) L, P cnxp) complex: the transform's
 // lds[e*S + c] layout, or, for a C2R, the staged half-spectrum rows.
-template <typename T, int RF, int RL>
+template <typename T, int RF, int RL, bool H>
 __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   using C = cplx_t<T>;
-  constexpr int E = Ept<T>::value;
+  constexpr int E = ept<T, RF, H>();
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
@@ -376,7 +383,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
     }
   }
   __syncthreads();
-  transform_tile_lds<T, false, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl);
+  transform_tile_lds<T, false, RF, RL, H>(lds, S, a.pshift, a.f, a.sd, twl);
   __syncthreads();
   // post-process and store cnxp complex per row
   for (int idx = threadIdx.x; idx < P * K; idx += kThreads) {
@@ -399,10 +406,10 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   }
 }
 
-template <typename T, int RF, int RL>
+template <typename T, int RF, int RL, bool H>
 __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   using C = cplx_t<T>;
-  constexpr int E = Ept<T>::value;
+  constexpr int E = ept<T, RF, H>();
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
@@ -467,7 +474,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
     }
   }
   __syncthreads();
-  transform_tile_lds<T, true, RF, RL>(lds, S, a.pshift, a.f, a.sd, twl);
+  transform_tile_lds<T, true, RF, RL, H>(lds, S, a.pshift, a.f, a.sd, twl);
   __syncthreads();
   // tile -> rows, scaled, plus the added grid
   if (a.xeven) {
@@ -528,7 +535,7 @@ struct ColArgs {
   int rev;             // walk the tiles backwards (xcd_remap)
 };
 
-template <typename T, bool INV, int RF, int RL>
+template <typename T, bool INV, int RF, int RL, bool H>
 __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
@@ -547,7 +554,7 @@ __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   auto store = [&](int c, int e, C v) {
     if (c < nc) base[(size_t)e * a.estride + c] = v;
   };
-  transform_tile<T, INV, false, false, RF, RL>(lds, TX, a.cshift, a.f, a.sd, twl, load, store);
+  transform_tile<T, INV, false, false, RF, RL, H>(lds, TX, a.cshift, a.f, a.sd, twl, load, store);
 }
 
 // ---- planning ----------------------------------------------------------------
@@ -573,6 +580,9 @@ bool is_pair(const std::vector<int>& rs) {
 #undef CRV_PAIR_IS
   return false;
 }
+
+// the axis runs in a two-stage kernel (the selection of rows_kernel / cols_kernel)
+bool axis_has_pair(const Fft1D& f) { return f.nst == 2 && is_pair(std::vector<int>(f.R, f.R + 2)); }
 
 // reverse: the stages of a length without a two-stage kernel run in the
 // opposite order (an autotuning choice)
@@ -622,26 +632,27 @@ size_t row_lds_bytes(const FftPlan* p) {
 // kernel for an axis plan: the two-stage kernel of its radix pair, or the
 // general one
 template <typename T>
-const void* rows_kernel(const Fft1D& f, bool inv) {
+const void* rows_kernel(const Fft1D& f, bool inv, bool half) {
   if (f.nst == 2) {
 #define CRV_PAIR_ROWS(A, B) \
     if (f.R[0] == (A) && f.R[1] == (B)) \
-      return inv ? (const void*)k_rows_c2r<T, A, B> : (const void*)k_rows_r2c<T, A, B>;
+      return inv ? (const void*)k_rows_c2r<T, A, B, false> : (const void*)k_rows_r2c<T, A, B, false>;
     CRV_FFT_PAIRS(CRV_PAIR_ROWS)
 #undef CRV_PAIR_ROWS
   }
-  return inv ? (const void*)k_rows_c2r<T, 0, 0> : (const void*)k_rows_r2c<T, 0, 0>;
+  if (half) return inv ? (const void*)k_rows_c2r<T, 0, 0, true> : (const void*)k_rows_r2c<T, 0, 0, true>;
+  return inv ? (const void*)k_rows_c2r<T, 0, 0, false> : (const void*)k_rows_r2c<T, 0, 0, false>;
 }
 
 template <typename T, bool INV>
-const void* cols_kernel(const Fft1D& f) {
+const void* cols_kernel(const Fft1D& f, bool half) {
   if (f.nst == 2) {
 #define CRV_PAIR_COLS(A, B) \
-    if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<T, INV, A, B>;
+    if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<T, INV, A, B, false>;
     CRV_FFT_PAIRS(CRV_PAIR_COLS)
 #undef CRV_PAIR_COLS
   }
-  return (const void*)k_cols<T, INV, 0, 0>;
+  return half ? (const void*)k_cols<T, INV, 0, 0, true> : (const void*)k_cols<T, INV, 0, 0, false>;
 }
 
 // Batches: nbatch grids bstride elements apart (real units), one launch.
@@ -674,7 +685,7 @@ int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const 
   const int nb = (a.nrows + p->rows_per_block - 1) / p->rows_per_block;
   void* args[] = {&a};
   ProfScope ps(ctx, inv ? "fft_x_c2r" : "fft_x_r2c");
-  CRV_HIP_CHECK(ctx, hipLaunchKernel(rows_kernel<T>(p->fx, inv), dim3(nb, bt.n), dim3(kThreads), args,
+  CRV_HIP_CHECK(ctx, hipLaunchKernel(rows_kernel<T>(p->fx, inv, p->half[0]), dim3(nb, bt.n), dim3(kThreads), args,
                                      row_lds_bytes<T>(p), ctx->stream));
   return CRV_OK;
 }
@@ -704,7 +715,8 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
   static const char* names[4] = {"fft_y_fwd", "fft_y_inv", "fft_z_fwd", "fft_z_inv"};
   ProfScope ps(ctx, names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
   void* args[] = {&a};
-  const void* kern = inv ? cols_kernel<T, true>(a.f) : cols_kernel<T, false>(a.f);
+  const bool half = p->half[zaxis ? 2 : 1];
+  const void* kern = inv ? cols_kernel<T, true>(a.f, half) : cols_kernel<T, false>(a.f, half);
   CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(kThreads), args, lds, ctx->stream));
   return CRV_OK;
 }
@@ -831,19 +843,29 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   const int maxe = kThreads * ((p->prec == CRV_FP64) ? Ept<double>::value : Ept<float>::value);
   const int txdef = (p->prec == CRV_FP64) ? 8 : 16;
   const int Lx = p->fx.L;
-  int P = 1;
   const int pmax = ctx->tune[CRV_TUNE_ROWS] > 0 ? ctx->tune[CRV_TUNE_ROWS] : 64;
   const int cnx = p->nxp / 2 + 1;
-  while (2 * P <= pmax && (2 * P + 1) * Lx <= maxe && 2 * P * cnx <= maxe) P *= 2;
-  if ((P + 1) * Lx > maxe || P * cnx > maxe) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nxp too large");
-  p->rows_per_block = P;
-  auto pick_tx = [&](int L, int want) {
-    int t = want > 0 ? pow2_floor(want) : txdef;
-    while (t > 1 && L * t > maxe) t /= 2;
-    return (L * t <= maxe) ? t : 0;
+  // x rows per block within a budget of complex values per tile; axes
+  // without a two-stage kernel use the half budget (and the kernel with half
+  // the registers) when their tiles fit it
+  auto pick_rows = [&](int budget) {
+    int P = 1;
+    while (2 * P <= pmax && (2 * P + 1) * Lx <= budget && 2 * P * cnx <= budget) P *= 2;
+    return ((P + 1) * Lx <= budget && P * cnx <= budget) ? P : 0;
   };
-  p->tx_y = pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y]);
-  p->tx_z = pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z]);
+  p->half[0] = !axis_has_pair(p->fx) && pick_rows(maxe / 2) > 0;
+  const int P = pick_rows(p->half[0] ? maxe / 2 : maxe);
+  if (!P) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nxp too large");
+  p->rows_per_block = P;
+  auto pick_tx = [&](int L, int want, int budget) {
+    int t = want > 0 ? pow2_floor(want) : txdef;
+    while (t > 1 && L * t > budget) t /= 2;
+    return (L * t <= budget) ? t : 0;
+  };
+  p->half[1] = !axis_has_pair(p->fy) && pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y], maxe / 2) > 0;
+  p->half[2] = !axis_has_pair(p->fz) && pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z], maxe / 2) > 0;
+  p->tx_y = pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y], p->half[1] ? maxe / 2 : maxe);
+  p->tx_z = pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z], p->half[2] ? maxe / 2 : maxe);
   if (!p->tx_y || !p->tx_z) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nyp or nzp too large");
   p->chunk_mb = ctx->tune[CRV_TUNE_CHUNK_MB];
   p->batch_mb = ctx->tune[CRV_TUNE_BATCH_MB];
@@ -1030,8 +1052,9 @@ extern "C" int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int pre
   if (rc == CRV_OK) {
     float best = 1e30f;
     int bestP = p->rows_per_block;
+    const int bx = p->half[0] ? maxe / 2 : maxe;
     for (int P = 1; P <= 64 && rc == CRV_OK; P *= 2) {
-      if ((P + 1) * p->fx.L > maxe || P * (p->nxp / 2 + 1) > maxe) break;
+      if ((P + 1) * p->fx.L > bx || P * (p->nxp / 2 + 1) > bx) break;
       p->rows_per_block = P;
       float ms = 0.0f;
       if ((rc = timed(rows, ms)) == CRV_OK && ms < best) { best = ms; bestP = P; }
@@ -1045,8 +1068,9 @@ extern "C" int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int pre
     int& tx = ax == 1 ? p->tx_y : p->tx_z;
     float best = 1e30f;
     int bestT = tx;
+    const int bt_ = p->half[ax] ? maxe / 2 : maxe;
     for (int t = 1; t <= 64 && rc == CRV_OK; t *= 2) {
-      if (L * t > maxe) break;
+      if (L * t > bt_) break;
       tx = t;
       float ms = 0.0f;
       if ((rc = timed([&]() { return cols(ax == 2); }, ms)) == CRV_OK && ms < best) { best = ms; bestT = t; }

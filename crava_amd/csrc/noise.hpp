@@ -18,6 +18,16 @@ struct NoiseGeom {
   int small;                // cnxp nyp nzp < 2^31: 32-bit index arithmetic
 };
 
+// the geometry of a grid of half-spectrum size cnxp x nyp x nzp without a
+// reference-stream prefix (counter noise only)
+inline NoiseGeom make_noise_geom_dims(int cnxp, int nyp, int nzp) {
+  NoiseGeom g;
+  g.cnxp = cnxp; g.nyp = nyp; g.nzp = nzp; g.prefix = nullptr;
+  g.dc = make_fastdiv(cnxp); g.dy = make_fastdiv(nyp);
+  g.small = ((long long)cnxp * nyp * nzp) < (1LL << 31);
+  return g;
+}
+
 inline NoiseGeom make_noise_geom(const FftPlan* p) {
   NoiseGeom g;
   g.cnxp = p->nxp / 2 + 1; g.nyp = p->nyp; g.nzp = p->nzp; g.prefix = p->noise_prefix;

@@ -31,6 +31,8 @@
 // when |k| is below the low-wavenumber fraction of Nyquist.
 #include "internal.hpp"
 #include "linalg.hpp"
+#include "noise.hpp"
+#include "simkern.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -82,7 +84,38 @@ struct PostArgs {
   // chunks of 64 wavenumbers per plane of the row range
   const float2* rec;
   int cpp;
+  // simulation fused into the FP32 passes (crv_invert): nsim realizations of
+  // the counter noise stream, realization r of parameter g written at
+  // sim + (3 r + g) sstride complex elements
+  float2* sim;
+  long long sstride;
+  int nsim;
+  unsigned long long seed, real0;
+  NoiseGeom geo;
 };
+
+// Simulation of one wavenumber from its covariance c as stored (FP32):
+// SIM 1 factors in float with the reference's rules, SIM 2 in FP64 with one
+// refinement step -- the work of k_simulate<float, SIM == 2> for this index.
+template <int SIM>
+__device__ __forceinline__ void fused_sim(const PostArgs<float>& a, long long idx, const float2 c[6]) {
+  if constexpr (SIM > 0) {
+    constexpr bool F64 = SIM == 2;
+    using U = typename std::conditional<F64, double, float>::type;
+    cplx_t<U> Lf[6];
+    const bool ok = sim_factor<float, F64>(c, Lf);
+    for (int r = 0; r < a.nsim; ++r) {
+      float2 o[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
+      if (ok) {
+        float re[3], im[3];
+        sim_noise3<float>(a.geo, a.seed, a.real0 + (unsigned long long)r, idx, re, im);
+        sim_apply<float, U>(Lf, re, im, o);
+      }
+#pragma unroll
+      for (int g = 0; g < 3; ++g) a.sim[(size_t)(3 * r + g) * a.sstride + idx] = o[g];
+    }
+  }
+}
 
 constexpr int kRecMaxN = 6;
 
@@ -276,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior(PostArgs<T> a) {
 // across the wave: the plane's record (E P complex, B 3n complex, K 3n
 // complex, D n, S 6, the band flag at float 127) is read with one float2
 // per lane and broadcast with readlane instead of per-lane table loads.
-template <int N, bool LIST>
+template <int N, bool LIST, int SIM>
 __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
   using C = float2;
   constexpr int P = N * (N + 1) / 2;
@@ -307,7 +340,15 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) s[q] = lam * R(OS + q);
     if (R(127) == 0.0f) {
-      if (live) store_prior(a, idx, s);
+      if (live) {
+        store_prior(a, idx, s);
+        if constexpr (SIM > 0) {
+          float2 c[6];
+#pragma unroll
+          for (int q = 0; q < 6; ++q) c[q] = make_float2(s[q], 0.0f);
+          fused_sim<SIM>(a, idx, c);
+        }
+      }
       continue;
     }
     C A[P];
@@ -340,7 +381,16 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
       }
     } else {
       if (!ok) {
-        if (live) { ++nfail; store_prior(a, idx, s); }
+        if (live) {
+          ++nfail;
+          store_prior(a, idx, s);
+          if constexpr (SIM > 0) {
+            float2 c[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) c[q] = make_float2(s[q], 0.0f);
+            fused_sim<SIM>(a, idx, c);
+          }
+        }
         continue;
       }
     }
@@ -377,13 +427,16 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
     }
     const float sb2 = sb * sb;
     constexpr int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    C cq[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       C x = cmake<C>(0.0f, 0.0f);
 #pragma unroll
       for (int l = 0; l < N; ++l) x = cadd(x, cmulc(B[l][pb[q]], B[l][pa[q]]));
-      a.cov[q][idx] = cmake<C>(s[q] - sb2 * x.x, -sb2 * x.y);
+      cq[q] = cmake<C>(s[q] - sb2 * x.x, -sb2 * x.y);
+      a.cov[q][idx] = cq[q];
     }
+    fused_sim<SIM>(a, idx, cq);
   }
   block_stats(a.stats, nfail, cmax);
 }
@@ -392,7 +445,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
 // wavenumbers (FP32 storage): per right-hand side b (columns of K S, then
 // d - K mu), x = M^-1 b, r = b - M x, x += M^-1 r, with M built in FP64 from
 // the FP64 tables.
-template <int N, bool SEP>
+template <int N, bool SEP, int SIM = 0>
 __global__ __launch_bounds__(kThreads) void k_posterior_list64(PostArgs<float> a) {
   using D = double2;
   constexpr int P = N * (N + 1) / 2;
@@ -423,7 +476,17 @@ __global__ __launch_bounds__(kThreads) void k_posterior_list64(PostArgs<float> a
 #pragma unroll
     for (int q = 0; q < P; ++q) A[q] = M[q];
     double rd[N], pmin = 0.0, pmax = 0.0;
-    if (!chol_rcp<double, N>(A, rd, pmin, pmax)) { ++nfail; store_prior(a, idx, s); continue; }
+    if (!chol_rcp<double, N>(A, rd, pmin, pmax)) {
+      ++nfail;
+      store_prior(a, idx, s);
+      if constexpr (SIM > 0) {
+        float2 c[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) c[q] = make_float2((float)s[q], 0.0f);
+        fused_sim<SIM>(a, idx, c);
+      }
+      continue;
+    }
     cmax = fmax(cmax, pmax / pmin);
     const D* Kk = a.t64.K + (size_t)k * N * 3;
     D mu[3], d[N];
@@ -489,8 +552,13 @@ __global__ __launch_bounds__(kThreads) void k_posterior_list64(PostArgs<float> a
       for (int m = 0; m < 3; ++m) x = csub(x, cmul(Kk[l * 3 + m], mp[m]));
       a.data[l][idx] = make_float2((float)x.x, (float)x.y);
     }
+    float2 cq[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) a.cov[q][idx] = make_float2((float)(s[q] - cv[q].x), (float)(-cv[q].y));
+    for (int q = 0; q < 6; ++q) {
+      cq[q] = make_float2((float)(s[q] - cv[q].x), (float)(-cv[q].y));
+      a.cov[q][idx] = cq[q];
+    }
+    fused_sim<SIM>(a, idx, cq);
   }
   block_stats(a.stats, nfail, cmax);
 }
@@ -510,10 +578,18 @@ template <typename T, int N, bool SEP, bool LIST>
 int launch_main(crv_ctx* ctx, const PostArgs<T>& a) {
   if constexpr (std::is_same<T, float>::value && SEP && N <= kRecMaxN) {
     if (a.rec) {
-      const int nb = persistent_blocks(ctx, k_posterior_rec<N, LIST>, a.ncr);
-      ProfScope ps(ctx, LIST ? "posterior_fp32" : "posterior");
-      hipLaunchKernelGGL((k_posterior_rec<N, LIST>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
-      CRV_HIP_CHECK(ctx, hipGetLastError());
+      auto go = [&](auto kern) {
+        const int nb = persistent_blocks(ctx, kern, a.ncr);
+        ProfScope ps(ctx, a.nsim ? (LIST ? "posterior_sim_fp32" : "posterior_sim")
+                                 : (LIST ? "posterior_fp32" : "posterior"));
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+        return hipGetLastError();
+      };
+      hipError_t e;
+      if (a.nsim == 0) e = go(k_posterior_rec<N, LIST, 0>);
+      else if (LIST) e = go(k_posterior_rec<N, LIST, 2>);
+      else e = go(k_posterior_rec<N, LIST, 1>);
+      if (e != hipSuccess) return crv_set_error(ctx, CRV_E_HIP, std::string("posterior: ") + hipGetErrorString(e));
       return CRV_OK;
     }
   }
@@ -527,7 +603,10 @@ int launch_main(crv_ctx* ctx, const PostArgs<T>& a) {
 template <int N, bool SEP>
 int launch_list(crv_ctx* ctx, const PostArgs<float>& a) {
   ProfScope ps(ctx, "posterior_fp64_list");
-  hipLaunchKernelGGL((k_posterior_list64<N, SEP>), dim3(256), dim3(kThreads), 0, ctx->stream, a);
+  if (a.nsim)
+    hipLaunchKernelGGL((k_posterior_list64<N, SEP, 2>), dim3(256), dim3(kThreads), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((k_posterior_list64<N, SEP>), dim3(256), dim3(kThreads), 0, ctx->stream, a);
   CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;
 }
@@ -812,8 +891,17 @@ __global__ void k_list_done(unsigned long long* stats) {
 
 // The posterior over y rows [j0, j1) of every kz plane (launches only).
 // reset clears the statistics first (the first range of an exec).
+// sim (FP32 plans with records only): the realizations' spectra, written by
+// the posterior passes themselves (see fused_sim)
+struct FusedSim {
+  float2* sim = nullptr;
+  long long sstride = 0;
+  int nsim = 0;
+  unsigned long long seed = 0, real0 = 0;
+};
+
 int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, crv_grid* err_corr,
-              crv_grid* const* cov, int j0, int j1, bool reset) {
+              crv_grid* const* cov, int j0, int j1, bool reset, const FusedSim& fs = FusedSim()) {
   crv_ctx* ctx = pl->ctx;
   const int n = pl->n;
   unsigned long long* dstats = (unsigned long long*)(pl->dtab + pl->oS);
@@ -853,6 +941,10 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
     if (pl->oR) {
       a.rec = (const float2*)(pl->dtab + pl->oR);
       a.cpp = (pl->cnxp * nyr + 63) / 64;
+      if (fs.nsim > 0) {
+        a.sim = fs.sim; a.sstride = fs.sstride; a.nsim = fs.nsim; a.seed = fs.seed; a.real0 = fs.real0;
+        a.geo = make_noise_geom_dims(pl->cnxp, pl->nyp, pl->nzp);
+      }
     }
     if (pl->refine) {
       a.t64 = tab_at<double>(pl->dtab, pl->o64);
@@ -1012,6 +1104,9 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   // back (the order of the one-call sequence); otherwise chunks of y rows
   const bool chunked = chunk_rows > 0 && chunk_rows < nyp;
   if (!chunked) chunk_rows = nyp;
+  // whole grids on an FP32 plan with per-plane records: the posterior
+  // passes write the realizations themselves (no re-read of the covariances)
+  const bool fused = !chunked && nsim > 0 && pl->oR != 0;
   // forward x and y passes of the data and the means
   if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src, !chunked)) != CRV_OK)
     return rc;
@@ -1027,7 +1122,12 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
       if ((rc = family_pass(ctx, fp, fd, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if ((rc = family_pass(ctx, fp, fm, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
     }
-    if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0)) != CRV_OK) return rc;
+    FusedSim fsim;
+    if (fused) {
+      fsim.sim = (float2*)sims[0]->d; fsim.sstride = (long long)sims[0]->rsize / 2; fsim.nsim = nreal;
+      fsim.seed = seed; fsim.real0 = realization0;
+    }
+    if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0, fsim)) != CRV_OK) return rc;
     if (chunked) {
       if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
                                             realization0, pl->refine ? 1 : 0, j0, j1)) != CRV_OK)
@@ -1043,7 +1143,7 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   // complete and before their inverse transforms
   if ((rc = family_xy(ctx, fp, fm, true, crv_scale_inverse(mean[0]), nullptr, 0, nullptr, !chunked)) != CRV_OK)
     return rc;
-  if (!chunked && nsim &&
+  if (!chunked && nsim && !fused &&
       (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed, realization0,
                                 pl->refine ? 1 : 0, 0, nyp)) != CRV_OK)
     return rc;
