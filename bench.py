@@ -87,38 +87,49 @@ def timed(ctx, G, kw, mode, steps, warmup, seed):
     return (time.perf_counter() - t0) / steps
 
 
-def transforms_per_step(name, s):
+def transforms_per_step(name, s, names=()):
     """3D transforms per step whose passes carry this kernel name: forward
     for the n data and 3 means, inverse for 3 means, 3 realization grids
-    and 6 covariance grids."""
+    and 6 covariance grids (when the covariances' z inverse generates the
+    planes outside the band, "fft_z_inv_prior", 6 of the 12 z inverse
+    passes carry that name)."""
     if name in ("fft_x_r2c", "fft_y_fwd", "fft_z_fwd"):
         return s.n + 3
-    if name in ("fft_x_c2r", "fft_y_inv", "fft_z_inv"):
+    if name in ("fft_x_c2r", "fft_y_inv"):
         return 12
+    if name == "fft_z_inv":
+        return 6 if "fft_z_inv_prior" in names else 12
+    if name == "fft_z_inv_prior":
+        return 6
     return None
 
 
-def launch_bytes(name, s, prec, ninv, launches_per_step):
+def launch_bytes(name, s, prec, ninv, launches_per_step, names=()):
     """Algorithmic bytes of one launch: a kernel whose work per step is split
     over several launches (chunks of rows, sub-batches of grids) moves its
     share of the step's figure."""
-    b = kernel_bytes(name, s, prec, ninv)
+    b = kernel_bytes(name, s, prec, ninv, names)
     if b is None:
         return None
-    t = transforms_per_step(name, s) or 1
+    t = transforms_per_step(name, s, names) or 1
     return b * t / launches_per_step
 
 
-def kernel_bytes(name, s, prec, ninv):
+def kernel_bytes(name, s, prec, ninv, names=()):
     """Algorithmic HBM bytes of one kernel over the whole grid (reads +
     writes of the data it must touch: one transform pass, or one posterior
-    or simulation launch), for the FP32 or FP64 grids of survey s."""
+    or simulation launch), for the FP32 or FP64 grids of survey s; ninv
+    wavenumbers lie in the inverted band."""
     e = 4 if prec == 32 else 8
     N = s.nxp * s.nyp * s.nzp
     Nc = s.cnxp * s.nyp * s.nzp
     n = s.n
+    prior_cov_written = "fft_z_inv_prior" not in names
     if name in ("fft_y_fwd", "fft_z_fwd", "fft_y_inv", "fft_z_inv"):
         return 2 * 2 * e * Nc
+    if name == "fft_z_inv_prior":
+        # reads the band's planes only, writes every plane
+        return 2 * e * ninv + 2 * e * Nc
     if name == "fft_x_r2c":
         return e * N + 2 * e * Nc
     if name == "fft_x_c2r":
@@ -129,7 +140,8 @@ def kernel_bytes(name, s, prec, ninv):
         return Nc * (6 + 3) * 2 * e
     if name == "posterior_sim_fp32":
         # the posterior plus the realization's 3 spectra it writes
-        return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e + Nc * 3 * 2 * e
+        outside = (Nc - ninv) * 6 * 2 * e if prior_cov_written else 0
+        return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + outside + Nc * 3 * 2 * e
     return None
 
 
@@ -233,7 +245,7 @@ KERNEL_SYMBOLS = {
     "fft_y_fwd": "k_cols<float, false, 25, 18, false>", "fft_z_fwd": "k_cols<float, false, 20, 16, false>",
     "fft_y_inv": "k_cols<float, true, 25, 18, false>", "fft_z_inv": "k_cols<float, true, 20, 16, false>",
     "posterior_fp32": "k_posterior_rec<5, true, 0>", "posterior_sim_fp32": "k_posterior_rec<5, true, 2>",
-    "simulate_scale_batch": "k_simulate<float, true>",
+    "simulate_scale_batch": "k_simulate<float, true>", "fft_z_inv_prior": "k_cols<float, true, 20, 16, false, true>",
 }
 
 
@@ -595,7 +607,7 @@ def main():
         ctx.synchronize()
         res[prec] = dict(sec=sec, prof=prof, grids=G)
         for k_, v_ in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-            b_ = launch_bytes(k_, s, prec, ninv, v_[0] / 3)
+            b_ = launch_bytes(k_, s, prec, ninv, v_[0] / 3, tuple(prof))
             gbs = f"{b_ / (v_[1] / v_[0] / 1e3) / 1e9:7.0f} GB/s" if b_ else ""
             log(f"[bench]   FP{prec} {k_:22s} x{v_[0] // 3:3d}/step  avg {v_[1] / v_[0]:.4f} ms  "
                 f"{v_[1] / 3:.3f} ms/step {gbs}")
@@ -645,13 +657,13 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (cnt, tot, mn, mx) = dom
     avg_s = tot / cnt / 1e3
-    kb = launch_bytes(name, s, 32, ninv, cnt / 3)
+    kb = launch_bytes(name, s, 32, ninv, cnt / 3, tuple(prof))
     roof = {"kernel": name, "bound": "hbm", "achieved": (kb / avg_s / 1e9) if kb else None, "peak": PEAK_HBM / 1e9,
             "unit": "GB/s", "frac": (kb / avg_s / PEAK_HBM) if kb else None, "traffic": read_traffic(name),
             "avg_ms": avg_s * 1e3, "bytes_per_launch": kb}
     kernels = {}
     for k, v in prof.items():
-        b_ = launch_bytes(k, s, 32, ninv, v[0] / 3)
+        b_ = launch_bytes(k, s, 32, ninv, v[0] / 3, tuple(prof))
         kernels[k] = {"launches_per_step": v[0] / 3, "avg_ms": v[1] / v[0], "ms_per_step": v[1] / 3,
                       "GB/s": (b_ / (v[1] / v[0] / 1e3) / 1e9) if b_ else None}
     bpv = pipeline_bytes_per_voxel(s.n, 32)

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace {
 
@@ -533,9 +534,16 @@ struct ColArgs {
   int cshift;          // TX = 1 << cshift
   long long bstride;   // elements between the grids of a batch (blockIdx.y)
   int rev;             // walk the tiles backwards (xcd_remap)
+  // SYN (z inverse of separable covariance spectra whose prior planes the
+  // posterior did not write): where band[e] is 0 the element of column x,
+  // y row o is lam[o cnxp + x] S[6 e + q], covariance q = q0 + blockIdx.y
+  const float* syn_lam;
+  const float* syn_S;
+  const int* syn_band;
+  int syn_q0;
 };
 
-template <typename T, bool INV, int RF, int RL, bool H>
+template <typename T, bool INV, int RF, int RL, bool H, bool SYN = false>
 __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
@@ -548,8 +556,26 @@ __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   const int x0 = t * TX;
   const int nc = min(TX, a.ncols - x0);
   C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
+  // SYN: the plane table of this covariance (S; NaN inside the band, where
+  // the element is read) and the tile's lam columns, staged in LDS once per
+  // block
+  float* synS = reinterpret_cast<float*>(twl + a.f.twn);
+  float* synL = synS + a.f.L;
+  if constexpr (SYN) {
+    const int q = a.syn_q0 + blockIdx.y;
+    for (int e = threadIdx.x; e < a.f.L; e += kThreads)
+      synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
+    for (int c = threadIdx.x; c < TX; c += kThreads)
+      synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
+    __syncthreads();
+  }
   auto load = [&](int c, int e) -> C {
-    return (c < nc) ? base[(size_t)e * a.estride + c] : cmake<C>((T)0, (T)0);
+    if (c >= nc) return cmake<C>((T)0, (T)0);
+    if constexpr (SYN) {
+      const float sv = synS[e];
+      if (!__builtin_isnan(sv)) return cmake<C>((T)(synL[c] * sv), (T)0);
+    }
+    return base[(size_t)e * a.estride + c];
   };
   auto store = [&](int c, int e, C v) {
     if (c < nc) base[(size_t)e * a.estride + c] = v;
@@ -644,6 +670,17 @@ const void* rows_kernel(const Fft1D& f, bool inv, bool half) {
   return inv ? (const void*)k_rows_c2r<T, 0, 0, false> : (const void*)k_rows_r2c<T, 0, 0, false>;
 }
 
+// z inverse with the prior planes generated (float grids)
+const void* cols_kernel_syn(const Fft1D& f, bool half) {
+  if (f.nst == 2) {
+#define CRV_PAIR_SYN(A, B) \
+    if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<float, true, A, B, false, true>;
+    CRV_FFT_PAIRS(CRV_PAIR_SYN)
+#undef CRV_PAIR_SYN
+  }
+  return half ? (const void*)k_cols<float, true, 0, 0, true, true> : (const void*)k_cols<float, true, 0, 0, false, true>;
+}
+
 template <typename T, bool INV>
 const void* cols_kernel(const Fft1D& f, bool half) {
   if (f.nst == 2) {
@@ -691,8 +728,16 @@ int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const 
 }
 
 // y pass over z planes [k0, k1) or z pass over y rows [j0, j1)
+struct ZSynth {
+  const float* lam = nullptr;
+  const float* S = nullptr;
+  const int* band = nullptr;
+  int q0 = 0;
+};
+
 template <typename T>
-int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data, const Batch& bt, int o0, int o1) {
+int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data, const Batch& bt, int o0, int o1,
+                const ZSynth* zs = nullptr) {
   using C = cplx_t<T>;
   const int cnxp = p->nxp / 2 + 1;
   ColArgs<T> a{};
@@ -711,12 +756,22 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
   a.cshift = ilog2(tx);
   a.ntx = (cnxp + tx - 1) / tx;
   const int nb = a.ntx * (o1 - o0);
-  const size_t lds = ((size_t)a.f.L * tx + a.f.twn) * sizeof(C);
+  size_t lds = ((size_t)a.f.L * tx + a.f.twn) * sizeof(C);
+  if (zs && zaxis && inv) lds += ((size_t)a.f.L + tx) * sizeof(float);
   static const char* names[4] = {"fft_y_fwd", "fft_y_inv", "fft_z_fwd", "fft_z_inv"};
-  ProfScope ps(ctx, names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
+  const bool syn = zs && zaxis && inv && std::is_same<T, float>::value;
+  ProfScope ps(ctx, syn ? "fft_z_inv_prior" : names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
   void* args[] = {&a};
   const bool half = p->half[zaxis ? 2 : 1];
   const void* kern = inv ? cols_kernel<T, true>(a.f, half) : cols_kernel<T, false>(a.f, half);
+  if (syn) {
+    // the generated elements of row o0 + o use the tables of rows from o0
+    a.syn_lam = zs->lam + (size_t)o0 * cnxp;
+    a.syn_S = zs->S;
+    a.syn_band = zs->band;
+    a.syn_q0 = zs->q0;
+    kern = cols_kernel_syn(a.f, half);
+  }
   CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(kThreads), args, lds, ctx->stream));
   return CRV_OK;
 }
@@ -944,7 +999,8 @@ int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src
 
 // One pass of a transform over part of a batch (see internal.hpp).
 int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
-                    int o0, int o1, double scale, const void* const* add, int nadd) {
+                    int o0, int o1, double scale, const void* const* add, int nadd, const float* syn_lam,
+                    const float* syn_S, const int* syn_band, int syn_q0) {
   if (nbatch < 1 || nbatch > 65535 || nadd < 0 || nadd > 3 || pass < 0 || pass > 5 || o0 < 0 || o1 < o0)
     return crv_set_error(ctx, CRV_E_INVALID, "bad pass");
   if (o1 == o0) return CRV_OK;
@@ -962,8 +1018,11 @@ int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void
     default: {
       const bool inv = pass >= 3, zaxis = (pass == 2 || pass == 3);
       if ((zaxis ? plan->nzp : plan->nyp) == 1) return CRV_OK;
+      ZSynth zs;
+      zs.lam = syn_lam; zs.S = syn_S; zs.band = syn_band; zs.q0 = syn_q0;
+      const ZSynth* zp = (pass == 3 && syn_lam && !f64) ? &zs : nullptr;
       return f64 ? launch_cols<double>(ctx, plan, inv, zaxis, (double2*)buf, bt, o0, o1)
-                 : launch_cols<float>(ctx, plan, inv, zaxis, (float2*)buf, bt, o0, o1);
+                 : launch_cols<float>(ctx, plan, inv, zaxis, (float2*)buf, bt, o0, o1, zp);
     }
   }
 }

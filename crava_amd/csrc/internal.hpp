@@ -157,14 +157,18 @@ int crv_launch_c2r_batch(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, lon
                          const void* const* add, int nadd);
 int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src, int nbatch, long long bstride,
                          double scale);
+// grids per launch of a batched transform of nbatch grids (CRV_TUNE_BATCH_MB)
+int crv_sub_batch(const FftPlan* p, int nbatch);
 // One pass of a transform over part of a batch of nbatch grids bstride reals
 // apart: pass 0 x R2C (rows [o0, o1), row = j + nyp k; reads src, scales),
 // 1 y forward (z planes [o0, o1)), 2 z forward (y rows [o0, o1)), 3 z
 // inverse, 4 y inverse, 5 x C2R (rows, in place, scales, adds add[b % nadd]).
-// grids per launch of a batched transform of nbatch grids (CRV_TUNE_BATCH_MB)
-int crv_sub_batch(const FftPlan* p, int nbatch);
+// With syn_lam (pass 3, FP32): the grids are covariances q0 + b of a
+// separable prior whose planes with syn_band[k] == 0 were not written; their
+// elements are generated as syn_lam[j cnxp + i] syn_S[6 k + q].
 int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
-                    int o0, int o1, double scale, const void* const* add, int nadd);
+                    int o0, int o1, double scale, const void* const* add, int nadd, const float* syn_lam = nullptr,
+                    const float* syn_S = nullptr, const int* syn_band = nullptr, int syn_q0 = 0);
 // Noise and Cholesky scaling of nreal counter-stream realizations over y
 // rows [j0, j1): realization r, parameter g written at out0 + (3 r + g)
 // gstride complex elements (simulate.hip).

@@ -92,6 +92,9 @@ struct PostArgs {
   int nsim;
   unsigned long long seed, real0;
   NoiseGeom geo;
+  // the covariances of planes outside the band are left unwritten (the z
+  // inverse pass generates them from the same tables, crv_invert)
+  int skip_prior_cov;
 };
 
 // Simulation of one wavenumber from its covariance c as stored (FP32):
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
     for (int q = 0; q < 6; ++q) s[q] = lam * R(OS + q);
     if (R(127) == 0.0f) {
       if (live) {
-        store_prior(a, idx, s);
+        if (!a.skip_prior_cov) store_prior(a, idx, s);
         if constexpr (SIM > 0) {
           float2 c[6];
 #pragma unroll
@@ -898,6 +901,7 @@ struct FusedSim {
   long long sstride = 0;
   int nsim = 0;
   unsigned long long seed = 0, real0 = 0;
+  bool skip_prior_cov = false;
 };
 
 int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, crv_grid* err_corr,
@@ -941,6 +945,7 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
     if (pl->oR) {
       a.rec = (const float2*)(pl->dtab + pl->oR);
       a.cpp = (pl->cnxp * nyr + 63) / 64;
+      a.skip_prior_cov = fs.skip_prior_cov ? 1 : 0;
       if (fs.nsim > 0) {
         a.sim = fs.sim; a.sstride = fs.sstride; a.nsim = fs.nsim; a.seed = fs.seed; a.real0 = fs.real0;
         a.geo = make_noise_geom_dims(pl->cnxp, pl->nyp, pl->nzp);
@@ -1026,8 +1031,14 @@ int family_pass(crv_ctx* ctx, FftPlan* fp, const Family& f, int pass, int o0, in
 // z pass too (forward last, inverse first), over a family in sub-batches
 // that fit the memory-side cache, so each pass of a sub-batch reads the
 // previous pass's output from there
+struct CovSynth {
+  const float* lam = nullptr;
+  const float* S = nullptr;
+  const int* band = nullptr;
+};
+
 int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double scale, const void* const* add,
-              int nadd, crv_grid* const* src = nullptr, bool with_z = false) {
+              int nadd, crv_grid* const* src = nullptr, bool with_z = false, const CovSynth* cs = nullptr) {
   const int nrows = fp->nyp * fp->nzp;
   // a forward x pass may read other grids (src): one launch per sub-batch
   // when they form a batch with the same stride, else one per grid
@@ -1050,7 +1061,21 @@ int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double s
       if ((rc = family_pass(ctx, fp, s, 1, 0, fp->nzp, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if (with_z && (rc = family_pass(ctx, fp, s, 2, 0, fp->nyp, 1.0, nullptr, 0)) != CRV_OK) return rc;
     } else {
-      if (with_z && (rc = family_pass(ctx, fp, s, 3, 0, fp->nyp, 1.0, nullptr, 0)) != CRV_OK) return rc;
+      if (with_z && cs) {
+        // covariances b0.. of the family: q = b0 + b
+        if (s.contiguous) {
+          rc = crv_launch_pass(ctx, fp, 3, s.g[0]->d, s.g[0]->d, s.n, (long long)s.g[0]->rsize, 0, fp->nyp, 1.0,
+                               nullptr, 0, cs->lam, cs->S, cs->band, b0);
+        } else {
+          rc = CRV_OK;
+          for (int b = 0; b < s.n && rc == CRV_OK; ++b)
+            rc = crv_launch_pass(ctx, fp, 3, s.g[b]->d, s.g[b]->d, 1, 0, 0, fp->nyp, 1.0, nullptr, 0, cs->lam, cs->S,
+                                 cs->band, b0 + b);
+        }
+        if (rc != CRV_OK) return rc;
+      } else if (with_z && (rc = family_pass(ctx, fp, s, 3, 0, fp->nyp, 1.0, nullptr, 0)) != CRV_OK) {
+        return rc;
+      }
       if ((rc = family_pass(ctx, fp, s, 4, 0, fp->nzp, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if ((rc = family_pass(ctx, fp, s, 5, 0, nrows, scale, ad, nadd)) != CRV_OK) return rc;
     }
@@ -1107,6 +1132,10 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   // whole grids on an FP32 plan with per-plane records: the posterior
   // passes write the realizations themselves (no re-read of the covariances)
   const bool fused = !chunked && nsim > 0 && pl->oR != 0;
+  // whole grids on an FP32 plan with records: the covariances of planes
+  // outside the band are the cleaned prior lam(i,j) S(k); the posterior
+  // leaves them unwritten and the z inverse pass generates them
+  const bool synth = !chunked && pl->oR != 0;
   // forward x and y passes of the data and the means
   if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src, !chunked)) != CRV_OK)
     return rc;
@@ -1127,6 +1156,7 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
       fsim.sim = (float2*)sims[0]->d; fsim.sstride = (long long)sims[0]->rsize / 2; fsim.nsim = nreal;
       fsim.seed = seed; fsim.real0 = realization0;
     }
+    fsim.skip_prior_cov = synth;
     if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0, fsim)) != CRV_OK) return rc;
     if (chunked) {
       if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
@@ -1153,7 +1183,13 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
     if ((rc = family_xy(ctx, fp, fs, true, crv_scale_inverse(sims[0]), add, 3, nullptr, !chunked)) != CRV_OK)
       return rc;
   }
-  if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked)) != CRV_OK)
+  CovSynth cs;
+  if (synth) {
+    const KzTab<float> t = tab_at<float>(pl->dtab, pl->oT);
+    cs.lam = t.lam; cs.S = t.S; cs.band = (const int*)(pl->dtab + pl->oI);
+  }
+  if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked,
+                      synth ? &cs : nullptr)) != CRV_OK)
     return rc;
   for (int l = 0; l < n; ++l) data[l]->transformed = true;
   for (int m = 0; m < 3; ++m) mean[m]->transformed = false;
