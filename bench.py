@@ -133,7 +133,9 @@ def kernel_bytes(name, s, prec, ninv, names=()):
     if name == "fft_x_r2c":
         return e * N + 2 * e * Nc
     if name == "fft_x_c2r":
-        return 2 * e * Nc + e * N            # (+ e*N when the mean is added)
+        # per transform, averaged over the step's 12: the realization's 3 also
+        # read the posterior mean they add
+        return 2 * e * Nc + e * N + e * N * 3 / 12
     if name in ("posterior", "posterior_fp32"):
         return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + (Nc - ninv) * 6 * 2 * e
     if name in ("simulate_scale", "simulate_scale_batch"):

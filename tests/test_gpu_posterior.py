@@ -159,3 +159,26 @@ def test_c0_end_to_end_fp32_vs_fp64_oracle(ctx):
         assert rel_rms(g.download_real()[:, :, :64] - mu, m[:, :, :64] - mu) < 1e-4
     for g, c in zip(r["cov"], o["cov"]):
         assert rel_rms(g.download_complex(), c) < 1e-4
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 8])
+def test_angle_counts_against_fp64_oracle(ctx, n):
+    """Angle counts other than the configs' 3 and 5: 1..6 run the per-plane
+    record kernel in FP32, 7 and 8 the general one.  FP64 equals the FP64
+    oracle; FP32 with refinement stays within the precision target; the
+    separable prior equals the grid path."""
+    case = Case(model.config_small(16, 12, 20, n))
+    o = oracle_posterior(case, np.float64)
+    r64 = gpu_posterior(ctx, case, prec=64)
+    for g, m in zip(r64["mean"], o["mean"]):
+        assert rel_rms(g.download_real()[:, :, :16], m[:, :, :16]) < 1e-11
+    for g, c in zip(r64["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-10
+    for g, d in zip(r64["data"], o["data"]):
+        assert rel_rms(g.download_complex(), d) < 1e-10
+    r32 = gpu_posterior(ctx, case, prec=32, mode=1, separable=True)
+    for g, m in zip(r32["mean"], o["mean"]):
+        assert rel_rms(g.download_real()[:, :, :16], m[:, :, :16]) < 1e-5
+    for g, c in zip(r32["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-4
+    assert r32["stats"].n_chol_fail == 0 and r32["stats"].n_inverted > 0
