@@ -96,3 +96,41 @@ extern "C" int crvh_invert(int nxp, int nyp, int nzp, int ntheta, const float* c
   crv_ctx_destroy(ctx);
   return rc;
 }
+
+// The output path of the host FFTGrid (ParameterOutput's vp output:
+// expTransf, then FFTGrid::writeCravaFile; reference src/parameteroutput.cpp:
+// 84-93, src/fftgrid.cpp:1726-1743,2411-2445), and readCravaFile of the file
+// written (src/fftgrid.cpp:2448-2498) into a second grid: log_real (rsize
+// floats) in, the exponentiated grid read back in exp_out.
+extern "C" int crvh_write_read_crava(int nxp, int nyp, int nzp, const float* log_real, const char* base_name,
+                                     float* exp_out, char* err, int errlen) {
+  crv_ctx* ctx = nullptr;
+  if (crv_ctx_create(0, &ctx) != CRV_OK) {
+    if (err && errlen) snprintf(err, errlen, "no GPU");
+    return CRV_E_HIP;
+  }
+  int rc = 0;
+  try {
+    const long long rsize = (long long)2 * (nxp / 2 + 1) * nyp * nzp;
+    crv_simbox_info sb{};
+    sb.dx = 25.0; sb.dy = 25.0; sb.nx = nxp; sb.ny = nyp;
+    {
+      FFTGrid g(ctx, nxp, nyp, nzp, nxp, nyp, nzp, CRV_FP32);
+      g.setType(FFTGrid::PARAMETER);
+      g.setRealValues(std::vector<float>(log_real, log_real + rsize));
+      g.expTransf();
+      g.writeCravaFile(base_name, sb);
+    }
+    FFTGrid h(ctx, nxp, nyp, nzp, nxp, nyp, nzp, CRV_FP32);
+    std::string errText;
+    h.readCravaFile(std::string(base_name) + ".crava", errText);
+    if (!errText.empty()) throw HipError(errText);
+    auto r = h.getRealValues();
+    std::memcpy(exp_out, r.data(), r.size() * sizeof(float));
+  } catch (const std::exception& e) {
+    if (err && errlen) snprintf(err, errlen, "%s", e.what());
+    rc = CRV_E_INVALID;
+  }
+  crv_ctx_destroy(ctx);
+  return rc;
+}

@@ -51,3 +51,24 @@ def test_host_classes_match_abi(ctx):
         assert rel_rms(got[..., 0] + 1j * got[..., 1], r["cov"][q].download_complex()) < 1e-5
     for a in range(3):
         assert np.all(np.isfinite(real_out[a]))
+
+
+def test_host_fftgrid_output_path(ctx, tmp_path):
+    """Host FFTGrid::expTransf + writeCravaFile + readCravaFile: the file is
+    CRAVA's binary format of exp(grid) and reads back unchanged."""
+    nxp, nyp, nzp = 30, 20, 16
+    rng = np.random.default_rng(5)
+    x = (np.log(2500.0) + 0.1 * rng.standard_normal((nzp, nyp, 2 * (nxp // 2 + 1)))).astype(np.float32)
+    host = C.CDLL(str(PKG_DIR / "libcrava_host.so"))
+    out = np.zeros_like(x)
+    ebuf = C.create_string_buffer(512)
+    base = str(tmp_path / "vp")
+    fp = C.POINTER(C.c_float)
+    rc = host.crvh_write_read_crava(nxp, nyp, nzp, x.ctypes.data_as(fp), base.encode(), out.ctypes.data_as(fp),
+                                    ebuf, 512)
+    assert rc == 0, ebuf.value
+    want = np.exp(x.astype(np.float64)).astype(np.float32)
+    assert np.max(np.abs(out - want) / want) <= 1.2e-7
+    raw = (tmp_path / "vp.crava").read_bytes()
+    assert raw.startswith(b"crava_fftgrid_binary\n")
+    assert raw[-out.size * 4:] == out.astype(">f4").tobytes()
