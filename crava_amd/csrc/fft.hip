@@ -185,6 +185,40 @@ __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int
   __syncthreads();
 }
 
+// first stage (Ns = 1, no twiddles) whose inputs come from load(c, e) --
+// values derived from LDS in another layout; ends with the tile written in
+// the lds[e*S + c] layout over that LDS (all reads finish before the writes)
+template <typename T, int R, bool INV, int E, typename Load>
+__device__ __forceinline__ void stage_first_lds(cplx_t<T>* lds, int S, int cshift, int L, Load load) {
+  using C = cplx_t<T>;
+  constexpr int MAXB = (E + R - 1) / R;
+  const int Lr = L / R;
+  const int total = Lr << cshift;
+  const int cm = (1 << cshift) - 1;
+  C v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      const int c = b & cm, j = b >> cshift;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = load(c, j + r * Lr);
+      Dft<T, R, INV>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * kThreads;
+    if (b < total) {
+      const int c = b & cm, j = b >> cshift;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[(j * R + r) * S + c] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
 // last stage (Ns = L/R, so k = j): LDS -> store(c, e)
 template <typename T, int R, bool INV, bool JFAST, int E, typename Store>
 __device__ __forceinline__ void stage_last(const cplx_t<T>* lds, int S, int cshift, int L, FastDiv dlr,
@@ -426,7 +460,10 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   T* dst0 = a.dst + boff + (size_t)row0 * a.rnxp;
   // the added grids are separate allocations, not members of the batch
   const T* add = a.nadd ? a.add[blockIdx.y % a.nadd] + (size_t)row0 * a.rnxp : nullptr;
-  // half-spectrum rows -> registers -> staging rows lds[c*K + k]
+  // half-spectrum rows -> registers -> staging rows lds[c*KS + k]; the
+  // two-stage kernels read the rows across lanes (one row per lane), so
+  // their stride is odd
+  const int KS = (RF > 0) ? (K | 1) : K;
   const int PK = P * K, PL = P * L;
   C v[E];
 #pragma unroll
@@ -440,42 +477,58 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int idx = threadIdx.x + q * kThreads;
-    if (idx < PK) lds[idx] = v[q];
-  }
-  __syncthreads();
-  // Z[k] of the half-length complex transform (or the full Hermitian row)
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int idx = threadIdx.x + q * kThreads;
-    if (idx < PL) {
-      const int c = fdiv(idx, a.divL), k = idx - c * L;
-      const C* X = lds + c * K;
-      if (a.xeven) {
-        C xa = X[k], xb = X[L - k];
-        if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
-        xb = cconj(xb);
-        const C d = csub(xa, xb);
-        const C t = cmulc(d, wpl[k]);  // d * exp(+2 pi i k / nxp)
-        // Z = (xa + xb) + i t
-        v[q] = cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
+    if (idx < PK) {
+      if constexpr (RF > 0) {
+        const int c = fdiv(idx, a.divK), k = idx - c * K;
+        lds[c * KS + k] = v[q];
       } else {
-        C z = (k < K) ? X[k] : cconj(X[L - k]);
-        if (k == 0) z.y = (T)0;
-        v[q] = z;
+        lds[idx] = v[q];
       }
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int idx = threadIdx.x + q * kThreads;
-    if (idx < PL) {
-      const int c = fdiv(idx, a.divL), k = idx - c * L;
-      lds[k * S + c] = v[q];
+  // Z[k] of the half-length complex transform (or the full Hermitian row)
+  // from staged row c
+  auto zval = [&](int c, int k) -> C {
+    const C* X = lds + c * KS;
+    if (a.xeven) {
+      C xa = X[k], xb = X[L - k];
+      if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
+      xb = cconj(xb);
+      const C d = csub(xa, xb);
+      const C t = cmulc(d, wpl[k]);  // d * exp(+2 pi i k / nxp)
+      // Z = (xa + xb) + i t
+      return cmake<C>(xa.x + xb.x - t.y, xa.y + xb.y + t.x);
     }
+    C z = (k < K) ? X[k] : cconj(X[L - k]);
+    if (k == 0) z.y = (T)0;
+    return z;
+  };
+  if constexpr (RF > 0) {
+    // the first stage reads Z from the staged rows; its output replaces them
+    stage_first_lds<T, RF, true, E>(lds, S, a.pshift, L, zval);
+    stage_mid<T, RL, true, true, E>(lds, S, a.pshift, L, a.f.Ns[1], a.sd.ns[1], twl + (a.f.tw[1] - a.f.tw0));
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      if (idx < PL) {
+        const int c = fdiv(idx, a.divL), k = idx - c * L;
+        v[q] = zval(c, k);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      if (idx < PL) {
+        const int c = fdiv(idx, a.divL), k = idx - c * L;
+        lds[k * S + c] = v[q];
+      }
+    }
+    __syncthreads();
+    transform_tile_lds<T, true, RF, RL, H>(lds, S, a.pshift, a.f, a.sd, twl);
   }
-  __syncthreads();
-  transform_tile_lds<T, true, RF, RL, H>(lds, S, a.pshift, a.f, a.sd, twl);
   __syncthreads();
   // tile -> rows, scaled, plus the added grid
   if (a.xeven) {
@@ -646,7 +699,7 @@ int ilog2(int v) { int s = 0; while ((1 << s) < v) ++s; return s; }
 // half-spectrum rows P cnxp of a C2R
 int row_tile_elems(const FftPlan* p) {
   const int cnxp = p->nxp / 2 + 1;
-  return std::max((p->rows_per_block + 1) * p->fx.L, p->rows_per_block * cnxp);
+  return std::max((p->rows_per_block + 1) * p->fx.L, p->rows_per_block * (cnxp | 1));
 }
 
 template <typename T>
@@ -903,10 +956,11 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   // x rows per block within a budget of complex values per tile; axes
   // without a two-stage kernel use the half budget (and the kernel with half
   // the registers) when their tiles fit it
+  const int cns = cnx | 1;  // staged C2R rows (odd stride)
   auto pick_rows = [&](int budget) {
     int P = 1;
-    while (2 * P <= pmax && (2 * P + 1) * Lx <= budget && 2 * P * cnx <= budget) P *= 2;
-    return ((P + 1) * Lx <= budget && P * cnx <= budget) ? P : 0;
+    while (2 * P <= pmax && (2 * P + 1) * Lx <= budget && 2 * P * cns <= budget) P *= 2;
+    return ((P + 1) * Lx <= budget && P * cns <= budget) ? P : 0;
   };
   p->half[0] = !axis_has_pair(p->fx) && pick_rows(maxe / 2) > 0;
   const int P = pick_rows(p->half[0] ? maxe / 2 : maxe);
@@ -1113,7 +1167,7 @@ extern "C" int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int pre
     int bestP = p->rows_per_block;
     const int bx = p->half[0] ? maxe / 2 : maxe;
     for (int P = 1; P <= 64 && rc == CRV_OK; P *= 2) {
-      if ((P + 1) * p->fx.L > bx || P * (p->nxp / 2 + 1) > bx) break;
+      if ((P + 1) * p->fx.L > bx || P * ((p->nxp / 2 + 1) | 1) > bx) break;
       p->rows_per_block = P;
       float ms = 0.0f;
       if ((rc = timed(rows, ms)) == CRV_OK && ms < best) { best = ms; bestP = P; }
