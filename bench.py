@@ -12,7 +12,12 @@ the 6 covariance grids (Crava::computePostMeanResidAndFFTCov, simulate and
 invFFTCovGrids; reference src/crava.cpp:205-300).  Inputs are resident in
 HBM when the timed region starts.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c0]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c0|c3|c4]
+                    [--path invert|sequence] [--chunk-rows R] [--order MASK] [--no-cpu] [--no-fp64]
+
+--config c3 times the batched workload (64 realizations of 256x256x200; 12
+uneven volumes in one HIP graph), --config c4 the learned rock-physics
+surrogate and the facies probabilities it feeds.
 """
 from __future__ import annotations
 

@@ -9,11 +9,13 @@
 //   cols : y, pencils of stride cnxp complex, tiles of TX consecutive x;
 //   cols : z, pencils of stride cnxp*nyp complex, tiles of TX consecutive x.
 //
-// Each tile goes through self-sorting (Stockham) mixed-radix stages.  The
-// first stage reads its butterflies straight from HBM into registers and
-// writes LDS; middle stages go LDS -> registers -> LDS; the last stage
-// writes HBM straight from registers (rows of an R2C go through LDS once
-// more for the real-to-complex post-processing).  Lengths that are a product
+// Each tile goes through self-sorting (Stockham) mixed-radix stages.  In
+// the column passes the first stage reads its butterflies straight from HBM
+// into registers and writes LDS, middle stages go LDS -> registers -> LDS,
+// and the last stage writes HBM straight from registers.  Row tiles move
+// between HBM and LDS with the lanes along the rows (coalesced) and every
+// stage runs LDS -> LDS; the real-to-complex post-processing (R2C) and
+// pre-processing (C2R) work on the staged rows.  Lengths that are a product
 // of two radices from a fixed list of pairs up to 25 x 18 (composite radices
 // done in registers, dft.hpp) -- the production grid's 225, 450 and 320
 // among them -- get kernels compiled for their pair: two stages, i.e. one
