@@ -122,7 +122,7 @@ __device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, i
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       int c, j;
       bmap<JFAST>(b, cshift, dlr, c, j);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, i
   }
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       int c, j;
       bmap<JFAST>(b, cshift, dlr, c, j);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
       v[q][0] = lds[j * S + c];
@@ -175,7 +175,7 @@ __device__ __forceinline__ void stage_mid(cplx_t<T>* lds, int S, int cshift, int
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
       const int k = TW ? j - fdiv(j, dns) * Ns : 0;
@@ -200,7 +200,7 @@ __device__ __forceinline__ void stage_first_lds(cplx_t<T>* lds, int S, int cshif
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
 #pragma unroll
@@ -211,7 +211,7 @@ __device__ __forceinline__ void stage_first_lds(cplx_t<T>* lds, int S, int cshif
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       const int c = b & cm, j = b >> cshift;
 #pragma unroll
@@ -231,7 +231,7 @@ __device__ __forceinline__ void stage_last(const cplx_t<T>* lds, int S, int cshi
   const int total = Lr << cshift;
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = threadIdx.x + q * kThreads;
+    const int b = threadIdx.x + q * (int)blockDim.x;
     if (b < total) {
       int c, j;
       bmap<JFAST>(b, cshift, dlr, c, j);
@@ -255,7 +255,7 @@ __device__ __forceinline__ void stage_single(int cshift, Load load, Store store)
   const int total = 1 << cshift;  // one butterfly per pencil
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int c = threadIdx.x + q * kThreads;
+    const int c = threadIdx.x + q * (int)blockDim.x;
     if (c < total) {
       C v[R];
 #pragma unroll
@@ -303,7 +303,7 @@ __device__ __forceinline__ void transform_tile(cplx_t<T>* lds, int S, int cshift
   if (nst == 0) {
     // length 1: the tile is its own transform
     const int total = 1 << cshift;
-    for (int b = threadIdx.x; b < total; b += kThreads) {
+    for (int b = threadIdx.x; b < total; b += (int)blockDim.x) {
       if (LAST_TO_LDS) lds[b] = load(b, 0);  // element 0 of pencil b
       else store(b, 0, load(b, 0));
     }
@@ -350,7 +350,7 @@ __device__ __forceinline__ void transform_tile_lds(cplx_t<T>* lds, int S, int cs
 
 template <typename T>
 __device__ __forceinline__ void copy_to_lds(cplx_t<T>* dst, const cplx_t<T>* src, int n) {
-  for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
+  for (int i = threadIdx.x; i < n; i += (int)blockDim.x) dst[i] = src[i];
 }
 
 // ---- x pass: rows ----------------------------------------------------------
@@ -598,8 +598,10 @@ struct ColArgs {
   int syn_q0;
 };
 
-template <typename T, bool INV, int RF, int RL, bool H, bool SYN = false>
-__global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
+// W: 512-thread blocks (tiles of up to 8192 complex FP32 values), so that
+// the pencil tiles of the two-stage lengths can be 16 columns (128 B) wide
+template <typename T, bool INV, int RF, int RL, bool H, bool SYN = false, bool W = false>
+__global__ __launch_bounds__(W ? 2 * kThreads : kThreads) void k_cols(ColArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -618,9 +620,9 @@ __global__ __launch_bounds__(kThreads) void k_cols(ColArgs<T> a) {
   float* synL = synS + a.f.L;
   if constexpr (SYN) {
     const int q = a.syn_q0 + blockIdx.y;
-    for (int e = threadIdx.x; e < a.f.L; e += kThreads)
+    for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x)
       synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
-    for (int c = threadIdx.x; c < TX; c += kThreads)
+    for (int c = threadIdx.x; c < TX; c += (int)blockDim.x)
       synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
     __syncthreads();
   }
@@ -695,6 +697,13 @@ bool plan_1d(int L, Fft1D& f, std::vector<long double>& twr, std::vector<long do
 }
 
 int pow2_floor(int v) { int p = 1; while (p * 2 <= v) p *= 2; return p; }
+
+// complex FP32 values of a 512-thread column tile: twice the 256-thread
+// budget, less what keeps the tile, the axis's twiddles and the generated
+// z-inverse tables (L + TX floats) within 64 KB of LDS
+int wide_budget(const Fft1D& f, int maxe) {
+  return std::min(2 * maxe, 65536 / (int)sizeof(float2) - f.twn - (f.L + 64) / 2);
+}
 int ilog2(int v) { int s = 0; while ((1 << s) < v) ++s; return s; }
 
 // complex elements of a row tile: the transform's (P+1) L, or the staged
@@ -725,11 +734,13 @@ const void* rows_kernel(const Fft1D& f, bool inv, bool half) {
   return inv ? (const void*)k_rows_c2r<T, 0, 0, false> : (const void*)k_rows_r2c<T, 0, 0, false>;
 }
 
-// z inverse with the prior planes generated (float grids)
-const void* cols_kernel_syn(const Fft1D& f, bool half) {
+// z inverse with the prior planes generated (float grids); wide: 512 threads
+const void* cols_kernel_syn(const Fft1D& f, bool half, bool wide) {
   if (f.nst == 2) {
-#define CRV_PAIR_SYN(A, B) \
-    if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<float, true, A, B, false, true>;
+#define CRV_PAIR_SYN(A, B)                                                                     \
+    if (f.R[0] == (A) && f.R[1] == (B))                                                        \
+      return wide ? (const void*)k_cols<float, true, A, B, false, true, true>                  \
+                  : (const void*)k_cols<float, true, A, B, false, true>;
     CRV_FFT_PAIRS(CRV_PAIR_SYN)
 #undef CRV_PAIR_SYN
   }
@@ -737,12 +748,21 @@ const void* cols_kernel_syn(const Fft1D& f, bool half) {
 }
 
 template <typename T, bool INV>
-const void* cols_kernel(const Fft1D& f, bool half) {
+const void* cols_kernel(const Fft1D& f, bool half, bool wide) {
   if (f.nst == 2) {
-#define CRV_PAIR_COLS(A, B) \
-    if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<T, INV, A, B, false>;
-    CRV_FFT_PAIRS(CRV_PAIR_COLS)
+    if constexpr (std::is_same<T, float>::value) {
+#define CRV_PAIR_COLS(A, B)                                                                    \
+      if (f.R[0] == (A) && f.R[1] == (B))                                                      \
+        return wide ? (const void*)k_cols<T, INV, A, B, false, false, true>                    \
+                    : (const void*)k_cols<T, INV, A, B, false>;
+      CRV_FFT_PAIRS(CRV_PAIR_COLS)
 #undef CRV_PAIR_COLS
+    } else {
+#define CRV_PAIR_COLS(A, B) \
+      if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_cols<T, INV, A, B, false>;
+      CRV_FFT_PAIRS(CRV_PAIR_COLS)
+#undef CRV_PAIR_COLS
+    }
   }
   return half ? (const void*)k_cols<T, INV, 0, 0, true> : (const void*)k_cols<T, INV, 0, 0, false>;
 }
@@ -818,16 +838,18 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
   ProfScope ps(ctx, syn ? "fft_z_inv_prior" : names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
   void* args[] = {&a};
   const bool half = p->half[zaxis ? 2 : 1];
-  const void* kern = inv ? cols_kernel<T, true>(a.f, half) : cols_kernel<T, false>(a.f, half);
+  const bool wide = p->wide[zaxis ? 2 : 1];
+  const void* kern = inv ? cols_kernel<T, true>(a.f, half, wide) : cols_kernel<T, false>(a.f, half, wide);
   if (syn) {
     // the generated elements of row o0 + o use the tables of rows from o0
     a.syn_lam = zs->lam + (size_t)o0 * cnxp;
     a.syn_S = zs->S;
     a.syn_band = zs->band;
     a.syn_q0 = zs->q0;
-    kern = cols_kernel_syn(a.f, half);
+    kern = cols_kernel_syn(a.f, half, wide);
   }
-  CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(kThreads), args, lds, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(wide ? 2 * kThreads : kThreads), args, lds,
+                                     ctx->stream));
   return CRV_OK;
 }
 
@@ -948,7 +970,9 @@ int crv_get_plan(crv_ctx* ctx, int nxp, int nyp, int nzp, int prec, FftPlan** ou
 // Launch shapes: a tile holds at most 4096 (FP32) / 2048 (FP64) complex
 // values (32 KB of LDS); x rows per block P with (P+1) L and P cnxp <= that; y/z tiles
 // of TX consecutive x columns, 16 (FP32) or 8 (FP64) at most, halved until
-// the pencil tile fits.  CRV_TUNE_* values override, within those limits.
+// the pencil tile fits.  FP32 two-stage y/z axes whose 16-column tile does
+// not fit 4096 values take the 512-thread kernel (up to 8192 values, 64 KB).
+// CRV_TUNE_* values override, within those limits.
 int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   const int maxe = kThreads * ((p->prec == CRV_FP64) ? Ept<double>::value : Ept<float>::value);
   const int txdef = (p->prec == CRV_FP64) ? 8 : 16;
@@ -975,8 +999,14 @@ int crv_plan_shapes(crv_ctx* ctx, FftPlan* p) {
   };
   p->half[1] = !axis_has_pair(p->fy) && pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y], maxe / 2) > 0;
   p->half[2] = !axis_has_pair(p->fz) && pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z], maxe / 2) > 0;
-  p->tx_y = pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y], p->half[1] ? maxe / 2 : maxe);
-  p->tx_z = pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z], p->half[2] ? maxe / 2 : maxe);
+  // FP32 two-stage axes may use 512-thread blocks (twice the budget), so
+  // that the tile is 16 columns (128 B per pencil row) wide
+  const bool wy = p->prec == CRV_FP32 && axis_has_pair(p->fy);
+  const bool wz = p->prec == CRV_FP32 && axis_has_pair(p->fz);
+  p->tx_y = pick_tx(p->nyp, ctx->tune[CRV_TUNE_TX_Y], p->half[1] ? maxe / 2 : (wy ? wide_budget(p->fy, maxe) : maxe));
+  p->tx_z = pick_tx(p->nzp, ctx->tune[CRV_TUNE_TX_Z], p->half[2] ? maxe / 2 : (wz ? wide_budget(p->fz, maxe) : maxe));
+  p->wide[1] = wy && p->tx_y * p->nyp > maxe;
+  p->wide[2] = wz && p->tx_z * p->nzp > maxe;
   if (!p->tx_y || !p->tx_z) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "nyp or nzp too large");
   p->chunk_mb = ctx->tune[CRV_TUNE_CHUNK_MB];
   p->batch_mb = ctx->tune[CRV_TUNE_BATCH_MB];
@@ -1183,14 +1213,19 @@ extern "C" int crv_fft_autotune(crv_ctx* ctx, int nxp, int nyp, int nzp, int pre
     int& tx = ax == 1 ? p->tx_y : p->tx_z;
     float best = 1e30f;
     int bestT = tx;
-    const int bt_ = p->half[ax] ? maxe / 2 : maxe;
+    const Fft1D& fa = ax == 1 ? p->fy : p->fz;
+    const bool wok = !f64 && axis_has_pair(fa);
+    const int bt_ = p->half[ax] ? maxe / 2 : (wok ? wide_budget(fa, maxe) : maxe);
+    const bool wide0 = p->wide[ax];
     for (int t = 1; t <= 64 && rc == CRV_OK; t *= 2) {
       if (L * t > bt_) break;
       tx = t;
+      p->wide[ax] = wok && L * t > maxe;
       float ms = 0.0f;
       if ((rc = timed([&]() { return cols(ax == 2); }, ms)) == CRV_OK && ms < best) { best = ms; bestT = t; }
     }
     tx = bestT;
+    p->wide[ax] = (rc == CRV_OK) ? (wok && L * bestT > maxe) : wide0;
   }
   if (rc == CRV_OK && shape) {
     shape->rows_per_tile = p->rows_per_block;

@@ -70,6 +70,7 @@ struct FftPlan {
   int batch_mb = 0;      // batched transforms: MB of grids per launch, 0 = 256
   int order = 0;         // CRV_TUNE_ORDER bits: passes that walk their tiles backwards
   bool half[3] = {false, false, false};  // x, y, z: general kernel with half the elements per thread
+  bool wide[3] = {false, false, false};  // y, z (index 1, 2): 512-thread column kernel
   int rev[3] = {0, 0, 0};  // radix order per axis (0 default, 1 reversed)
   // reference noise: per-row prefix of draws (nyp*nzp+1 int64), lazily made
   long long* noise_prefix = nullptr;
