@@ -120,6 +120,31 @@ __device__ __forceinline__ void fused_sim(const PostArgs<float>& a, long long id
   }
 }
 
+// Simulation of a wavenumber outside the inverted band, whose covariance is
+// its plane's prior lam S with S real: the factor is sqrt(lam) L, L the
+// plane's factor of S (real, packed lower 00 10 11 20 21 22; zero where S is
+// rejected), made once per plane on the host.  o = sqrt(lam) L z in FP32.
+__device__ __forceinline__ void fused_sim_prior(const PostArgs<float>& a, long long idx, float lam,
+                                                const float L[6]) {
+  const bool ok = L[0] > 0.0f && lam > 0.0f;
+  const float sl = sqrtf(lam);
+  float l[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) l[q] = sl * L[q];
+  for (int r = 0; r < a.nsim; ++r) {
+    float2 o[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
+    if (ok) {
+      float re[3], im[3];
+      sim_noise3<float>(a.geo, a.seed, a.real0 + (unsigned long long)r, idx, re, im);
+      o[0] = make_float2(l[0] * re[0], l[0] * im[0]);
+      o[1] = make_float2(l[1] * re[0] + l[2] * re[1], l[1] * im[0] + l[2] * im[1]);
+      o[2] = make_float2(l[3] * re[0] + l[4] * re[1] + l[5] * re[2], l[3] * im[0] + l[4] * im[1] + l[5] * im[2]);
+    }
+#pragma unroll
+    for (int g = 0; g < 3; ++g) a.sim[(size_t)(3 * r + g) * a.sstride + idx] = o[g];
+  }
+}
+
 constexpr int kRecMaxN = 6;
 
 // wavenumber t of the main pass's row range -> (i, j, k) and its index
@@ -312,12 +337,15 @@ __global__ __launch_bounds__(kThreads) void k_posterior(PostArgs<T> a) {
 // across the wave: the plane's record (E P complex, B 3n complex, K 3n
 // complex, D n, S 6, the band flag at float 127) is read with one float2
 // per lane and broadcast with readlane instead of per-lane table loads.
+// Where the record has room (n <= 5) it also holds the plane's factor of S
+// (OL), which the FP64-refine simulation (SIM 2) uses outside the band.
 template <int N, bool LIST, int SIM>
 __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
   using C = float2;
   constexpr int P = N * (N + 1) / 2;
-  constexpr int OE = 0, OB = 2 * P, OK = OB + 6 * N, OD = OK + 6 * N, OS = OD + N;
+  constexpr int OE = 0, OB = 2 * P, OK = OB + 6 * N, OD = OK + 6 * N, OS = OD + N, OL = OS + 6;
   static_assert(OS + 6 <= 127, "record overflow");
+  constexpr bool PF = SIM == 2 && OL + 6 <= 127;
   unsigned long long nfail = 0;
   double cmax = 0.0;
   const int lane = threadIdx.x & 63;
@@ -345,7 +373,12 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
     if (R(127) == 0.0f) {
       if (live) {
         if (!a.skip_prior_cov) store_prior(a, idx, s);
-        if constexpr (SIM > 0) {
+        if constexpr (PF) {
+          float Lp[6];
+#pragma unroll
+          for (int q = 0; q < 6; ++q) Lp[q] = R(OL + q);
+          fused_sim_prior(a, idx, lam, Lp);
+        } else if constexpr (SIM > 0) {
           float2 c[6];
 #pragma unroll
           for (int q = 0; q < 6; ++q) c[q] = make_float2(s[q], 0.0f);
@@ -689,6 +722,28 @@ int validate_params(crv_ctx* ctx, const crv_post_params* p) {
   return CRV_OK;
 }
 
+// Factor L (packed lower 00 10 11 20 21 22) of a plane's prior S as the
+// FP32 record holds it (aa bb rr ab ar br), in extended precision with
+// lib_matrCholCpx's acceptance rule (lib_matr.c:78-167, the rule of
+// chol3_f64); all zero where S is rejected.
+void prior_factor(const float* S, float* L) {
+  using E = long double;
+  for (int q = 0; q < 6; ++q) L[q] = 0.0f;
+  const E p00 = S[0], p10 = S[3], p11 = S[1], p20 = S[4], p21 = S[5], p22 = S[2];
+  if (p00 <= 0) return;
+  const E tol = (E)kCholTol * p00;
+  if (p00 <= tol) return;
+  const E l00 = sqrtl(p00), l10 = p10 / l00, l20 = p20 / l00;
+  const E d1 = p11 - l10 * l10;
+  if (d1 <= tol) return;
+  const E l11 = sqrtl(d1), l21 = (p21 - l20 * l10) / l11;
+  const E d2 = p22 - l20 * l20 - l21 * l21;
+  if (d2 <= tol) return;
+  const E l22 = sqrtl(d2);
+  const E v[6] = {l00, l10, l11, l20, l21, l22};
+  for (int q = 0; q < 6; ++q) L[q] = (float)v[q];
+}
+
 // kz tables (crava.cpp:1104-1121, 1336-1364, 1441-1469) in FP64
 HostTables build_tables(const crv_post_params* p, int nyp, int nzp, int cnxp, long long& ninv_planes) {
   const int n = p->ntheta;
@@ -829,6 +884,7 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
       for (int q = 0; q < 6 * n; ++q) rk[o++] = (float)h.K[(size_t)k * 6 * n + q];
       for (int q = 0; q < n; ++q) rk[o++] = (float)h.D[(size_t)k * n + q];
       for (int q = 0; q < 6; ++q) rk[o++] = (float)h.S[(size_t)k * 6 + q];
+      if (o + 6 <= 127) prior_factor(rk + o - 6, rk + o);
       rk[127] = h.inv[k] ? 1.0f : 0.0f;
     }
   }

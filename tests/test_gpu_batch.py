@@ -212,8 +212,12 @@ def test_autotuned_shapes_keep_the_transform(ctx, dims):
 @pytest.mark.parametrize("chunk_rows,batched,src", [(0, False, False), (7, False, False), (1, True, False),
                                                     (0, True, True), (7, True, True), (5, False, True)])
 def test_invert_matches_sequence(ctx, chunk_rows, batched, src):
-    """crv_invert (transforms, posterior, realizations and inverse transforms
-    pipelined by chunks of y rows) equals the one-call sequence bit for bit."""
+    """crv_invert (transforms, posterior, realizations and inverse transforms,
+    whole grids or pipelined by chunks of y rows) equals the one-call
+    sequence bit for bit, but for the realizations of the whole-grid call:
+    outside the inverted band its fused simulation scales the noise by
+    sqrt(lam) times the plane's factor of the prior S, where the simulation
+    kernel factors the stored FP32 product lam S (rel RMS below 1e-6)."""
     from crava_amd.hip import FFTGrid, PostPlan, fft_c2r_batch, grid_batch, simulate_batch
     case = Case(model.config_small(64, 48, 40, 5))
     s = case.s
@@ -268,8 +272,13 @@ def test_invert_matches_sequence(ctx, chunk_rows, batched, src):
     got = ([g.download_real() for g in mean2], [g.download_complex() for g in data2],
            [g.download_real() for g in cov2], [g.download_real() for g in sims2])
     nx = s.nxp
-    for w, g in zip(want[0] + want[2] + want[3], got[0] + got[2] + got[3]):
+    for w, g in zip(want[0] + want[2], got[0] + got[2]):
         assert np.array_equal(w[:, :, :nx], g[:, :, :nx])
+    for w, g in zip(want[3], got[3]):
+        if chunk_rows > 0:
+            assert np.array_equal(w[:, :, :nx], g[:, :, :nx])
+        else:
+            assert 0 < rel_rms(g[:, :, :nx], w[:, :, :nx]) < 1e-6
     for w, g in zip(want[1], got[1]):
         assert np.array_equal(w, g)
     assert st.n_fp64 == st_seq.n_fp64 and st.n_chol_fail == st_seq.n_chol_fail
