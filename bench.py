@@ -612,7 +612,12 @@ def main():
         # reference step for accuracy (same realization for both precisions)
         step(ctx, G, kw, mode, 77, s.seed_sim)
         ctx.synchronize()
-        res[prec] = dict(sec=sec, prof=prof, grids=G)
+        st = G["plan"].stats()
+        res[prec] = dict(sec=sec, prof=prof, grids=G,
+                         stats={"wavenumbers": st.n_wavenumbers, "inverted": st.n_inverted, "fp64_solves": st.n_fp64,
+                                "rejected": st.n_chol_fail, "max_condition_estimate": st.max_cond})
+        log(f"[bench] FP{prec} posterior: {st.n_inverted} of {st.n_wavenumbers} wavenumbers inverted, "
+            f"{st.n_fp64} FP64 solves, {st.n_chol_fail} rejected, max condition estimate {st.max_cond:.3g}")
         for k_, v_ in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             b_ = launch_bytes(k_, s, prec, ninv, v_[0] / 3, tuple(prof))
             gbs = f"{b_ / (v_[1] / v_[0] / 1e3) / 1e9:7.0f} GB/s" if b_ else ""
@@ -689,6 +694,7 @@ def main():
         "fp64": ({"value": s.voxels / res[64]["sec"], "ms_per_step": res[64]["sec"] * 1e3,
                   "speedup_fp32_refine": res[64]["sec"] / sec} if 64 in res else None),
         "errors_vs_fp64": errors,
+        "posterior_stats": res[32]["stats"],
         "transfers": transfers,
         "kernels": kernels,
         "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}

@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <type_traits>
 
@@ -84,6 +85,10 @@ struct PostArgs {
   // chunks of 64 wavenumbers per plane of the row range
   const float2* rec;
   int cpp;
+  // FP32 refine mode, separable prior, n <= kRecMaxN, D > 0 on the band's
+  // planes: the plane's eigen-basis record of kEigRec floats at erec +
+  // kEigRec k (see k_posterior_eig); nullptr: none
+  const float* erec;
   // simulation fused into the FP32 passes (crv_invert): nsim realizations of
   // the counter noise stream, realization r of parameter g written at
   // sim + (3 r + g) sstride complex elements
@@ -477,6 +482,165 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
   block_stats(a.stats, nfail, cmax);
 }
 
+// ---- eigen-basis posterior (FP32 refine mode, separable prior) -------------
+// On a plane k of the band M(lam) = lam E + D with D diagonal and positive.
+// With F = D^-1/2 E D^-1/2 = U diag(l) U^H (made on the host in FP64),
+// T = U^H D^-1/2 and V = T B (B = |ct| K Sigma0): M^-1 = T^H diag(w) T with
+// w_p = 1 / (lam l_p + 1), so with r = d - K mu and z = T r
+//   mu_post = mu + lam V^H diag(w) z,
+//   S_post  = lam S - lam^2 V^H diag(w) V = lam S - lam^2 sum_p w_p G_p,
+//   res     = r - K (mu_post - mu),
+// where G_p = conj(V_pa) V_pb per covariance q = (a, b).  The solve is
+// diagonal in that basis.  The condition estimate is the bound
+// cond(D) (lam l_max + 1) / (lam l_min + 1) >= cond(M).
+constexpr int kEigRec = 256;  // floats per plane record
+template <int N>
+struct EigLayout {
+  static constexpr int OT = 0;              // T, N x N complex [p][l]
+  static constexpr int OV = OT + 2 * N * N; // V, N x 3 complex [p][m]
+  static constexpr int OK = OV + 6 * N;     // K, N x 3 complex [l][m]
+  static constexpr int OG = OK + 6 * N;     // G_p: 3 real, then 3 complex (ab, ar, br)
+  static constexpr int OW = OG + 9 * N;     // l_p
+  static constexpr int OS = OW + N;         // |ct| Sigma0 (aa bb rr ab ar br)
+  static constexpr int OF = OS + 6;         // factor of S (fused_sim_prior)
+  static constexpr int OC = OF + 6;         // cond(D), l_max, l_min
+  static constexpr int OBAND = kEigRec - 1; // 1: inverted plane
+  static_assert(OC + 3 <= OBAND, "record overflow");
+};
+
+// the record of plane k as scalar (wave-uniform) loads
+typedef const __attribute__((address_space(4))) float* EigRecPtr;
+
+template <int N, int SIM>
+__global__ __launch_bounds__(kThreads) void k_posterior_eig(PostArgs<float> a) {
+  using C = float2;
+  using Lo = EigLayout<N>;
+  double cmax = 0.0;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
+  const int nwv = (int)gridDim.x * (kThreads / 64);
+  const int plane = a.dplr.d, cpp = a.cpp, nch = cpp * a.nzp;
+  for (int ch = wv; ch < nch; ch += nwv) {
+    const int k = ch / cpp;
+    const int p0 = (ch - k * cpp) * 64;
+    const EigRecPtr er = (EigRecPtr)(a.erec + (size_t)k * kEigRec);
+    // lanes past the plane's end repeat its last wavenumber and store nothing
+    const bool live = p0 + lane < plane;
+    const int pp = live ? p0 + lane : plane - 1;
+    const int jl = fdiv(pp, a.dcx);
+    const int i = pp - jl * a.cnxp, j = a.j0 + jl;
+    const long long idx = (long long)i + (long long)a.cnxp * ((long long)j + (long long)a.nyp * k);
+    const float lam = a.t.lam[(size_t)j * a.cnxp + i];
+    float s[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s[q] = lam * er[Lo::OS + q];
+    if (er[Lo::OBAND] == 0.0f) {
+      if (live) {
+        if (!a.skip_prior_cov) store_prior(a, idx, s);
+        if constexpr (SIM == 2) {
+          float Lp[6];
+#pragma unroll
+          for (int q = 0; q < 6; ++q) Lp[q] = er[Lo::OF + q];
+          fused_sim_prior(a, idx, lam, Lp);
+        }
+      }
+      continue;
+    }
+    const float ce = er[Lo::OC] * (lam * er[Lo::OC + 1] + 1.0f) / (lam * er[Lo::OC + 2] + 1.0f);
+    const int ky = min(j, a.nyp - j), kz = min(k, a.nzp - k);
+    const float fx = (float)i * a.rx, fy = (float)ky * a.ry, fz = (float)kz * a.rz;
+    const bool lowk = (fx * fx + fy * fy + fz * fz) < a.lowk2;
+    if (lowk || !(ce <= a.cond_thr)) {
+      if (live) {
+        const unsigned int slot = atomicAdd(a.list_count, 1u);
+        a.list[slot] = (unsigned int)idx;
+      }
+      continue;
+    }
+    if (!live) continue;
+    cmax = fmax(cmax, (double)ce);
+    C mu[3], r[N];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      C x = a.data[l][idx];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const float kr = er[Lo::OK + 2 * (l * 3 + m)], ki = er[Lo::OK + 2 * (l * 3 + m) + 1];
+        x.x = fmaf(-kr, mu[m].x, fmaf(ki, mu[m].y, x.x));
+        x.y = fmaf(-kr, mu[m].y, fmaf(-ki, mu[m].x, x.y));
+      }
+      r[l] = x;
+    }
+    // y = lam diag(w) T r
+    C y[N];
+    float w[N];
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+      C z = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        const float tr = er[Lo::OT + 2 * (p * N + l)], ti = er[Lo::OT + 2 * (p * N + l) + 1];
+        z.x = fmaf(tr, r[l].x, fmaf(-ti, r[l].y, z.x));
+        z.y = fmaf(tr, r[l].y, fmaf(ti, r[l].x, z.y));
+      }
+      w[p] = __builtin_amdgcn_rcpf(fmaf(lam, er[Lo::OW + p], 1.0f));
+      const float g = lam * w[p];
+      y[p] = make_float2(g * z.x, g * z.y);
+    }
+    // mean update V^H y, residual r - K dm
+    C dm[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      C x = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int p = 0; p < N; ++p) {
+        const float vr = er[Lo::OV + 2 * (p * 3 + m)], vi = er[Lo::OV + 2 * (p * 3 + m) + 1];
+        x.x = fmaf(vr, y[p].x, fmaf(vi, y[p].y, x.x));
+        x.y = fmaf(vr, y[p].y, fmaf(-vi, y[p].x, x.y));
+      }
+      dm[m] = x;
+      a.mean[m][idx] = make_float2(mu[m].x + x.x, mu[m].y + x.y);
+    }
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      C x = r[l];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const float kr = er[Lo::OK + 2 * (l * 3 + m)], ki = er[Lo::OK + 2 * (l * 3 + m) + 1];
+        x.x = fmaf(-kr, dm[m].x, fmaf(ki, dm[m].y, x.x));
+        x.y = fmaf(-kr, dm[m].y, fmaf(-ki, dm[m].x, x.y));
+      }
+      a.data[l][idx] = x;
+    }
+    // covariances lam S - lam^2 sum_p w_p G_p
+    const float l2 = lam * lam;
+    C cq[6];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float x = 0.0f;
+#pragma unroll
+      for (int p = 0; p < N; ++p) x = fmaf(w[p], er[Lo::OG + 9 * p + q], x);
+      cq[q] = make_float2(fmaf(-l2, x, s[q]), 0.0f);
+    }
+#pragma unroll
+    for (int q = 3; q < 6; ++q) {
+      float xr = 0.0f, xi = 0.0f;
+#pragma unroll
+      for (int p = 0; p < N; ++p) {
+        xr = fmaf(w[p], er[Lo::OG + 9 * p + 3 + 2 * (q - 3)], xr);
+        xi = fmaf(w[p], er[Lo::OG + 9 * p + 4 + 2 * (q - 3)], xi);
+      }
+      cq[q] = make_float2(fmaf(-l2, xr, s[q]), -l2 * xi);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.cov[q][idx] = cq[q];
+    fused_sim<SIM>(a, idx, cq);
+  }
+  block_stats(a.stats, 0ull, cmax);
+}
+
 // FP64 solve with one step of iterative refinement for the queued
 // wavenumbers (FP32 storage): per right-hand side b (columns of K S, then
 // d - K mu), x = M^-1 b, r = b - M x, x += M^-1 r, with M built in FP64 from
@@ -612,6 +776,16 @@ int persistent_blocks(crv_ctx* ctx, F kernel, long long work) {
 
 template <typename T, int N, bool SEP, bool LIST>
 int launch_main(crv_ctx* ctx, const PostArgs<T>& a) {
+  if constexpr (std::is_same<T, float>::value && SEP && LIST && N <= kRecMaxN) {
+    if (a.erec) {
+      auto kern = a.nsim ? k_posterior_eig<N, 2> : k_posterior_eig<N, 0>;
+      const int nb = persistent_blocks(ctx, kern, a.ncr);
+      ProfScope ps(ctx, a.nsim ? "posterior_sim_fp32" : "posterior_fp32");
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+      CRV_HIP_CHECK(ctx, hipGetLastError());
+      return CRV_OK;
+    }
+  }
   if constexpr (std::is_same<T, float>::value && SEP && N <= kRecMaxN) {
     if (a.rec) {
       auto go = [&](auto kern) {
@@ -829,6 +1003,129 @@ HostTables build_tables(const crv_post_params* p, int nyp, int nzp, int cnxp, lo
 
 }  // namespace
 
+// Eigen-decomposition A = U diag(l) U^H of a Hermitian n x n matrix (row
+// major, overwritten) by cyclic Jacobi rotations; false if it does not
+// converge (non-finite input).
+bool herm_eig(int n, std::vector<std::complex<double>>& A, std::vector<double>& l,
+              std::vector<std::complex<double>>& U) {
+  using Cx = std::complex<double>;
+  U.assign((size_t)n * n, Cx(0.0, 0.0));
+  for (int p = 0; p < n; ++p) U[(size_t)p * n + p] = 1.0;
+  bool done = false;
+  for (int sweep = 0; sweep < 60 && !done; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) (p == q ? dia : off) += std::norm(A[(size_t)p * n + q]);
+    if (!std::isfinite(off) || !std::isfinite(dia)) return false;
+    if (off <= 1e-30 * dia) { done = true; break; }
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const Cx b = A[(size_t)p * n + q];
+        const double ab = std::abs(b);
+        if (ab == 0.0) continue;
+        // J = diag(1, conj(ph)) [[c, s], [-s, c]] on (p, q) makes A_pq zero
+        const Cx ph = b / ab;
+        const double tau = (A[(size_t)q * n + q].real() - A[(size_t)p * n + p].real()) / (2.0 * ab);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+        auto cols = [&](std::vector<Cx>& X) {
+          for (int r = 0; r < n; ++r) {
+            const Cx x = X[(size_t)r * n + p], y = X[(size_t)r * n + q];
+            X[(size_t)r * n + p] = c * x - sn * std::conj(ph) * y;
+            X[(size_t)r * n + q] = sn * x + c * std::conj(ph) * y;
+          }
+        };
+        cols(A);
+        for (int r = 0; r < n; ++r) {
+          const Cx x = A[(size_t)p * n + r], y = A[(size_t)q * n + r];
+          A[(size_t)p * n + r] = c * x - sn * ph * y;
+          A[(size_t)q * n + r] = sn * x + c * ph * y;
+        }
+        cols(U);
+      }
+  }
+  if (!done) return false;
+  l.resize(n);
+  for (int p = 0; p < n; ++p) l[p] = A[(size_t)p * n + p].real();
+  return true;
+}
+
+// Eigen-basis records (k_posterior_eig's EigLayout) of every plane: S, its
+// factor and K on all planes, T, V, G, l and the condition data on the
+// band's.  false (no records) where a band plane's noise diagonal D is not
+// positive or its decomposition fails.
+bool eig_records(const HostTables& h, int n, int nzp, std::vector<float>& out) {
+  using Cx = std::complex<double>;
+  const int P = n * (n + 1) / 2;
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+  const int OT = 0, OV = OT + 2 * n * n, OK = OV + 6 * n, OG = OK + 6 * n, OW = OG + 9 * n, OS = OW + n,
+            OF = OS + 6, OC = OF + 6;
+  if (OC + 3 > kEigRec - 1) return false;
+  out.assign((size_t)nzp * kEigRec, 0.0f);
+  std::vector<double> dh(n), lv;
+  std::vector<Cx> F((size_t)n * n), U, T((size_t)n * n), V((size_t)n * 3);
+  for (int k = 0; k < nzp; ++k) {
+    float* rk = out.data() + (size_t)k * kEigRec;
+    for (int q = 0; q < 6; ++q) rk[OS + q] = (float)h.S[(size_t)k * 6 + q];
+    prior_factor(rk + OS, rk + OF);
+    for (int q = 0; q < 6 * n; ++q) rk[OK + q] = (float)h.K[(size_t)k * 6 * n + q];
+    if (!h.inv[k]) continue;
+    rk[kEigRec - 1] = 1.0f;
+    double dmin = 0.0, dmax = 0.0;
+    for (int l = 0; l < n; ++l) {
+      const double d = h.D[(size_t)k * n + l];
+      if (!(d > 0.0) || !std::isfinite(d)) return false;
+      dh[l] = 1.0 / std::sqrt(d);
+      dmin = l ? std::min(dmin, d) : d;
+      dmax = l ? std::max(dmax, d) : d;
+    }
+    for (int l = 0; l < n; ++l)
+      for (int m = 0; m <= l; ++m) {
+        const size_t o = ((size_t)k * P + l * (l + 1) / 2 + m) * 2;
+        const Cx f = dh[l] * Cx(h.E[o], l == m ? 0.0 : h.E[o + 1]) * dh[m];
+        F[(size_t)l * n + m] = f;
+        F[(size_t)m * n + l] = std::conj(f);
+      }
+    if (!herm_eig(n, F, lv, U)) return false;
+    for (int p = 0; p < n; ++p)
+      for (int l = 0; l < n; ++l) T[(size_t)p * n + l] = std::conj(U[(size_t)l * n + p]) * dh[l];
+    for (int p = 0; p < n; ++p)
+      for (int m = 0; m < 3; ++m) {
+        Cx v = 0.0;
+        for (int l = 0; l < n; ++l) {
+          const size_t o = (((size_t)k * n + l) * 3 + m) * 2;
+          v += T[(size_t)p * n + l] * Cx(h.B[o], h.B[o + 1]);
+        }
+        V[(size_t)p * 3 + m] = v;
+      }
+    double lmin = 0.0, lmax = 0.0;
+    for (int p = 0; p < n; ++p) {
+      const double lp = std::max(lv[p], 0.0);
+      lmin = p ? std::min(lmin, lp) : lp;
+      lmax = p ? std::max(lmax, lp) : lp;
+      rk[OW + p] = (float)lp;
+      for (int l = 0; l < n; ++l) {
+        rk[OT + 2 * (p * n + l)] = (float)T[(size_t)p * n + l].real();
+        rk[OT + 2 * (p * n + l) + 1] = (float)T[(size_t)p * n + l].imag();
+      }
+      for (int m = 0; m < 3; ++m) {
+        rk[OV + 2 * (p * 3 + m)] = (float)V[(size_t)p * 3 + m].real();
+        rk[OV + 2 * (p * 3 + m) + 1] = (float)V[(size_t)p * 3 + m].imag();
+      }
+      for (int q = 0; q < 3; ++q) rk[OG + 9 * p + q] = (float)std::norm(V[(size_t)p * 3 + q]);
+      for (int q = 3; q < 6; ++q) {
+        const Cx g = std::conj(V[(size_t)p * 3 + pa[q]]) * V[(size_t)p * 3 + pb[q]];
+        rk[OG + 9 * p + 3 + 2 * (q - 3)] = (float)g.real();
+        rk[OG + 9 * p + 4 + 2 * (q - 3)] = (float)g.imag();
+      }
+    }
+    rk[OC] = (float)(dmax / dmin);
+    rk[OC + 1] = (float)lmax;
+    rk[OC + 2] = (float)lmin;
+  }
+  return true;
+}
+
 struct crv_post_plan {
   crv_ctx* ctx = nullptr;
   int n = 0, nxp = 0, nyp = 0, nzp = 0, cnxp = 0, precision = 0;
@@ -837,6 +1134,7 @@ struct crv_post_plan {
   unsigned char* dtab = nullptr;  // [tables T][tables FP64 (refine)][inv][stats 256 B]
   TabOffsets oT, o64;
   size_t oI = 0, oS = 0, oR = 0;   // oR: FP32 separable records (0: none)
+  size_t oE = 0;                   // eigen-basis records (0: none)
   unsigned int* dlist = nullptr;
   double sig0[6] = {0, 0, 0, 0, 0, 0};
   float cond_thr = 1e5f;
@@ -886,6 +1184,14 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
       for (int q = 0; q < 6; ++q) rk[o++] = (float)h.S[(size_t)k * 6 + q];
       if (o + 6 <= 127) prior_factor(rk + o - 6, rk + o);
       rk[127] = h.inv[k] ? 1.0f : 0.0f;
+    }
+  }
+  if (precision == CRV_FP32 && pl->separable && refine && p->ntheta <= kRecMaxN) {
+    std::vector<float> er;
+    if (eig_records(h, p->ntheta, nzp, er)) {
+      pl->oE = (blob.size() + 255) & ~(size_t)255;
+      blob.resize(pl->oE + er.size() * sizeof(float), 0);
+      memcpy(blob.data() + pl->oE, er.data(), er.size() * sizeof(float));
     }
   }
   pl->oI = blob.size();
@@ -998,6 +1304,7 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
     a.dplane = make_fastdiv(pl->cnxp * pl->nyp); a.dcx = make_fastdiv(pl->cnxp);
     a.j0 = j0; a.nyr = nyr; a.ncr = (long long)pl->cnxp * nyr * pl->nzp; a.dplr = make_fastdiv(pl->cnxp * nyr);
     a.stats = dstats;
+    if (pl->oE) a.erec = (const float*)(pl->dtab + pl->oE);
     if (pl->oR) {
       a.rec = (const float2*)(pl->dtab + pl->oR);
       a.cpp = (pl->cnxp * nyr + 63) / 64;
