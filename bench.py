@@ -249,10 +249,14 @@ KERNEL_SYMBOLS = {
     # kernels of the FP32-refine step at 450x450x320 (x 225 = 15 x 15 as a
     # complex half-length transform, y 450 = 25 x 18, z 320 = 20 x 16)
     "fft_x_r2c": "k_rows_r2c<float, 15, 15, false>", "fft_x_c2r": "k_rows_c2r<float, 15, 15, false>",
-    "fft_y_fwd": "k_cols<float, false, 25, 18, false>", "fft_z_fwd": "k_cols<float, false, 20, 16, false>",
-    "fft_y_inv": "k_cols<float, true, 25, 18, false>", "fft_z_inv": "k_cols<float, true, 20, 16, false>",
-    "posterior_fp32": "k_posterior_rec<5, true, 0>", "posterior_sim_fp32": "k_posterior_rec<5, true, 2>",
-    "simulate_scale_batch": "k_simulate<float, true>", "fft_z_inv_prior": "k_cols<float, true, 20, 16, false, true>",
+    # (the y and z tiles are 16 columns wide: 512-thread kernels)
+    "fft_y_fwd": "k_cols<float, false, 25, 18, false, false, true>",
+    "fft_z_fwd": "k_cols<float, false, 20, 16, false, false, true>",
+    "fft_y_inv": "k_cols<float, true, 25, 18, false, false, true>",
+    "fft_z_inv": "k_cols<float, true, 20, 16, false, false, true>",
+    "fft_z_inv_prior": "k_cols<float, true, 20, 16, false, true, true>",
+    "posterior_fp32": "k_posterior_eig<5, 0>", "posterior_sim_fp32": "k_posterior_eig<5, 2>",
+    "simulate_scale_batch": "k_simulate<float, true>",
 }
 
 

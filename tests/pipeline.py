@@ -20,7 +20,7 @@ class Case:
         self.err_real = model.err_corr_grid(s, np.float64)
 
 
-def gpu_posterior(ctx, case: Case, prec=32, mode=0, separable=False, skip_mean_inverse=False):
+def gpu_posterior(ctx, case: Case, prec=32, mode=0, separable=False, skip_mean_inverse=False, **kw):
     from crava_amd.hip import FFTGrid, computePostMeanResidAndFFTCov
     s = case.s
     mk = lambda ct: FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=prec, cubetype=ct)
@@ -40,7 +40,7 @@ def gpu_posterior(ctx, case: Case, prec=32, mode=0, separable=False, skip_mean_i
     st = computePostMeanResidAndFFTCov(ctx, mean, data, err, cov, A=p.A, theta=p.theta, tables=p.tables,
                                        nz=s.nz, lz=s.lz, low_cut=s.low_cut, high_cut=s.high_cut, wnc=s.wnc,
                                        sigma0=p.sigma0, mode=mode, separable=separable, cxy=p.cxy, ct=p.ct,
-                                       skip_mean_inverse=skip_mean_inverse)
+                                       skip_mean_inverse=skip_mean_inverse, **kw)
     return dict(mean=mean, data=data, cov=cov, stats=st)
 
 

@@ -182,3 +182,23 @@ def test_angle_counts_against_fp64_oracle(ctx, n):
     for g, c in zip(r32["cov"], o["cov"]):
         assert rel_rms(g.download_complex(), c) < 1e-4
     assert r32["stats"].n_chol_fail == 0 and r32["stats"].n_inverted > 0
+
+
+@pytest.mark.parametrize("separable", [True, False])
+def test_fp64_list_takes_every_wavenumber(ctx, separable):
+    """FP32 refine mode with a condition threshold below 1 (every estimate is
+    at least 1) queues every inverted wavenumber for the FP64 solve with one
+    refinement step: the count equals the inverted count, and the results,
+    FP64 solves of the FP32 spectra, stay within FP32 rounding of the FP64
+    oracle."""
+    case = Case(model.config_small(16, 12, 20, 5))
+    o = oracle_posterior(case, np.float64)
+    r = gpu_posterior(ctx, case, prec=32, mode=1, separable=separable, cond_threshold=0.5)
+    st = r["stats"]
+    assert st.n_fp64 == st.n_inverted > 0 and st.n_chol_fail == 0
+    for g, m in zip(r["mean"], o["mean"]):
+        assert rel_rms(g.download_real()[:, :, :16], m[:, :, :16]) < 1e-6
+    for g, c in zip(r["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-5
+    for g, d in zip(r["data"], o["data"]):
+        assert rel_rms(g.download_complex(), d) < 1e-4
