@@ -34,7 +34,11 @@ struct SimArgs {
   const cplx_t<T>* host[3];  // host-supplied noise (uploaded)
   unsigned long long seed, real;
   long long nc;
-  long long rstride;         // batches: complex elements between realizations (blockIdx.y)
+  // realizations real, real + 1, ..., real + nreal - 1 (counter stream;
+  // 1 otherwise), written rstride complex elements apart: each wavenumber's
+  // factor is made once for all of them
+  int nreal;
+  long long rstride;
   // wavenumbers covered: y rows [j0, j0 + nyr) of every kz plane, ncr of
   // them (the whole grid: j0 = 0, nyr = nyp, ncr = nc); dplr divides by
   // cnxp nyr, dcx by cnxp (indices below 2^31)
@@ -60,8 +64,6 @@ __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
   using U = typename std::conditional<F64, double, T>::type;
   using CU = cplx_t<U>;
   const long long stride = (long long)gridDim.x * kThreads;
-  const long long roff = (long long)blockIdx.y * a.rstride;
-  const unsigned long long real = a.real + blockIdx.y;
   for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < a.ncr; t += stride) {
     const int k = fdiv((int)t, a.dplr);
     const int rem = (int)t - k * a.dplr.d;
@@ -70,28 +72,30 @@ __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
     const C cv[6] = {a.cov[0][i], a.cov[1][i], a.cov[2][i], a.cov[3][i], a.cov[4][i], a.cov[5][i]};
     CU Lf[6];
     const bool ok = sim_factor<T, F64>(cv, Lf);
-    C o[3] = {cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0)};
-    if (ok) {
-      T re[3], im[3];
-      if (a.mode == CRV_NOISE_COUNTER) {
-        sim_noise3<T>(a.geo, a.seed, real, i, re, im);
-      } else {
+    for (int r = 0; r < a.nreal; ++r) {
+      C o[3] = {cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0)};
+      if (ok) {
+        T re[3], im[3];
+        if (a.mode == CRV_NOISE_COUNTER) {
+          sim_noise3<T>(a.geo, a.seed, a.real + (unsigned long long)r, i, re, im);
+        } else {
 #pragma unroll
-        for (int g = 0; g < 3; ++g) {
-          if (a.mode == CRV_NOISE_REFERENCE) {
-            double r_, i_;
-            noise_ref_value(a.geo, a.normals + g * a.draws, i, r_, i_);
-            re[g] = (T)r_; im[g] = (T)i_;
-          } else {
-            const C h = a.host[g][i];
-            re[g] = h.x; im[g] = h.y;
+          for (int g = 0; g < 3; ++g) {
+            if (a.mode == CRV_NOISE_REFERENCE) {
+              double r_, i_;
+              noise_ref_value(a.geo, a.normals + g * a.draws, i, r_, i_);
+              re[g] = (T)r_; im[g] = (T)i_;
+            } else {
+              const C h = a.host[g][i];
+              re[g] = h.x; im[g] = h.y;
+            }
           }
         }
+        sim_apply<T, U>(Lf, re, im, o);
       }
-      sim_apply<T, U>(Lf, re, im, o);
-    }
 #pragma unroll
-    for (int g = 0; g < 3; ++g) a.out[g][roff + i] = o[g];
+      for (int g = 0; g < 3; ++g) a.out[g][(long long)r * a.rstride + i] = o[g];
+    }
   }
 }
 
@@ -203,7 +207,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     for (int g = 0; g < 3; ++g) a.out[g] = (double2*)out[g]->d;
     a.geo = make_noise_geom(plan); a.mode = noise->mode; a.normals = ctx->normals; a.draws = draws;
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const double2*)((char*)hostbuf + g * gb) : nullptr;
-    a.seed = noise->seed; a.real = noise->realization; a.nc = nc;
+    a.seed = noise->seed; a.real = noise->realization; a.nc = nc; a.nreal = 1;
     sim_rows(a, plan->nxp / 2 + 1, plan->nyp, plan->nzp, 0, plan->nyp);
     hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   } else {
@@ -212,7 +216,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     for (int g = 0; g < 3; ++g) a.out[g] = (float2*)out[g]->d;
     a.geo = make_noise_geom(plan); a.mode = noise->mode; a.normals = ctx->normals; a.draws = draws;
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const float2*)((char*)hostbuf + g * gb) : nullptr;
-    a.seed = noise->seed; a.real = noise->realization; a.nc = nc;
+    a.seed = noise->seed; a.real = noise->realization; a.nc = nc; a.nreal = 1;
     sim_rows(a, plan->nxp / 2 + 1, plan->nyp, plan->nzp, 0, plan->nyp);
     if (fp64_chol)
       hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
@@ -265,20 +269,20 @@ int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], v
     for (int q = 0; q < 6; ++q) a.cov[q] = (const double2*)cov[q];
     for (int g = 0; g < 3; ++g) a.out[g] = (double2*)out0 + g * gstride;
     a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
-    a.seed = seed; a.real = real0; a.nc = nc; a.rstride = 3 * gstride;
+    a.seed = seed; a.real = real0; a.nc = nc; a.nreal = nreal; a.rstride = 3 * gstride;
     sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
-    hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   } else {
     SimArgs<float> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q];
     for (int g = 0; g < 3; ++g) a.out[g] = (float2*)out0 + g * gstride;
     a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
-    a.seed = seed; a.real = real0; a.nc = nc; a.rstride = 3 * gstride;
+    a.seed = seed; a.real = real0; a.nc = nc; a.nreal = nreal; a.rstride = 3 * gstride;
     sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
     if (fp64_chol)
-      hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
     else
-      hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb, nreal), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   }
   CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;
