@@ -392,7 +392,10 @@ def bench_c3(a):
     """Batched and graph-captured runs: 64 realizations of the 256x256x200
     5-angle inversion (one launch per stage for all realizations), and 12
     uneven volumes in one captured graph; the FFT launch shapes of every
-    grid size and the posterior's blocks per CU come from the autotuner."""
+    grid size and the posterior's blocks per CU come from the autotuner.
+    --path invert (default) runs each volume as one crv_invert call (the
+    realizations fused into the posterior), --path sequence as the one-call
+    entry points (posterior, then the batched simulation)."""
     import torch
     from crava_amd import _lib as L
     from crava_amd import hip, model
@@ -415,6 +418,11 @@ def bench_c3(a):
             g.copyFrom(src)
 
     def step(G, plan, s, nreal):
+        if a.path == "invert":
+            # one call: transforms, posterior with the realizations fused, inverse transforms
+            plan.invert(G["mean"], G["data"], G["cov"], G["sims"], mean_src=G["mean_in"], data_src=G["data_in"],
+                        seed=s.seed_sim, realization0=0)
+            return
         for w, src in zip(G["data"], G["data_in"]):
             w.fftFrom(src)
         for w, src in zip(G["mean"], G["mean_in"]):
@@ -467,6 +475,12 @@ def bench_c3(a):
     shape = hip.autotune(ctx, s.nxp, s.nyp, s.nzp, 32, reps=3)
     t_tune = time.perf_counter() - t0
     eager = timeit(lambda: step(G, plan, s, nreal), a.steps)
+    ctx.profile(True)
+    step(G, plan, s, nreal)
+    ctx.profile(False)
+    prof1 = ctx.profile_report()
+    for k_, v_ in sorted(prof1.items(), key=lambda kv: -kv[1][1])[:8]:
+        log(f"[bench c3] part 1: {k_:22s} x{v_[0]:4d}  {v_[1]:.3f} ms")
     graph = hip.Graph.begin(ctx)
     step(G, plan, s, nreal)
     graph.end()
@@ -477,7 +491,8 @@ def bench_c3(a):
                     "ms_per_batch_graph": graphed * 1e3, "ms_per_batch_eager": eager * 1e3,
                     "realizations_per_s": nreal / graphed,
                     "voxels_per_s": nreal * s.voxels / graphed, "autotune": shape, "autotune_s": t_tune,
-                    "posterior_blocks_per_cu": best[1], "batch_mb": best_b[1]}
+                    "posterior_blocks_per_cu": best[1], "batch_mb": best_b[1], "path": a.path,
+                    "kernels_ms": {k_: v_[1] for k_, v_ in prof1.items()}}
     log(f"[bench c3] part 1: graph {graphed * 1e3:.2f} ms, eager {eager * 1e3:.2f} ms per batch; "
         f"{nreal / graphed:.1f} realizations/s")
     for g in sum((v for v in G.values()), []):

@@ -151,12 +151,37 @@ __device__ __forceinline__ void sim_words(unsigned long long seed, unsigned long
   philox10(w, (unsigned)seed, (unsigned)(seed >> 32));
 }
 
+// sin(2 pi u), cos(2 pi u) for u = (2m + 1) 2^-24 in (0, 1).  With
+// q = rint(4u), v = u - q/4 is exact and within [-1/8, 1/8], so
+// t = 2 pi v is within pi/4, where the Taylor series to t^9 (sin) and t^10
+// (cos) stay within 2e-9 of the functions; q mod 4 rotates the pair by
+// quarter turns.
+__device__ __forceinline__ void sincos_2pi_unit(float u, float& sn, float& cs) {
+  const float q = rintf(4.0f * u);
+  const float t = 6.28318530717958647692f * fmaf(-0.25f, q, u);
+  const float t2 = t * t;
+  float ps = fmaf(t2, 2.7557319e-6f, -1.9841270e-4f);  // 1/9!, -1/7!
+  ps = fmaf(t2, ps, 8.3333333e-3f);                    // 1/5!
+  ps = fmaf(t2, ps, -1.6666667e-1f);                   // -1/3!
+  const float s = fmaf(t * t2, ps, t);
+  float pc = fmaf(t2, -2.7557319e-7f, 2.4801587e-5f);  // -1/10!, 1/8!
+  pc = fmaf(t2, pc, -1.3888889e-3f);                   // -1/6!
+  pc = fmaf(t2, pc, 4.1666667e-2f);                    // 1/4!
+  pc = fmaf(t2, pc, -0.5f);
+  const float c = fmaf(t2, pc, 1.0f);
+  const int k = (int)q & 3;
+  sn = (k == 0) ? s : (k == 1) ? c : (k == 2) ? -s : -c;
+  cs = (k == 0) ? c : (k == 1) ? -s : (k == 2) ? -c : s;
+}
+
 __device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, float& n0, float& n1) {
   const float u1 = (float)(((a >> 9) << 1) | 1u) * 0x1p-24f;
   const float u2 = (float)(((b >> 9) << 1) | 1u) * 0x1p-24f;
-  const float r = sqrtf(-2.0f * logf(u1));
+  // -2 ln u1 >= 1.1e-7: a normal number, where the hardware square root is
+  // within 1 ulp
+  const float r = __builtin_amdgcn_sqrtf(-2.0f * logf(u1));
   float sn, cs;
-  sincospif(2.0f * u2, &sn, &cs);
+  sincos_2pi_unit(u2, sn, cs);
   n0 = r * cs;
   n1 = r * sn;
 }
