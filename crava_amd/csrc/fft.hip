@@ -599,9 +599,16 @@ struct ColArgs {
 };
 
 // W: 512-thread blocks (tiles of up to 8192 complex FP32 values), so that
-// the pencil tiles of the two-stage lengths can be 16 columns (128 B) wide
+// the pencil tiles of the two-stage lengths can be 16 columns (128 B) wide.
+// The generating z inverse (SYN) of lengths up to 320 holds 80 VGPRs or
+// fewer (6 waves per SIMD), so that three of its 45 KB blocks share a CU:
+// it reads only the band's planes, and waits on HBM latency rather than
+// bandwidth, so the third block's loads in flight shorten it.
+constexpr int cols_waves_per_eu(bool wide, bool syn, int L) { return wide && syn && L <= 320 ? 6 : 1; }
+
 template <typename T, bool INV, int RF, int RL, bool H, bool SYN = false, bool W = false>
-__global__ __launch_bounds__(W ? 2 * kThreads : kThreads) void k_cols(ColArgs<T> a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThreads),
+                          amdgpu_waves_per_eu(cols_waves_per_eu(W, SYN, RF * RL)))) void k_cols(ColArgs<T> a) {
   using C = cplx_t<T>;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);

@@ -202,3 +202,18 @@ def test_fp64_list_takes_every_wavenumber(ctx, separable):
         assert rel_rms(g.download_complex(), c) < 1e-5
     for g, d in zip(r["data"], o["data"]):
         assert rel_rms(g.download_complex(), d) < 1e-4
+
+
+def test_separable_refine_without_white_noise(ctx):
+    """wnc = 0 leaves the noise diagonal D at zero, so the plan has no
+    eigen-basis records (M = lam E has no D to scale by) and runs the
+    Cholesky record kernel; the results still match the FP64 oracle."""
+    import dataclasses
+    case = Case(dataclasses.replace(model.config_small(16, 12, 20, 3), wnc=0.0))
+    o = oracle_posterior(case, np.float64)
+    r = gpu_posterior(ctx, case, prec=32, mode=1, separable=True)
+    for g, m in zip(r["mean"], o["mean"]):
+        assert rel_rms(g.download_real()[:, :, :16], m[:, :, :16]) < 1e-5
+    for g, c in zip(r["cov"], o["cov"]):
+        assert rel_rms(g.download_complex(), c) < 1e-4
+    assert r["stats"].n_inverted > 0
