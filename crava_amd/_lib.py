@@ -34,6 +34,7 @@ EXPORTS = [
     "crv_fft_autotune", "crv_grid_create_batch", "crv_simulate_batch", "crv_graph_begin", "crv_graph_end",
     "crv_graph_launch", "crv_graph_destroy", "crv_facies_density", "crv_facies_prob",
     "crv_grid_derived", "crv_crava_header", "crv_grid_write_crava", "crv_grid_read_crava",
+    "crv_storm_header", "crv_grid_write_storm",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
@@ -81,6 +82,12 @@ class SimboxInfo(C.Structure):
                 ("nx", C.c_int), ("ny", C.c_int), ("il0", C.c_double), ("xl0", C.c_double),
                 ("il_step_x", C.c_double), ("il_step_y", C.c_double), ("xl_step_x", C.c_double),
                 ("xl_step_y", C.c_double), ("angle", C.c_double)]
+
+
+class StormInfo(C.Structure):
+    _fields_ = [("xmin", C.c_double), ("lx", C.c_double), ("ymin", C.c_double), ("ly", C.c_double),
+                ("lz", C.c_double), ("angle", C.c_double), ("top_name", C.c_char_p), ("bot_name", C.c_char_p),
+                ("flat", C.c_int)]
 
 
 CRV_OUT_EXP, CRV_OUT_AI, CRV_OUT_SI, CRV_OUT_VPVS, CRV_OUT_POISSON = 0, 1, 2, 3, 4
@@ -189,6 +196,8 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_crava_header": (i, [C.POINTER(SimboxInfo), i, i, i, C.c_char_p, C.c_size_t]),
         "crv_grid_write_crava": (i, [vp, C.c_char_p, C.POINTER(SimboxInfo)]),
         "crv_grid_read_crava": (i, [vp, C.c_char_p, C.POINTER(SimboxInfo)]),
+        "crv_storm_header": (i, [C.POINTER(StormInfo), i, i, i, i, i, C.c_char_p, C.c_size_t]),
+        "crv_grid_write_storm": (i, [vp, C.c_char_p, C.POINTER(StormInfo), i, i, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -75,6 +75,16 @@ void FFTGrid::writeCravaFile(const std::string& fileName, const crv_simbox_info&
   check(crv_grid_write_crava(grid_, (fileName + ".crava").c_str(), &simbox), "writeCravaFile");
 }
 
+// the reference appends IO::SuffixStormBinary() (".storm") or IO::SuffixGeneralData() (".dat")
+void FFTGrid::writeStormFile(const std::string& fileName, const crv_storm_info& storm, bool ascii, bool padding,
+                             bool flat, bool scientific_format) {
+  crv_storm_info si = storm;
+  si.flat = flat ? 1 : 0;
+  const std::string path = fileName + (ascii ? ".dat" : ".storm");
+  check(crv_grid_write_storm(grid_, path.c_str(), &si, ascii ? 1 : 0, padding ? 1 : 0, scientific_format ? 1 : 0),
+        "writeStormFile");
+}
+
 void FFTGrid::readCravaFile(const std::string& fileName, std::string& errText) {
   if (crv_grid_read_crava(grid_, fileName.c_str(), nullptr) != CRV_OK)
     errText += std::string("Error: ") + crv_last_error(ctx_) + "\n";

@@ -53,6 +53,10 @@ class FFTGrid {
   // fftgrid.cpp:2411-2498: CRAVA's binary grid format with the simbox geometry
   void writeCravaFile(const std::string& fileName, const crv_simbox_info& simbox);
   void readCravaFile(const std::string& fileName, std::string& errText);
+  // fftgrid.cpp:2117-2176: appends ".storm" (binary) or ".dat" (text) to fileName;
+  // the simbox's geometry and surface names come in storm (its flat field is set from flat)
+  void writeStormFile(const std::string& fileName, const crv_storm_info& storm, bool ascii, bool padding = false,
+                      bool flat = false, bool scientific_format = false);
 
   // host copies in the reference layout (rvalue_ with pads / cvalue_)
   void setRealValues(const std::vector<float>& r);

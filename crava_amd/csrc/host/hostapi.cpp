@@ -101,7 +101,9 @@ extern "C" int crvh_invert(int nxp, int nyp, int nzp, int ntheta, const float* c
 // expTransf, then FFTGrid::writeCravaFile; reference src/parameteroutput.cpp:
 // 84-93, src/fftgrid.cpp:1726-1743,2411-2445), and readCravaFile of the file
 // written (src/fftgrid.cpp:2448-2498) into a second grid: log_real (rsize
-// floats) in, the exponentiated grid read back in exp_out.
+// floats) in, the exponentiated grid read back in exp_out.  The exponentiated
+// grid is also written with writeStormFile (binary, padded, flat simbox of
+// 25 x 25 x 4 cells from the origin) to base_name + ".storm".
 extern "C" int crvh_write_read_crava(int nxp, int nyp, int nzp, const float* log_real, const char* base_name,
                                      float* exp_out, char* err, int errlen) {
   crv_ctx* ctx = nullptr;
@@ -120,6 +122,10 @@ extern "C" int crvh_write_read_crava(int nxp, int nyp, int nzp, const float* log
       g.setRealValues(std::vector<float>(log_real, log_real + rsize));
       g.expTransf();
       g.writeCravaFile(base_name, sb);
+      // the padded grid as a flat STORM binary file (fftgrid.cpp:2117-2150)
+      crv_storm_info si{};
+      si.lx = 25.0 * nxp; si.ly = 25.0 * nyp; si.lz = 4.0 * nzp;
+      g.writeStormFile(base_name, si, false, true, true);
     }
     FFTGrid h(ctx, nxp, nyp, nzp, nxp, nyp, nzp, CRV_FP32);
     std::string errText;

@@ -17,12 +17,17 @@
 // default, libs/nrlib/iotools/fileio.hpp:266-288).  The grid moves through
 // two pinned staging buffers: the copy of one chunk overlaps the byte
 // swapping and file I/O of the other.
+//
+// crv_grid_write_storm writes FFTGrid::writeStormFile's STORM files (binary
+// or text) through the same staging.
 #include "internal.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
+#include <vector>
 
 namespace {
 
@@ -234,5 +239,111 @@ extern "C" int crv_grid_read_crava(crv_grid* g, const char* path, crv_simbox_inf
   if (e != hipSuccess) return crv_set_error(ctx, CRV_E_HIP, std::string("read_crava: ") + hipGetErrorString(e));
   if (!io_ok) return crv_set_error(ctx, CRV_E_INVALID, std::string("read_crava: file too short: ") + path);
   g->transformed = false;
+  return CRV_OK;
+}
+
+// ---- STORM grid files --------------------------------------------------------
+// FFTGrid::writeStormFile (src/fftgrid.cpp:2117-2176) with the header of
+// Simbox::getStormHeader (src/simbox.cpp:408-458): reals as NRLib::ToString
+// (value, 6) prints them (fixed notation, 6 decimals), the values as
+// big-endian floats or as text (std::fixed with 6 decimals, or
+// std::scientific), one text line per x row, then the line "0".
+
+namespace {
+
+std::string fixed6(double v) {
+  char b[64];
+  snprintf(b, sizeof(b), "%.6f", v);
+  return b;
+}
+
+std::string storm_header(const crv_storm_info* si, int cubetype, int nx, int ny, int nz, bool ascii) {
+  const double pi = 3.14159265358979323846;
+  std::string h = ascii ? "storm_petro_ascii\n" : "storm_petro_binary\n";
+  h += "0 " + std::to_string(cubetype) + " " + fixed6(kRMissing) + "\n";
+  h += "FFTGrid\n";
+  const std::string xy = fixed6(si->xmin) + " " + fixed6(si->lx) + " " + fixed6(si->ymin) + " " + fixed6(si->ly);
+  if (si->flat)
+    h += xy + " 0.0 " + fixed6(si->lz) + " 0.0 0.0\n";
+  else
+    h += xy + " " + si->top_name + " " + si->bot_name + " 0.0 0.0\n";
+  h += fixed6(si->lz) + " " + fixed6(si->angle * 180 / pi) + "\n\n";
+  h += std::to_string(nx) + " " + std::to_string(ny) + " " + std::to_string(nz) + "\n";
+  return h;
+}
+
+}  // namespace
+
+extern "C" int crv_storm_header(const crv_storm_info* si, int cubetype, int nx, int ny, int nz, int ascii, char* buf,
+                                size_t len) {
+  if (!si || (!si->flat && (!si->top_name || !si->bot_name)) || nx < 1 || ny < 1 || nz < 1) return CRV_E_INVALID;
+  const std::string h = storm_header(si, cubetype, nx, ny, nz, ascii != 0);
+  if (!buf) return (int)h.size();
+  if (len < h.size()) return CRV_E_INVALID;
+  memcpy(buf, h.data(), h.size());
+  return (int)h.size();
+}
+
+extern "C" int crv_grid_write_storm(crv_grid* g, const char* path, const crv_storm_info* si, int ascii, int padding,
+                                    int scientific) {
+  if (!g || !path || !si) return CRV_E_INVALID;
+  crv_ctx* ctx = g->ctx;
+  if (!si->flat && (!si->top_name || !si->bot_name))
+    return crv_set_error(ctx, CRV_E_INVALID, "write_storm: top and base surface names needed (flat = 0)");
+  if (g->precision != CRV_FP32) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "write_storm: FP32 grids only");
+  if (g->transformed) return crv_set_error(ctx, CRV_E_STATE, "write_storm: grid is transformed");
+  const int nx = padding ? g->nxp : g->nx, ny = padding ? g->nyp : g->ny, nz = padding ? g->nzp : g->nz;
+  const size_t rnxp = 2 * (size_t)(g->nxp / 2 + 1);
+  const std::string head = storm_header(si, g->cubetype, nx, ny, nz, ascii != 0);
+  FILE* f = fopen(path, ascii ? "w" : "wb");
+  if (!f) return crv_set_error(ctx, CRV_E_INVALID, std::string("write_storm: cannot open ") + path);
+  Staging st;
+  hipError_t e = st.init();
+  bool io_ok = fwrite(head.data(), 1, head.size(), f) == head.size();
+  // rows r = j + nyp k of the planes k < nz, kChunk bytes of whole rows per
+  // chunk; rows j >= ny of a plane are staged and skipped
+  const size_t row_bytes = rnxp * sizeof(float);
+  const size_t nrows = (size_t)g->nyp * nz, rows_per = std::max<size_t>(1, kChunk / row_bytes);
+  const size_t nchunk = (nrows + rows_per - 1) / rows_per;
+  if (row_bytes > kChunk) e = hipErrorInvalidValue;
+  auto queue = [&](size_t c) {
+    const size_t r0 = c * rows_per, n = std::min(rows_per, nrows - r0);
+    hipError_t r = hipMemcpyAsync(st.buf[c & 1], (const char*)g->d + r0 * row_bytes, n * row_bytes,
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    if (r == hipSuccess) r = hipEventRecord(st.ev[c & 1], ctx->stream);
+    return r;
+  };
+  std::vector<unsigned char> out;
+  if (e == hipSuccess && nchunk) e = queue(0);
+  for (size_t c = 0; c < nchunk && e == hipSuccess && io_ok; ++c) {
+    if (c + 1 < nchunk && (e = queue(c + 1)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(st.ev[c & 1])) != hipSuccess) break;
+    const size_t r0 = c * rows_per, n = std::min(rows_per, nrows - r0);
+    const float* b = (const float*)st.buf[c & 1];
+    out.clear();
+    char txt[64];
+    for (size_t q = 0; q < n; ++q) {
+      if ((int)((r0 + q) % (size_t)g->nyp) >= ny) continue;
+      const float* row = b + q * rnxp;
+      for (int i = 0; i < nx; ++i) {
+        if (ascii) {
+          const int m = snprintf(txt, sizeof(txt), scientific ? "%e" : "%.6f", (double)row[i]);
+          out.insert(out.end(), txt, txt + m);
+          out.push_back(i + 1 < nx ? ' ' : '\n');
+        } else {
+          unsigned char v[4];
+          put_be(v, &row[i], 4);
+          out.insert(out.end(), v, v + 4);
+        }
+      }
+    }
+    io_ok = fwrite(out.data(), 1, out.size(), f) == out.size();
+  }
+  if (io_ok) io_ok = fwrite("0\n", 1, 2, f) == 2;
+  const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = e2;
+  io_ok = (fclose(f) == 0) && io_ok;
+  if (e != hipSuccess) return crv_set_error(ctx, CRV_E_HIP, std::string("write_storm: ") + hipGetErrorString(e));
+  if (!io_ok) return crv_set_error(ctx, CRV_E_INVALID, std::string("write_storm: write failed: ") + path);
   return CRV_OK;
 }

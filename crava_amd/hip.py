@@ -489,6 +489,26 @@ def write_crava(g: FFTGrid, path: str, sb: "L.SimboxInfo"):
     g.ctx.check(g.ctx.lib.crv_grid_write_crava(g.h, str(path).encode(), C.byref(sb)), "write_crava")
 
 
+def storm_header(si: "L.StormInfo", cubetype: int, nx: int, ny: int, nz: int, ascii: bool = False) -> bytes:
+    """The STORM header (crv_storm_header); needs no GPU."""
+    lib = L.lib()
+    n = lib.crv_storm_header(C.byref(si), cubetype, nx, ny, nz, int(ascii), None, 0)
+    if n < 0:
+        raise CravaError("storm_header: invalid arguments")
+    buf = C.create_string_buffer(n)
+    if lib.crv_storm_header(C.byref(si), cubetype, nx, ny, nz, int(ascii), buf, n) != n:
+        raise CravaError("storm_header failed")
+    return buf.raw[:n]
+
+
+def write_storm(g: FFTGrid, path: str, si: "L.StormInfo", ascii: bool = False, padding: bool = False,
+                scientific: bool = False):
+    """FFTGrid::writeStormFile through crv_grid_write_storm (path complete,
+    suffix included)."""
+    g.ctx.check(g.ctx.lib.crv_grid_write_storm(g.h, str(path).encode(), C.byref(si), int(ascii), int(padding),
+                                               int(scientific)), "write_storm")
+
+
 def read_crava(g: FFTGrid, path: str) -> "L.SimboxInfo":
     """FFTGrid::readCravaFile through crv_grid_read_crava; returns the geometry."""
     sb = L.SimboxInfo()

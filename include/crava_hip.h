@@ -531,6 +531,33 @@ typedef struct {
 int crv_crava_header(const crv_simbox_info* sb, int rnxp, int nyp, int nzp, unsigned char* buf, size_t len);
 int crv_grid_write_crava(crv_grid* g, const char* path, const crv_simbox_info* sb);
 int crv_grid_read_crava(crv_grid* g, const char* path, crv_simbox_info* sb);
+/* STORM grid files (FFTGrid::writeStormFile, src/fftgrid.cpp:2117-2176; the
+ * header is Simbox::getStormHeader, src/simbox.cpp:408-458):
+ *   storm_petro_binary            (storm_petro_ascii for text files)
+ *   0 <cubetype> -99999.000000
+ *   FFTGrid
+ *   <xmin> <lx> <ymin> <ly> <top_name> <bot_name> 0.0 0.0
+ *                                 (flat: <xmin> <lx> <ymin> <ly> 0.0 <lz> 0.0 0.0)
+ *   <lz> <angle in degrees>
+ *   (empty line)
+ *   <nx> <ny> <nz>
+ * reals in fixed notation with 6 decimals; then the nx ny nz values of the
+ * real grid, x fastest (the padded nxp nyp nzp with padding != 0), as
+ * big-endian floats, or as text (6 decimals fixed, or scientific), one line
+ * per x row; then the line "0".  FP32 real grids.  The caller appends the
+ * reference's suffix (".storm" binary, ".dat" text) to path.
+ * crv_storm_header encodes the header into buf (NULL: the length only) and
+ * returns its length.                                                      */
+typedef struct {
+  double xmin, lx, ymin, ly, lz, angle;  /* angle in radians */
+  const char* top_name;                  /* surfaces named in the header (flat = 0) */
+  const char* bot_name;
+  int flat;
+} crv_storm_info;
+int crv_storm_header(const crv_storm_info* si, int cubetype, int nx, int ny, int nz, int ascii, char* buf,
+                     size_t len);
+int crv_grid_write_storm(crv_grid* g, const char* path, const crv_storm_info* si, int ascii, int padding,
+                         int scientific);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,25 @@ def test_crava_header_layout():
     assert got == want
 
 
+def test_storm_header_text():
+    """The STORM header (Simbox::getStormHeader, reference src/simbox.cpp:
+    408-458; reals through NRLib::ToString(x, 6), i.e. fixed notation with 6
+    decimals): named surfaces, binary; flat, text."""
+    from crava_amd import _lib as L
+    from crava_amd.hip import storm_header
+    si = L.StormInfo(xmin=456000.5, lx=11250.0, ymin=6700000.25, ly=5625.0, lz=1280.0, angle=0.3,
+                     top_name=b"top.irap", bot_name=b"base.irap", flat=0)
+    want = ("storm_petro_binary\n0 3 -99999.000000\nFFTGrid\n"
+            "456000.500000 11250.000000 6700000.250000 5625.000000 top.irap base.irap 0.0 0.0\n"
+            "1280.000000 17.188734\n\n450 451 320\n")
+    assert storm_header(si, 3, 450, 451, 320) == want.encode()
+    si.flat = 1
+    want = ("storm_petro_ascii\n0 2 -99999.000000\nFFTGrid\n"
+            "456000.500000 11250.000000 6700000.250000 5625.000000 0.0 1280.000000 0.0 0.0\n"
+            "1280.000000 17.188734\n\n64 64 128\n")
+    assert storm_header(si, 2, 64, 64, 128, ascii=True) == want.encode()
+
+
 def test_context_without_gpu_fails_cleanly():
     import torch
     if torch.cuda.is_available():

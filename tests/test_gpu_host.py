@@ -55,7 +55,8 @@ def test_host_classes_match_abi(ctx):
 
 def test_host_fftgrid_output_path(ctx, tmp_path):
     """Host FFTGrid::expTransf + writeCravaFile + readCravaFile: the file is
-    CRAVA's binary format of exp(grid) and reads back unchanged."""
+    CRAVA's binary format of exp(grid) and reads back unchanged; the same
+    grid through writeStormFile is the STORM binary file of the padded grid."""
     nxp, nyp, nzp = 30, 20, 16
     rng = np.random.default_rng(5)
     x = (np.log(2500.0) + 0.1 * rng.standard_normal((nzp, nyp, 2 * (nxp // 2 + 1)))).astype(np.float32)
@@ -72,3 +73,10 @@ def test_host_fftgrid_output_path(ctx, tmp_path):
     raw = (tmp_path / "vp.crava").read_bytes()
     assert raw.startswith(b"crava_fftgrid_binary\n")
     assert raw[-out.size * 4:] == out.astype(">f4").tobytes()
+    # FFTGrid::writeStormFile of the same grid (binary, padded, flat)
+    from crava_amd import _lib as L
+    from crava_amd.hip import storm_header
+    si = L.StormInfo(xmin=0.0, lx=25.0 * nxp, ymin=0.0, ly=25.0 * nyp, lz=4.0 * nzp, angle=0.0, flat=1)
+    head = storm_header(si, L.CRV_PARAMETER, nxp, nyp, nzp)
+    storm = (tmp_path / "vp.storm").read_bytes()
+    assert storm == head + out[:, :, :nxp].astype(">f4").tobytes() + b"0\n"

@@ -87,3 +87,34 @@ def test_crava_file_round_trip(ctx, tmp_path):
     wrong = FFTGrid(ctx, 450, 64, 98)
     with pytest.raises(Exception):
         read_crava(wrong, tmp_path / "grid.crava")
+
+
+@pytest.mark.parametrize("ascii,padding,scientific", [(False, False, False), (False, True, False),
+                                                      (True, False, False), (True, True, True)])
+def test_storm_file(ctx, tmp_path, ascii, padding, scientific):
+    """FFTGrid::writeStormFile (reference src/fftgrid.cpp:2117-2176): the
+    header, the grid (padded or not) x fastest as big-endian floats, or as
+    text (fixed with 6 decimals, or scientific) with one line per x row,
+    then the line "0"."""
+    from crava_amd.hip import FFTGrid, storm_header, write_storm
+    nx, ny, nz, nxp, nyp, nzp = 12, 10, 7, 15, 12, 9
+    g = FFTGrid(ctx, nxp, nyp, nzp, cubetype=L.CRV_PARAMETER, nx=nx, ny=ny, nz=nz)
+    X = (np.random.default_rng(5).standard_normal((nzp, nyp, g.rnxp)) * 100).astype(np.float32)
+    g.upload_real(X)
+    si = L.StormInfo(xmin=10.0, lx=300.0, ymin=-20.0, ly=250.0, lz=28.0, angle=0.0, top_name=b"top",
+                     bot_name=b"base", flat=0)
+    p = tmp_path / ("g.dat" if ascii else "g.storm")
+    write_storm(g, p, si, ascii=ascii, padding=padding, scientific=scientific)
+    NX, NY, NZ = (nxp, nyp, nzp) if padding else (nx, ny, nz)
+    raw = p.read_bytes()
+    head = storm_header(si, L.CRV_PARAMETER, NX, NY, NZ, ascii)
+    assert raw[:len(head)] == head
+    body = raw[len(head):]
+    vals = X[:NZ, :NY, :NX]
+    if ascii:
+        fmt = "%e" if scientific else "%.6f"
+        want = "".join(" ".join(fmt % v for v in row) + "\n" for row in vals.reshape(-1, NX).astype(np.float64))
+        assert body.decode() == want + "0\n"
+    else:
+        assert body[-2:] == b"0\n"
+        assert body[:-2] == vals.astype(">f4").tobytes()
