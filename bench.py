@@ -298,6 +298,7 @@ def bench_c4(a):
     err = np.abs(S.validation_errors(ctx, r.mlp, r.val_x, r.val_y))
     table = S.ErrorTable.build(ctx, r.mlp, r.val_x, r.val_y, nb=6)
     out["training"] = {"samples": 1_000_000, "epochs": 400, "steps": r.steps, "train_s": r.seconds,
+                       "step_kernels_ms": r.step_kernels_ms,
                        "with_data_generation_s": t_all, "mfma_tflops": r.mfma_tflops,
                        "mfma_util_bf16_dense": r.mfma_util,
                        "val_abs_err": {nm: {"median": float(np.median(err[:, c])),
@@ -305,6 +306,7 @@ def bench_c4(a):
                                        for c, nm in enumerate(("ln_vp", "ln_vs", "ln_rho"))}}
     log(f"[bench c4] trained in {r.seconds:.1f} s ({r.steps} steps), MFMA {r.mfma_tflops:.1f} TFLOP/s "
         f"({100 * r.mfma_util:.2f} % of 2.5 PFLOP/s); val |ln vp| median {np.median(err[:, 0]):.2e}")
+    log("[bench c4] one training step: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in r.step_kernels_ms.items()))
     rng = np.random.default_rng(21)
     xs = [torch.from_numpy(S.sample_inputs(10_000_000, rng, facies=f)).to(dev) for f in facies.FACIES]
     # warm-up of every kernel involved

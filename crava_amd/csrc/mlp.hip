@@ -68,7 +68,7 @@ constexpr int kLd = kKa + 8;     // LDS row stride (bf16) of activations and W2.
 constexpr int kLd1 = kK1 + 8;    // LDS row stride of W1
 constexpr int kOffW1 = 0, kOffW2 = kHid * kK1, kOffW3 = kOffW2 + kHid * kKa, kOffW4 = kOffW3 + kHid * kKa;
 constexpr int kParams = kOffW4 + kNo * kKa;  // 43520
-constexpr int kKChunk = 1024;    // rows per split-K slice of the weight gradients
+constexpr int kKChunk = 256;     // rows per split-K slice of the weight gradients (enough blocks to fill the chip)
 
 __device__ __forceinline__ float tanh_fast(float x) {
   const float e = __expf(-2.0f * fabsf(x));
@@ -237,10 +237,12 @@ struct GemmArgs {
   float* P;
   __bf16* C; int ldc;
   const __bf16* H; int ldh;
+  int vec;                                 // A, B 16-byte aligned, lda, ldb multiples of 8: 8-wide loads
 };
 
-constexpr int kGT = 256;   // 4 waves, 64 x 64 tile, K step 32
-constexpr int kGLd = 32 + 8;
+constexpr int kGT = 256;   // 4 waves, 64 x 64 tile
+constexpr int kGK = 128;   // K per LDS tile: 8 loads of 16 B in flight per thread, 8 MFMA steps per wave
+constexpr int kGLd = kGK + 8;
 
 __global__ __launch_bounds__(kGT) void k_gemm(GemmArgs g) {
   __shared__ __bf16 As[64 * kGLd];
@@ -251,46 +253,84 @@ __global__ __launch_bounds__(kGT) void k_gemm(GemmArgs g) {
   const long long ke = (g.K < kb + g.kchunk) ? g.K : kb + g.kchunk;
   const int wm = wave & 1, wn = wave >> 1;
   floatx16 acc = floatx16{};
-  for (long long k0 = kb; k0 < ke; k0 += 32) {
-    // A tile 64 x 32 -> As[m][k]
-    if (!g.a_trans) {
-      const int m = tid >> 2, k8 = 8 * (tid & 3);
+  for (long long k0 = kb; k0 < ke; k0 += kGK) {
+    // chunks of 32 and MFMA steps of 16 past the end of K are skipped
+    const int kt = (int)((ke - k0 < kGK) ? ke - k0 : kGK);
+    // A tile 64 x kGK -> As[m][k], 32 values per thread as 4 chunks of 8
+    bf16x8 va[kGK / 32], vb[kGK / 32];
+#pragma unroll
+    for (int c = 0; c < kGK / 32; ++c) {
       bf16x8 v = bf16x8{};
-      const long long gm = m0 + m, gk = k0 + k8;
-      if (gm < g.M) {
-        if (gk + 8 <= ke) v = *reinterpret_cast<const bf16x8*>(g.A + gm * g.lda + gk);
-        else
+      if (32 * c >= kt) { va[c] = v; continue; }
+      if (!g.a_trans) {
+        // m-major: 8 consecutive k of one m
+        const int m = tid >> 2, k8 = 8 * (tid & 3) + 32 * c;
+        const long long gm = m0 + m, gk = k0 + k8;
+        if (gm < g.M) {
+          if (g.vec && gk + 8 <= ke) v = *reinterpret_cast<const bf16x8*>(g.A + gm * g.lda + gk);
+          else
 #pragma unroll
-          for (int j = 0; j < 8; ++j) v[j] = (gk + j < ke) ? g.A[gm * g.lda + gk + j] : (__bf16)0.0f;
+            for (int j = 0; j < 8; ++j) v[j] = (gk + j < ke) ? g.A[gm * g.lda + gk + j] : (__bf16)0.0f;
+        }
+      } else {
+        // k-major: 8 consecutive m of one k
+        const int k = (tid >> 3) + 32 * c, m8 = 8 * (tid & 7);
+        const long long gk = k0 + k, gm = m0 + m8;
+        if (gk < ke) {
+          if (g.vec && gm + 8 <= g.M) v = *reinterpret_cast<const bf16x8*>(g.A + gk * g.lda + gm);
+          else
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = (gm + j < g.M) ? g.A[gk * g.lda + gm + j] : (__bf16)0.0f;
+        }
       }
-      *reinterpret_cast<bf16x8*>(As + m * kGLd + k8) = v;
-    } else {
-      const int k = tid >> 3, m8 = 8 * (tid & 7);
-      const long long gk = k0 + k, gm = m0 + m8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        As[(m8 + j) * kGLd + k] = (gk < ke && gm + j < g.M) ? g.A[gk * g.lda + gm + j] : (__bf16)0.0f;
+      va[c] = v;
     }
-    // B tile 32 x 64 -> Bs[n][k]
-    if (g.b_rowk) {
-      const int k = tid >> 3, n8 = 8 * (tid & 7);
-      const long long gk = k0 + k, gn = n0 + n8;
+    // B tile kGK x 64 -> Bs[n][k]
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        Bs[(n8 + j) * kGLd + k] = (gk < ke && gn + j < g.N) ? g.B[gk * g.ldb + gn + j] : (__bf16)0.0f;
-    } else {
-      const int n = tid >> 2, k8 = 8 * (tid & 3);
+    for (int c = 0; c < kGK / 32; ++c) {
       bf16x8 v = bf16x8{};
-      const long long gn = n0 + n, gk = k0 + k8;
-      if (gn < g.N) {
+      if (32 * c >= kt) { vb[c] = v; continue; }
+      if (g.b_rowk) {
+        const int k = (tid >> 3) + 32 * c, n8 = 8 * (tid & 7);
+        const long long gk = k0 + k, gn = n0 + n8;
+        if (gk < ke) {
+          if (g.vec && gn + 8 <= g.N) v = *reinterpret_cast<const bf16x8*>(g.B + gk * g.ldb + gn);
+          else
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (gk + j < ke) ? g.B[gn * g.ldb + gk + j] : (__bf16)0.0f;
+            for (int j = 0; j < 8; ++j) v[j] = (gn + j < g.N) ? g.B[gk * g.ldb + gn + j] : (__bf16)0.0f;
+        }
+      } else {
+        const int n = tid >> 2, k8 = 8 * (tid & 3) + 32 * c;
+        const long long gn = n0 + n, gk = k0 + k8;
+        if (gn < g.N) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[j] = (gk + j < ke) ? g.B[gn * g.ldb + gk + j] : (__bf16)0.0f;
+        }
       }
-      *reinterpret_cast<bf16x8*>(Bs + n * kGLd + k8) = v;
+      vb[c] = v;
+    }
+#pragma unroll
+    for (int c = 0; c < kGK / 32; ++c) {
+      if (32 * c >= kt) continue;
+      if (!g.a_trans) {
+        *reinterpret_cast<bf16x8*>(As + (tid >> 2) * kGLd + 8 * (tid & 3) + 32 * c) = va[c];
+      } else {
+        const int k = (tid >> 3) + 32 * c, m8 = 8 * (tid & 7);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) As[(m8 + j) * kGLd + k] = va[c][j];
+      }
+      if (g.b_rowk) {
+        const int k = (tid >> 3) + 32 * c, n8 = 8 * (tid & 7);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bs[(n8 + j) * kGLd + k] = vb[c][j];
+      } else {
+        *reinterpret_cast<bf16x8*>(Bs + (tid >> 2) * kGLd + 8 * (tid & 3) + 32 * c) = vb[c];
+      }
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kGK / 16; ++s) {
+      if (16 * s >= kt) break;
       const bf16x8 av = *reinterpret_cast<const bf16x8*>(As + (32 * wm + (lane & 31)) * kGLd + 16 * s + 8 * (lane >> 5));
       const bf16x8 bv = *reinterpret_cast<const bf16x8*>(Bs + (32 * wn + (lane & 31)) * kGLd + 16 * s + 8 * (lane >> 5));
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
@@ -319,9 +359,15 @@ __global__ void k_reduce(ReduceArgs a) {
     int l = 0;
     while (l < 3 && p >= a.s[l + 1].off) ++l;
     const int q = p - a.s[l].off;
-    float acc = 0.0f;
-    for (int z = 0; z < a.s[l].slices; ++z) acc += a.s[l].P[(size_t)z * a.s[l].size + q];
-    a.g[p] = acc;
+    // eight independent partial sums keep eight loads in flight
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const int ns = a.s[l].slices;
+    int z = 0;
+    for (; z + 8 <= ns; z += 8)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += a.s[l].P[(size_t)(z + u) * a.s[l].size + q];
+    for (; z < ns; ++z) acc[0] += a.s[l].P[(size_t)z * a.s[l].size + q];
+    a.g[p] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   }
 }
 
@@ -368,8 +414,10 @@ int launch_forward(crv_mlp* h, const float* x, const float* t, float* y, long lo
   return CRV_OK;
 }
 
-int launch_gemm(crv_ctx* ctx, const GemmArgs& g, int slices, const char* name) {
+int launch_gemm(crv_ctx* ctx, GemmArgs g, int slices, const char* name) {
   dim3 grid((g.M + 63) / 64, (g.N + 63) / 64, slices);
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  g.vec = al16(g.A) && al16(g.B) && g.lda % 8 == 0 && g.ldb % 8 == 0;
   ProfScope ps(ctx, name);
   hipLaunchKernelGGL(k_gemm, grid, dim3(kGT), 0, ctx->stream, g);
   CRV_HIP_CHECK(ctx, hipGetLastError());

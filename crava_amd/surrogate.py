@@ -176,6 +176,7 @@ class TrainResult:
     train_rows: int = 0
     mfma_tflops: float = 0.0
     mfma_util: float = 0.0
+    step_kernels_ms: dict = field(default_factory=dict)   # one training step, per kernel
 
 
 def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int = 16384, lr: float = 2e-3,
@@ -236,6 +237,7 @@ def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int =
     mlp.train_step(xtr.data_ptr(), ytr.data_ptr(), batch, 1e-5, step + 1)
     ctx.profile(False)
     rep = ctx.profile_report()
+    res.step_kernels_ms = {k: v[1] for k, v in rep.items()}
     ms = sum(v[1] for k, v in rep.items() if k.startswith("mlp_") and k not in ("mlp_reduce", "mlp_adam"))
     f, b_ = mlp.flops()
     if ms > 0:
