@@ -366,10 +366,14 @@ def bench_c4(a):
     keep = lambda y: y[torch.isfinite(y).all(1)].contiguous()
     prior = [0.4, 0.4, 0.2]
     t0 = time.perf_counter()
+    ctx.profile(True)
     d_nn = [facies.density(ctx, keep(g[0]), box, Sm, bins) for g in gated]
     P_nn = facies.probabilities(ctx, d_nn, prior, box, mean)
     ctx.synchronize()
     t_fac = time.perf_counter() - t0
+    ctx.profile(False)
+    log("[bench c4] densities + probabilities kernels: " +
+        ", ".join(f"{k} x{v[0]} {v[1]:.2f} ms" for k, v in sorted(ctx.profile_report().items(), key=lambda kv: -kv[1][1])))
     d_rp = [facies.density(ctx, keep(y), box, Sm, bins) for y in ys_rp]
     P_rp = facies.probabilities(ctx, d_rp, prior, box, mean)
     nx = sv_s.nx
