@@ -92,6 +92,70 @@ def timed(ctx, G, kw, mode, steps, warmup, seed):
     return (time.perf_counter() - t0) / steps
 
 
+def general_leg(ctx, s, sv, kw, steps, warmup):
+    """The drop-in path as the reference feeds it: six real-space prior
+    covariance grids and the error-correlation grid go in and are transformed
+    inside (FFTCovGrids, seismicparametersholder.cpp:273-322; errCorr,
+    crava.cpp:1062), the general posterior kernel reads them per wavenumber,
+    and the realization comes from the simulation kernel over the stored
+    covariances.  Timed like the headline step (FP32 refine mode, inputs
+    resident); the prior grids are rebuilt from pristine copies by the
+    forward transforms themselves (r2c from src)."""
+    from crava_amd import _lib as L
+    from crava_amd import hip
+    G = make_grids(ctx, s, 32)
+    mk = lambda n, ct: hip.grid_batch(ctx, n, s.nxp, s.nyp, s.nzp, 32, ct, s.nx, s.ny, s.nz)
+    cov_in, err_in, err = mk(6, L.CRV_COVARIANCE), mk(1, L.CRV_COVARIANCE), mk(1, L.CRV_COVARIANCE)
+    for g, src in zip(G["mean_in"], sv.mean):
+        g.copyFrom(src)
+    for g, src in zip(G["data_in"], sv.data):
+        g.copyFrom(src)
+    # prior covariance spectra -> real-space grids; errCorr: the lateral
+    # correlation spectrum on every kz plane (what the separable plan uses)
+    hip.fill_prior_spectra(ctx, sv.sigma0, sv.cxy, sv.ct, cov_in)
+    hip.fft_c2r_batch(cov_in)
+    hip.fill_prior_spectra(ctx, np.eye(3), sv.cxy, np.ones(s.nzp), G["cov"])
+    err_in[0].copyFrom(G["cov"][0])
+    hip.fft_c2r_batch(err_in)
+    gkw = {k: v for k, v in kw.items() if k not in ("cxy", "ct")}
+    plan = hip.PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=L.CRV_MODE_FP32_REFINE, separable=False,
+                        skip_mean_inverse=True, **gkw)
+
+    def one(r):
+        hip.fft_r2c_batch(G["data"], src=G["data_in"])
+        hip.fft_r2c_batch(G["mean"], src=G["mean_in"])
+        hip.fft_r2c_batch(G["cov"], src=cov_in)
+        hip.fft_r2c_batch(err, src=err_in)
+        plan.exec(G["mean"], G["data"], err[0], G["cov"])
+        hip.fft_c2r_batch(G["mean"])
+        hip.simulate_batch(ctx, G["cov"], G["mean"], G["sim"], seed=s.seed_sim, realization0=r, fp64_chol=True)
+        hip.fft_c2r_batch(G["cov"])
+
+    for r in range(warmup):
+        one(r)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for r in range(steps):
+        one(1000 + r)
+    ctx.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    ctx.profile(True)
+    one(5000)
+    ctx.profile(False)
+    prof = {k: v[1] for k, v in ctx.profile_report().items()}
+    st = plan.stats()
+    plan.close()
+    for g in cov_in + err_in + err + sum((G[k] for k in ("mean_in", "data_in", "mean", "data", "cov", "sim")), []):
+        g.close()
+    # algorithmic bytes per voxel: 15 forward + 12 inverse transforms (8 B
+    # each), the general posterior (reads n + 3 + 7 complex, writes 3 + n + 6
+    # complex per wavenumber, half a wavenumber per voxel), the simulation
+    # (reads 6, writes 3 complex) and the mean add (12 B)
+    bpv = 8 * (s.n + 3 + 7 + 12) + (8 * (s.n + 10) + 8 * (s.n + 9)) // 2 + 36 + 12
+    return {"ms_per_step": sec * 1e3, "voxels_per_s": s.voxels / sec, "bytes_per_voxel": bpv,
+            "GB/s": bpv * s.voxels / sec / 1e9, "fp64_solves": st.n_fp64, "kernels_ms": prof}
+
+
 def transforms_per_step(name, s, names=()):
     """3D transforms per step whose passes carry this kernel name: forward
     for the n data and 3 means, inverse for 3 means, 3 realization grids
@@ -556,6 +620,8 @@ def main():
                     help="c1: headline 450x450x320 inversion; c0: 64x64x128; c3: batched realizations and volumes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-general", action="store_true",
+                    help="skip the drop-in general path (prior covariance grids + errCorr, transformed inside)")
     ap.add_argument("--fp32-mode", choices=["refine", "plain"], default="refine",
                     help="FP32 path: refinement + FP64 fallback (the metric) or the reference's plain FP32")
     ap.add_argument("--chunk-mb", type=int, default=0, help="CRV_TUNE_CHUNK_MB for the transforms")
@@ -650,6 +716,13 @@ def main():
                 f"{v_[1] / 3:.3f} ms/step {gbs}")
         log(f"[bench] FP{prec}: {sec * 1e3:.2f} ms/step, {s.voxels / sec:.3e} voxels/s")
 
+    general = None
+    if not a.no_general:
+        log("[bench] drop-in general path (6 prior covariance grids + errCorr transformed inside)")
+        general = general_leg(ctx, s, sv, kw, max(1, a.steps // 2), 1)
+        log(f"[bench] general path: {general['ms_per_step']:.2f} ms/step, {general['voxels_per_s']:.3e} voxels/s, "
+            f"{general['bytes_per_voxel']} B/voxel algorithmic ({general['GB/s']:.0f} GB/s)")
+
     # host transfers, reported beside the resident-input figure: the 8 input
     # cubes uploaded from pageable host arrays (as FFTGrid's rvalue_) and the
     # step's outputs (3 means, 6 covariances, 1 realization, 5 residual
@@ -720,6 +793,7 @@ def main():
                   "speedup_fp32_refine": res[64]["sec"] / sec} if 64 in res else None),
         "errors_vs_fp64": errors,
         "posterior_stats": res[32]["stats"],
+        "general_path": general,
         "transfers": transfers,
         "kernels": kernels,
         "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}
