@@ -188,6 +188,53 @@ def test_graph_replays_match_eager(ctx):
     graph.close()
 
 
+def test_invert_graph_replays_match_eager(ctx):
+    """crv_invert (one call: transforms, eigen-basis posterior with the
+    realizations fused, generated prior planes, inverse transforms) captured
+    in a graph: every replay, with eager work and synchronisations between
+    replays, equals the eager call bit for bit."""
+    from crava_amd.hip import FFTGrid, Graph, PostPlan, grid_batch
+    case = Case(model.config_small(20, 18, 30, 5))
+    s = case.s
+    _, _, kw = _inputs(ctx, case)
+    plan = PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=1, **kw)
+    batch = lambda n, ct: grid_batch(ctx, n, s.nxp, s.nyp, s.nzp, 32, ct)
+    mean_in, data_in = batch(3, 2), batch(s.n, 1)
+    for g, a in zip(mean_in, case.syn.mean):
+        g.upload_real(a)
+    for g, a in zip(data_in, case.syn.data):
+        g.upload_real(a)
+
+    def grids():
+        return batch(3, 2), batch(s.n, 1), batch(6, 3), batch(6, 2)
+
+    def run(mean, data, cov, sims):
+        plan.invert(mean, data, cov, sims, mean_src=mean_in, data_src=data_in, seed=9, realization0=2)
+
+    def read(mean, data, cov, sims):
+        return ([g.download_real() for g in mean] + [g.download_complex() for g in data]
+                + [g.download_real() for g in cov + sims])
+
+    eager = grids()
+    run(*eager)
+    ctx.synchronize()
+    want = read(*eager)
+    cap = grids()
+    graph = Graph.begin(ctx)
+    run(*cap)
+    graph.end()
+    for rep in range(3):
+        junk = FFTGrid(ctx, s.nxp, s.nyp, s.nzp, cubetype=2)
+        junk.upload_real(np.full((s.nzp, s.nyp, s.rnxp), float(rep), np.float32))
+        junk.fftInPlace()
+        ctx.synchronize()
+        graph.launch()
+        ctx.synchronize()
+        for a, b in zip(read(*cap), want):
+            assert np.array_equal(a, b)
+    graph.close()
+
+
 @pytest.mark.parametrize("dims", [(80, 60, 50), (450, 450, 320)])
 def test_autotuned_shapes_keep_the_transform(ctx, dims):
     from crava_amd.hip import FFTGrid, autotune
