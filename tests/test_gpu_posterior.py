@@ -161,10 +161,11 @@ def test_c0_end_to_end_fp32_vs_fp64_oracle(ctx):
         assert rel_rms(g.download_complex(), c) < 1e-4
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 8])
+@pytest.mark.parametrize("n", [1, 2, 4, 6, 7, 8])
 def test_angle_counts_against_fp64_oracle(ctx, n):
     """Angle counts other than the configs' 3 and 5: 1..6 run the per-plane
-    record kernel in FP32, 7 and 8 the general one.  FP64 equals the FP64
+    kernels in FP32 (the eigen-basis kernel with refinement; 6 fills its
+    largest record), 7 and 8 the general one.  FP64 equals the FP64
     oracle; FP32 with refinement stays within the precision target; the
     separable prior equals the grid path."""
     case = Case(model.config_small(16, 12, 20, n))
