@@ -7,7 +7,7 @@
 // facies-conditional elastic samples are generated for faciesprob
 // (reference rplib/distributionsrockdem.cpp:125-156,
 // src/faciesprob.cpp:2494-2559); crv_rock_forward stays the fallback and the
-// judge of its error.
+// measure of its error.
 //
 // Parameters live in augmented matrices whose last used column holds the
 // bias, so every layer is one product with a constant-1 input column:

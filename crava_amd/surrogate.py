@@ -7,7 +7,7 @@ ln vp, ln vs, ln rho) stands in for the per-sample rplib chain that
 DistributionsRockDEM runs when faciesprob generates synthetic elastic
 samples (reference rplib/distributionsrockdem.cpp:125-156,
 src/faciesprob.cpp:2494-2559).  crv_rock_forward, the GPU port of that chain,
-produces the training targets, judges the network's error and re-evaluates
+produces the training targets, measures the network's error and re-evaluates
 every sample the error gate does not trust.
 
 Device memory is held in torch tensors (plumbing); every computation is a
