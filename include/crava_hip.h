@@ -316,8 +316,8 @@ int crv_post_plan_stats(crv_post_plan* plan, crv_post_stats* stats);
  * the work when chunk_rows <= 0; with 0 < chunk_rows < nyp each chunk of
  * chunk_rows y rows goes through the z transforms, the posterior and the
  * noise in turn (smaller working sets, but smaller launches).  One
- * exception: with chunk_rows <= 0 on an FP32 refine plan of up to 5 angles,
- * the realizations outside the inverted band scale the noise by
+ * exception: with chunk_rows <= 0 on an FP32 refine plan (separable prior,
+ * up to 6 angles), the realizations outside the inverted band scale the noise by
  * sqrt(lam) times the plane's factor of the prior, where
  * crv_simulate_batch factors the stored FP32 product lam S: those values
  * differ by rounding (about 1e-7 relative).  Launches only (capturable);
