@@ -409,7 +409,15 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
       const int c = fdiv(idx, a.divL), e = idx - c * L;
       if (c < nr) {
         const T* row = src0 + (size_t)c * a.rnxp;
-        v[q] = a.xeven ? *reinterpret_cast<const C*>(row + 2 * e) : cmake<C>(row[e], (T)0);
+        // the input cube is read once: streaming loads keep it from
+        // displacing the spectrum this pass writes (the next pass's input)
+        // in the memory-side cache
+        if (a.xeven) {
+          const T* p2 = row + 2 * e;
+          v[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
+        } else {
+          v[q] = cmake<C>(__builtin_nontemporal_load(row + e), (T)0);
+        }
       }
     }
   }
