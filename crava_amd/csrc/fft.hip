@@ -551,16 +551,25 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       w[q] = cmake<C>((T)0, (T)0);
       if (add && idx < PL) {
         const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr) w[q] = *reinterpret_cast<const C*>(add + (size_t)c * a.rnxp + 2 * n);
+        if (c < nr) {
+          const T* p2 = add + (size_t)c * a.rnxp + 2 * n;
+          w[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
+        }
       }
     }
+    // the real grid is the transform's final output, not read by the next
+    // pass: streaming stores leave the memory-side cache to the passes
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int idx = threadIdx.x + q * kThreads;
       if (idx < PL) {
         const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr)
-          *reinterpret_cast<C*>(dst0 + (size_t)c * a.rnxp + 2 * n) = cadd(cscale(lds[n * S + c], a.scale), w[q]);
+        if (c < nr) {
+          const C o = cadd(cscale(lds[n * S + c], a.scale), w[q]);
+          T* p2 = dst0 + (size_t)c * a.rnxp + 2 * n;
+          __builtin_nontemporal_store(o.x, p2);
+          __builtin_nontemporal_store(o.y, p2 + 1);
+        }
       }
     }
   } else {
@@ -571,7 +580,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       w[q] = (T)0;
       if (add && idx < PL) {
         const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr) w[q] = add[(size_t)c * a.rnxp + n];
+        if (c < nr) w[q] = __builtin_nontemporal_load(add + (size_t)c * a.rnxp + n);
       }
     }
 #pragma unroll
@@ -579,7 +588,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       const int idx = threadIdx.x + q * kThreads;
       if (idx < PL) {
         const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr) dst0[(size_t)c * a.rnxp + n] = lds[n * S + c].x * a.scale + w[q];
+        if (c < nr) __builtin_nontemporal_store(lds[n * S + c].x * a.scale + w[q], dst0 + (size_t)c * a.rnxp + n);
       }
     }
   }
