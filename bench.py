@@ -472,6 +472,8 @@ def bench_c3(a):
     from crava_amd.survey import GpuSurvey
     ctx = hip.Context(0)
     refine = L.CRV_MODE_FP32_REFINE
+    if a.order >= 0:
+        ctx.tune(L.CRV_TUNE_ORDER, a.order + 1)
 
     def grids(s, nreal):
         mk = lambda ct: hip.FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=32, cubetype=ct, nx=s.nx, ny=s.ny, nz=s.nz)
