@@ -188,6 +188,47 @@ def test_graph_replays_match_eager(ctx):
     graph.close()
 
 
+def test_invert_fp64_matches_sequence(ctx):
+    """crv_invert on an FP64 plan (no per-plane records: the general
+    posterior kernel, the simulation kernel, every covariance plane written)
+    equals the FP64 one-call sequence bit for bit, realizations included."""
+    from crava_amd.hip import FFTGrid, PostPlan, grid_batch, simulate_batch
+    case = Case(model.config_small(32, 24, 20, 3))
+    s = case.s
+    _, _, kw = _inputs(ctx, case)
+    plan = PostPlan(ctx, s.nxp, s.nyp, s.nzp, 64, s.n, mode=0, **kw)
+    batch = lambda n, ct: grid_batch(ctx, n, s.nxp, s.nyp, s.nzp, 64, ct)
+
+    def inputs():
+        mean, data = batch(3, 2), batch(s.n, 1)
+        for g, a in zip(mean, case.syn.mean):
+            g.upload_real(a)
+        for g, a in zip(data, case.syn.data):
+            g.upload_real(a)
+        return mean, data
+
+    mean, data = inputs()
+    cov, sims = batch(6, 3), batch(3, 2)
+    for g in data + mean:
+        g.fftInPlace()
+    plan.exec(mean, data, None, cov)
+    for g in mean:
+        g.invFFTInPlace()
+    simulate_batch(ctx, cov, mean, sims, seed=4, realization0=1, fp64_chol=False)
+    for g in cov:
+        g.invFFTInPlace()
+    ctx.synchronize()
+    want = ([g.download_real() for g in mean + cov + sims], [g.download_complex() for g in data])
+    mean2, data2 = inputs()
+    cov2, sims2 = batch(6, 3), batch(3, 2)
+    plan.invert(mean2, data2, cov2, sims2, seed=4, realization0=1)
+    got = ([g.download_real() for g in mean2 + cov2 + sims2], [g.download_complex() for g in data2])
+    for w, g in zip(want[0], got[0]):
+        assert np.array_equal(w[:, :, :s.nxp], g[:, :, :s.nxp])
+    for w, g in zip(want[1], got[1]):
+        assert np.array_equal(w, g)
+
+
 def test_invert_graph_replays_match_eager(ctx):
     """crv_invert (one call: transforms, eigen-basis posterior with the
     realizations fused, generated prior planes, inverse transforms) captured
