@@ -77,6 +77,29 @@ def test_storm_header_text():
     assert storm_header(si, 2, 64, 64, 128, ascii=True) == want.encode()
 
 
+def test_header_encoders_reject_bad_arguments():
+    """The header encoders validate before writing: no simbox, a non-flat
+    STORM header without surface names, empty dimensions, or a buffer that
+    is too short give CRV_E_INVALID and leave the buffer alone."""
+    import ctypes as C
+    from crava_amd import _lib as L
+    lib = L.lib()
+    inval = -1  # CRV_E_INVALID (include/crava_hip.h)
+    si = L.StormInfo(xmin=0.0, lx=1.0, ymin=0.0, ly=1.0, lz=1.0, angle=0.0, flat=0)
+    assert lib.crv_storm_header(C.byref(si), 2, 4, 4, 4, 0, None, 0) == inval      # no surface names
+    si.flat = 1
+    assert lib.crv_storm_header(C.byref(si), 2, 0, 4, 4, 0, None, 0) == inval      # empty grid
+    n = lib.crv_storm_header(C.byref(si), 2, 4, 4, 4, 0, None, 0)
+    buf = C.create_string_buffer(b"x" * n, n)
+    assert lib.crv_storm_header(C.byref(si), 2, 4, 4, 4, 0, buf, n - 1) == inval  # too short
+    assert buf.raw == b"x" * n
+    assert lib.crv_storm_header(None, 2, 4, 4, 4, 0, None, 0) == inval
+    sb = L.SimboxInfo()
+    m = lib.crv_crava_header(C.byref(sb), 10, 4, 4, None, 0)
+    assert m > 0 and lib.crv_crava_header(C.byref(sb), 10, 4, 4, C.create_string_buffer(m), m - 1) == inval
+    assert lib.crv_crava_header(None, 10, 4, 4, None, 0) == inval
+
+
 def test_context_without_gpu_fails_cleanly():
     import torch
     if torch.cuda.is_available():
