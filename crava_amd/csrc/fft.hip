@@ -488,6 +488,23 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       v[q] = (c < nr) ? reinterpret_cast<const C*>(src0 + (size_t)c * a.rnxp)[k] : cmake<C>((T)0, (T)0);
     }
   }
+  // rows of the added grid (even lengths): loaded before the transform so
+  // that their latency hides behind it.  The registers they hold take the
+  // kernel from four waves per SIMD to three, and it is still faster: it
+  // waits on memory latency, not on issue slots
+  C w[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int idx = threadIdx.x + q * kThreads;
+    w[q] = cmake<C>((T)0, (T)0);
+    if (a.xeven && add && idx < PL) {
+      const int c = fdiv(idx, a.divL), n = idx - c * L;
+      if (c < nr) {
+        const T* p2 = add + (size_t)c * a.rnxp + 2 * n;
+        w[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
+      }
+    }
+  }
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int idx = threadIdx.x + q * kThreads;
@@ -546,19 +563,6 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   __syncthreads();
   // tile -> rows, scaled, plus the added grid
   if (a.xeven) {
-    C w[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int idx = threadIdx.x + q * kThreads;
-      w[q] = cmake<C>((T)0, (T)0);
-      if (add && idx < PL) {
-        const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr) {
-          const T* p2 = add + (size_t)c * a.rnxp + 2 * n;
-          w[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
-        }
-      }
-    }
     // the real grid is the transform's final output, not read by the next
     // pass: streaming stores leave the memory-side cache to the passes
 #pragma unroll
