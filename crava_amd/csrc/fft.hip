@@ -488,23 +488,28 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       v[q] = (c < nr) ? reinterpret_cast<const C*>(src0 + (size_t)c * a.rnxp)[k] : cmake<C>((T)0, (T)0);
     }
   }
-  // rows of the added grid (even lengths): loaded before the transform so
-  // that their latency hides behind it.  The registers they hold take the
-  // kernel from four waves per SIMD to three, and it is still faster: it
-  // waits on memory latency, not on issue slots
+  // rows of the added grid (even lengths)
   C w[E];
+  auto load_add = [&]() {
 #pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int idx = threadIdx.x + q * kThreads;
-    w[q] = cmake<C>((T)0, (T)0);
-    if (a.xeven && add && idx < PL) {
-      const int c = fdiv(idx, a.divL), n = idx - c * L;
-      if (c < nr) {
-        const T* p2 = add + (size_t)c * a.rnxp + 2 * n;
-        w[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
+    for (int q = 0; q < E; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      w[q] = cmake<C>((T)0, (T)0);
+      if (a.xeven && add && idx < PL) {
+        const int c = fdiv(idx, a.divL), n = idx - c * L;
+        if (c < nr) {
+          const T* p2 = add + (size_t)c * a.rnxp + 2 * n;
+          w[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
+        }
       }
     }
-  }
+  };
+  // FP32: loaded before the transform, so that their latency hides behind
+  // it; the kernel stays within 128 registers, at the four waves per SIMD
+  // its LDS allows.  FP64 rows take twice the registers and load after the
+  // transform, which measures faster for them.
+  constexpr bool early_add = sizeof(T) == 4;
+  if constexpr (early_add) load_add();
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int idx = threadIdx.x + q * kThreads;
@@ -563,6 +568,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   __syncthreads();
   // tile -> rows, scaled, plus the added grid
   if (a.xeven) {
+    if constexpr (!early_add) load_add();
     // the real grid is the transform's final output, not read by the next
     // pass: streaming stores leave the memory-side cache to the passes
 #pragma unroll
