@@ -585,14 +585,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       }
     }
   } else {
-    T w[E];
+    T wr[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int idx = threadIdx.x + q * kThreads;
-      w[q] = (T)0;
+      wr[q] = (T)0;
       if (add && idx < PL) {
         const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr) w[q] = __builtin_nontemporal_load(add + (size_t)c * a.rnxp + n);
+        if (c < nr) wr[q] = __builtin_nontemporal_load(add + (size_t)c * a.rnxp + n);
       }
     }
 #pragma unroll
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
       const int idx = threadIdx.x + q * kThreads;
       if (idx < PL) {
         const int c = fdiv(idx, a.divL), n = idx - c * L;
-        if (c < nr) __builtin_nontemporal_store(lds[n * S + c].x * a.scale + w[q], dst0 + (size_t)c * a.rnxp + n);
+        if (c < nr) __builtin_nontemporal_store(lds[n * S + c].x * a.scale + wr[q], dst0 + (size_t)c * a.rnxp + n);
       }
     }
   }
