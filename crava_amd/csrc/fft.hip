@@ -393,8 +393,6 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   const int L = a.f.L, K = a.cnxp;
   C* twl = lds + a.tile;
   C* wpl = twl + a.f.twn;
-  copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
-  if (a.xeven) copy_to_lds<T>(wpl, a.wpost, K);
   const int row0 = xcd_remap(blockIdx.x, gridDim.x, a.rev) << a.pshift;
   const int nr = min(P, a.nrows - row0);
   const size_t boff = (size_t)blockIdx.y * a.bstride;
@@ -423,6 +421,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
       }
     }
   }
+  // the twiddles go to LDS while the rows are in flight
+  copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
+  if (a.xeven) copy_to_lds<T>(wpl, a.wpost, K);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int idx = threadIdx.x + q * kThreads;
