@@ -47,6 +47,8 @@ int crv_ctx_create(int device, crv_ctx** out) {
     c->num_cu = ncu;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CRV_E_HIP; }
   c->own_stream = true;
+  // a failed lookup is reported by the allocation-free entry point itself
+  (void)crv_noalloc_setup(device);
   *out = c;
   return CRV_OK;
 }

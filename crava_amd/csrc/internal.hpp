@@ -188,6 +188,10 @@ int crv_reference_normals(crv_ctx* ctx, unsigned int* state, long long nnormals)
 // Fill one grid from ctx->normals starting at draw offset `first`.
 int crv_launch_fill_noise(crv_ctx* ctx, FftPlan* plan, crv_grid* g, long long first);
 
+// Per-device address of the staging array of crv_fill_prior_spectra_noalloc
+// (synth.hip); called by crv_ctx_create with the device current.
+int crv_noalloc_setup(int device);
+
 // FFT scale factors by grid type and precision (capi.hip)
 double crv_scale_forward(const crv_grid* g);
 double crv_scale_inverse(const crv_grid* g);

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -122,41 +123,112 @@ __global__ void k_moments(const T* g, int nx, int ny, int nz, int rnxp, int nyp,
   }
 }
 
-}  // namespace
-
-extern "C" int crv_fill_prior_spectra(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct,
-                                      crv_grid* cov[6]) {
-  if (!ctx) return CRV_E_INVALID;
+// Argument checks shared by the two prior-spectra entry points.
+int check_prior_args(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct, crv_grid* cov[6]) {
   if (!sigma0 || !cxy || !ct || !cov) return crv_set_error(ctx, CRV_E_INVALID, "NULL argument");
   crv_grid* g0 = cov[0];
   for (int q = 0; q < 6; ++q)
     if (!cov[q] || cov[q]->nxp != g0->nxp || cov[q]->nyp != g0->nyp || cov[q]->nzp != g0->nzp ||
         cov[q]->precision != g0->precision)
       return crv_set_error(ctx, CRV_E_INVALID, "cov grids inconsistent");
-  const size_t bx = (size_t)g0->cnxp * g0->nyp * 8, bt = (size_t)g0->nzp * 8;
-  double* d = nullptr;
-  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&d, bx + bt, ctx->stream));
-  CRV_HIP_CHECK(ctx, hipMemcpyAsync(d, cxy, bx, hipMemcpyHostToDevice, ctx->stream));
-  CRV_HIP_CHECK(ctx, hipMemcpyAsync((char*)d + bx, ct, bt, hipMemcpyHostToDevice, ctx->stream));
+  return CRV_OK;
+}
+
+// Copies cxy and ct to d (device, cnxp*nyp + nzp doubles) and fills the six
+// spectra from there; stream-ordered, so d may be released or reused by a
+// later call on the same stream.
+int launch_prior_spectra(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct, crv_grid* cov[6],
+                         double* d) {
+  crv_grid* g0 = cov[0];
+  const size_t nxy = (size_t)g0->cnxp * g0->nyp;
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(d, cxy, nxy * 8, hipMemcpyHostToDevice, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(d + nxy, ct, (size_t)g0->nzp * 8, hipMemcpyHostToDevice, ctx->stream));
   const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
   double s[6];
   for (int q = 0; q < 6; ++q) s[q] = sigma0[pa[q] * 3 + pb[q]];
   const long long nc = (long long)g0->csize;
-  {
-    ProfScope ps(ctx, "prior_spectra");
-    if (g0->precision == CRV_FP64)
-      hipLaunchKernelGGL(k_prior_spectra<double>, dim3(nblocks(nc)), dim3(kThreads), 0, ctx->stream,
-                         (double2*)cov[0]->d, (double2*)cov[1]->d, (double2*)cov[2]->d, (double2*)cov[3]->d,
-                         (double2*)cov[4]->d, (double2*)cov[5]->d, d, (const double*)((char*)d + bx), s[0], s[1],
-                         s[2], s[3], s[4], s[5], g0->cnxp, g0->nyp, nc);
-    else
-      hipLaunchKernelGGL(k_prior_spectra<float>, dim3(nblocks(nc)), dim3(kThreads), 0, ctx->stream,
-                         (float2*)cov[0]->d, (float2*)cov[1]->d, (float2*)cov[2]->d, (float2*)cov[3]->d,
-                         (float2*)cov[4]->d, (float2*)cov[5]->d, d, (const double*)((char*)d + bx), s[0], s[1],
-                         s[2], s[3], s[4], s[5], g0->cnxp, g0->nyp, nc);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
+  ProfScope ps(ctx, "prior_spectra");
+  if (g0->precision == CRV_FP64)
+    hipLaunchKernelGGL(k_prior_spectra<double>, dim3(nblocks(nc)), dim3(kThreads), 0, ctx->stream,
+                       (double2*)cov[0]->d, (double2*)cov[1]->d, (double2*)cov[2]->d, (double2*)cov[3]->d,
+                       (double2*)cov[4]->d, (double2*)cov[5]->d, d, d + nxy, s[0], s[1], s[2], s[3], s[4], s[5],
+                       g0->cnxp, g0->nyp, nc);
+  else
+    hipLaunchKernelGGL(k_prior_spectra<float>, dim3(nblocks(nc)), dim3(kThreads), 0, ctx->stream,
+                       (float2*)cov[0]->d, (float2*)cov[1]->d, (float2*)cov[2]->d, (float2*)cov[3]->d,
+                       (float2*)cov[4]->d, (float2*)cov[5]->d, d, d + nxy, s[0], s[1], s[2], s[3], s[4], s[5],
+                       g0->cnxp, g0->nyp, nc);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
+// Staging area of crv_fill_prior_spectra_noalloc: a file-scope device array
+// (each GPU holds its own copy until the process ends).  In host code its
+// name is only a host-side stand-in; host code uses the per-device address
+// that hipGetSymbolAddress returns (crv_noalloc_setup) instead.
+constexpr size_t kPriorScratchDoubles = (size_t)1 << 19;   // 4 MiB
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+int g_scratch_state[kMaxDevices] = {};   // 0 not looked up, 1 ok, -1 lookup failed
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kPriorScratchDoubles];
+
+// Called by crv_ctx_create with `device` current: looks up this device's
+// address of the staging array once per device and keeps it.
+int crv_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return CRV_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_state[device] == 0) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+      g_scratch_ptr[device] = (double*)p;
+      g_scratch_state[device] = 1;
+    } else {
+      g_scratch_state[device] = -1;
+    }
   }
+  return g_scratch_state[device] == 1 ? CRV_OK : CRV_E_HIP;
+}
+
+extern "C" int crv_fill_prior_spectra(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct,
+                                      crv_grid* cov[6]) {
+  if (!ctx) return CRV_E_INVALID;
+  int rc = check_prior_args(ctx, sigma0, cxy, ct, cov);
+  if (rc != CRV_OK) return rc;
+  crv_grid* g0 = cov[0];
+  double* d = nullptr;
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&d, ((size_t)g0->cnxp * g0->nyp + g0->nzp) * 8, ctx->stream));
+  rc = launch_prior_spectra(ctx, sigma0, cxy, ct, cov, d);
   CRV_HIP_CHECK(ctx, hipFreeAsync(d, ctx->stream));
+  if (rc != CRV_OK) return rc;
+  for (int q = 0; q < 6; ++q) { cov[q]->transformed = true; cov[q]->cubetype = CRV_COVARIANCE; }
+  return CRV_OK;
+}
+
+extern "C" int crv_fill_prior_spectra_noalloc(crv_ctx* ctx, const double* sigma0, const double* cxy,
+                                              const double* ct, crv_grid* cov[6]) {
+  if (!ctx) return CRV_E_INVALID;
+  int rc = check_prior_args(ctx, sigma0, cxy, ct, cov);
+  if (rc != CRV_OK) return rc;
+  crv_grid* g0 = cov[0];
+  const size_t need = (size_t)g0->cnxp * g0->nyp + g0->nzp;
+  if (need > kPriorScratchDoubles)
+    return crv_set_error(ctx, CRV_E_UNSUPPORTED,
+                         "fill_prior_spectra_noalloc: cnxp*nyp + nzp = " + std::to_string(need) +
+                             " doubles exceeds the staging area of " + std::to_string(kPriorScratchDoubles));
+  double* d = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < kMaxDevices && g_scratch_state[ctx->device] == 1)
+      d = g_scratch_ptr[ctx->device];
+  }
+  if (!d) return crv_set_error(ctx, CRV_E_HIP, "fill_prior_spectra_noalloc: staging area address unavailable");
+  rc = launch_prior_spectra(ctx, sigma0, cxy, ct, cov, d);
+  if (rc != CRV_OK) return rc;
   for (int q = 0; q < 6; ++q) { cov[q]->transformed = true; cov[q]->cubetype = CRV_COVARIANCE; }
   return CRV_OK;
 }

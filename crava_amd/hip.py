@@ -254,13 +254,15 @@ def simulate(ctx: Context, cov, post_mean, out, *, noise_mode=L.CRV_NOISE_COUNTE
     return st.value
 
 
-def fill_prior_spectra(ctx: Context, sigma0, cxy, ct, cov):
+def fill_prior_spectra(ctx: Context, sigma0, cxy, ct, cov, noalloc=False):
+    """crv_fill_prior_spectra, or with noalloc its allocation-free variant."""
     s0 = np.ascontiguousarray(sigma0, np.float64)
     cx = np.ascontiguousarray(cxy, np.float64)
     c_t = np.ascontiguousarray(ct, np.float64)
     covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
-    ctx.check(ctx.lib.crv_fill_prior_spectra(ctx.h, _arr(s0, C.c_double), _arr(cx, C.c_double),
-                                             _arr(c_t, C.c_double), covs), "fill_prior_spectra")
+    fn = ctx.lib.crv_fill_prior_spectra_noalloc if noalloc else ctx.lib.crv_fill_prior_spectra
+    ctx.check(fn(ctx.h, _arr(s0, C.c_double), _arr(cx, C.c_double), _arr(c_t, C.c_double), covs),
+              "fill_prior_spectra")
 
 
 def grid_fill(g: FFTGrid, value: float):

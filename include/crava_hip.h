@@ -382,6 +382,17 @@ long long crv_noise_draws(int nxp, int nyp, int nzp);
  * COVARIANCE.  cxy is cnxp*nyp doubles (row ky), ct is nzp doubles.        */
 int crv_fill_prior_spectra(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct,
                            crv_grid* cov[6]);
+/* crv_fill_prior_spectra without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  cxy and ct are staged in a fixed device array of 4 MiB per
+ * GPU (kept until the process ends) instead of a per-call allocation; a grid
+ * with cnxp*nyp + nzp above 524288 is refused with CRV_E_UNSUPPORTED.  The
+ * results equal crv_fill_prior_spectra's bit for bit.  Every call on one GPU
+ * shares that array: calls on different streams of the same GPU that
+ * overlap in time race (calls on one context's stream are ordered).        */
+int crv_fill_prior_spectra_noalloc(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct,
+                                   crv_grid* cov[6]);
 /* Real grid set to value in columns i < nxp, 0 in the pad (untransformed).
  * FFTGrid::fillInConstant (fftgrid.cpp:875-892) with a zero pad.           */
 int crv_grid_fill(crv_grid* g, double value);
