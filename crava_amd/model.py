@@ -64,6 +64,8 @@ class Survey:
     seed_truth: int = 1
     seed_noise: int = 2
     seed_sim: int = 0
+    phase_deg: tuple = ()                # per-angle constant phase rotation (empty: zero phase)
+    err_phase_deg: tuple = ()            # phase of the error-norm wavelets (empty: same as the forward ones)
     nx: int = 0
     ny: int = 0
     nz: int = 0
@@ -148,6 +150,30 @@ def config_small(nxp=12, nyp=10, nzp=16, n=3) -> Survey:
     return Survey(nxp, nyp, nzp, angles_deg=ang, peak_hz=(30.0,) * n, sn=(5.0,) * n)
 
 
+PHASES_DEG = (0.0, 25.0, -40.0, 60.0, 90.0, -75.0, 15.0, -120.0)
+
+
+ERR_PHASES_DEG = (10.0, -30.0, 45.0, 20.0, -60.0, 80.0, -5.0, 35.0)
+
+
+def config_phase(nxp=20, nyp=18, nzp=30, n=5, own_error_wavelets=False) -> Survey:
+    """Wavelets that differ between angles in peak frequency (35 -> 28 Hz
+    from near to far, as the production survey) and in phase (each Ricker
+    rotated by its own constant phase, as wavelets estimated from wells
+    are): the forward operators and the error covariance of different
+    angles then differ in phase (Im Sigma_e != 0 off the diagonal).  With one
+    wavelet per angle the phases cancel from the posterior covariance
+    (K = conj(W) D1 A and e1 = conj(W)/|W| share W), which stays real; with
+    own_error_wavelets the error terms use wavelets of other phases, as
+    Wavelet3D's error norm uses the average wavelet instead of the local one
+    (reference src/wavelet3D.cpp:449-460), and the posterior covariance is
+    complex as well."""
+    ang = tuple(np.linspace(5.0, 33.0, n))
+    peaks = tuple(35.0 - 7.0 * i / max(n - 1, 1) for i in range(n))
+    return Survey(nxp, nyp, nzp, angles_deg=ang, peak_hz=peaks, sn=(4.0,) * n, phase_deg=PHASES_DEG[:n],
+                  err_phase_deg=ERR_PHASES_DEG[:n] if own_error_wavelets else ())
+
+
 # ----------------------------------------------------------------------------
 # padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622)
 def find_closest_factorable(leastint: int) -> int:
@@ -192,12 +218,23 @@ def ricker(t_ms, peak_hz):
     return (1.0 - 2.0 * c) * np.exp(-c)
 
 
-def wavelet_fft_order(nzp: int, dt: float, peak_hz: float) -> np.ndarray:
-    """Resampled Ricker in FFT order (float32), zero outside +-119 ms."""
+def wavelet_fft_order(nzp: int, dt: float, peak_hz: float, phase_deg: float = 0.0) -> np.ndarray:
+    """Resampled Ricker in FFT order (float32), zero outside +-119 ms,
+    optionally rotated by a constant phase: w cos(p) - H[w] sin(p) with H the
+    discrete Hilbert transform (multiplier -i sgn(f), 0 at DC and Nyquist), so
+    the rotated wavelet is real and its spectrum is W(f) e^{i p sgn(f)} away
+    from DC and Nyquist."""
     k = np.arange(nzp)
     t = np.where(k < nzp // 2 + 1, dt * k, dt * (k - nzp)).astype(np.float32).astype(np.float64)
     w = ricker(t, peak_hz)
     w[np.abs(t) > 119.0] = 0.0
+    if phase_deg:
+        p = math.radians(phase_deg)
+        f = np.fft.fftfreq(nzp)
+        mult = math.cos(p) + 1j * math.sin(p) * np.sign(f)
+        if nzp % 2 == 0:
+            mult[nzp // 2] = math.cos(p)
+        w = np.fft.ifft(np.fft.fft(w) * mult).real
     return w.astype(np.float32)
 
 
@@ -264,15 +301,19 @@ def kz_tables(s: Survey) -> KzTables:
     em3 = np.ones((nzp, n), complex)
     wl = []
     for l in range(n):
-        w = wavelet_fft_order(nzp, s.dt, s.peak_hz[l])
+        w = wavelet_fft_order(nzp, s.dt, s.peak_hz[l], s.phase_deg[l] if s.phase_deg else 0.0)
         wl.append(w)
         W = full_spectrum(w)
-        nw = norm_time(w)
-        e3 = diff_central(diff_backward(diff_forward(w)))
+        kwd[:, l] = D1 * np.conj(W)
+        # the error norm's wavelet (createWavelet1DForErrorNorm): the same one,
+        # or for own error wavelets one of another phase
+        we = w if not s.err_phase_deg else wavelet_fft_order(nzp, s.dt, s.peak_hz[l], s.err_phase_deg[l])
+        We = full_spectrum(we)
+        nw = norm_time(we)
+        e3 = diff_central(diff_backward(diff_forward(we)))
         E3 = full_spectrum(e3)
         n3 = norm_time(e3)
-        kwd[:, l] = D1 * np.conj(W)
-        em1[:, l] = np.conj(W) / nw
+        em1[:, l] = np.conj(We) / nw
         em2[:, l] = np.conj(E3) / n3
     return KzTables(kwd, em1, em2, em3, wl)
 

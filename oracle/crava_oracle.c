@@ -291,6 +291,8 @@ void or_axeqb_flat_f(int n, float* L, float* B, int ncol) {
 
 /* ------------------------------------------------------ posterior loops */
 #define PL_NAME or_posterior_f
+#define PL_ERRVAR or_errvar_f
+#define PL_PARVAR or_parvar_f
 #define PL_REAL float
 #define PL_CPX ocf
 #define PL_PROD or_prod_f
@@ -306,6 +308,8 @@ void or_axeqb_flat_f(int n, float* L, float* B, int ncol) {
 #define PL_ADDVEC or_addvec_f
 #include "post_loop.inc"
 #undef PL_NAME
+#undef PL_ERRVAR
+#undef PL_PARVAR
 #undef PL_REAL
 #undef PL_CPX
 #undef PL_PROD
@@ -321,6 +325,8 @@ void or_axeqb_flat_f(int n, float* L, float* B, int ncol) {
 #undef PL_ADDVEC
 
 #define PL_NAME or_posterior_d
+#define PL_ERRVAR or_errvar_d
+#define PL_PARVAR or_parvar_d
 #define PL_REAL double
 #define PL_CPX ocd
 #define PL_PROD or_prod_d
@@ -335,6 +341,28 @@ void or_axeqb_flat_f(int n, float* L, float* B, int ncol) {
 #define PL_ADJMATVEC or_adjmatvec_d
 #define PL_ADDVEC or_addvec_d
 #include "post_loop.inc"
+
+/* The posterior loop's own error-variance and parameter-covariance helpers,
+   exposed for direct checks.  e1, e2, e3: n complex (interleaved), theta
+   n x n, out: n x n complex (interleaved, row-major); covre: the real parts
+   of the six covariance spectra, out 3 x 3 complex. */
+void or_error_variance_d(int n, const double* e1, const double* e2, const double* e3, const double* theta,
+                         double lam, float wnc, double* out) {
+  ocd *rows[8];
+  for (int l = 0; l < n; l++) rows[l] = (ocd*)(out + 2 * n * l);
+  or_errvar_d(n, (const ocd*)e1, (const ocd*)e2, (const ocd*)e3, theta, fabs(lam), wnc, rows);
+}
+void or_error_variance_f(int n, const float* e1, const float* e2, const float* e3, const double* theta, float lam,
+                         float wnc, float* out) {
+  ocf *rows[8];
+  for (int l = 0; l < n; l++) rows[l] = (ocf*)(out + 2 * n * l);
+  or_errvar_f(n, (const ocf*)e1, (const ocf*)e2, (const ocf*)e3, theta, (float)sqrt(lam * lam), wnc, rows);
+}
+void or_parameter_covariance_d(const double* covre, const double* sigma0, double* out) {
+  ocd *rows[3];
+  for (int a = 0; a < 3; a++) rows[a] = (ocd*)(out + 6 * a);
+  or_parvar_d(covre, sigma0, rows);
+}
 
 /* ------------------------------------------ simulate scaling (crava.cpp:1617-1656) */
 #define DEFINE_SIMSCALE(SUF, R, C)                                                                    \

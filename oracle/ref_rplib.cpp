@@ -1,10 +1,16 @@
 // ORACLE (test infrastructure only).  C entry points over the reference's
-// rplib DEM and fluid routines, compiled from rplib/dem.cpp,
-// rplib/orddiffeqsolver.cpp and rplib/demmodelling.cpp into
-// oracle/_ref/libcravaref.so by oracle/Makefile.  No reference code here.
+// rplib DEM, fluid and Gassmann routines, compiled from rplib/dem.cpp,
+// rplib/orddiffeqsolver.cpp, rplib/demmodelling.cpp, rplib/rockgassmann.cpp
+// and the rock/dry-rock/fluid classes it is built from (rplib/rock.cpp,
+// dryrock.cpp, fluid.cpp, dryrocktabulatedmodulus.cpp,
+// fluidtabulatedmodulus.cpp) into oracle/_ref/libcravaref.so by
+// oracle/Makefile.  No reference code here.
 #include <vector>
 
 #include "rplib/demmodelling.h"
+#include "rplib/dryrocktabulatedmodulus.h"
+#include "rplib/fluidtabulatedmodulus.h"
+#include "rplib/rockgassmann.h"
 
 extern "C" {
 
@@ -33,6 +39,22 @@ double ref_hill(int n, const double* prop, const double* frac) {
 
 void ref_seismic_from_elastic(double k, double mu, double rho, double* vp, double* vs) {
   DEMTools::CalcSeismicParamsFromElasticParams(k, mu, rho, *vp, *vs);
+}
+
+// RockGassmann(fluid, dryrock) -> ComputeSeismicAndElasticParams
+// (rockgassmann.cpp:8-17,79-97) on a dry rock and a fluid of given moduli,
+// as the reference builds a Gassmann rock from DistributionsRockGassmann.
+// Units as rplib: moduli kPa, density g/cm3, velocities m/s.
+void ref_gassmann(double k_dry, double mu_dry, double rho_dry, double porosity, double k_mineral, double k_fluid,
+                  double rho_fluid, double* out /* k, mu, rho, vp, vs */) {
+  const std::vector<double> u;
+  DryRockTabulatedModulus dry(k_dry, mu_dry, rho_dry, porosity, k_mineral, u);
+  FluidTabulatedModulus fluid(k_fluid, rho_fluid, u);
+  RockGassmann rock(&fluid, &dry);
+  double k, mu, rho, vp, vs, rho2;
+  rock.GetElasticParams(k, mu, rho);
+  rock.GetSeismicParams(vp, vs, rho2);
+  out[0] = k; out[1] = mu; out[2] = rho; out[3] = vp; out[4] = vs;
 }
 
 }  // extern "C"

@@ -96,6 +96,8 @@ void ref_axeqb_flat(int n, float* L, float* B, int ncol) {
 }
 
 #define PL_NAME ref_posterior
+#define PL_ERRVAR ref_errvar
+#define PL_PARVAR ref_parvar
 #define PL_REAL float
 #define PL_CPX fftw_complex
 #define PL_PROD lib_matrProdCpx

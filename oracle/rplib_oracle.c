@@ -193,6 +193,40 @@ void or_gas_props(double T, double P, double G, double* rho, double* k_kpa) {
   *k_kpa = k_mpa * 1E3;
 }
 
+/* ---------------------------------------------- mixing and substitution */
+/* Hill average of n constituents (rplib/demmodelling.cpp:178-236): the mean
+   of the Voigt (arithmetic) and Reuss (harmonic) averages by volume fraction. */
+double or_hill(int n, const double* prop, const double* frac) {
+  double v = 0.0, r = 0.0;
+  int zero = 0;
+  for (int i = 0; i < n; i++) {
+    v += frac[i] * prop[i];
+    if (prop[i] == 0.0) zero = 1;
+  }
+  /* the Reuss average is 0 when a constituent has zero modulus */
+  if (!zero) {
+    for (int i = 0; i < n; i++) r += frac[i] / prop[i];
+    r = 1.0 / r;
+  }
+  return 0.5 * (r + v);
+}
+
+/* RockGassmann::ComputeSeismicAndElasticParams in the reference's form
+   (rplib/rockgassmann.cpp:79-97), including its "+ k_dry / K_min^2" term in
+   the denominator; shear modulus unchanged, density rho_dry + phi rho_fl,
+   then vp, vs (rplib/demmodelling.cpp:338-354).  out: k, mu, rho, vp, vs. */
+void or_gassmann(double k_dry, double mu_dry, double rho_dry, double phi, double k_min, double k_fl, double rho_fl,
+                 double* out) {
+  const double t1 = (1 - k_dry / k_min) * (1 - k_dry / k_min);
+  const double k = k_dry + t1 / (phi / k_fl + (1 - phi) / k_min + k_dry / (k_min * k_min));
+  const double rho = rho_dry + phi * rho_fl;
+  out[0] = k;
+  out[1] = mu_dry;
+  out[2] = rho;
+  out[3] = sqrt((k + 4.0 / 3.0 * mu_dry) / rho);
+  out[4] = sqrt(mu_dry / rho);
+}
+
 /* ------------------------------------------------------ one rock sample */
 typedef struct {
   double kq, gq, rhoq;     /* quartz: kPa, kPa, g/cm3 */
@@ -210,9 +244,8 @@ int or_rock_forward(const or_rock_params* p, const double in[5], double out[3]) 
   const double phi = in[0], clay = in[1], sw = in[2], asp = in[3], pe = in[4];
   /* mineral: Hill average of quartz and clay */
   const double vq = 1.0 - clay, vc = clay;
-  const double kv = vq * p->kq + vc * p->kc, kr = 1.0 / (vq / p->kq + vc / p->kc);
-  const double gv = vq * p->gq + vc * p->gc, gr = 1.0 / (vq / p->gq + vc / p->gc);
-  const double kmin = 0.5 * (kv + kr), gmin = 0.5 * (gv + gr);
+  const double fr[2] = {vq, vc}, ks[2] = {p->kq, p->kc}, gs[2] = {p->gq, p->gc};
+  const double kmin = or_hill(2, ks, fr), gmin = or_hill(2, gs, fr);
   const double rhomin = vq * p->rhoq + vc * p->rhoc;
   /* dry frame: DEM with empty spheroidal pores of aspect asp */
   double k0 = 0.0, g0 = 0.0, a = asp, c = phi, kd, gd;
@@ -224,15 +257,13 @@ int or_rock_forward(const or_rock_params* p, const double in[5], double out[3]) 
   or_gas_props(p->temp_c, pe, p->gas_gravity, &rg, &kg);
   const double kfl = 1.0 / (sw / kb + (1.0 - sw) / kg);
   const double rfl = sw * rb + (1.0 - sw) * rg;
-  /* Gassmann in the reference's form (rockgassmann.cpp:90) */
-  const double t1 = (1 - kd / kmin) * (1 - kd / kmin);
-  const double k = kd + t1 / (phi / kfl + (1 - phi) / kmin + kd / (kmin * kmin));
-  const double rho = (1 - phi) * rhomin + phi * rfl;
-  const double vp = sqrt((k + 4.0 / 3.0 * gd) / rho);
-  const double vs = sqrt(gd / rho);
-  out[0] = log(vp);
-  out[1] = log(vs);
-  out[2] = log(rho * 1000.0);
+  /* Gassmann in the reference's form (rockgassmann.cpp:90); the dry frame
+     of empty pores has density (1 - phi) rho_mineral */
+  double gm[5];
+  or_gassmann(kd, gd, (1 - phi) * rhomin, phi, kmin, kfl, rfl, gm);
+  out[0] = log(gm[3]);
+  out[1] = log(gm[4]);
+  out[2] = log(gm[2] * 1000.0);
   return 0;
 }
 

@@ -159,6 +159,45 @@ def main() -> None:
     g["brine_out"] = np.array([[r.ref_brine_velocity(*t), r.ref_brine_density(*t), r.ref_brine_modulus(*t)]
                                for t in tps])
 
+    # rplib: Hill mix (demmodelling.cpp:178-236) and Gassmann
+    # (rockgassmann.cpp:79-97) from the compiled reference classes
+    r.ref_hill.restype = C.c_double
+    r.ref_hill.argtypes = [C.c_int, dp, dp]
+    r.ref_gassmann.argtypes = [C.c_double] * 7 + [dp]
+    hill_in, hill_out = [], []
+    while len(hill_in) < 200:
+        c = float(rng.uniform(0.0, 1.0))
+        if (1.0 - c) + c != 1.0:          # the reference asserts fractions summing to exactly 1
+            continue
+        p1, p2 = rng.uniform(2e6, 4.5e7, 2)
+        if len(hill_in) % 50 == 0:
+            p2 = 0.0                       # a zero modulus: Reuss average 0
+        P = (C.c_double * 2)(p1, p2)
+        F = (C.c_double * 2)(1.0 - c, c)
+        hill_in.append([p1, p2, 1.0 - c, c])
+        hill_out.append(r.ref_hill(2, P, F))
+    g["hill_in"], g["hill_out"] = np.array(hill_in), np.array(hill_out)
+    gas_in, gas_out = [], []
+    for t in range(300):
+        kmin = rng.uniform(1.5e7, 4.0e7)
+        kdry = rng.uniform(0.02, 0.98) * kmin
+        mudry = rng.uniform(0.1, 1.2) * kdry
+        phi = rng.uniform(0.01, 0.4)
+        rhodry = (1 - phi) * rng.uniform(2.4, 2.8)
+        kfl = 10 ** rng.uniform(4.0, 6.6)
+        rhofl = rng.uniform(0.05, 1.2)
+        if t == 0:
+            kdry = 0.0
+        elif t == 1:
+            kdry = kmin
+        elif t == 2:
+            phi = 1e-9
+        out = np.zeros(5)
+        r.ref_gassmann(kdry, mudry, rhodry, phi, kmin, kfl, rhofl, out.ctypes.data_as(dp))
+        gas_in.append([kdry, mudry, rhodry, phi, kmin, kfl, rhofl])
+        gas_out.append(out)
+    g["gassmann_in"], g["gassmann_out"] = np.array(gas_in), np.array(gas_out)
+
     np.savez_compressed(OUT, **g)
     print(f"wrote {OUT} ({OUT.stat().st_size / 1024:.0f} KiB, {len(g)} arrays)")
 

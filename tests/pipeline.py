@@ -44,14 +44,25 @@ def gpu_posterior(ctx, case: Case, prec=32, mode=0, separable=False, skip_mean_i
     return dict(mean=mean, data=data, cov=cov, stats=st)
 
 
-def oracle_posterior(case: Case, dtype=np.float64, nthreads=4):
+def oracle_posterior(case: Case, dtype=np.float64, nthreads=4, separable=False):
     """Oracle pipeline: FP64 transforms (numpy) + the restated loop in dtype.
-    Returns real posterior means, residual spectra and covariance spectra."""
+    Returns real posterior means, residual spectra and covariance spectra.
+    separable: the prior covariance and error spectra are the exact products
+    Sigma0[a][b] cxy ct and cxy (what a separable plan uses) instead of the
+    transforms of the float real-space grids, whose rounding leaves a noise
+    floor of about 1e-8 of the peak in the spectra."""
     s = case.s
     mean = [O.fft_r2c(m, s.nxp, 2) for m in case.syn.mean]
     data = [O.fft_r2c(d, s.nxp, 1) for d in case.syn.data]
-    err = O.fft_r2c(case.err_real, s.nxp, 3)
-    cov = [O.fft_r2c(c, s.nxp, 3) for c in case.cov_real]
+    if separable:
+        cxy, ct = model.prior_spectra(s)
+        base = ct[:, None, None] * cxy[None, :, :]
+        S0 = case.pin.sigma0
+        cov = [S0[a, b] * base + 0j for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+        err = np.broadcast_to(cxy[None, :, :], base.shape) + 0j
+    else:
+        err = O.fft_r2c(case.err_real, s.nxp, 3)
+        cov = [O.fft_r2c(c, s.nxp, 3) for c in case.cov_real]
     t = case.pin.tables
     pm, pd, pc, nf = O.posterior_loop(mean, data, err, cov, kwd=t.kwd, em1=t.em1, em2=t.em2, em3=t.em3,
                                       A=case.pin.A, theta=case.pin.theta, sigma0=case.pin.sigma0,

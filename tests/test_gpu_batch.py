@@ -297,10 +297,13 @@ def test_autotuned_shapes_keep_the_transform(ctx, dims):
     assert rel_rms(b.download_real()[:, :, :nx], x[:, :, :nx]) < 1e-6
 
 
-@pytest.mark.parametrize("chunk_rows,batched,src,n", [(0, False, False, 5), (7, False, False, 5),
-                                                      (1, True, False, 5), (0, True, True, 5), (7, True, True, 5),
-                                                      (5, False, True, 5), (0, True, False, 6), (3, True, False, 2)])
-def test_invert_matches_sequence(ctx, chunk_rows, batched, src, n):
+@pytest.mark.parametrize("chunk_rows,batched,src,n,dims",
+                         [(0, False, False, 5, (64, 48, 40)), (7, False, False, 5, (64, 48, 40)),
+                          (1, True, False, 5, (64, 48, 40)), (0, True, True, 5, (64, 48, 40)),
+                          (7, True, True, 5, (64, 48, 40)), (5, False, True, 5, (64, 48, 40)),
+                          (0, True, False, 6, (64, 48, 40)), (3, True, False, 2, (64, 48, 40)),
+                          (0, True, False, 5, (21, 18, 30)), (4, False, True, 3, (21, 18, 30))])
+def test_invert_matches_sequence(ctx, chunk_rows, batched, src, n, dims):
     """crv_invert (transforms, posterior, realizations and inverse transforms,
     whole grids or pipelined by chunks of y rows) equals the one-call
     sequence bit for bit, but for the realizations of the whole-grid call:
@@ -308,7 +311,7 @@ def test_invert_matches_sequence(ctx, chunk_rows, batched, src, n):
     sqrt(lam) times the plane's factor of the prior S, where the simulation
     kernel factors the stored FP32 product lam S (rel RMS below 1e-6)."""
     from crava_amd.hip import FFTGrid, PostPlan, fft_c2r_batch, grid_batch, simulate_batch
-    case = Case(model.config_small(64, 48, 40, n))
+    case = Case(model.config_small(*dims, n))
     s = case.s
     _, _, kw = _inputs(ctx, case)
     plan = PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=1, **kw)

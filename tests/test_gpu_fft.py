@@ -76,12 +76,15 @@ def test_fft_vs_fp64_oracle(ctx, shape, prec):
     assert rel_rms(y, x[:, :, :nx]) < tol
 
 
+@pytest.mark.parametrize("shape", [(24, 18, 20), (21, 18, 20), (343, 6, 5), (450, 8, 6)])
 @pytest.mark.parametrize("prec", [32, 64])
-def test_out_of_place_and_fused_add(ctx, prec):
-    shape = (24, 18, 20)
+def test_out_of_place_and_fused_add(ctx, prec, shape):
+    """Out-of-place R2C and the C2R that adds a real grid in its last pass,
+    over even and odd x lengths (odd lengths take the C2R pass's own
+    branch for the added values)."""
     nx, ny, nz = shape
     rng = np.random.default_rng(5)
-    x = np.zeros((nz, ny, 26))
+    x = np.zeros((nz, ny, 2 * (nx // 2 + 1)))
     x[:, :, :nx] = rng.standard_normal((nz, ny, nx))
     src = _grid(ctx, shape, prec, 2)
     dst = _grid(ctx, shape, prec, 2)

@@ -48,8 +48,27 @@ def test_noise_fill_long_stream(ctx, shape, seed):
     assert st == st_ref
     got = g.download_complex()
     mism = np.count_nonzero(got.astype(np.complex64) != ref.astype(np.complex64))
-    # exp/log of the rejection tails may differ in the last double ulp
-    assert mism <= max(2, ref.size // 10 ** 6)
+    print(f"noise {shape} seed {seed}: {mism} of {ref.size} values differ")
+    assert mism == 0
+
+
+def test_noise_fill_c1_nyquist_plane(ctx):
+    """The reference stream at the 450x450x320 grid (cnxp 226, even): every
+    point of the x = 225 plane is "special" under the global-parity rule
+    (fftgrid.cpp:1201), so the plane is exactly Hermitian and the fill draws
+    N normals: the final LCG state equals the oracle stream's after N draws."""
+    from crava_amd.hip import FFTGrid
+    nx, ny, nz = 450, 450, 320
+    g = FFTGrid(ctx, nx, ny, nz, precision=32)
+    st = g.fillInComplexNoise(231171)
+    _, st_ref = O.rnorm_seq(231171, nx * ny * nz)
+    assert st == st_ref
+    c = g.download_complex()
+    p = c[:, :, -1]
+    pc = np.conj(p[(-np.arange(nz)) % nz][:, (-np.arange(ny)) % ny])
+    assert np.array_equal(p, pc)
+    p0 = c[:, :, 0]
+    assert np.array_equal(p0, np.conj(p0[(-np.arange(nz)) % nz][:, (-np.arange(ny)) % ny]))
 
 
 def _sim_case():

@@ -70,6 +70,44 @@ def test_noise_hermitian_where_paired():
     assert np.allclose(p, np.conj(p[kk][:, jj]))
 
 
+def _nyquist_pairs(c, nyp, nzp):
+    """For the x = cnxp - 1 plane: mask of (k, j) whose value is the exact
+    conjugate of its Hermitian partner's (nyp - j, nzp - k)."""
+    p = c[:, :, -1]
+    jj = (-np.arange(nyp)) % nyp
+    kk = (-np.arange(nzp)) % nzp
+    return p == np.conj(p[kk][:, jj])
+
+
+def test_noise_nyquist_parity_quirk_tables():
+    """fillInComplexNoise treats x = cnxp - 1 as a Hermitian ("special") column
+    only where the global complex index is odd (fftgrid.cpp:1201), not where
+    nxp is even.  Index i = x + cnxp (j + nyp k), so:
+      * cnxp odd (the 64x64x128 grid, cnxp 33): i is odd exactly where j is
+        odd (nyp even); the odd-j half of the x = 32 plane pairs with its
+        conjugates (partners of odd j have odd j), the even-j half draws two
+        independent values per point, so the fill uses N + 64*128/2 draws;
+      * cnxp even (the 450x450x320 grid, cnxp 226): i is always odd at
+        x = 225, the whole plane is Hermitian and a real field's N draws
+        suffice.
+    A rule keyed on nxp parity gives N draws for both, one keyed on x alone
+    makes the 64x64x128 plane fully Hermitian."""
+    assert O.noise_draws(64, 64, 128) == 64 * 64 * 128 + 64 * 128 // 2
+    assert O.noise_draws(450, 450, 320) == 450 * 450 * 320
+    c, _ = O.fill_complex_noise(64, 64, 128, 3, np.float64)
+    herm = _nyquist_pairs(c, 64, 128)
+    odd = (np.arange(64) % 2 == 1)[None, :].repeat(128, 0)
+    assert np.all(herm[odd])
+    assert not np.any(herm[~odd])
+    # x = 0 is Hermitian everywhere, whatever the parity
+    p0 = c[:, :, 0]
+    assert np.array_equal(p0, np.conj(p0[(-np.arange(128)) % 128][:, (-np.arange(64)) % 64]))
+    # an even cnxp (nxp 10, cnxp 6) at small size: the whole plane is Hermitian
+    c, _ = O.fill_complex_noise(10, 8, 12, 5, np.float64)
+    assert np.all(_nyquist_pairs(c, 8, 12))
+    assert O.noise_draws(10, 8, 12) == 10 * 8 * 12
+
+
 # ------------------------------------------------------------- lib_matr
 def test_cholesky_bit_exact(golden):
     for a, f, L in zip(golden["chol_in"], golden["chol_flag"], golden["chol_L"]):
@@ -167,6 +205,158 @@ def test_posterior_fp64_matches_closed_form(golden):
         assert np.allclose(got, want, rtol=1e-8, atol=1e-14 * np.abs(Sm).max())
         checked += 1
     assert checked > 50
+
+
+def _errvar(e1, e2, e3, theta, lam, wnc):
+    """The oracle loop's getNextErrorVariance (FP64 instance) on one
+    wavenumber; returns the n x n complex Sigma_e."""
+    import ctypes as C
+    o = O.olib()
+    dp = C.POINTER(C.c_double)
+    o.or_error_variance_d.argtypes = [C.c_int, dp, dp, dp, dp, C.c_double, C.c_float, dp]
+    n = len(e1)
+    il = lambda z: np.ascontiguousarray(np.stack([np.real(z), np.imag(z)], -1), np.float64)
+    a, b, c = il(e1), il(e2), il(e3)
+    th = np.ascontiguousarray(theta, np.float64)
+    out = np.zeros((n, n, 2))
+    o.or_error_variance_d(n, a.ctypes.data_as(dp), b.ctypes.data_as(dp), c.ctypes.data_as(dp), th.ctypes.data_as(dp),
+                          float(lam), float(wnc), out.ctypes.data_as(dp))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def _theta(rng, n):
+    """errThetaCov-shaped: sigma_l sigma_m exp(-|theta_l - theta_m| / r)."""
+    sd = rng.uniform(0.5, 2.0, n)
+    ang = np.sort(rng.uniform(0, 0.6, n))
+    return np.outer(sd, sd) * np.exp(-np.abs(ang[:, None] - ang[None, :]) / 0.17)
+
+
+def test_error_variance_hermitian_psd():
+    """Sigma_e = 0.5 (1 - wnc) lam theta o (e1 e1^H + e2 e2^H) + wnc diag(theta_ll Re(e3_l)^2)
+    is a sum of Schur products of positive semi-definite matrices: Hermitian
+    and PSD for any multipliers (crava.cpp:1441-1469)."""
+    rng = np.random.default_rng(21)
+    for n in (1, 3, 5, 8):
+        for _ in range(20):
+            e1 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            e2 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            e3 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            S = _errvar(e1, e2, e3, _theta(rng, n), rng.uniform(0, 3), rng.uniform(0, 1))
+            assert np.allclose(S, S.conj().T, rtol=0, atol=1e-14 * np.abs(S).max())
+            assert np.all(np.diag(S).imag == 0)
+            w = np.linalg.eigvalsh(0.5 * (S + S.conj().T))
+            assert w.min() >= -1e-12 * w.max()
+
+
+def test_error_variance_white_noise_only():
+    """wnc = 1 leaves the white part alone: diag(theta_ll Re(e3_l)^2)."""
+    rng = np.random.default_rng(22)
+    n = 5
+    e = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3)]
+    th = _theta(rng, n)
+    S = _errvar(*e, th, 1.7, 1.0)
+    assert np.allclose(S, np.diag(np.diag(th) * e[2].real ** 2), rtol=1e-14, atol=0)
+
+
+def test_error_variance_phase_differences():
+    """e1_l = a_l exp(i phi_l), e2 = 0: the coloured part is
+    0.5 (1 - wnc) lam theta_lm a_l a_m exp(i (phi_l - phi_m)), so its imaginary
+    part is 0.5 (1 - wnc) lam theta_lm a_l a_m sin(phi_l - phi_m) -- the sign and
+    the conjugated index a transposed operand or a dropped conj would flip."""
+    rng = np.random.default_rng(23)
+    n = 5
+    a = rng.uniform(0.2, 2.0, n)
+    phi = rng.uniform(-np.pi, np.pi, n)
+    e1 = a * np.exp(1j * phi)
+    e3 = rng.standard_normal(n) + 0j
+    th = _theta(rng, n)
+    lam, wnc = 0.8, float(np.float32(0.3))      # wnc is a float in the reference
+    S = _errvar(e1, np.zeros(n, complex), e3, th, lam, wnc)
+    c = 0.5 * (1 - wnc) * lam * th * np.outer(a, a)
+    dphi = phi[:, None] - phi[None, :]
+    assert np.allclose(S.imag, c * np.sin(dphi), rtol=1e-12, atol=1e-15)
+    assert np.allclose(S.real, c * np.cos(dphi) + np.diag(wnc * np.diag(th) * e3.real ** 2), rtol=1e-12,
+                       atol=1e-15)
+    # lam enters as |Re errCorr|
+    assert np.allclose(_errvar(e1, np.zeros(n, complex), e3, th, -lam, wnc), S, rtol=1e-15, atol=0)
+
+
+def test_parameter_covariance_cleaning():
+    """getParameterCovariance (seismicparametersholder.cpp:408-426) on the
+    spectrum of a separable prior, c_ab(k) = Sigma0_ab s(k) with any real s:
+    the result is Sigma0 |s| (a PSD matrix, imaginary part 0) -- the sign of
+    the spectrum is dropped, that of Sigma0 kept; imaginary input is ignored."""
+    import ctypes as C
+    o = O.olib()
+    dp = C.POINTER(C.c_double)
+    o.or_parameter_covariance_d.argtypes = [dp, dp, dp]
+    rng = np.random.default_rng(24)
+    G = rng.standard_normal((3, 3))
+    S0 = G @ G.T + 0.1 * np.eye(3)
+    S0[0, 2] = S0[2, 0] = -abs(S0[0, 2])          # a negative correlation
+    pa, pb = (0, 1, 2, 0, 0, 1), (0, 1, 2, 1, 2, 2)
+    for sc in (2.5, -0.7, 1e-6, -3e3):
+        covre = np.array([S0[pa[q], pb[q]] * sc for q in range(6)])
+        out = np.zeros((3, 3, 2))
+        o.or_parameter_covariance_d(covre.ctypes.data_as(dp), np.ascontiguousarray(S0).ctypes.data_as(dp),
+                                    out.ctypes.data_as(dp))
+        P = out[..., 0] + 1j * out[..., 1]
+        assert np.allclose(P, S0 * abs(sc), rtol=1e-14, atol=0)
+        assert np.all(P.imag == 0)
+        assert np.linalg.eigvalsh(P.real).min() > 0
+
+
+@pytest.mark.parametrize("own_error_wavelets", [False, True])
+def test_posterior_with_phase_matches_closed_form(own_error_wavelets):
+    """Per-angle wavelets with different peak frequencies and phases make
+    Sigma_e complex: the FP64 loop still equals the closed-form Gaussian
+    update built with numpy from the same multipliers.  When the error terms
+    use the forward wavelets, the wavelet phases cancel from the posterior
+    covariance, which is then real up to rounding (a dropped or misplaced
+    conjugate would leave an imaginary part); with error wavelets of other
+    phases it is complex."""
+    from pipeline import Case
+    case = Case(model.config_phase(12, 10, 24, 5, own_error_wavelets=own_error_wavelets))
+    s, p = case.s, case.pin
+    mean = [O.fft_r2c(m, s.nxp, 2) for m in case.syn.mean]
+    data = [O.fft_r2c(d, s.nxp, 1) for d in case.syn.data]
+    err = O.fft_r2c(case.err_real, s.nxp, 3)
+    cov = [O.fft_r2c(c, s.nxp, 3) for c in case.cov_real]
+    t = p.tables
+    pm, pd, pc, nf = O.posterior_loop(mean, data, err, cov, kwd=t.kwd, em1=t.em1, em2=t.em2, em3=t.em3, A=p.A,
+                                      theta=p.theta, sigma0=p.sigma0, invert=p.invert, wnc=s.wnc, dtype=np.float64)
+    assert nf == 0
+    pa, pb = (0, 1, 2, 0, 0, 1), (0, 1, 2, 1, 2, 2)
+    wnc = float(np.float32(s.wnc))
+    rng = np.random.default_rng(4)
+    checked, imag_peak, imag_err = 0, 0.0, 0.0
+    for _ in range(200):
+        k, j, i = rng.integers(s.nzp), rng.integers(s.nyp), rng.integers(s.cnxp)
+        if not p.invert[k]:
+            continue
+        Sm = np.zeros((3, 3))
+        for q in range(6):
+            Sm[pa[q], pb[q]] = Sm[pb[q], pa[q]] = abs(cov[q][k, j, i].real)
+        Sm *= np.sign(p.sigma0)
+        K = t.kwd[k][:, None] * p.A.astype(np.float64)
+        Se = _errvar(t.em1[k], t.em2[k], t.em3[k], p.theta, err[k, j, i].real, wnc)
+        imag_err = max(imag_err, float(np.abs(Se.imag).max() / np.abs(Se).max()))
+        mu = np.array([m[k, j, i] for m in mean])
+        d = np.array([x[k, j, i] for x in data])
+        mp, Sp, res = O.gaussian_update(Sm, K, Se, mu, d)
+        assert np.allclose([m[k, j, i] for m in pm], mp, rtol=1e-9, atol=1e-12)
+        assert np.allclose([x[k, j, i] for x in pd], res, rtol=1e-9, atol=1e-12)
+        got = np.array([c[k, j, i] for c in pc])
+        want = np.array([Sp[pa[q], pb[q]] for q in range(6)])
+        assert np.allclose(got, want, rtol=1e-8, atol=1e-13 * np.abs(Sm).max())
+        imag_peak = max(imag_peak, float(np.abs(got.imag).max() / np.abs(got).max()))
+        checked += 1
+    assert checked > 50
+    assert imag_err > 0.05
+    if own_error_wavelets:
+        assert imag_peak > 1e-3
+    else:
+        assert imag_peak < 1e-9
 
 
 def test_posterior_fp32_close_to_fp64(golden):
@@ -354,6 +544,84 @@ def test_gassmann_and_gas_limits():
     r2, k2 = C.c_double(), C.c_double()
     o.or_gas_props(80.0, 40.0, 0.6, C.byref(r2), C.byref(k2))
     assert r2.value > rho.value and k2.value > k.value
+
+
+def _gm():
+    import ctypes as C
+    o = O.olib()
+    dp = C.POINTER(C.c_double)
+    o.or_hill.restype = C.c_double
+    o.or_hill.argtypes = [C.c_int, dp, dp]
+    o.or_gassmann.argtypes = [C.c_double] * 7 + [dp]
+
+    def gassmann(*a):
+        out = np.zeros(5)
+        o.or_gassmann(*[float(v) for v in a], out.ctypes.data_as(dp))
+        return out
+
+    def hill(prop, frac):
+        return o.or_hill(len(prop), (C.c_double * len(prop))(*prop), (C.c_double * len(frac))(*frac))
+    return gassmann, hill
+
+
+def test_hill_bit_exact_vs_reference(golden):
+    """DEMTools::CalcEffectiveElasticModuliUsingHill compiled from the
+    reference (200 two-mineral mixes, zero moduli included)."""
+    _, hill = _gm()
+    for (p1, p2, f1, f2), want in zip(golden["hill_in"], golden["hill_out"]):
+        assert hill([p1, p2], [f1, f2]) == want
+
+
+def test_hill_closed_forms():
+    """Hill = mean of Voigt (arithmetic) and Reuss (harmonic) averages; one
+    constituent gives its own modulus."""
+    _, hill = _gm()
+    rng = np.random.default_rng(8)
+    for _ in range(50):
+        p = rng.uniform(1e6, 5e7, 3)
+        f = rng.dirichlet([1.0, 1.0, 1.0])
+        voigt = float(np.dot(f, p))
+        reuss = 1.0 / float(np.dot(f, 1.0 / p))
+        assert hill(list(p), list(f)) == pytest.approx(0.5 * (voigt + reuss), rel=1e-14)
+        assert hill([p[0]], [1.0]) == pytest.approx(p[0], rel=1e-15)
+
+
+def test_gassmann_bit_exact_vs_reference(golden):
+    """RockGassmann::ComputeSeismicAndElasticParams compiled from the
+    reference on tabulated dry rocks and fluids (300 cases, including
+    K_dry = 0, K_dry = K_min and porosity -> 0)."""
+    gassmann, _ = _gm()
+    for a, want in zip(golden["gassmann_in"], golden["gassmann_out"]):
+        assert np.array_equal(gassmann(*a), want)
+
+
+def test_gassmann_closed_forms():
+    """Limits that the reference's form fixes (its denominator carries
+    + K_dry / K_min^2):
+      K_dry = 0      -> the Reuss average of fluid and mineral,
+      K_dry = K_min  -> K_min,
+      phi -> 0       -> K_dry + (K_min - K_dry)^2 / (K_min + K_dry)
+    (the textbook form, with - K_dry / K_min^2, would give K_min there);
+    the shear modulus is unchanged and the density is rho_dry + phi rho_fl."""
+    gassmann, _ = _gm()
+    rng = np.random.default_rng(9)
+    for _ in range(40):
+        kmin = rng.uniform(1.5e7, 4e7)
+        kdry = rng.uniform(0.05, 0.95) * kmin
+        mu = rng.uniform(0.2, 1.0) * kdry
+        phi = rng.uniform(0.05, 0.35)
+        rho, kfl, rfl = 2.0, 10 ** rng.uniform(4, 6.5), rng.uniform(0.1, 1.1)
+        k0 = gassmann(0.0, mu, rho, phi, kmin, kfl, rfl)[0]
+        assert k0 == pytest.approx(1.0 / (phi / kfl + (1 - phi) / kmin), rel=1e-13)
+        assert gassmann(kmin, mu, rho, phi, kmin, kfl, rfl)[0] == pytest.approx(kmin, rel=1e-15)
+        out = gassmann(kdry, mu, rho, 1e-12, kmin, kfl, rfl)
+        assert out[0] == pytest.approx(kdry + (kmin - kdry) ** 2 / (kmin + kdry), rel=1e-7)
+        out = gassmann(kdry, mu, rho, phi, kmin, kfl, rfl)
+        assert out[1] == mu and out[2] == pytest.approx(rho + phi * rfl, rel=1e-15)
+        assert out[3] == pytest.approx(math.sqrt((out[0] + 4 * mu / 3) / out[2]), rel=1e-15)
+        assert out[4] == pytest.approx(math.sqrt(mu / out[2]), rel=1e-15)
+        # a stiffer fluid stiffens the saturated rock
+        assert gassmann(kdry, mu, rho, phi, kmin, 2 * kfl, rfl)[0] > out[0]
 
 
 def test_philox_known_answers():
