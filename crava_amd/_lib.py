@@ -36,7 +36,7 @@ EXPORTS = [
     "crv_grid_derived", "crv_crava_header", "crv_grid_write_crava", "crv_grid_read_crava",
     "crv_storm_header", "crv_grid_write_storm",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
-    "crv_fill_prior_spectra_noalloc",
+    "crv_fill_prior_spectra_noalloc", "crv_circ_corr_t",
 ]
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
 CRV_TUNE_BATCH_MB = 6
@@ -158,6 +158,7 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_ctx_profile_report": (i, [vp, C.c_char_p, C.c_size_t]),
         "crv_fill_prior_spectra": (i, [vp, _dp, _dp, _dp, C.POINTER(vp)]),
         "crv_fill_prior_spectra_noalloc": (i, [vp, _dp, _dp, _dp, C.POINTER(vp)]),
+        "crv_circ_corr_t": (i, [C.POINTER(C.c_float), i, i, i, C.POINTER(C.c_float)]),
         "crv_grid_fill": (i, [vp, C.c_double]),
         "crv_forward_model": (i, [vp, i, C.POINTER(C.c_float), _dp, C.POINTER(vp), C.POINTER(vp)]),
         "crv_add_white_noise": (i, [vp, C.c_double, C.c_ulonglong, C.c_ulonglong]),

@@ -395,6 +395,51 @@ int crv_padding_size(int n, double pad_fac) {
   return crv_find_closest_factorable(leastint);
 }
 
+// Circular temporal correlation made positive definite
+// (SeismicParametersHolder::computeCircCorrT + makeCircCorrTPosDef,
+// seismicparametersholder.cpp:430-498).  The transforms are direct DFTs in
+// double (nzp is a trace length); the reference's unnormalised transforms
+// and its final division by the lag-0 value give the same result up to
+// rounding.
+int crv_circ_corr_t(const float* corr, int ncorr, int nzp, int low_int_cut, float* out) {
+  if (!corr || !out || ncorr < 1 || nzp < 1 || corr[0] == 0.0f) return CRV_E_INVALID;
+  std::vector<double> c(nzp);
+  for (int k = 0; k < nzp; ++k) {
+    const int refk = k < nzp / 2 + 1 ? k : nzp - k;
+    c[k] = refk < ncorr ? (double)corr[refk] : 0.0;
+  }
+  // half spectrum of the even sequence; its modulus, zero up to the cut
+  const int nh = nzp / 2 + 1;
+  std::vector<double> a(nh);
+  const double w = 2.0 * M_PI / nzp;
+  for (int f = 0; f < nh; ++f) {
+    double re = 0.0, im = 0.0;
+    for (int t = 0; t < nzp; ++t) {
+      const long long r = ((long long)f * t) % nzp;
+      re += c[t] * std::cos(w * (double)r);
+      im -= c[t] * std::sin(w * (double)r);
+    }
+    a[f] = f <= low_int_cut ? 0.0 : std::sqrt(re * re + im * im);
+  }
+  // inverse real transform of the (real, even) half spectrum, then the
+  // division by the value at lag 0
+  std::vector<double> x(nzp);
+  for (int t = 0; t < nzp; ++t) {
+    double v = a[0];
+    for (int f = 1; f < nh; ++f) {
+      const long long r = ((long long)f * t) % nzp;
+      const double m = (2 * f == nzp) ? 1.0 : 2.0;
+      v += m * a[f] * std::cos(w * (double)r);
+    }
+    x[t] = v / nzp;
+  }
+  // the reference stops when the lag-0 value (of its unnormalised inverse,
+  // nzp times this one) is not above 1e-5: too few layers for the correlation
+  if (x[0] * nzp <= 1e-5) return CRV_E_INVALID;
+  for (int t = 0; t < nzp; ++t) out[t] = (float)(x[t] / x[0]);
+  return CRV_OK;
+}
+
 }  // extern "C"
 
 extern "C" int crv_ctx_profile(crv_ctx* ctx, int enable) {

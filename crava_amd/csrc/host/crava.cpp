@@ -28,6 +28,15 @@ void SeismicParametersHolder::invFFTCovGrids() {
     if (g->getIsTransformed()) g->invFFTInPlace();
 }
 
+std::vector<float> SeismicParametersHolder::computeCircCorrT(const std::vector<float>& priorCorrT, int minIntFq,
+                                                            int nzp) {
+  std::vector<float> out(nzp);
+  if (priorCorrT.empty() ||
+      crv_circ_corr_t(priorCorrT.data(), (int)priorCorrT.size(), nzp, minIntFq, out.data()) != CRV_OK)
+    throw HipError("computeCircCorrT: the circular temporal correlation is undefined (too few layers)");
+  return out;
+}
+
 int Crava::computePostMeanResidAndFFTCov(SeismicParametersHolder& sp, FFTGrid*& postAlpha, FFTGrid*& postBeta,
                                          FFTGrid*& postRho, FFTGrid*& errCorr, FFTGrid** seisData,
                                          const KzTables& t, double** errThetaCov, float** A, int ntheta,

@@ -30,6 +30,9 @@ class SeismicParametersHolder {
   const double* priorVar0() const { return priorVar0_; }
   void FFTCovGrids();     // seismicparametersholder.cpp:299-322
   void invFFTCovGrids();  // seismicparametersholder.cpp:273-296
+  // computeCircCorrT + makeCircCorrTPosDef (seismicparametersholder.cpp:
+  // 430-498): nzp values; throws when the correlation is undefined.
+  static std::vector<float> computeCircCorrT(const std::vector<float>& priorCorrT, int minIntFq, int nzp);
 
  private:
   FFTGrid* cov_[6];

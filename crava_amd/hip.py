@@ -300,6 +300,18 @@ def padding_size(n: int, frac: float) -> int:
     return int(L.lib().crv_padding_size(n, frac))
 
 
+def circ_corr_t(corr, nzp: int, low_int_cut: int) -> np.ndarray:
+    """SeismicParametersHolder::computeCircCorrT + makeCircCorrTPosDef
+    (crv_circ_corr_t): nzp floats; raises when the correlation is undefined."""
+    c = np.ascontiguousarray(corr, np.float32)
+    out = np.empty(nzp, np.float32)
+    rc = L.lib().crv_circ_corr_t(c.ctypes.data_as(C.POINTER(C.c_float)), c.size, nzp, int(low_int_cut),
+                                 out.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        raise CravaError(f"circ_corr_t failed ({rc}): circular temporal correlation undefined")
+    return out
+
+
 # rock physics constants of the learned-surrogate workload: quartz and clay
 # moduli in kPa, densities in g/cm3, fluids at 80 C
 ROCK_DEFAULTS = dict(k_quartz=37.0e6, g_quartz=44.0e6, rho_quartz=2.65, k_clay=21.0e6, g_clay=7.0e6,

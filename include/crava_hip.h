@@ -507,6 +507,18 @@ int crv_find_closest_factorable(int leastint);
 /* ModelGeneral::setPaddingSize: findClosestFactorable(ceil(n*(1+pad_fac))). */
 int crv_padding_size(int n, double pad_fac);
 
+/* ---- prior temporal correlation ------------------------------------------
+ * SeismicParametersHolder::computeCircCorrT + makeCircCorrTPosDef
+ * (seismicparametersholder.cpp:430-498): the prior temporal correlation
+ * corr (ncorr lags, corr[0] != 0) laid out circularly on nzp points (lag
+ * min(k, nzp - k), 0 beyond ncorr), its spectrum replaced by its modulus and
+ * set to 0 for frequency indices <= low_int_cut (ModelGeneral's lowIntCut =
+ * floor(lowCut nzp dt / 1000), modelgeneral.cpp:4081), transformed back and
+ * divided by its lag-0 value.  out: nzp floats.  CRV_E_INVALID when the
+ * unnormalised lag-0 value is not above 1e-5 (the reference aborts: too few
+ * layers).  Host only (no device work).                                  */
+int crv_circ_corr_t(const float* corr, int ncorr, int nzp, int low_int_cut, float* out);
+
 /* ---- output (next to the path) ------------------------------------------
  * Derived parameters of ParameterOutput::writeParameters
  * (src/parameteroutput.cpp:14-420) from the real log-parameter grids
