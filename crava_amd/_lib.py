@@ -36,8 +36,9 @@ EXPORTS = [
     "crv_grid_derived", "crv_crava_header", "crv_grid_write_crava", "crv_grid_read_crava",
     "crv_storm_header", "crv_grid_write_storm",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
-    "crv_fill_prior_spectra_noalloc", "crv_circ_corr_t",
+    "crv_fill_prior_spectra_noalloc", "crv_circ_corr_t", "crv_facies_density_kernel",
 ]
+CRV_SMOOTH_FOURIER, CRV_SMOOTH_SAMPLED = 0, 1
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
 CRV_TUNE_BATCH_MB = 6
 CRV_TUNE_ORDER = 7
@@ -191,6 +192,7 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_graph_launch": (i, [vp]),
         "crv_graph_destroy": (i, [vp]),
         "crv_facies_density": (i, [vp, vp, C.c_longlong, C.POINTER(FaciesBox), _dp, vp]),
+        "crv_facies_density_kernel": (i, [vp, vp, C.c_longlong, C.POINTER(FaciesBox), _dp, i, vp]),
         "crv_fft_r2c_batch": (i, [C.POINTER(vp), i]),
         "crv_fft_r2c_batch_from": (i, [C.POINTER(vp), C.POINTER(vp), i]),
         "crv_fft_c2r_batch": (i, [C.POINTER(vp), i]),

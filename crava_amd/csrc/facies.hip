@@ -103,6 +103,35 @@ __global__ void k_sum(const float* __restrict__ d, int nx, long long rows, int r
   if (threadIdx.x == 0) atomicAdd(out, sh[0]);
 }
 
+// PosteriorElasticPDF3D::SetupSmoothingGaussian3D (reference
+// src/posteriorelasticpdf3d.cpp:369-420): the Gaussian exp(-x^T Si x / 2)
+// sampled at the cyclic bin lags (j for j <= n/2, j - n above), stored in
+// float; the sum is taken separately and the kernel divided by it.
+__global__ void k_smoother(float* __restrict__ s, int nx, int ny, int nz, int rnx, double dx, double dy, double dz,
+                           double i00, double i11, double i22, double i10, double i20, double i21) {
+  const long long n = (long long)rnx * ny * nz;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % rnx);
+    const long long r = t / rnx;
+    const int k = (int)(r % ny), l = (int)(r / ny);
+    if (j >= nx) { s[t] = 0.0f; continue; }
+    const double x = (j <= nx / 2 ? j : j - nx) * dx;
+    const double y = (k <= ny / 2 ? k : k - ny) * dy;
+    const double z = (l <= nz / 2 ? l : l - nz) * dz;
+    const double q = x * x * i00 + y * y * i11 + z * z * i22 + 2.0 * (x * y * i10 + x * z * i20 + y * z * i21);
+    s[t] = (float)exp(-0.5 * q);
+  }
+}
+
+// spectrum of the histogram times spectrum of the kernel, the kernel's
+// normalisation folded in
+__global__ void k_cmul(float2* __restrict__ a, const float2* __restrict__ b, long long nc, float scale) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
+    const float2 x = a[i], y = b[i];
+    a[i] = make_float2((y.x * x.x - y.y * x.y) * scale, (y.y * x.x + y.x * x.y) * scale);
+  }
+}
+
 __device__ __forceinline__ float trilinear(const float* __restrict__ d, const Box& b, float v0, float v1, float v2) {
   float t[3] = {(v0 - b.lo[0]) * b.inv_w[0] - 0.5f, (v1 - b.lo[1]) * b.inv_w[1] - 0.5f,
                 (v2 - b.lo[2]) * b.inv_w[2] - 0.5f};
@@ -205,6 +234,94 @@ extern "C" int crv_facies_density(crv_ctx* ctx, const float* d_samples, long lon
                      scale);
   CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;
+}
+
+// PosteriorElasticPDF3D's density (reference src/posteriorelasticpdf3d.cpp:
+// 12-131): the histogram of the samples divided by their number, circularly
+// convolved with the sampled Gaussian kernel normalised to sum 1 (products of
+// unscaled transforms, 1/N on the inverse): a probability mass per bin that
+// sums to 1 over the box.  The kernel's sum is taken in double on the device.
+static int facies_density_sampled(crv_ctx* ctx, const float* d_samples, long long n, const crv_facies_box* box,
+                                  const double* sigma, crv_grid* density) {
+  // inverse of the 3x3 kernel covariance (InvertSquareMatrix), by cofactors
+  const double* a = sigma;
+  const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+  const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  if (!(det > 0.0)) return crv_set_error(ctx, CRV_E_INVALID, "facies_density: kernel covariance not positive definite");
+  double inv[9];
+  inv[0] = c00 / det; inv[1] = (a[2] * a[7] - a[1] * a[8]) / det; inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+  inv[3] = c01 / det; inv[4] = (a[0] * a[8] - a[2] * a[6]) / det; inv[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+  inv[6] = c02 / det; inv[7] = (a[1] * a[6] - a[0] * a[7]) / det; inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+  const Box b = make_box(box, density);
+  const int nx = density->nxp, ny = density->nyp, nz = density->nzp;
+  float* g = (float*)density->d;
+  FftPlan* plan = nullptr;
+  int rc = crv_get_plan(ctx, nx, ny, nz, CRV_FP32, &plan);
+  if (rc != CRV_OK) return rc;
+  const size_t rbytes = density->rsize * sizeof(float);
+  float* sm = nullptr;
+  double* dsum = nullptr;
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&sm, rbytes, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dsum, sizeof(double), ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMemsetAsync(g, 0, rbytes, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMemsetAsync(dsum, 0, sizeof(double), ctx->stream));
+  {
+    ProfScope ps(ctx, "facies_hist");
+    if (n > 0) hipLaunchKernelGGL(k_hist, dim3(nblocks(n)), dim3(kThreads), 0, ctx->stream, d_samples, n, b, g);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  const double dx = (box->hi[0] - box->lo[0]) / nx, dy = (box->hi[1] - box->lo[1]) / ny,
+               dz = (box->hi[2] - box->lo[2]) / nz;
+  {
+    ProfScope ps(ctx, "facies_kernel");
+    hipLaunchKernelGGL(k_smoother, dim3(nblocks((long long)density->rsize)), dim3(kThreads), 0, ctx->stream, sm, nx,
+                       ny, nz, density->rnxp, dx, dy, dz, inv[0], inv[4], inv[8], inv[3], inv[6], inv[7]);
+    hipLaunchKernelGGL(k_sum, dim3(256), dim3(kThreads), 0, ctx->stream, (const float*)sm, nx, (long long)ny * nz,
+                       density->rnxp, dsum);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  double ksum = 0.0;
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(&ksum, dsum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!(ksum > 0.0)) {
+    (void)hipFreeAsync(sm, ctx->stream);
+    (void)hipFreeAsync(dsum, ctx->stream);
+    return crv_set_error(ctx, CRV_E_INVALID, "facies_density: empty smoothing kernel");
+  }
+  // unscaled forward transforms; the product carries 1/(n sum N): the
+  // histogram's 1/n, the kernel's 1/sum and the inverse transform's 1/N
+  rc = crv_launch_r2c(ctx, plan, g, g, 1.0);
+  if (rc == CRV_OK) rc = crv_launch_r2c(ctx, plan, sm, sm, 1.0);
+  if (rc == CRV_OK) {
+    ProfScope ps(ctx, "facies_smooth");
+    const double N = (double)nx * ny * nz;
+    const float scale = (float)(1.0 / ((n > 0 ? (double)n : 1.0) * ksum * N));
+    hipLaunchKernelGGL(k_cmul, dim3(nblocks((long long)density->csize)), dim3(kThreads), 0, ctx->stream, (float2*)g,
+                       (const float2*)sm, (long long)density->csize, scale);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = crv_set_error(ctx, CRV_E_HIP, hipGetErrorString(e));
+  }
+  if (rc == CRV_OK) rc = crv_launch_c2r(ctx, plan, g, g, 1.0, nullptr);
+  CRV_HIP_CHECK(ctx, hipFreeAsync(sm, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipFreeAsync(dsum, ctx->stream));
+  if (rc != CRV_OK) return rc;
+  density->cubetype = CRV_PARAMETER;
+  density->transformed = false;
+  return CRV_OK;
+}
+
+extern "C" int crv_facies_density_kernel(crv_ctx* ctx, const float* d_samples, long long n,
+                                         const crv_facies_box* box, const double* smooth_cov, int kernel,
+                                         crv_grid* density) {
+  if (kernel == CRV_SMOOTH_FOURIER) return crv_facies_density(ctx, d_samples, n, box, smooth_cov, density);
+  if (!ctx) return CRV_E_INVALID;
+  if (kernel != CRV_SMOOTH_SAMPLED) return crv_set_error(ctx, CRV_E_INVALID, "facies_density: unknown kernel");
+  if (!box || !density || !smooth_cov || n < 0 || (n > 0 && !d_samples))
+    return crv_set_error(ctx, CRV_E_INVALID, "facies_density: bad args");
+  if (density->precision != CRV_FP32) return crv_set_error(ctx, CRV_E_INVALID, "facies_density: FP32 grid needed");
+  for (int d = 0; d < 3; ++d)
+    if (!(box->hi[d] > box->lo[d])) return crv_set_error(ctx, CRV_E_INVALID, "facies_density: empty box");
+  return facies_density_sampled(ctx, d_samples, n, box, smooth_cov, density);
 }
 
 extern "C" int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,

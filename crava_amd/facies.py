@@ -23,17 +23,20 @@ def make_box(lo, hi) -> L.FaciesBox:
     return b
 
 
-def density(ctx: Context, d_samples, box: L.FaciesBox, smooth_cov, bins=(100, 100, 50)) -> FFTGrid:
-    """Smoothed, normalised density of device samples (n, 3) float on the box
-    (a torch tensor: torch's work on it ends before the library's stream
-    reads it, and the library is done with it on return)."""
+def density(ctx: Context, d_samples, box: L.FaciesBox, smooth_cov, bins=(100, 100, 50),
+            kernel=L.CRV_SMOOTH_FOURIER) -> FFTGrid:
+    """Smoothed density of device samples (n, 3) float on the box (a torch
+    tensor: torch's work on it ends before the library's stream reads it, and
+    the library is done with it on return).  kernel CRV_SMOOTH_FOURIER: the
+    Gaussian's transform, normalised to unit integral; CRV_SMOOTH_SAMPLED:
+    the reference's sampled kernel, a mass per bin summing to 1."""
     import torch
     g = FFTGrid(ctx, bins[0], bins[1], bins[2], precision=32, cubetype=L.CRV_OPERATOR)
     S = np.ascontiguousarray(smooth_cov, np.float64).reshape(3, 3)
     torch.cuda.synchronize()
-    ctx.check(ctx.lib.crv_facies_density(ctx.h, C.c_void_p(d_samples.data_ptr()), int(d_samples.shape[0]),
-                                         C.byref(box), S.ctypes.data_as(C.POINTER(C.c_double)), g.h),
-              "facies_density")
+    ctx.check(ctx.lib.crv_facies_density_kernel(ctx.h, C.c_void_p(d_samples.data_ptr()), int(d_samples.shape[0]),
+                                                C.byref(box), S.ctypes.data_as(C.POINTER(C.c_double)), int(kernel),
+                                                g.h), "facies_density")
     ctx.synchronize()
     return g
 

@@ -500,6 +500,22 @@ int crv_facies_density(crv_ctx* ctx, const float* d_samples, long long n, const 
                        const double* smooth_cov, crv_grid* density);
 int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
                     const crv_facies_box* box, crv_grid* mean[3], crv_grid** prob);
+/* The density with a choice of smoothing kernel:
+ *   CRV_SMOOTH_FOURIER  crv_facies_density (the Gaussian's Fourier transform,
+ *                       clamped at 0, normalised to unit integral);
+ *   CRV_SMOOTH_SAMPLED  PosteriorElasticPDF3D's estimate (reference
+ *                       src/posteriorelasticpdf3d.cpp:12-131, 369-420): the
+ *                       histogram divided by n, circularly convolved with the
+ *                       Gaussian exp(-x^T S^-1 x / 2) sampled at the bins'
+ *                       cyclic lags and normalised to sum 1 -- a probability
+ *                       mass per bin that sums to 1 (densities of one box
+ *                       differ from it by the bin volume only, so
+ *                       crv_facies_prob takes either).  Samples outside the
+ *                       box are skipped (the reference does not check).  */
+#define CRV_SMOOTH_FOURIER 0
+#define CRV_SMOOTH_SAMPLED 1
+int crv_facies_density_kernel(crv_ctx* ctx, const float* d_samples, long long n, const crv_facies_box* box,
+                              const double* smooth_cov, int kernel, crv_grid* density);
 
 /* ---- padding (src/fftgrid.cpp:2857-2885, src/modelgeneral.cpp:616-622) -- */
 /* Smallest 2^a 3^b 5^c 7^d >= leastint (FFTGrid::findClosestFactorableNumber). */

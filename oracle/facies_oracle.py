@@ -70,3 +70,32 @@ def probabilities(dens, prior, lo, hi, bins, m):
     q = np.stack([p * trilinear(d, lo, hi, bins, m) for p, d in zip(pr, dens)], 0)
     s = q.sum(0)
     return np.where(s > 0, q / np.where(s > 0, s, 1.0), 1.0 / len(dens))
+
+
+def density_pdf3d(samples, lo, hi, sigma, bins):
+    """PosteriorElasticPDF3D's density (reference
+    src/posteriorelasticpdf3d.cpp:12-131, 369-420), (nz, ny, nx): the
+    histogram of the samples (bin floor((d - lo) / w)) times 1/n, circularly
+    convolved with exp(-x^T sigma^-1 x / 2) sampled at the cyclic bin lags
+    (j for j <= n/2, j - n above) as float and divided by its float running
+    sum in the reference's loop order (z, y outer, x fastest).  Samples
+    outside the box are dropped (the reference writes past its grid)."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    nb = np.asarray(bins)
+    w = (hi - lo) / nb
+    s = np.asarray(samples, np.float64)
+    idx = np.floor((s - lo) / w).astype(np.int64)
+    ok = np.all((idx >= 0) & (idx < nb), axis=1)
+    h = np.zeros((bins[2], bins[1], bins[0]))
+    np.add.at(h, (idx[ok, 2], idx[ok, 1], idx[ok, 0]), 1.0)
+    h = h.astype(np.float32) * np.float32(1.0 / len(s))
+    si = np.linalg.inv(np.asarray(sigma, np.float64).reshape(3, 3))
+    lag = lambda n: np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)
+    z, y, x = np.meshgrid(lag(bins[2]) * w[2], lag(bins[1]) * w[1], lag(bins[0]) * w[0], indexing="ij")
+    q = (x * x * si[0, 0] + y * y * si[1, 1] + z * z * si[2, 2]
+         + 2 * x * y * si[1, 0] + 2 * x * z * si[2, 0] + 2 * y * z * si[2, 1])
+    k = np.exp(-0.5 * q).astype(np.float32)
+    ksum = np.cumsum(k.ravel(), dtype=np.float32)[-1]
+    k = k / ksum
+    return np.fft.irfftn(np.fft.rfftn(h.astype(np.float64)) * np.fft.rfftn(k.astype(np.float64)), s=h.shape)

@@ -35,6 +35,34 @@ def test_density_matches_oracle(ctx, bins):
     assert np.max(np.abs(got - want)) / want.max() < 2e-4
 
 
+@pytest.mark.parametrize("bins", [(40, 36, 30), (100, 100, 50)])
+def test_sampled_kernel_density_matches_reference_restatement(ctx, bins):
+    """PosteriorElasticPDF3D's density (histogram / n convolved with the
+    sampled, sum-normalised Gaussian kernel; reference
+    src/posteriorelasticpdf3d.cpp:12-131, 369-420) against its CPU
+    restatement.  The restatement keeps the reference's float running sum of
+    the kernel, the library sums it in double: a common scale factor that
+    differs by less than 1e-4.  The result is a mass per bin: its sum is 1."""
+    import torch
+    from crava_amd import _lib as L, facies
+    x, mu, C = _cloud(200_000, 4)
+    lo, hi = mu - 6 * np.sqrt(np.diag(C)), mu + 6 * np.sqrt(np.diag(C))
+    S = 0.3 * C
+    box = facies.make_box(lo, hi)
+    d_x = torch.from_numpy(x).to(torch.device("cuda", 0))
+    torch.cuda.synchronize()
+    g = facies.density(ctx, d_x, box, S, bins, kernel=L.CRV_SMOOTH_SAMPLED)
+    got = g.download_real()[:, :, :bins[0]].astype(np.float64)
+    want = F.density_pdf3d(x, lo, hi, S, bins)
+    assert np.max(np.abs(got - want)) / want.max() < 1e-4
+    assert got.sum() == pytest.approx(1.0, abs=1e-4)
+    # against the Fourier-Gaussian mode: the same smoothing up to the
+    # kernel's sampling (its width is 2-3 bins here) and the unit change
+    f = facies.density(ctx, d_x, box, S, bins).download_real()[:, :, :bins[0]].astype(np.float64)
+    cell = np.prod((hi - lo) / np.array(bins))
+    assert np.max(np.abs(f * cell - got)) / got.max() < 2e-2
+
+
 def test_probabilities_match_oracle(ctx):
     import torch
     from crava_amd import facies

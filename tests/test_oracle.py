@@ -712,6 +712,40 @@ def test_facies_density_gaussian_closed_form():
     assert d.sum() * np.prod(w) == pytest.approx(1.0, rel=1e-12)
 
 
+def test_facies_density_pdf3d_closed_forms():
+    """The restated PosteriorElasticPDF3D density: the kernel of a single
+    sample is the sampled Gaussian itself centred on the sample's bin (sum
+    1, its ratios exp(-x^T S^-1 x / 2)); a convolution with a sum-1 kernel keeps
+    the total mass 1; a Gaussian cloud gives approximately the Gaussian mass
+    of covariance C + S."""
+    from oracle import facies_oracle as F
+    lo, hi, bins = np.array([0.0, 0.0, 0.0]), np.array([1.0, 1.2, 0.8]), (20, 24, 16)
+    w = (hi - lo) / np.array(bins)
+    S = np.array([[0.004, 0.001, 0.0], [0.001, 0.006, 0.0005], [0.0, 0.0005, 0.002]])
+    pt = lo + (np.array([7, 9, 5]) + 0.5) * w
+    d = F.density_pdf3d(pt[None, :], lo, hi, S, bins)
+    # 1 up to the reference's float running sum of the kernel
+    assert d.sum() == pytest.approx(1.0, rel=1e-4)
+    Si = np.linalg.inv(S)
+    for off in ((0, 0, 0), (1, 0, 0), (0, -2, 1), (3, 2, -1)):
+        x = np.array(off) * w
+        ratio = d[5 + off[2], 9 + off[1], 7 + off[0]] / d[5, 9, 7]
+        assert ratio == pytest.approx(math.exp(-0.5 * x @ Si @ x), rel=1e-5)
+    rng = np.random.default_rng(6)
+    C = np.array([[0.01, 0.003, 0.0], [0.003, 0.012, 0.001], [0.0, 0.001, 0.005]])
+    mu = (lo + hi) / 2
+    cloud = rng.multivariate_normal(mu, C, 400_000)
+    d = F.density_pdf3d(cloud, lo, hi, S, bins)
+    assert d.sum() == pytest.approx(1.0, rel=1e-4)
+    T = C + S + np.diag(w ** 2 / 12)
+    cz, cy, cx = np.meshgrid(*[lo[k] + (np.arange(bins[k]) + 0.5) * w[k] for k in (2, 1, 0)], indexing="ij")
+    p = np.stack([cx, cy, cz], -1) - mu
+    quad = np.einsum("...i,ij,...j->...", p, np.linalg.inv(T), p)
+    mass = np.exp(-0.5 * quad) / np.sqrt((2 * np.pi) ** 3 * np.linalg.det(T)) * np.prod(w)
+    core = quad < 1.0
+    assert np.max(np.abs(d[core] - mass[core]) / mass[core]) < 0.05
+
+
 def test_facies_probabilities_logistic():
     """Two Gaussian facies densities with one covariance T: P(1 | m) is the
     logistic function of the linear discriminant."""
