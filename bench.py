@@ -74,7 +74,7 @@ def step(ctx, G, kw, mode, realization, seed):
     plan.exec(G["mean"], G["data"], None, G["cov"])
     hip.fft_c2r_batch(G["mean"])
     hip.simulate_batch(ctx, G["cov"], G["mean"], G["sim"], seed=seed, realization0=realization,
-                       fp64_chol=(mode == L.CRV_MODE_FP32_REFINE))
+                       fp64_chol=2 if mode == L.CRV_MODE_FP32_REFINE else 0)
     hip.fft_c2r_batch(G["cov"])
 
 
@@ -128,7 +128,7 @@ def general_leg(ctx, s, sv, kw, steps, warmup):
         hip.fft_r2c_batch(err, src=err_in)
         plan.exec(G["mean"], G["data"], err[0], G["cov"])
         hip.fft_c2r_batch(G["mean"])
-        hip.simulate_batch(ctx, G["cov"], G["mean"], G["sim"], seed=s.seed_sim, realization0=r, fp64_chol=True)
+        hip.simulate_batch(ctx, G["cov"], G["mean"], G["sim"], seed=s.seed_sim, realization0=r, fp64_chol=2)
         hip.fft_c2r_batch(G["cov"])
 
     for r in range(warmup):
@@ -502,7 +502,7 @@ def bench_c3(a):
         plan.exec(G["mean"], G["data"], None, G["cov"])
         for m in G["mean"]:
             m.invFFTInPlace()
-        hip.simulate_batch(ctx, G["cov"], G["mean"], G["sims"], seed=s.seed_sim, realization0=0, fp64_chol=True)
+        hip.simulate_batch(ctx, G["cov"], G["mean"], G["sims"], seed=s.seed_sim, realization0=0, fp64_chol=2)
         for c in G["cov"]:
             c.invFFTInPlace()
 

@@ -26,6 +26,7 @@
 // multiply-shift division, no runtime modulo.
 #include "internal.hpp"
 #include "dft.hpp"
+#include "eiggen.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -608,6 +609,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
 }
 
 // ---- y / z passes: strided complex pencils ---------------------------------
+constexpr int kRecMaxNGen = 6;   // angles of the eigen-basis records
 template <typename T>
 struct ColArgs {
   cplx_t<T>* data;
@@ -623,20 +625,25 @@ struct ColArgs {
   int rev;             // walk the tiles backwards (xcd_remap)
   // SYN (z inverse of separable covariance spectra whose prior planes the
   // posterior did not write): where band[e] is 0 the element of column x,
-  // y row o is lam[o cnxp + x] S[6 e + q], covariance q = q0 + blockIdx.y
+  // y row o is lam[o cnxp + x] S[6 e + q], covariance q = q0 + blockIdx.y;
+  // with syn_rows the band's planes are generated as well (CovGen)
   const float* syn_lam;
   const float* syn_S;
   const int* syn_band;
   int syn_q0;
+  const int* syn_bidx;
+  const float* syn_rows;
+  int syn_nband, syn_rw, syn_n, syn_j0;
+  int syn_nrows;       // y rows of the launch (persistent SYN blocks walk ntx x nrows tiles)
+  EigQueueGeo syn_geo;
 };
 
 // W: 512-thread blocks (tiles of up to 8192 complex FP32 values), so that
 // the pencil tiles of the two-stage lengths can be 16 columns (128 B) wide.
-// The generating z inverse (SYN) of lengths up to 320 holds 80 VGPRs or
-// fewer (6 waves per SIMD), so that three of its 45 KB blocks share a CU:
-// it reads only the band's planes, and waits on HBM latency rather than
-// bandwidth, so the third block's loads in flight shorten it.
-constexpr int cols_waves_per_eu(bool wide, bool syn, int L) { return wide && syn && L <= 320 ? 6 : 1; }
+// The generating z inverse (SYN) with its plane tables takes up to 64 KB of
+// LDS, so two of its blocks share a CU: it may hold 128 VGPRs (4 waves per
+// SIMD) without spilling.
+constexpr int cols_waves_per_eu(bool wide, bool syn, int L) { return wide && syn ? 4 : 1; }
 
 template <typename T, bool INV, int RF, int RL, bool H, bool SYN = false, bool W = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThreads),
@@ -647,30 +654,84 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
   const int TX = 1 << a.cshift;
   C* twl = lds + a.f.L * TX;
   copy_to_lds<T>(twl, a.tw + a.f.tw0, a.f.twn);
+  if constexpr (SYN) {
+    // Persistent blocks: the tables of this covariance (S, NaN inside the
+    // band; with the band generated too, each plane's row index and the band
+    // planes' rows) are staged in LDS once, then the block walks the tiles
+    // vb = blockIdx.x + i gridDim.x (gridDim.x a multiple of the 8 XCDs, so
+    // that xcd_remap keeps each XCD on a contiguous range).  Each tile is
+    // made in LDS in natural order with the lanes along its columns (the
+    // lanes of a wave share one or a few planes), transformed LDS -> LDS and
+    // stored with the lanes along the columns.
+    float* synS = reinterpret_cast<float*>(twl + a.f.twn);
+    float* synL = synS + a.f.L;
+    int* synB = reinterpret_cast<int*>(synL + TX);
+    float* synT = reinterpret_cast<float*>(synB + a.f.L);
+    const int q = a.syn_q0 + blockIdx.y;
+    for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x)
+      synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
+    if (a.syn_rows) {
+      for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x) synB[e] = a.syn_bidx[e];
+      const float* rq = a.syn_rows + (size_t)q * a.syn_nband * a.syn_rw;
+      for (int t = threadIdx.x; t < a.syn_nband * a.syn_rw; t += (int)blockDim.x) synT[t] = rq[t];
+    }
+    const int n = a.syn_n;
+    const int ntiles = a.ntx * a.syn_nrows;
+    for (int vb = blockIdx.x; vb < ntiles; vb += gridDim.x) {
+      const int tile = xcd_remap(vb, ntiles, a.rev);
+      const int o = tile / a.ntx, tt = tile - o * a.ntx;
+      const int x0 = tt * TX;
+      const int nc = min(TX, a.ncols - x0);
+      C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
+      // the previous tile's stores have read the LDS tile and synL
+      __syncthreads();
+      for (int c = threadIdx.x; c < TX; c += (int)blockDim.x)
+        synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
+      __syncthreads();
+      for (int t = threadIdx.x; t < a.f.L * TX; t += (int)blockDim.x) {
+        const int c = t & (TX - 1), e = t >> a.cshift;
+        C v = cmake<C>((T)0, (T)0);
+        if (c < nc) {
+          const float sv = synS[e];
+          if (!__builtin_isnan(sv)) {
+            v = cmake<C>((T)(synL[c] * sv), (T)0);
+          } else {
+            bool gen = false;
+            if (a.syn_rows) {
+              // a band plane: the eigen-basis posterior's covariance entry
+              const float lam = synL[c];
+              const float* tb = synT + synB[e] * a.syn_rw;
+              if (!eig_queued(a.syn_geo, lam, tb[1], tb[2], tb[3], x0 + c, a.syn_j0 + o, e)) {
+                float w[kRecMaxNGen];
+#pragma unroll
+                for (int p = 0; p < kRecMaxNGen; ++p) w[p] = p < n ? eig_weight(lam, tb[4 + p]) : 0.0f;
+                const float2 g = eig_cov_entry<kRecMaxNGen>(n, q < 3, lam * tb[0], lam * lam, w, tb + 4 + n,
+                                                            tb + 4 + 2 * n);
+                v = cmake<C>((T)g.x, (T)g.y);
+                gen = true;
+              }
+            }
+            if (!gen) v = base[(size_t)e * a.estride + c];
+          }
+        }
+        lds[e * TX + c] = v;
+      }
+      __syncthreads();
+      transform_tile_lds<T, INV, RF, RL, H>(lds, TX, a.cshift, a.f, a.sd, twl);
+      for (int t = threadIdx.x; t < a.f.L * TX; t += (int)blockDim.x) {
+        const int c = t & (TX - 1), e = t >> a.cshift;
+        if (c < nc) base[(size_t)e * a.estride + c] = lds[e * TX + c];
+      }
+    }
+    return;
+  }
   const int tile = xcd_remap(blockIdx.x, gridDim.x, a.rev);
   const int o = tile / a.ntx, t = tile - o * a.ntx;
   const int x0 = t * TX;
   const int nc = min(TX, a.ncols - x0);
   C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
-  // SYN: the plane table of this covariance (S; NaN inside the band, where
-  // the element is read) and the tile's lam columns, staged in LDS once per
-  // block
-  float* synS = reinterpret_cast<float*>(twl + a.f.twn);
-  float* synL = synS + a.f.L;
-  if constexpr (SYN) {
-    const int q = a.syn_q0 + blockIdx.y;
-    for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x)
-      synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
-    for (int c = threadIdx.x; c < TX; c += (int)blockDim.x)
-      synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
-    __syncthreads();
-  }
   auto load = [&](int c, int e) -> C {
     if (c >= nc) return cmake<C>((T)0, (T)0);
-    if constexpr (SYN) {
-      const float sv = synS[e];
-      if (!__builtin_isnan(sv)) return cmake<C>((T)(synL[c] * sv), (T)0);
-    }
     return base[(size_t)e * a.estride + c];
   };
   auto store = [&](int c, int e, C v) {
@@ -841,17 +902,21 @@ int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const 
   return CRV_OK;
 }
 
-// y pass over z planes [k0, k1) or z pass over y rows [j0, j1)
-struct ZSynth {
-  const float* lam = nullptr;
-  const float* S = nullptr;
-  const int* band = nullptr;
-  int q0 = 0;
-};
+// LDS of a column tile of TX columns, with the generating z inverse's
+// tables (gen != nullptr: S, lam columns; rows: row indices and band rows)
+size_t cols_lds_bytes(const Fft1D& f, int tx, size_t elem, const CovGen* gen) {
+  size_t lds = ((size_t)f.L * tx + f.twn) * elem;
+  if (gen) {
+    lds += ((size_t)f.L + tx) * sizeof(float);
+    if (gen->rows) lds += (size_t)f.L * sizeof(int) + (size_t)gen->nband * gen->rw * sizeof(float);
+  }
+  return lds;
+}
 
+// y pass over z planes [k0, k1) or z pass over y rows [j0, j1)
 template <typename T>
 int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data, const Batch& bt, int o0, int o1,
-                const ZSynth* zs = nullptr) {
+                const CovGen* zs = nullptr) {
   using C = cplx_t<T>;
   const int cnxp = p->nxp / 2 + 1;
   ColArgs<T> a{};
@@ -870,10 +935,9 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
   a.cshift = ilog2(tx);
   a.ntx = (cnxp + tx - 1) / tx;
   const int nb = a.ntx * (o1 - o0);
-  size_t lds = ((size_t)a.f.L * tx + a.f.twn) * sizeof(C);
-  if (zs && zaxis && inv) lds += ((size_t)a.f.L + tx) * sizeof(float);
-  static const char* names[4] = {"fft_y_fwd", "fft_y_inv", "fft_z_fwd", "fft_z_inv"};
   const bool syn = zs && zaxis && inv && std::is_same<T, float>::value;
+  const size_t lds = cols_lds_bytes(a.f, tx, sizeof(C), syn ? zs : nullptr);
+  static const char* names[4] = {"fft_y_fwd", "fft_y_inv", "fft_z_fwd", "fft_z_inv"};
   ProfScope ps(ctx, syn ? "fft_z_inv_prior" : names[(zaxis ? 2 : 0) + (inv ? 1 : 0)]);
   void* args[] = {&a};
   const bool half = p->half[zaxis ? 2 : 1];
@@ -885,7 +949,21 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
     a.syn_S = zs->S;
     a.syn_band = zs->band;
     a.syn_q0 = zs->q0;
+    a.syn_bidx = zs->bidx;
+    a.syn_rows = zs->rows;
+    a.syn_nband = zs->nband; a.syn_rw = zs->rw; a.syn_n = zs->n; a.syn_j0 = o0;
+    a.syn_geo = EigQueueGeo{zs->rx, zs->ry, zs->rz, zs->lowk2, zs->cond_thr, p->nyp, p->nzp};
+    a.syn_nrows = o1 - o0;
     kern = cols_kernel_syn(a.f, half, wide);
+    // persistent: as many blocks as fit at once (a multiple of 8, the XCDs)
+    const int threads = wide ? 2 * kThreads : kThreads;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    const int want = std::max(8, (per_cu * ctx->num_cu / std::max(1, bt.n) + 7) / 8 * 8);
+    const int gx = std::min(nb, want);
+    CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(gx, bt.n), dim3(threads), args, lds, ctx->stream));
+    return CRV_OK;
   }
   CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(wide ? 2 * kThreads : kThreads), args, lds,
                                      ctx->stream));
@@ -1122,10 +1200,16 @@ int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src
   return CRV_OK;
 }
 
+bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw) {
+  CovGen g;
+  g.rows = (const float*)p;  // any non-null pointer: count the band tables
+  g.nband = nband; g.rw = rw;
+  return cols_lds_bytes(p->fz, p->tx_z, sizeof(float2), &g) <= 65536;
+}
+
 // One pass of a transform over part of a batch (see internal.hpp).
 int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
-                    int o0, int o1, double scale, const void* const* add, int nadd, const float* syn_lam,
-                    const float* syn_S, const int* syn_band, int syn_q0) {
+                    int o0, int o1, double scale, const void* const* add, int nadd, const CovGen* gen) {
   if (nbatch < 1 || nbatch > 65535 || nadd < 0 || nadd > 3 || pass < 0 || pass > 5 || o0 < 0 || o1 < o0)
     return crv_set_error(ctx, CRV_E_INVALID, "bad pass");
   if (o1 == o0) return CRV_OK;
@@ -1143,9 +1227,7 @@ int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void
     default: {
       const bool inv = pass >= 3, zaxis = (pass == 2 || pass == 3);
       if ((zaxis ? plan->nzp : plan->nyp) == 1) return CRV_OK;
-      ZSynth zs;
-      zs.lam = syn_lam; zs.S = syn_S; zs.band = syn_band; zs.q0 = syn_q0;
-      const ZSynth* zp = (pass == 3 && syn_lam && !f64) ? &zs : nullptr;
+      const CovGen* zp = (pass == 3 && gen && gen->lam && !f64) ? gen : nullptr;
       return f64 ? launch_cols<double>(ctx, plan, inv, zaxis, (double2*)buf, bt, o0, o1)
                  : launch_cols<float>(ctx, plan, inv, zaxis, (float2*)buf, bt, o0, o1, zp);
     }

@@ -160,16 +160,37 @@ int crv_launch_r2c_batch(crv_ctx* ctx, FftPlan* plan, void* dst, const void* src
                          double scale);
 // grids per launch of a batched transform of nbatch grids (CRV_TUNE_BATCH_MB)
 int crv_sub_batch(const FftPlan* p, int nbatch);
+// Covariance spectra that the z inverse pass generates instead of reading
+// (FP32, separable prior, crv_invert): the grids of a launch are the
+// covariances q0 + b.  Outside the band (band[k] == 0) the element of column
+// i, y row j is lam[j cnxp + i] S[6 k + q].  With rows (eigen-basis plans)
+// the band's planes are generated too: plane k's row bidx[k] of covariance
+// q, rows + (q nband + bidx[k]) rw, holds S_q, cond(D), l_max, l_min, the n
+// eigenvalues l_p and the real and imaginary parts of the n G_pq, and the
+// element is lam S_q - lam^2 sum_p w_p G_pq unless the posterior queued the
+// wavenumber for its FP64 solve (same rule), whose value is read.
+struct CovGen {
+  const float* lam = nullptr;
+  const float* S = nullptr;
+  const int* band = nullptr;
+  int q0 = 0;
+  const int* bidx = nullptr;
+  const float* rows = nullptr;
+  int nband = 0, rw = 0, n = 0;
+  float rx = 0, ry = 0, rz = 0, lowk2 = 0, cond_thr = 0;
+};
+// LDS of the generating z inverse of a plan with nband band planes of rw
+// floats each fits the column kernels' 64 KB
+bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw);
+
 // One pass of a transform over part of a batch of nbatch grids bstride reals
 // apart: pass 0 x R2C (rows [o0, o1), row = j + nyp k; reads src, scales),
 // 1 y forward (z planes [o0, o1)), 2 z forward (y rows [o0, o1)), 3 z
 // inverse, 4 y inverse, 5 x C2R (rows, in place, scales, adds add[b % nadd]).
-// With syn_lam (pass 3, FP32): the grids are covariances q0 + b of a
-// separable prior whose planes with syn_band[k] == 0 were not written; their
-// elements are generated as syn_lam[j cnxp + i] syn_S[6 k + q].
+// gen (pass 3, FP32): the grids are covariances whose spectra are generated
+// where CovGen says so.
 int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
-                    int o0, int o1, double scale, const void* const* add, int nadd, const float* syn_lam = nullptr,
-                    const float* syn_S = nullptr, const int* syn_band = nullptr, int syn_q0 = 0);
+                    int o0, int o1, double scale, const void* const* add, int nadd, const CovGen* gen = nullptr);
 // Noise and Cholesky scaling of nreal counter-stream realizations over y
 // rows [j0, j1): realization r, parameter g written at out0 + (3 r + g)
 // gstride complex elements (simulate.hip).

@@ -136,22 +136,32 @@ __device__ __forceinline__ void counter_pair(unsigned long long seed, unsigned l
 }
 
 // Simulation noise of the three grids at one element.  One Philox call per
-// (element, realization, call tag) gives four words; grid g takes words
-// 2(g&1), 2(g&1)+1 of call g>>1 and turns them into 23-bit uniforms
-// (2 (w >> 9) + 1) 2^-24 in (0, 1), exact in FP32, and a Box-Muller pair
-// r cos(2 pi u2), r sin(2 pi u2) with r = sqrt(-2 ln u1), evaluated in the
-// precision U of the grids.  An element that copies its Hermitian partner
-// draws with the partner's index.
-__device__ __forceinline__ void sim_words(unsigned long long seed, unsigned long long real, unsigned tag,
-                                          long long s, unsigned w[4]) {
+// (element, realization) gives four 32-bit words, cut into six 21-bit
+// fields: the top 21 bits of each word, f0..f3, and from their low 11 bits
+// f4 = low0 * 2^10 + low1 / 2 and f5 = low2 * 2^10 + low3 / 2.  Each field
+// f gives the uniform (2 f + 1) 2^-22 in (0, 1), exact in FP32; grid g takes
+// the Box-Muller pair of fields (2g, 2g + 1): r cos(2 pi u2), r sin(2 pi u2)
+// with r = sqrt(-2 ln u1), evaluated in the precision U of the grids.  The
+// radius field limits |N| to sqrt(2 ln 2^22) = 5.5, a tail of probability
+// 4e-8.  An element that copies its Hermitian partner draws with the
+// partner's index.
+__device__ __forceinline__ void sim_words(unsigned long long seed, unsigned long long real, long long s,
+                                          unsigned w[4]) {
   w[0] = (unsigned)s;
   w[1] = (unsigned)((unsigned long long)s >> 32);
   w[2] = (unsigned)real;
-  w[3] = ((unsigned)(real >> 32) & 0x00FFFFFFu) | (tag << 24);
+  w[3] = (unsigned)(real >> 32) & 0x00FFFFFFu;
   philox10(w, (unsigned)seed, (unsigned)(seed >> 32));
 }
 
-// sin(2 pi u), cos(2 pi u) for u = (2m + 1) 2^-24 in (0, 1).  With
+__device__ __forceinline__ void sim_fields(const unsigned w[4], unsigned f[6]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) f[q] = w[q] >> 11;
+  f[4] = ((w[0] & 0x7FFu) << 10) | ((w[1] & 0x7FFu) >> 1);
+  f[5] = ((w[2] & 0x7FFu) << 10) | ((w[3] & 0x7FFu) >> 1);
+}
+
+// sin(2 pi u), cos(2 pi u) for u = (2m + 1) 2^-22 in (0, 1).  With
 // q = rint(4u), v = u - q/4 is exact and within [-1/8, 1/8], so
 // t = 2 pi v is within pi/4, where the Taylor series to t^9 (sin) and t^10
 // (cos) stay within 2e-9 of the functions; q mod 4 rotates the pair by
@@ -174,10 +184,11 @@ __device__ __forceinline__ void sincos_2pi_unit(float u, float& sn, float& cs) {
   cs = (k == 0) ? c : (k == 1) ? -s : (k == 2) ? -c : s;
 }
 
+// Box-Muller on two 21-bit fields
 __device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, float& n0, float& n1) {
-  const float u1 = (float)(((a >> 9) << 1) | 1u) * 0x1p-24f;
-  const float u2 = (float)(((b >> 9) << 1) | 1u) * 0x1p-24f;
-  // -2 ln u1 >= 1.1e-7: a normal number, where the hardware square root is
+  const float u1 = (float)((a << 1) | 1u) * 0x1p-22f;
+  const float u2 = (float)((b << 1) | 1u) * 0x1p-22f;
+  // -2 ln u1 >= 4.7e-7: a normal number, where the hardware square root is
   // within 1 ulp
   const float r = __builtin_amdgcn_sqrtf(-2.0f * logf(u1));
   float sn, cs;
@@ -186,8 +197,8 @@ __device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, float& n0
   n1 = r * sn;
 }
 __device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, double& n0, double& n1) {
-  const double u1 = (double)(((a >> 9) << 1) | 1u) * 0x1p-24;
-  const double u2 = (double)(((b >> 9) << 1) | 1u) * 0x1p-24;
+  const double u1 = (double)((a << 1) | 1u) * 0x1p-22;
+  const double u2 = (double)((b << 1) | 1u) * 0x1p-22;
   const double r = sqrt(-2.0 * log(u1));
   double sn, cs;
   sincospi(2.0 * u2, &sn, &cs);
@@ -204,13 +215,12 @@ __device__ __forceinline__ void sim_noise3(const NoiseGeom& g, unsigned long lon
   int x;
   const int kind = noise_kind(g, i, partner, jk, x);
   const long long s = (kind == 3) ? partner : i;
-  unsigned w0[4], w1[4];
-  sim_words(seed, real, 0u, s, w0);
-  sim_words(seed, real, 1u, s, w1);
+  unsigned w[4], f[6];
+  sim_words(seed, real, s, w);
+  sim_fields(w, f);
   U n[3][2];
-  sim_box_muller(w0[0], w0[1], n[0][0], n[0][1]);
-  sim_box_muller(w0[2], w0[3], n[1][0], n[1][1]);
-  sim_box_muller(w1[0], w1[1], n[2][0], n[2][1]);
+#pragma unroll
+  for (int g = 0; g < 3; ++g) sim_box_muller(f[2 * g], f[2 * g + 1], n[g][0], n[g][1]);
   const U sd = (U)noise_std();
 #pragma unroll
   for (int q = 0; q < 3; ++q) {

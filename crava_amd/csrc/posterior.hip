@@ -30,6 +30,7 @@
 // estimate (largest over smallest Cholesky pivot) exceeds the threshold, or
 // when |k| is below the low-wavenumber fraction of Nyquist.
 #include "internal.hpp"
+#include "eiggen.hpp"
 #include "linalg.hpp"
 #include "noise.hpp"
 #include "simkern.hpp"
@@ -100,24 +101,26 @@ struct PostArgs {
   // the covariances of planes outside the band are left unwritten (the z
   // inverse pass generates them from the same tables, crv_invert)
   int skip_prior_cov;
+  // eigen-basis kernel: the covariances inside the band are left unwritten
+  // too, but for the wavenumbers of the FP64 list (generated likewise)
+  int skip_band_cov;
 };
 
 // Simulation of one wavenumber from its covariance c as stored (FP32):
-// SIM 1 factors in float with the reference's rules, SIM 2 in FP64 with one
-// refinement step -- the work of k_simulate<float, SIM == 2> for this index.
+// SIM 1 factors in float with the reference's rules; SIM 2 likewise, and in
+// FP64 with one refinement step where the float factorisation is rejected
+// or ill-conditioned (sim_factor_mixed).
 template <int SIM>
 __device__ __forceinline__ void fused_sim(const PostArgs<float>& a, long long idx, const float2 c[6]) {
   if constexpr (SIM > 0) {
-    constexpr bool F64 = SIM == 2;
-    using U = typename std::conditional<F64, double, float>::type;
-    cplx_t<U> Lf[6];
-    const bool ok = sim_factor<float, F64>(c, Lf);
+    float2 Lf[6];
+    const bool ok = (SIM == 2) ? sim_factor_mixed(c, Lf) : sim_factor<float, false>(c, Lf);
     for (int r = 0; r < a.nsim; ++r) {
       float2 o[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
       if (ok) {
         float re[3], im[3];
         sim_noise3<float>(a.geo, a.seed, a.real0 + (unsigned long long)r, idx, re, im);
-        sim_apply<float, U>(Lf, re, im, o);
+        sim_apply<float, float>(Lf, re, im, o);
       }
 #pragma unroll
       for (int g = 0; g < 3; ++g) a.sim[(size_t)(3 * r + g) * a.sstride + idx] = o[g];
@@ -546,11 +549,8 @@ __global__ __launch_bounds__(kThreads) void k_posterior_eig(PostArgs<float> a) {
       }
       continue;
     }
-    const float ce = er[Lo::OC] * (lam * er[Lo::OC + 1] + 1.0f) / (lam * er[Lo::OC + 2] + 1.0f);
-    const int ky = min(j, a.nyp - j), kz = min(k, a.nzp - k);
-    const float fx = (float)i * a.rx, fy = (float)ky * a.ry, fz = (float)kz * a.rz;
-    const bool lowk = (fx * fx + fy * fy + fz * fz) < a.lowk2;
-    if (lowk || !(ce <= a.cond_thr)) {
+    const EigQueueGeo qg{a.rx, a.ry, a.rz, a.lowk2, a.cond_thr, a.nyp, a.nzp};
+    if (eig_queued(qg, lam, er[Lo::OC], er[Lo::OC + 1], er[Lo::OC + 2], i, j, k)) {
       if (live) {
         const unsigned int slot = atomicAdd(a.list_count, 1u);
         a.list[slot] = (unsigned int)idx;
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior_eig(PostArgs<float> a) {
       continue;
     }
     if (!live) continue;
-    cmax = fmax(cmax, (double)ce);
+    cmax = fmax(cmax, (double)eig_cond(lam, er[Lo::OC], er[Lo::OC + 1], er[Lo::OC + 2]));
     C mu[3], r[N];
 #pragma unroll
     for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(kThreads) void k_posterior_eig(PostArgs<float> a) {
         z.x = fmaf(tr, r[l].x, fmaf(-ti, r[l].y, z.x));
         z.y = fmaf(tr, r[l].y, fmaf(ti, r[l].x, z.y));
       }
-      w[p] = __builtin_amdgcn_rcpf(fmaf(lam, er[Lo::OW + p], 1.0f));
+      w[p] = eig_weight(lam, er[Lo::OW + p]);
       const float g = lam * w[p];
       y[p] = make_float2(g * z.x, g * z.y);
     }
@@ -618,24 +618,19 @@ __global__ __launch_bounds__(kThreads) void k_posterior_eig(PostArgs<float> a) {
     const float l2 = lam * lam;
     C cq[6];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      float x = 0.0f;
-#pragma unroll
-      for (int p = 0; p < N; ++p) x = fmaf(w[p], er[Lo::OG + 9 * p + q], x);
-      cq[q] = make_float2(fmaf(-l2, x, s[q]), 0.0f);
-    }
-#pragma unroll
-    for (int q = 3; q < 6; ++q) {
-      float xr = 0.0f, xi = 0.0f;
+    for (int q = 0; q < 6; ++q) {
+      float gr[N], gi[N];
 #pragma unroll
       for (int p = 0; p < N; ++p) {
-        xr = fmaf(w[p], er[Lo::OG + 9 * p + 3 + 2 * (q - 3)], xr);
-        xi = fmaf(w[p], er[Lo::OG + 9 * p + 4 + 2 * (q - 3)], xi);
+        gr[p] = er[Lo::OG + 9 * p + (q < 3 ? q : 3 + 2 * (q - 3))];
+        gi[p] = q < 3 ? 0.0f : er[Lo::OG + 9 * p + 4 + 2 * (q - 3)];
       }
-      cq[q] = make_float2(fmaf(-l2, xr, s[q]), -l2 * xi);
+      cq[q] = eig_cov_entry<N>(N, q < 3, s[q], l2, w, gr, gi);
     }
+    if (!a.skip_band_cov) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) a.cov[q][idx] = cq[q];
+      for (int q = 0; q < 6; ++q) a.cov[q][idx] = cq[q];
+    }
     fused_sim<SIM>(a, idx, cq);
   }
   block_stats(a.stats, 0ull, cmax);
@@ -1135,6 +1130,10 @@ struct crv_post_plan {
   TabOffsets oT, o64;
   size_t oI = 0, oS = 0, oR = 0;   // oR: FP32 separable records (0: none)
   size_t oE = 0;                   // eigen-basis records (0: none)
+  // with eigen-basis records: the band planes' tables of the generating z
+  // inverse (CovGen): row index per plane, then 6 x nband rows of gw floats
+  size_t oGI = 0, oGR = 0;
+  int nband = 0, gw = 0;
   unsigned int* dlist = nullptr;
   double sig0[6] = {0, 0, 0, 0, 0, 0};
   float cond_thr = 1e5f;
@@ -1192,6 +1191,38 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
       pl->oE = (blob.size() + 255) & ~(size_t)255;
       blob.resize(pl->oE + er.size() * sizeof(float), 0);
       memcpy(blob.data() + pl->oE, er.data(), er.size() * sizeof(float));
+      // the generating z inverse's copy of the band planes' values: the same
+      // floats as the records, per covariance q: S_q, cond(D), l_max, l_min,
+      // l_p, Re G_pq, Im G_pq (CovGen)
+      const int n = p->ntheta;
+      const int OG = 2 * n * n + 12 * n, OW = OG + 9 * n, OS = OW + n, OC = OS + 12;
+      std::vector<int> bidx(nzp, -1);
+      int nb = 0;
+      for (int k = 0; k < nzp; ++k)
+        if (h.inv[k]) bidx[k] = nb++;
+      const int gw = 4 + 3 * n;
+      std::vector<float> rows((size_t)6 * nb * gw, 0.0f);
+      for (int k = 0; k < nzp; ++k) {
+        if (bidx[k] < 0) continue;
+        const float* rk = er.data() + (size_t)k * kEigRec;
+        for (int q = 0; q < 6; ++q) {
+          float* t = rows.data() + ((size_t)q * nb + bidx[k]) * gw;
+          t[0] = rk[OS + q];
+          t[1] = rk[OC]; t[2] = rk[OC + 1]; t[3] = rk[OC + 2];
+          for (int pp = 0; pp < n; ++pp) {
+            t[4 + pp] = rk[OW + pp];
+            t[4 + n + pp] = rk[OG + 9 * pp + (q < 3 ? q : 3 + 2 * (q - 3))];
+            t[4 + 2 * n + pp] = q < 3 ? 0.0f : rk[OG + 9 * pp + 4 + 2 * (q - 3)];
+          }
+        }
+      }
+      pl->nband = nb; pl->gw = gw;
+      pl->oGI = (blob.size() + 255) & ~(size_t)255;
+      blob.resize(pl->oGI + (size_t)nzp * sizeof(int), 0);
+      memcpy(blob.data() + pl->oGI, bidx.data(), (size_t)nzp * sizeof(int));
+      pl->oGR = (blob.size() + 255) & ~(size_t)255;
+      blob.resize(pl->oGR + rows.size() * sizeof(float), 0);
+      memcpy(blob.data() + pl->oGR, rows.data(), rows.size() * sizeof(float));
     }
   }
   pl->oI = blob.size();
@@ -1264,6 +1295,7 @@ struct FusedSim {
   int nsim = 0;
   unsigned long long seed = 0, real0 = 0;
   bool skip_prior_cov = false;
+  bool skip_band_cov = false;
 };
 
 int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, crv_grid* err_corr,
@@ -1309,6 +1341,7 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
       a.rec = (const float2*)(pl->dtab + pl->oR);
       a.cpp = (pl->cnxp * nyr + 63) / 64;
       a.skip_prior_cov = fs.skip_prior_cov ? 1 : 0;
+      a.skip_band_cov = (fs.skip_band_cov && pl->oE) ? 1 : 0;
       if (fs.nsim > 0) {
         a.sim = fs.sim; a.sstride = fs.sstride; a.nsim = fs.nsim; a.seed = fs.seed; a.real0 = fs.real0;
         a.geo = make_noise_geom_dims(pl->cnxp, pl->nyp, pl->nzp);
@@ -1393,15 +1426,10 @@ int family_pass(crv_ctx* ctx, FftPlan* fp, const Family& f, int pass, int o0, in
 // x and y passes (forward: x then y; inverse: y then x), and with with_z the
 // z pass too (forward last, inverse first), over a family in sub-batches
 // that fit the memory-side cache, so each pass of a sub-batch reads the
-// previous pass's output from there
-struct CovSynth {
-  const float* lam = nullptr;
-  const float* S = nullptr;
-  const int* band = nullptr;
-};
-
+// previous pass's output from there; cs: the family is the covariances, whose
+// spectra the z inverse generates where CovGen says so
 int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double scale, const void* const* add,
-              int nadd, crv_grid* const* src = nullptr, bool with_z = false, const CovSynth* cs = nullptr) {
+              int nadd, crv_grid* const* src = nullptr, bool with_z = false, const CovGen* cs = nullptr) {
   const int nrows = fp->nyp * fp->nzp;
   // a forward x pass may read other grids (src): one launch per sub-batch
   // when they form a batch with the same stride, else one per grid
@@ -1426,14 +1454,17 @@ int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double s
     } else {
       if (with_z && cs) {
         // covariances b0.. of the family: q = b0 + b
+        CovGen g = *cs;
         if (s.contiguous) {
+          g.q0 = b0;
           rc = crv_launch_pass(ctx, fp, 3, s.g[0]->d, s.g[0]->d, s.n, (long long)s.g[0]->rsize, 0, fp->nyp, 1.0,
-                               nullptr, 0, cs->lam, cs->S, cs->band, b0);
+                               nullptr, 0, &g);
         } else {
           rc = CRV_OK;
-          for (int b = 0; b < s.n && rc == CRV_OK; ++b)
-            rc = crv_launch_pass(ctx, fp, 3, s.g[b]->d, s.g[b]->d, 1, 0, 0, fp->nyp, 1.0, nullptr, 0, cs->lam, cs->S,
-                                 cs->band, b0 + b);
+          for (int b = 0; b < s.n && rc == CRV_OK; ++b) {
+            g.q0 = b0 + b;
+            rc = crv_launch_pass(ctx, fp, 3, s.g[b]->d, s.g[b]->d, 1, 0, 0, fp->nyp, 1.0, nullptr, 0, &g);
+          }
         }
         if (rc != CRV_OK) return rc;
       } else if (with_z && (rc = family_pass(ctx, fp, s, 3, 0, fp->nyp, 1.0, nullptr, 0)) != CRV_OK) {
@@ -1499,6 +1530,9 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   // outside the band are the cleaned prior lam(i,j) S(k); the posterior
   // leaves them unwritten and the z inverse pass generates them
   const bool synth = !chunked && pl->oR != 0;
+  // with eigen-basis records the band's planes as well (but for the
+  // wavenumbers of the FP64 list), when their tables fit the z inverse's LDS
+  const bool synth_band = synth && pl->oE != 0 && pl->oGR != 0 && crv_cov_gen_fits(fp, pl->nband, pl->gw);
   // forward x and y passes of the data and the means
   if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src, !chunked)) != CRV_OK)
     return rc;
@@ -1520,10 +1554,11 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
       fsim.seed = seed; fsim.real0 = realization0;
     }
     fsim.skip_prior_cov = synth;
+    fsim.skip_band_cov = synth_band;
     if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0, fsim)) != CRV_OK) return rc;
     if (chunked) {
       if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
-                                            realization0, pl->refine ? 1 : 0, j0, j1)) != CRV_OK)
+                                            realization0, pl->refine ? 2 : 0, j0, j1)) != CRV_OK)
         return rc;
       if ((rc = family_pass(ctx, fp, fm, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if ((rc = family_pass(ctx, fp, fc, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
@@ -1538,7 +1573,7 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
     return rc;
   if (!chunked && nsim && !fused &&
       (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed, realization0,
-                                pl->refine ? 1 : 0, 0, nyp)) != CRV_OK)
+                                pl->refine ? 2 : 0, 0, nyp)) != CRV_OK)
     return rc;
   if (nsim) {
     for (int b = 0; b < nsim; ++b) sims[b]->cubetype = CRV_PARAMETER;
@@ -1546,10 +1581,17 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
     if ((rc = family_xy(ctx, fp, fs, true, crv_scale_inverse(sims[0]), add, 3, nullptr, !chunked)) != CRV_OK)
       return rc;
   }
-  CovSynth cs;
+  CovGen cs;
   if (synth) {
     const KzTab<float> t = tab_at<float>(pl->dtab, pl->oT);
     cs.lam = t.lam; cs.S = t.S; cs.band = (const int*)(pl->dtab + pl->oI);
+    if (synth_band) {
+      cs.bidx = (const int*)(pl->dtab + pl->oGI);
+      cs.rows = (const float*)(pl->dtab + pl->oGR);
+      cs.nband = pl->nband; cs.rw = pl->gw; cs.n = pl->n;
+      cs.rx = 2.0f / (float)pl->nxp; cs.ry = 2.0f / (float)pl->nyp; cs.rz = 2.0f / (float)pl->nzp;
+      cs.lowk2 = (float)pl->lowk2; cs.cond_thr = pl->cond_thr;
+    }
   }
   if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked,
                       synth ? &cs : nullptr)) != CRV_OK)

@@ -129,6 +129,41 @@ __device__ __forceinline__ bool sim_factor(const cplx_t<T> c[6],
   return ok;
 }
 
+// The factor of an FP32 covariance c = (aa, bb, rr, ab, ar, br) in float
+// without divisions (reciprocal square roots), with lib_matrCholCpx's
+// acceptance rule applied to the unscaled pivots (p <= 1e-20 A00, the form of
+// its test on the matrix divided by A00); where a pivot is rejected or the
+// pivot ratio max p / min p (the squared ratio of the factor's diagonal)
+// exceeds kSimCondF32 the factor is made again in FP64 with one refinement
+// step and rounded to float.  false: rejected in FP64 as well (the
+// realization is zero there).
+constexpr float kSimCondF32 = 1e3f;
+__device__ __forceinline__ bool sim_factor_mixed(const float2 c[6], float2 L[6]) {
+  const float a00 = c[0].x;
+  const float tol = (float)kCholTol * a00;
+  const float r0 = __builtin_amdgcn_rsqf(a00);
+  L[0] = make_float2(a00 * r0, 0.0f);
+  L[1] = make_float2(c[3].x * r0, -c[3].y * r0);
+  const float p1 = c[1].x - (L[1].x * L[1].x + L[1].y * L[1].y);
+  const float r1 = __builtin_amdgcn_rsqf(p1);
+  L[2] = make_float2(p1 * r1, 0.0f);
+  L[3] = make_float2(c[4].x * r0, -c[4].y * r0);
+  // P21 - L20 conj(L10), with P21 = conj(br)
+  const float2 x = make_float2(c[5].x - (L[3].x * L[1].x + L[3].y * L[1].y),
+                               -c[5].y - (L[3].y * L[1].x - L[3].x * L[1].y));
+  L[4] = make_float2(x.x * r1, x.y * r1);
+  const float p2 = c[2].x - (L[3].x * L[3].x + L[3].y * L[3].y) - (L[4].x * L[4].x + L[4].y * L[4].y);
+  L[5] = make_float2(p2 * __builtin_amdgcn_rsqf(p2), 0.0f);
+  const float pmin = fminf(a00, fminf(p1, p2)), pmax = fmaxf(a00, fmaxf(p1, p2));
+  const bool ok = a00 > 0.0f && a00 > tol && p1 > tol && p2 > tol;
+  if (ok && pmax <= kSimCondF32 * pmin) return true;
+  double2 Ld[6];
+  if (!sim_factor<float, true>(c, Ld)) return false;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) L[q] = make_float2((float)Ld[q].x, (float)Ld[q].y);
+  return true;
+}
+
 // o = L z for the noise z = re + i im, rounded to T
 template <typename T, typename U>
 __device__ __forceinline__ void sim_apply(const cplx_t<U> Lf[6], const T re[3], const T im[3], cplx_t<T> o[3]) {

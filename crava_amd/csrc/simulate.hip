@@ -6,8 +6,10 @@
 // 3 x 3 posterior covariance is factored with lib_matrCholCpx's rules and the
 // noise vector multiplied by L (lib_matrProdCholVec, lib_matr.c:453-470), or
 // zeroed when the factorisation is rejected (crava.cpp:1639-1651).  With
-// fp64_chol the factor of FP32 covariances is computed in FP64 and refined
-// by one Newton step before it scales the noise.  The three
+// fp64_chol 1 the factor of FP32 covariances is computed in FP64 and refined
+// by one Newton step before it scales the noise; with 2 it is made in float
+// as the reference does and again in FP64 only where the float one is
+// rejected or ill-conditioned.  The three
 // inverse transforms then add the posterior means in their last pass
 // (FFTGrid::add, fftgrid.cpp:1870-1891).
 #include "internal.hpp"
@@ -55,12 +57,14 @@ void sim_rows(SimArgs<T>& a, int cnxp, int nyp, int nzp, int j0, int j1) {
   a.dcx = make_fastdiv(cnxp);
 }
 
-// F64: factor in FP64 with one refinement step (FP32 grids, refine mode);
-// otherwise the reference's factorisation in the grids' precision (without
-// divisions for FP64 grids).
-template <typename T, bool F64>
+// FM (factor mode): 0 the reference's factorisation in the grids' precision
+// (without divisions for FP64 grids); 1 in FP64 with one refinement step
+// (FP32 grids); 2 in float with the reference's rules, and in FP64 with
+// refinement where that is rejected or ill-conditioned (FP32 grids).
+template <typename T, int FM>
 __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
   using C = cplx_t<T>;
+  constexpr bool F64 = FM == 1;
   using U = typename std::conditional<F64, double, T>::type;
   using CU = cplx_t<U>;
   const long long stride = (long long)gridDim.x * kThreads;
@@ -71,7 +75,9 @@ __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
     const long long i = (long long)(rem - jl * a.cnxp) + (long long)a.cnxp * ((long long)(a.j0 + jl) + (long long)a.nyp * k);
     const C cv[6] = {a.cov[0][i], a.cov[1][i], a.cov[2][i], a.cov[3][i], a.cov[4][i], a.cov[5][i]};
     CU Lf[6];
-    const bool ok = sim_factor<T, F64>(cv, Lf);
+    bool ok;
+    if constexpr (FM == 2) ok = sim_factor_mixed(cv, Lf);
+    else ok = sim_factor<T, F64>(cv, Lf);
     for (int r = 0; r < a.nreal; ++r) {
       C o[3] = {cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0), cmake<C>((T)0, (T)0)};
       if (ok) {
@@ -209,7 +215,7 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const double2*)((char*)hostbuf + g * gb) : nullptr;
     a.seed = noise->seed; a.real = noise->realization; a.nc = nc; a.nreal = 1;
     sim_rows(a, plan->nxp / 2 + 1, plan->nyp, plan->nzp, 0, plan->nyp);
-    hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_simulate<double, 0>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   } else {
     SimArgs<float> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q]->d;
@@ -218,10 +224,12 @@ extern "C" int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[
     for (int g = 0; g < 3; ++g) a.host[g] = hostbuf ? (const float2*)((char*)hostbuf + g * gb) : nullptr;
     a.seed = noise->seed; a.real = noise->realization; a.nc = nc; a.nreal = 1;
     sim_rows(a, plan->nxp / 2 + 1, plan->nyp, plan->nzp, 0, plan->nyp);
-    if (fp64_chol)
-      hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    if (fp64_chol == 2)
+      hipLaunchKernelGGL((k_simulate<float, 2>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    else if (fp64_chol)
+      hipLaunchKernelGGL((k_simulate<float, 1>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
     else
-      hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_simulate<float, 0>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   }
   }
   CRV_HIP_CHECK(ctx, hipGetLastError());
@@ -271,7 +279,7 @@ int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], v
     a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
     a.seed = seed; a.real = real0; a.nc = nc; a.nreal = nreal; a.rstride = 3 * gstride;
     sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
-    hipLaunchKernelGGL((k_simulate<double, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_simulate<double, 0>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   } else {
     SimArgs<float> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q];
@@ -279,10 +287,12 @@ int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], v
     a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
     a.seed = seed; a.real = real0; a.nc = nc; a.nreal = nreal; a.rstride = 3 * gstride;
     sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
-    if (fp64_chol)
-      hipLaunchKernelGGL((k_simulate<float, true>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    if (fp64_chol == 2)
+      hipLaunchKernelGGL((k_simulate<float, 2>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+    else if (fp64_chol)
+      hipLaunchKernelGGL((k_simulate<float, 1>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
     else
-      hipLaunchKernelGGL((k_simulate<float, false>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL((k_simulate<float, 0>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
   }
   CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;

@@ -411,7 +411,9 @@ def grid_batch(ctx: Context, count, nxp, nyp, nzp, precision=32, cubetype=L.CRV_
 
 
 def simulate_batch(ctx: Context, cov, post_mean, out, *, seed=0, realization0=0, fp64_chol=False):
-    """len(out) // 3 realizations with the counter stream; out from grid_batch."""
+    """len(out) // 3 realizations with the counter stream; out from grid_batch.
+    fp64_chol: 0 float factor, 1 FP64 with refinement, 2 float with FP64
+    where the float factor is rejected or ill-conditioned."""
     nreal = len(out) // 3
     covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
     pm = (C.c_void_p * 3)(*[g.h.value for g in post_mean]) if post_mean else None

@@ -361,7 +361,11 @@ typedef struct crv_noise_spec {
 /* cov: posterior covariance spectra (aa, bb, rr, ab, ar, br), transformed.
  * post_mean: posterior means in real space (untransformed) or NULL.
  * out[3]: destination grids (alpha, beta, rho realization), real on return.
- * fp64_chol: 1 = factor the 3x3 matrices in FP64 (FP32 grids only).        */
+ * fp64_chol (FP32 grids): 0 = factor the 3x3 matrices in float with
+ * lib_matrCholCpx's rules, as the reference does; 1 = in FP64 with one
+ * Newton refinement step; 2 = in float, and in FP64 with refinement only
+ * where the float factorisation is rejected or its pivot ratio squared
+ * exceeds 1e3 (the precision of crv_invert's realizations on refine plans). */
 int crv_simulate(crv_ctx* ctx, crv_grid* cov[6], crv_grid* post_mean[3],
                  const crv_noise_spec* noise, int fp64_chol, crv_grid* out[3]);
 

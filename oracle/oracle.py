@@ -292,15 +292,20 @@ def counter_noise(nxp, nyp, nzp, seed, realization, grid):
     src = np.where(special & (jkcc < jk), jkcc * cnxp + x, i)
 
     def pair(idx):
-        # call tag grid >> 1, words 2 (grid & 1) and 2 (grid & 1) + 1,
-        # 23-bit uniforms (2 (w >> 9) + 1) 2^-24
-        hi = (np.uint64(realization >> 32) & np.uint64(0xFFFFFF)) | np.uint64((grid >> 1) << 24)
+        # one call per element; six 21-bit fields: the top 21 bits of each
+        # word, then low0 2^10 + low1 / 2 and low2 2^10 + low3 / 2; grid g
+        # takes fields 2g, 2g + 1 as the uniforms (2 f + 1) 2^-22
+        hi = np.uint64(realization >> 32) & np.uint64(0xFFFFFF)
         c = philox10(idx.astype(np.uint64) & np.uint64(M32), idx.astype(np.uint64) >> np.uint64(32),
                      np.full(idx.shape, realization & M32, np.uint64), np.full(idx.shape, hi, np.uint64),
                      seed & M32, (seed >> 32) & M32)
-        wa, wb = c[2 * (grid & 1)], c[2 * (grid & 1) + 1]
-        u1 = ((wa >> np.uint64(9)) * np.uint64(2) + np.uint64(1)).astype(np.float64) / 16777216.0
-        u2 = ((wb >> np.uint64(9)) * np.uint64(2) + np.uint64(1)).astype(np.float64) / 16777216.0
+        lo = [w & np.uint64(0x7FF) for w in c]
+        f = [w >> np.uint64(11) for w in c]
+        f.append((lo[0] << np.uint64(10)) | (lo[1] >> np.uint64(1)))
+        f.append((lo[2] << np.uint64(10)) | (lo[3] >> np.uint64(1)))
+        fa, fb = f[2 * grid], f[2 * grid + 1]
+        u1 = (fa * np.uint64(2) + np.uint64(1)).astype(np.float64) / 4194304.0
+        u2 = (fb * np.uint64(2) + np.uint64(1)).astype(np.float64) / 4194304.0
         r = np.sqrt(-2.0 * np.log(u1))
         return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
 

@@ -112,7 +112,7 @@ def test_batched_transforms_match_single_grids(ctx, batch_mb):
 
 
 @pytest.mark.parametrize("batch_mb", [0, 1], indirect=True)
-@pytest.mark.parametrize("fp64_chol", [False, True])
+@pytest.mark.parametrize("fp64_chol", [0, 1, 2])
 def test_simulate_batch_matches_single_calls(ctx, fp64_chol, batch_mb):
     from crava_amd import _lib as L
     from crava_amd.hip import FFTGrid, grid_batch, simulate, simulate_batch
@@ -340,7 +340,7 @@ def test_invert_matches_sequence(ctx, chunk_rows, batched, src, n, dims):
     st_seq = plan.stats()
     for g in mean:
         g.invFFTInPlace()
-    simulate_batch(ctx, cov, mean, sims, seed=17, realization0=3, fp64_chol=True)
+    simulate_batch(ctx, cov, mean, sims, seed=17, realization0=3, fp64_chol=2)
     for g in cov:
         g.invFFTInPlace()
     ctx.synchronize()

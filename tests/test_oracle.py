@@ -635,6 +635,31 @@ def test_philox_known_answers():
         assert tuple(int(x) for x in O.philox10(*c, *k)) == want
 
 
+def test_counter_noise_fields_are_uniform_and_independent():
+    """The six 21-bit fields cut from one Philox-4x32-10 call (the top 21 bits
+    of each word; the low 11 bits of words 0/1 and 2/3 joined) are uniform on
+    [0, 2^21) and uncorrelated, so the three Box-Muller pairs of a
+    wavenumber are independent."""
+    n = 1 << 18
+    i = np.arange(n, dtype=np.uint64)
+    c = O.philox10(i, np.zeros(n, np.uint64), np.full(n, 7, np.uint64), np.zeros(n, np.uint64), 123, 0)
+    lo = [w & np.uint64(0x7FF) for w in c]
+    f = [w >> np.uint64(11) for w in c]
+    f.append((lo[0] << np.uint64(10)) | (lo[1] >> np.uint64(1)))
+    f.append((lo[2] << np.uint64(10)) | (lo[3] >> np.uint64(1)))
+    u = np.stack([x.astype(np.float64) / 2 ** 21 for x in f])
+    assert u.max() < 1.0 and u.min() >= 0.0
+    # mean 1/2, variance 1/12, to 5 sigma; histogram of 64 bins, chi^2
+    assert np.all(np.abs(u.mean(1) - 0.5) < 5 * math.sqrt(1 / 12 / n))
+    assert np.all(np.abs(u.var(1) - 1 / 12) < 5 * math.sqrt(1 / 180 / n))
+    for row in u:
+        h = np.bincount((row * 64).astype(int), minlength=64)
+        chi2 = float(((h - n / 64) ** 2 / (n / 64)).sum())
+        assert chi2 < 63 + 6 * math.sqrt(2 * 63)
+    cc = np.corrcoef(u)
+    assert np.max(np.abs(cc - np.eye(6))) < 5 / math.sqrt(n)
+
+
 def test_counter_noise_statistics():
     """The simulation noise is N(0,1)-distributed per component (1/2 per
     component of a complex (N, N)/sqrt(2) value), Hermitian-consistent, and
