@@ -622,6 +622,8 @@ def main():
                     help="c1: headline 450x450x320 inversion; c0: 64x64x128; c3: batched realizations and volumes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-refstream", action="store_true",
+                    help="skip the step with the reference RandomGen stream for the realization")
     ap.add_argument("--no-general", action="store_true",
                     help="skip the drop-in general path (prior covariance grids + errCorr, transformed inside)")
     ap.add_argument("--fp32-mode", choices=["refine", "plain"], default="refine",
@@ -721,6 +723,47 @@ def main():
                 f"{v_[1] / 3:.3f} ms/step {gbs}")
         log(f"[bench] FP{prec}: {sec * 1e3:.2f} ms/step, {s.voxels / sec:.3e} voxels/s")
 
+    # the reference stream (RandomGen) in the one-call inversion: the same
+    # step with the realization's noise decoded from an LCG state first
+    # (crv_invert_noise; the decode synchronises, so this is not captured)
+    refstream = None
+    if not a.no_refstream:
+        G = res[32]["grids"]
+        plan = G["plan"]
+
+        def ref_step(st):
+            return plan.invert(G["mean"], G["data"], G["cov"], G["sim"], mean_src=G["mean_in"],
+                               data_src=G["data_in"], lcg_state=st, chunk_rows=G["chunk_rows"])
+
+        st = ref_step(1)
+        ctx.synchronize()
+        nref = max(2, a.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(nref):
+            st = ref_step(st)
+        ctx.synchronize()
+        sec_ref = (time.perf_counter() - t0) / nref
+        ctx.profile(True)
+        st = ref_step(st)
+        ctx.profile(False)
+        prof_ref = ctx.profile_report()
+        info = hip.noise_guard(ctx)
+        draws = 3 * hip.noise_draws(s.nxp, s.nyp, s.nzp)
+        dec = {k: v for k, v in prof_ref.items() if k.startswith("lcg_")}
+        emit_ms = dec["lcg_emit"][1] if "lcg_emit" in dec else None
+        refstream = {"ms_per_step": sec_ref * 1e3, "voxels_per_s": s.voxels / sec_ref, "steps": nref,
+                     "normals_per_step": draws,
+                     "decode_ms": sum(v[1] for v in dec.values()),
+                     "decode_kernels": {k: {"launches": v[0], "ms": v[1]} for k, v in dec.items()},
+                     "emit_GB/s": (8.0 * draws / (emit_ms / 1e3) / 1e9) if emit_ms else None,
+                     "fix_passes": info[0], "host_resolved": info[1], "decode_runs": info[3]}
+        log(f"[bench] reference stream: {sec_ref * 1e3:.2f} ms/step (decode {refstream['decode_ms']:.3f} ms: "
+            + ", ".join(f"{k} {v[1]:.3f}" for k, v in dec.items()) + f"), {info[1]} normals resolved on the host")
+        # the FP32 outputs again as the error table below reads them (the
+        # counter-stream realization the FP64 leg made too)
+        step(ctx, G, kw, mode32, 77, s.seed_sim)
+        ctx.synchronize()
+
     general = None
     if not a.no_general:
         log("[bench] drop-in general path (6 prior covariance grids + errCorr transformed inside)")
@@ -799,6 +842,7 @@ def main():
         "errors_vs_fp64": errors,
         "posterior_stats": res[32]["stats"],
         "general_path": general,
+        "reference_stream": refstream,
         "transfers": transfers,
         "kernels": kernels,
         "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}

@@ -100,6 +100,12 @@ struct crv_ctx {
   size_t lcg_scratch_bytes = 0;
   double* normals = nullptr;
   size_t  normals_cap = 0;
+  // reference-noise decode: scale of the guard margins (1 by default) and
+  // what the last decode did: fix passes, normals resolved on the host
+  // (decisions, values), decode runs
+  double noise_guard = 1.0;
+  bool noise_force_table = false;
+  long long noise_info[4] = {0, 0, 0, 0};
 };
 
 struct crv_grid {
@@ -191,11 +197,14 @@ bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw);
 // where CovGen says so.
 int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
                     int o0, int o1, double scale, const void* const* add, int nadd, const CovGen* gen = nullptr);
-// Noise and Cholesky scaling of nreal counter-stream realizations over y
-// rows [j0, j1): realization r, parameter g written at out0 + (3 r + g)
-// gstride complex elements (simulate.hip).
+// Noise and Cholesky scaling of nreal realizations over y rows [j0, j1):
+// realization r, parameter g written at out0 + (3 r + g) gstride complex
+// elements (simulate.hip).  Noise: the counter stream (seed, real0), or with
+// normals the decoded reference stream, grid g of realization r from draw
+// (3 r + g) noise_draws (the plan's reference-stream prefix must exist).
 int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], void* out0, long long gstride,
-                        int nreal, unsigned long long seed, unsigned long long real0, int fp64_chol, int j0, int j1);
+                        int nreal, unsigned long long seed, unsigned long long real0, int fp64_chol, int j0, int j1,
+                        const double* normals = nullptr);
 
 // elementwise (simulate.hip) ---------------------------------------------------
 int crv_launch_axpy(crv_ctx* ctx, int prec, void* dst, const void* src, size_t n);

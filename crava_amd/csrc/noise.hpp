@@ -37,8 +37,10 @@ inline NoiseGeom make_noise_geom(const FftPlan* p) {
 }
 
 // classification of element i: 0 ordinary, 1 self-conjugate, 2 owns draws
-// (partner later), 3 copies the partner (returned in partner)
-__device__ __forceinline__ int noise_kind(const NoiseGeom& g, long long i, long long& partner, long long& jk, int& x) {
+// (partner later), 3 copies the partner (returned in partner, its row in
+// pjk); jk and x: the row and column of element i
+__device__ __forceinline__ int noise_kind(const NoiseGeom& g, long long i, long long& partner, long long& jk, int& x,
+                                          long long& pjk) {
   if (g.small) {
     const int q = fdiv((int)i, g.dc);
     x = (int)i - q * g.cnxp;
@@ -61,6 +63,7 @@ __device__ __forceinline__ int noise_kind(const NoiseGeom& g, long long i, long 
   const long long jkcc = jcc + (long long)kcc * g.nyp;
   if (jkcc == jk) return 1;
   if (jkcc > jk) return 2;
+  pjk = jkcc;
   partner = jkcc * g.cnxp + x;
   return 3;
 }
@@ -73,7 +76,14 @@ __device__ __forceinline__ long long noise_ref_offset(const NoiseGeom& g, long l
   const long long r = g.prefix[jk];
   if (x == 0) return r;
   // draws of element 0 of this row
-  const int j = (int)(jk % g.nyp), k = (int)(jk / g.nyp);
+  int j, k;
+  if (g.small) {
+    k = fdiv((int)jk, g.dy);
+    j = (int)jk - k * g.nyp;
+  } else {
+    j = (int)(jk % g.nyp);
+    k = (int)(jk / g.nyp);
+  }
   const int jcc = (j == 0) ? 0 : g.nyp - j, kcc = (k == 0) ? 0 : g.nzp - k;
   const long long jkcc = jcc + (long long)kcc * g.nyp;
   const int e0 = jkcc == jk ? 1 : (jkcc > jk ? 2 : 0);
@@ -84,22 +94,45 @@ __device__ __forceinline__ long long noise_ref_offset(const NoiseGeom& g, long l
 // grid's first draw
 __device__ __forceinline__ void noise_ref_value(const NoiseGeom& g, const double* normals, long long i, double& re,
                                                 double& im) {
-  long long partner = 0, jk;
+  long long partner = 0, jk, pjk = 0;
   int x;
-  const int kind = noise_kind(g, i, partner, jk, x);
+  const int kind = noise_kind(g, i, partner, jk, x, pjk);
   const double sd = noise_std();
   if (kind == 1) {
     re = (double)(float)normals[noise_ref_offset(g, jk, x)];
     im = 0.0;
-  } else if (kind == 3) {
-    const long long pjk = partner / g.cnxp;
-    const long long o = noise_ref_offset(g, pjk, x);
-    re = (double)(float)(sd * normals[o]);
-    im = -(double)(float)(sd * normals[o + 1]);
   } else {
-    const long long o = noise_ref_offset(g, jk, x);
+    const long long o = noise_ref_offset(g, kind == 3 ? pjk : jk, x);
     re = (double)(float)(sd * normals[o]);
     im = (double)(float)(sd * normals[o + 1]);
+    if (kind == 3) im = -im;
+  }
+}
+
+// reference-stream noise of the three grids of realization r at element i:
+// grid q of realization r starts at draw (3 r + q) draws of the decoded
+// normals (Crava::simulate fills seed0, seed1, seed2 of each realization in
+// turn from one generator, crava.cpp:1598-1603); values rounded to float as
+// the reference's grids hold them
+template <typename U>
+__device__ __forceinline__ void ref_noise3(const NoiseGeom& g, const double* normals, long long draws, int r,
+                                           long long i, U re[3], U im[3]) {
+  long long partner = 0, jk, pjk = 0;
+  int x;
+  const int kind = noise_kind(g, i, partner, jk, x, pjk);
+  const long long o = noise_ref_offset(g, kind == 3 ? pjk : jk, x);
+  const double sd = noise_std();
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double* nq = normals + (long long)(3 * r + q) * draws + o;
+    if (kind == 1) {
+      re[q] = (U)(float)nq[0];
+      im[q] = (U)0;
+    } else {
+      re[q] = (U)(float)(sd * nq[0]);
+      const float v = (float)(sd * nq[1]);
+      im[q] = (U)(kind == 3 ? -v : v);
+    }
   }
 }
 
@@ -211,9 +244,9 @@ __device__ __forceinline__ void sim_box_muller(unsigned a, unsigned b, double& n
 template <typename U>
 __device__ __forceinline__ void sim_noise3(const NoiseGeom& g, unsigned long long seed, unsigned long long real,
                                            long long i, U re[3], U im[3]) {
-  long long partner = 0, jk;
+  long long partner = 0, jk, pjk = 0;
   int x;
-  const int kind = noise_kind(g, i, partner, jk, x);
+  const int kind = noise_kind(g, i, partner, jk, x, pjk);
   const long long s = (kind == 3) ? partner : i;
   unsigned w[4], f[6];
   sim_words(seed, real, s, w);

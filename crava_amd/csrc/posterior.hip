@@ -98,6 +98,10 @@ struct PostArgs {
   int nsim;
   unsigned long long seed, real0;
   NoiseGeom geo;
+  // with normals: the decoded reference stream instead of the counter one
+  // (grid g of realization r from draw (3 r + g) draws; geo has the prefix)
+  const double* normals;
+  long long draws;
   // the covariances of planes outside the band are left unwritten (the z
   // inverse pass generates them from the same tables, crv_invert)
   int skip_prior_cov;
@@ -105,6 +109,14 @@ struct PostArgs {
   // too, but for the wavenumbers of the FP64 list (generated likewise)
   int skip_band_cov;
 };
+
+// the noise of realization r at wavenumber idx: the counter stream, or the
+// decoded reference stream
+__device__ __forceinline__ void fused_noise(const PostArgs<float>& a, int r, long long idx, float re[3],
+                                            float im[3]) {
+  if (a.normals) ref_noise3<float>(a.geo, a.normals, a.draws, r, idx, re, im);
+  else sim_noise3<float>(a.geo, a.seed, a.real0 + (unsigned long long)r, idx, re, im);
+}
 
 // Simulation of one wavenumber from its covariance c as stored (FP32):
 // SIM 1 factors in float with the reference's rules; SIM 2 likewise, and in
@@ -119,7 +131,7 @@ __device__ __forceinline__ void fused_sim(const PostArgs<float>& a, long long id
       float2 o[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
       if (ok) {
         float re[3], im[3];
-        sim_noise3<float>(a.geo, a.seed, a.real0 + (unsigned long long)r, idx, re, im);
+        fused_noise(a, r, idx, re, im);
         sim_apply<float, float>(Lf, re, im, o);
       }
 #pragma unroll
@@ -143,7 +155,7 @@ __device__ __forceinline__ void fused_sim_prior(const PostArgs<float>& a, long l
     float2 o[3] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
     if (ok) {
       float re[3], im[3];
-      sim_noise3<float>(a.geo, a.seed, a.real0 + (unsigned long long)r, idx, re, im);
+      fused_noise(a, r, idx, re, im);
       o[0] = make_float2(l[0] * re[0], l[0] * im[0]);
       o[1] = make_float2(l[1] * re[0] + l[2] * re[1], l[1] * im[0] + l[2] * im[1]);
       o[2] = make_float2(l[3] * re[0] + l[4] * re[1] + l[5] * re[2], l[3] * im[0] + l[4] * im[1] + l[5] * im[2]);
@@ -1294,6 +1306,10 @@ struct FusedSim {
   long long sstride = 0;
   int nsim = 0;
   unsigned long long seed = 0, real0 = 0;
+  // reference stream: the decoded normals, draws per grid, row prefix
+  const double* normals = nullptr;
+  long long draws = 0;
+  const long long* prefix = nullptr;
   bool skip_prior_cov = false;
   bool skip_band_cov = false;
 };
@@ -1345,6 +1361,7 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
       if (fs.nsim > 0) {
         a.sim = fs.sim; a.sstride = fs.sstride; a.nsim = fs.nsim; a.seed = fs.seed; a.real0 = fs.real0;
         a.geo = make_noise_geom_dims(pl->cnxp, pl->nyp, pl->nzp);
+        a.normals = fs.normals; a.draws = fs.draws; a.geo.prefix = fs.prefix;
       }
     }
     if (pl->refine) {
@@ -1479,13 +1496,19 @@ int family_xy(crv_ctx* ctx, FftPlan* fp, const Family& f, bool inverse, double s
 
 }  // namespace
 
-extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
-                          crv_grid* const* data_src, crv_grid* cov[6], unsigned long long seed,
-                          unsigned long long realization0, int nreal, crv_grid** sims, int chunk_rows) {
+extern "C" int crv_invert_noise(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
+                                crv_grid* const* data_src, crv_grid* cov[6], const crv_noise_spec* noise, int nreal,
+                                crv_grid** sims, int chunk_rows) {
   if (!pl) return CRV_E_INVALID;
   crv_ctx* ctx = pl->ctx;
-  if (!mean || !data || !cov || nreal < 0 || nreal > 21845 || (nreal > 0 && !sims))
+  if (!mean || !data || !cov || nreal < 0 || nreal > 21845 || (nreal > 0 && !sims) || !noise)
     return crv_set_error(ctx, CRV_E_INVALID, "crv_invert: bad arguments");
+  if (noise->mode != CRV_NOISE_COUNTER && noise->mode != CRV_NOISE_REFERENCE)
+    return crv_set_error(ctx, CRV_E_INVALID, "crv_invert: noise must be the counter or the reference stream");
+  if (noise->mode == CRV_NOISE_REFERENCE && !noise->lcg_state)
+    return crv_set_error(ctx, CRV_E_INVALID, "reference noise needs lcg_state");
+  const bool refnoise = noise->mode == CRV_NOISE_REFERENCE;
+  const unsigned long long seed = noise->seed, realization0 = noise->realization;
   if (!pl->separable) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "crv_invert needs a separable-prior plan");
   const int n = pl->n;
   auto fits = [&](const crv_grid* g) {
@@ -1516,6 +1539,14 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   FftPlan* fp = nullptr;
   int rc = crv_get_plan(ctx, pl->nxp, pl->nyp, pl->nzp, pl->precision, &fp);
   if (rc != CRV_OK) return rc;
+  // reference stream: the 3 nreal grids' normals decoded up front, in the
+  // order Crava::simulate draws them (crava.cpp:1598-1603)
+  const double* normals = nullptr;
+  if (refnoise && nsim > 0) {
+    if ((rc = crv_noise_prefix(ctx, fp)) != CRV_OK) return rc;
+    if ((rc = crv_reference_normals(ctx, noise->lcg_state, (long long)nsim * fp->noise_draws)) != CRV_OK) return rc;
+    normals = ctx->normals;
+  }
   const int nyp = pl->nyp;
   const Family fd = make_family(data, n), fm = make_family(mean, 3), fc = make_family(cov, 6);
   const Family fs = nsim ? make_family(sims, nsim) : Family();
@@ -1552,13 +1583,14 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
     if (fused) {
       fsim.sim = (float2*)sims[0]->d; fsim.sstride = (long long)sims[0]->rsize / 2; fsim.nsim = nreal;
       fsim.seed = seed; fsim.real0 = realization0;
+      fsim.normals = normals; fsim.draws = fp->noise_draws; fsim.prefix = fp->noise_prefix;
     }
     fsim.skip_prior_cov = synth;
     fsim.skip_band_cov = synth_band;
     if ((rc = exec_rows(pl, mean, data, nullptr, cov, j0, j1, j0 == 0, fsim)) != CRV_OK) return rc;
     if (chunked) {
       if (nsim && (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed,
-                                            realization0, pl->refine ? 2 : 0, j0, j1)) != CRV_OK)
+                                            realization0, pl->refine ? 2 : 0, j0, j1, normals)) != CRV_OK)
         return rc;
       if ((rc = family_pass(ctx, fp, fm, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
       if ((rc = family_pass(ctx, fp, fc, 3, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
@@ -1573,7 +1605,7 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
     return rc;
   if (!chunked && nsim && !fused &&
       (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed, realization0,
-                                pl->refine ? 2 : 0, 0, nyp)) != CRV_OK)
+                                pl->refine ? 2 : 0, 0, nyp, normals)) != CRV_OK)
     return rc;
   if (nsim) {
     for (int b = 0; b < nsim; ++b) sims[b]->cubetype = CRV_PARAMETER;
@@ -1601,6 +1633,17 @@ extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data,
   for (int q = 0; q < 6; ++q) cov[q]->transformed = false;
   for (int b = 0; b < nsim; ++b) sims[b]->transformed = false;
   return CRV_OK;
+}
+
+extern "C" int crv_invert(crv_post_plan* pl, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
+                          crv_grid* const* data_src, crv_grid* cov[6], unsigned long long seed,
+                          unsigned long long realization0, int nreal, crv_grid** sims, int chunk_rows) {
+  crv_noise_spec ns;
+  memset(&ns, 0, sizeof(ns));
+  ns.mode = CRV_NOISE_COUNTER;
+  ns.seed = seed;
+  ns.realization = realization0;
+  return crv_invert_noise(pl, mean, data, mean_src, data_src, cov, &ns, nreal, sims, chunk_rows);
 }
 
 extern "C" int crv_post_plan_stats(crv_post_plan* pl, crv_post_stats* stats) {

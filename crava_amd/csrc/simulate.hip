@@ -31,7 +31,7 @@ struct SimArgs {
   cplx_t<T>* out[3];
   NoiseGeom geo;
   int mode;                  // CRV_NOISE_*
-  const double* normals;     // reference: 3 consecutive grids of D draws each
+  const double* normals;     // reference: grid g of realization r at (3 r + g) D
   long long draws;           // D
   const cplx_t<T>* host[3];  // host-supplied noise (uploaded)
   unsigned long long seed, real;
@@ -84,17 +84,13 @@ __global__ __launch_bounds__(kThreads) void k_simulate(SimArgs<T> a) {
         T re[3], im[3];
         if (a.mode == CRV_NOISE_COUNTER) {
           sim_noise3<T>(a.geo, a.seed, a.real + (unsigned long long)r, i, re, im);
+        } else if (a.mode == CRV_NOISE_REFERENCE) {
+          ref_noise3<T>(a.geo, a.normals, a.draws, r, i, re, im);
         } else {
 #pragma unroll
           for (int g = 0; g < 3; ++g) {
-            if (a.mode == CRV_NOISE_REFERENCE) {
-              double r_, i_;
-              noise_ref_value(a.geo, a.normals + g * a.draws, i, r_, i_);
-              re[g] = (T)r_; im[g] = (T)i_;
-            } else {
-              const C h = a.host[g][i];
-              re[g] = h.x; im[g] = h.y;
-            }
+            const C h = a.host[g][i];
+            re[g] = h.x; im[g] = h.y;
           }
         }
         sim_apply<T, U>(Lf, re, im, o);
@@ -263,7 +259,8 @@ extern "C" int crv_fill_complex_noise(crv_grid* g, unsigned int* lcg_state) {
 // rows [j0, j1) of every kz plane: realization r, parameter g at
 // out0 + (3 r + g) gstride complex elements (see internal.hpp).
 int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], void* out0, long long gstride,
-                        int nreal, unsigned long long seed, unsigned long long real0, int fp64_chol, int j0, int j1) {
+                        int nreal, unsigned long long seed, unsigned long long real0, int fp64_chol, int j0, int j1,
+                        const double* normals) {
   const int cnxp = plan->nxp / 2 + 1;
   const long long nc = (long long)cnxp * plan->nyp * plan->nzp;
   if (nc >= (1LL << 31)) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "simulate: grid of 2^31 or more wavenumbers");
@@ -276,7 +273,8 @@ int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], v
     SimArgs<double> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const double2*)cov[q];
     for (int g = 0; g < 3; ++g) a.out[g] = (double2*)out0 + g * gstride;
-    a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
+    a.geo = make_noise_geom(plan); a.mode = normals ? CRV_NOISE_REFERENCE : CRV_NOISE_COUNTER;
+    a.normals = normals; a.draws = plan->noise_draws;
     a.seed = seed; a.real = real0; a.nc = nc; a.nreal = nreal; a.rstride = 3 * gstride;
     sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
     hipLaunchKernelGGL((k_simulate<double, 0>), dim3(nb), dim3(kThreads), 0, ctx->stream, a);
@@ -284,7 +282,8 @@ int crv_launch_sim_rows(crv_ctx* ctx, FftPlan* plan, const void* const cov[6], v
     SimArgs<float> a{};
     for (int q = 0; q < 6; ++q) a.cov[q] = (const float2*)cov[q];
     for (int g = 0; g < 3; ++g) a.out[g] = (float2*)out0 + g * gstride;
-    a.geo = make_noise_geom(plan); a.mode = CRV_NOISE_COUNTER;
+    a.geo = make_noise_geom(plan); a.mode = normals ? CRV_NOISE_REFERENCE : CRV_NOISE_COUNTER;
+    a.normals = normals; a.draws = plan->noise_draws;
     a.seed = seed; a.real = real0; a.nc = nc; a.nreal = nreal; a.rstride = 3 * gstride;
     sim_rows(a, cnxp, plan->nyp, plan->nzp, j0, j1);
     if (fp64_chol == 2)

@@ -254,6 +254,15 @@ def simulate(ctx: Context, cov, post_mean, out, *, noise_mode=L.CRV_NOISE_COUNTE
     return st.value
 
 
+def noise_guard(ctx: Context, scale=0.0, force_table=False):
+    """crv_noise_guard: set the reference decode's guard scale (0 keeps it)
+    and whether the next decode tables every normal near a margin; returns
+    the last decode's (fix passes, host-evaluated, patched, runs)."""
+    info = (C.c_longlong * 4)()
+    ctx.check(ctx.lib.crv_noise_guard(ctx.h, float(scale), int(bool(force_table)), info), "noise_guard")
+    return tuple(int(v) for v in info)
+
+
 def fill_prior_spectra(ctx: Context, sigma0, cxy, ct, cov, noalloc=False):
     """crv_fill_prior_spectra, or with noalloc its allocation-free variant."""
     s0 = np.ascontiguousarray(sigma0, np.float64)
@@ -366,15 +375,28 @@ class PostPlan:
                        "posterior_exec")
 
     def invert(self, mean, data, cov, sims=None, *, mean_src=None, data_src=None, seed=0, realization0=0,
-               chunk_rows=0):
+               chunk_rows=0, lcg_state=None):
         """crv_invert: forward transforms of the real means and data (or of
         mean_src / data_src into them), the posterior, len(sims) // 3
         realizations and the inverse transforms, pipelined by chunks of y
-        rows."""
+        rows.  With lcg_state the realizations draw the reference stream
+        from that state (crv_invert_noise) and the updated state is
+        returned."""
         arr = lambda gs: (C.c_void_p * len(gs))(*[g.h.value for g in gs]) if gs else None
         sims = sims or []
-        self.ctx.check(self.ctx.lib.crv_invert(self.h, arr(mean), arr(data), arr(mean_src), arr(data_src), arr(cov),
-                                               seed, realization0, len(sims) // 3, arr(sims), chunk_rows), "invert")
+        if lcg_state is None:
+            self.ctx.check(self.ctx.lib.crv_invert(self.h, arr(mean), arr(data), arr(mean_src), arr(data_src),
+                                                   arr(cov), seed, realization0, len(sims) // 3, arr(sims),
+                                                   chunk_rows), "invert")
+            return None
+        spec = L.NoiseSpec()
+        spec.mode = L.CRV_NOISE_REFERENCE
+        st = C.c_uint(int(lcg_state) & 0xFFFFFFFF)
+        spec.lcg_state = C.pointer(st)
+        self.ctx.check(self.ctx.lib.crv_invert_noise(self.h, arr(mean), arr(data), arr(mean_src), arr(data_src),
+                                                     arr(cov), C.byref(spec), len(sims) // 3, arr(sims),
+                                                     chunk_rows), "invert_noise")
+        return st.value
 
     def stats(self) -> PostResult:
         st = L.PostStats()

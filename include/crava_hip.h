@@ -358,6 +358,22 @@ typedef struct crv_noise_spec {
   const void*        host_noise[3];/* CRV_NOISE_HOST                          */
 } crv_noise_spec;
 
+/* crv_invert with the realizations' noise chosen by noise (crv_noise_spec
+ * above): CRV_NOISE_COUNTER (seed, realization: crv_invert itself) or
+ * CRV_NOISE_REFERENCE, the reference's RandomGen stream from
+ * *noise->lcg_state (updated), drawn as Crava::simulate draws it
+ * (crava.cpp:1598-1603: seed0, seed1, seed2 of realization 0, then of
+ * realization 1, ...).  With the reference stream the 3 nreal grids'
+ * normals are decoded first (crv_fill_complex_noise's decode; it
+ * synchronises the stream, so this form is not capturable), and the
+ * realizations equal crv_simulate's with the same state on the same
+ * covariances and means (fp64_chol 2 on FP32 refine plans, 0 otherwise),
+ * bit for bit, but for the out-of-band rounding exception stated for
+ * crv_invert.  CRV_NOISE_HOST is not accepted here.                       */
+int crv_invert_noise(crv_post_plan* plan, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
+                     crv_grid* const* data_src, crv_grid* cov[6], const crv_noise_spec* noise, int nreal,
+                     crv_grid** sims, int chunk_rows);
+
 /* cov: posterior covariance spectra (aa, bb, rr, ab, ar, br), transformed.
  * post_mean: posterior means in real space (untransformed) or NULL.
  * out[3]: destination grids (alpha, beta, rho realization), real on return.
@@ -377,6 +393,21 @@ int crv_fill_complex_noise(crv_grid* g, unsigned int* lcg_state);
 /* Number of N(0,1) draws FFTGrid::fillInComplexNoise makes on one grid of
  * this padded size (a function of nxp, nyp, nzp only). */
 long long crv_noise_draws(int nxp, int nyp, int nzp);
+
+/* The reference stream's decode (used by crv_fill_complex_noise,
+ * crv_simulate and crv_invert_noise) evaluates RandomGen::rnorm01
+ * (libs/lib/random.cpp:38-111) on the device; a normal whose accept/reject
+ * decision (the exp of g(x), the log of the polar branch) or float-rounded
+ * value lies within a guard margin of where the device and the host libm
+ * could disagree is evaluated again on the host, so the decoded stream
+ * equals the host's sequential one.  No reference counterpart: a test hook.
+ * scale > 0 multiplies the margins (1 by default; larger sends more normals
+ * to the host), 0 keeps the current scale; force_table != 0 makes the next
+ * decode put every normal near a margin into its host table and decode
+ * again (the path a differing decision takes).  info (may be NULL)
+ * receives the last decode's fix passes, host-evaluated normals, values
+ * patched from the host and decode runs. */
+int crv_noise_guard(crv_ctx* ctx, double scale, int force_table, long long info[4]);
 
 /* ---- inputs of the path generated on the GPU ----------------------------
  * crv_fill_prior_spectra: the six prior covariance spectra that

@@ -52,6 +52,37 @@ def test_noise_fill_long_stream(ctx, shape, seed):
     assert mism == 0
 
 
+@pytest.mark.parametrize("force", [False, True])
+def test_noise_guard_host_resolution(ctx, force):
+    """The decode's guard sends normals whose decisions or float roundings
+    lie near where the device and host libm could differ to the host.
+    With the margins widened 1e9 times hundreds of normals take that path:
+    their values are patched from the host, or (force) the decode runs
+    again with them in its table.  Either way the stream equals the
+    oracle's sequential one and ends in the same state."""
+    from crava_amd.hip import FFTGrid, noise_guard
+    nx, ny, nz, seed = 64, 48, 40, 99
+    ref, st_ref = O.fill_complex_noise(nx, ny, nz, seed)
+    try:
+        noise_guard(ctx, 1e9, force)
+        g = FFTGrid(ctx, nx, ny, nz, precision=32)
+        st = g.fillInComplexNoise(seed)
+        passes, host, patched, runs = noise_guard(ctx)
+    finally:
+        noise_guard(ctx, 1.0, False)
+    print(f"guard x1e9 force={force}: {passes} fix passes, {host} host normals, {patched} patched, {runs} runs")
+    assert st == st_ref
+    assert np.array_equal(g.download_complex().astype(np.complex64), ref.astype(np.complex64))
+    assert host > 100
+    if force:
+        assert runs == 2 and patched == 0
+    else:
+        assert patched > 50
+    g2 = FFTGrid(ctx, nx, ny, nz, precision=32)
+    assert g2.fillInComplexNoise(seed) == st_ref
+    assert noise_guard(ctx)[1] == 0
+
+
 def test_noise_fill_c1_nyquist_plane(ctx):
     """The reference stream at the 450x450x320 grid (cnxp 226, even): every
     point of the x = 225 plane is "special" under the global-parity rule
