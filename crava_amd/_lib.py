@@ -37,7 +37,7 @@ EXPORTS = [
     "crv_storm_header", "crv_grid_write_storm",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
     "crv_fill_prior_spectra_noalloc", "crv_circ_corr_t", "crv_facies_density_kernel",
-    "crv_invert_noise", "crv_noise_guard",
+    "crv_invert_noise", "crv_noise_guard", "crv_krig_post",
 ]
 CRV_SMOOTH_FOURIER, CRV_SMOOTH_SAMPLED = 0, 1
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
@@ -105,6 +105,21 @@ class FftShape(C.Structure):
 class MlpNorm(C.Structure):
     _fields_ = [("in_lo", C.c_float * 5), ("in_scale", C.c_float * 5), ("in_log_mask", C.c_int),
                 ("out_mean", C.c_float * 3), ("out_std", C.c_float * 3)]
+
+
+class WellPoint(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("k", C.c_int), ("alpha", C.c_float), ("beta", C.c_float),
+                ("rho", C.c_float)]
+
+
+class KrigStats(C.Structure):
+    _fields_ = [("block", C.c_int * 3), ("block_ext", C.c_int * 3), ("range", C.c_int * 3),
+                ("data_points", C.c_int), ("blocks", C.c_int), ("empty_blocks", C.c_int * 3),
+                ("max_block_data", C.c_int * 3), ("cells_updated", C.c_longlong * 3),
+                ("cells_missing", C.c_longlong * 3)]
+
+
+CRV_WELL_MISSING = -99999.0
 
 
 class NoiseSpec(C.Structure):
@@ -187,6 +202,7 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_invert_noise": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                  C.POINTER(NoiseSpec), i, C.POINTER(vp), i]),
         "crv_noise_guard": (i, [vp, C.c_double, i, C.POINTER(C.c_longlong)]),
+        "crv_krig_post": (i, [C.POINTER(vp), C.POINTER(vp), C.POINTER(WellPoint), i, i, i, C.POINTER(KrigStats)]),
         "crv_fft_autotune": (i, [vp, i, i, i, i, i, C.POINTER(FftShape)]),
         "crv_grid_create_batch": (i, [vp, i, i, i, i, i, i, i, i, i, C.POINTER(vp)]),
         "crv_simulate_batch": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.c_ulonglong, C.c_ulonglong, i, i,

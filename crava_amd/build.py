@@ -19,7 +19,7 @@ LIB = PKG / "libcrava_hip.so"
 HOSTLIB = PKG / "libcrava_host.so"
 
 HIP_SOURCES = ["capi.hip", "fft.hip", "posterior.hip", "noise.hip", "simulate.hip", "synth.hip",
-               "rockphysics.hip", "mlp.hip", "facies.hip", "output.hip"]
+               "rockphysics.hip", "mlp.hip", "facies.hip", "output.hip", "krig.hip"]
 HOST_SOURCES = ["host/fftgrid.cpp", "host/crava.cpp", "host/model.cpp", "host/hostapi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

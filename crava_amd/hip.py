@@ -254,6 +254,26 @@ def simulate(ctx: Context, cov, post_mean, out, *, noise_mode=L.CRV_NOISE_COUNTE
     return st.value
 
 
+def krig_post(ctx: Context, mean, cov, points, data_target, smooth=False) -> dict:
+    """crv_krig_post (Crava::doPostKriging): the real posterior means mean[3]
+    kriged in place to the well points (rows i, j, k, log Vp, log Vs,
+    log rho; L.CRV_WELL_MISSING where absent) with the six real posterior
+    covariance grids cov[6].  Returns the planner's statistics."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 6)
+    arr = (L.WellPoint * max(1, len(pts)))()
+    for n, r in enumerate(pts):
+        arr[n] = L.WellPoint(int(r[0]), int(r[1]), int(r[2]), float(r[3]), float(r[4]), float(r[5]))
+    st = L.KrigStats()
+    means = (C.c_void_p * 3)(*[g.h.value for g in mean])
+    covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+    ctx.check(ctx.lib.crv_krig_post(means, covs, arr, len(pts), int(data_target), int(bool(smooth)), C.byref(st)),
+              "krig_post")
+    return dict(block=tuple(st.block), block_ext=tuple(st.block_ext), range=tuple(st.range),
+                data_points=st.data_points, blocks=st.blocks, empty_blocks=tuple(st.empty_blocks),
+                max_block_data=tuple(st.max_block_data), cells_updated=tuple(st.cells_updated),
+                cells_missing=tuple(st.cells_missing))
+
+
 def noise_guard(ctx: Context, scale=0.0, force_table=False):
     """crv_noise_guard: set the reference decode's guard scale (0 keeps it)
     and whether the next decode tables every normal near a margin; returns

@@ -638,6 +638,39 @@ int crv_storm_header(const crv_storm_info* si, int cubetype, int nx, int ny, int
 int crv_grid_write_storm(crv_grid* g, const char* path, const crv_storm_info* si, int ascii, int padding,
                          int scientific);
 
+/* ---- prediction kriging to wells ---------------------------------------
+ * Crava::doPostKriging (src/crava.cpp:1736-1765): the posterior means
+ * mean[3] (real, updated in place) kriged to well data with the separable
+ * covariances of the six real posterior covariance grids cov[6] (aa, bb,
+ * rr, ab, ar, br), as CKrigingAdmin::KrigAll does it (src/krigingadmin.cpp:
+ * block size and data neighbourhood from the covariance ranges, per block
+ * a kriging system over up to data_target (+-10 %) data, every cell
+ * updated with k(cell) . x).  pts: the blocked-log points, log Vp, log Vs,
+ * log rho per cell of the simbox (0 <= i < nx ...), CRV_WELL_MISSING where
+ * a value is absent; points of one cell are averaged.  The block systems
+ * are solved on the host (FP64 Cholesky, where the reference calls LAPACK
+ * posv); the cells are updated on the device.  smooth != 0 (the
+ * smooth-kriged-parameters option, off by default in the reference) is not
+ * built and returns CRV_E_UNSUPPORTED.  Synchronises.                     */
+#define CRV_WELL_MISSING (-99999.0f)
+typedef struct crv_well_point {
+  int   i, j, k;
+  float alpha, beta, rho;
+} crv_well_point;
+typedef struct crv_krig_stats {
+  int       block[3];          /* dxBlock, dyBlock, dzBlock               */
+  int       block_ext[3];      /* data neighbourhood beyond the block     */
+  int       range[3];          /* covariance ranges used (cells)          */
+  int       data_points;       /* after merging points of one cell        */
+  int       blocks;            /* blocks per parameter                    */
+  int       empty_blocks[3];   /* blocks without data, per parameter      */
+  int       max_block_data[3]; /* largest system, per parameter           */
+  long long cells_updated[3];
+  long long cells_missing[3];  /* trend RMISSING: left as it is           */
+} crv_krig_stats;
+int crv_krig_post(crv_grid* mean[3], crv_grid* cov[6], const crv_well_point* pts, int npts, int data_target,
+                  int smooth, crv_krig_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
