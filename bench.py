@@ -156,6 +156,37 @@ def general_leg(ctx, s, sv, kw, steps, warmup):
             "GB/s": bpv * s.voxels / sec / 1e9, "fp64_solves": st.n_fp64, "kernels_ms": prof}
 
 
+def krig_leg(ctx, s, G, nwells=8, data_target=250):
+    """Prediction kriging of the FP32 posterior means to synthetic wells
+    (crv_krig_post, Crava::doPostKriging): nwells vertical wells on a
+    regular pattern, a sample at every other cell, values the posterior mean
+    there plus N(0, 0.03^2) (log units); the covariances are the posterior's.
+    Reports the planner's blocks and the time of the call (host planning and
+    block solves included) and of its cell kernel."""
+    from crava_amd import hip
+    rng = np.random.default_rng(11)
+    means = [g.download_real() for g in G["mean"]]
+    cols = [(int((2 * (w % 4) + 1) * s.nx / 8), int((2 * (w // 4) + 1) * s.ny / (2 * ((nwells + 3) // 4))))
+            for w in range(nwells)]
+    pts = [[i, j, k] + [float(means[a][k, j, i]) + float(rng.normal(0, 0.03)) for a in range(3)]
+           for (i, j) in cols for k in range(0, s.nz, 2)]
+    ctx.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    st = hip.krig_post(ctx, G["mean"], G["cov"], pts, data_target)
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    ctx.profile(False)
+    prof = ctx.profile_report()
+    kern = prof.get("krig_cells", (0, 0.0))
+    log(f"[bench] kriging: {len(pts)} well points, blocks {st['block']} x{st['blocks']}, largest systems "
+        f"{st['max_block_data']}, call {sec * 1e3:.1f} ms, cell kernel {kern[1]:.2f} ms")
+    return {"wells": nwells, "points": len(pts), "data_target": data_target, "block": st["block"],
+            "block_ext": st["block_ext"], "blocks": st["blocks"], "max_block_data": st["max_block_data"],
+            "cells_updated": st["cells_updated"], "call_ms": sec * 1e3, "cell_kernel_ms": kern[1],
+            "cell_kernel_launches": kern[0]}
+
+
 def transforms_per_step(name, s, names=()):
     """3D transforms per step whose passes carry this kernel name: forward
     for the n data and 3 means, inverse for 3 means, 3 realization grids
@@ -622,6 +653,7 @@ def main():
                     help="c1: headline 450x450x320 inversion; c0: 64x64x128; c3: batched realizations and volumes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-krig", action="store_true", help="skip the kriging of the means to synthetic wells")
     ap.add_argument("--no-refstream", action="store_true",
                     help="skip the step with the reference RandomGen stream for the realization")
     ap.add_argument("--no-general", action="store_true",
@@ -809,6 +841,9 @@ def main():
         errors = {k: {"rel_rms": v[0], "max_rel": v[1]} for k, v in errors.items()}
         log("[bench] FP32+refine vs FP64: " + ", ".join(f"{k} {v['rel_rms']:.1e}" for k, v in errors.items()))
 
+    # kriging to wells last: it updates the FP32 means in place
+    kriging = None if a.no_krig else krig_leg(ctx, s, res[32]["grids"])
+
     sec = res[32]["sec"]
     value = s.voxels / sec
     prof = res[32]["prof"]
@@ -843,6 +878,7 @@ def main():
         "posterior_stats": res[32]["stats"],
         "general_path": general,
         "reference_stream": refstream,
+        "kriging": kriging,
         "transfers": transfers,
         "kernels": kernels,
         "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}

@@ -54,7 +54,7 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
     # host C++ mirror of the reference classes, linked against the C ABI only
     host_srcs = [CSRC / s for s in HOST_SOURCES if (CSRC / s).exists()]

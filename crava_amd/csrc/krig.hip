@@ -24,7 +24,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -64,11 +66,9 @@ int cov_sep_from_grid(crv_grid* g, CovSep& c) {
     for (int i = 0; i < c.nxp; ++i) c.xy[i + (size_t)j * c.nxp] = val(plane.data(), i + (size_t)g->rnxp * j);
   std::vector<float> col(c.nzp);
   const size_t kstride = (size_t)g->rnxp * g->nyp * eb;
-  for (int k = 0; k < c.nzp; ++k) {
-    char v[8];
-    CRV_HIP_CHECK(ctx, hipMemcpy(v, (const char*)g->d + k * kstride, eb, hipMemcpyDeviceToHost));
-    col[k] = val(v, 0);
-  }
+  std::vector<char> colv((size_t)c.nzp * eb);
+  CRV_HIP_CHECK(ctx, hipMemcpy2D(colv.data(), eb, g->d, kstride, eb, c.nzp, hipMemcpyDeviceToHost));
+  for (int k = 0; k < c.nzp; ++k) col[k] = val(colv.data(), k);
   // the first local maximum of |C(0, 0, k)| from k = 0 (its signed value
   // normalises the column)
   float maxval = std::fabs(col[0]);
@@ -217,6 +217,7 @@ struct Plan {
   int nx, ny, nz;
   CovSep cov[6];  // aa bb rr ab ar br
   std::vector<WellPt> pts;
+  std::vector<int> by_i, ival;  // point indices sorted by i (and index); their i
   int nvalid = 0;
   float rangeX = 0, rangeY = 0, rangeZ = 0;
   int rA[3], rB[3], rR[3];
@@ -332,14 +333,23 @@ void block_size(Plan& p) {  // EstimateSizeOfBlock, krigingadmin.cpp:768-923
 // point indices per parameter; a point whose own parameter g is valid adds
 // only that one, else its other valid parameters
 enum Fit { kRight, kSmall, kBig };
-Fit find_data(const Plan& p, int g, const Box& box, std::vector<int> lists[3]) {
+Fit find_data(const Plan& p, int g, const Box& box, std::vector<int> lists[3], std::vector<int>& hit) {
   for (int a = 0; a < 3; ++a) lists[a].clear();
+  // the points inside, in their order (through the index by i)
+  hit.clear();
+  const auto lo = std::lower_bound(p.ival.begin(), p.ival.end(), box.i0);
+  const auto hi = std::upper_bound(p.ival.begin(), p.ival.end(), box.i1);
+  for (auto it = lo; it != hi; ++it) {
+    const int m = p.by_i[it - p.ival.begin()];
+    const WellPt& w = p.pts[m];
+    if (box.inside(w.i, w.j, w.k)) hit.push_back(m);
+  }
+  std::sort(hit.begin(), hit.end());
   int total = 0;
   const int cmin = (int)(p.target * (1.0f - kTolPercent / 100.0f));
   const int cmax = (int)(p.target * (1.0f + kTolPercent / 100.0f));
-  for (int m = 0; m < (int)p.pts.size(); ++m) {
+  for (const int m : hit) {
     const WellPt& w = p.pts[m];
-    if (!box.inside(w.i, w.j, w.k)) continue;
     if (w.nv[g] > 0) {
       lists[g].push_back(m);
       ++total;
@@ -357,18 +367,19 @@ Fit find_data(const Plan& p, int g, const Box& box, std::vector<int> lists[3]) {
 
 // FindDataInDataBlockLoop, krigingadmin.cpp:519-575: the lists of the last
 // search made
-void find_data_loop(const Plan& p, int g, const Box& block, Box data, std::vector<int> lists[3]) {
-  Fit cur = find_data(p, g, data, lists);
+void find_data_loop(const Plan& p, int g, const Box& block, Box data, std::vector<int> lists[3],
+                    std::vector<int>& hit) {
+  Fit cur = find_data(p, g, data, lists, hit);
   Box minB = data;
   Box maxB = make_box(block.i0 - (int)p.rangeX, block.j0 - (int)p.rangeY, block.k0 - (int)p.rangeZ,
                       block.i1 + (int)p.rangeX, block.j1 + (int)p.rangeY, block.k1 + (int)p.rangeZ, nullptr);
   if (cur == kSmall) {
-    if (find_data(p, g, maxB, lists) != kBig) {
+    if (find_data(p, g, maxB, lists, hit) != kBig) {
       data = maxB;
       cur = kRight;
     }
   } else if (cur == kBig) {
-    if (find_data(p, g, minB, lists) != kSmall) {
+    if (find_data(p, g, minB, lists, hit) != kSmall) {
       data = minB;
       cur = kRight;
     }
@@ -382,7 +393,7 @@ void find_data_loop(const Plan& p, int g, const Box& block, Box data, std::vecto
       move_half(data, minB);
     }
     if (data == maxB || data == minB) break;
-    cur = find_data(p, g, data, lists);
+    cur = find_data(p, g, data, lists, hit);
   }
 }
 
@@ -416,67 +427,68 @@ bool chol_solve(std::vector<double>& K, int n, std::vector<double>& x) {
 
 // ---- device ---------------------------------------------------------------
 
-struct KTile { int block, i0, j0, k0, ni, nj, nk; };
 struct KBlock { int off, n; };
 struct KTabs {
   const float* xy[3];   // per data type (alpha, beta, rho data): the table used
   const float* z[3];
   int flip[3];          // 1: covariance from the datum to the cell
 };
+struct KGeo {
+  int nx, ny, nz;       // simbox
+  int dxB, dyB, dzB, nbx, nby;
+  int nxp, nyp, nzp;    // padded (the covariance window)
+  long long sj, sk;     // grid strides of y and z
+};
 
 constexpr int kKrigThreads = 256;
-constexpr int kKrigStage = 512;
 
-// one tile of cells of one block: result = trend + sum_d gamma_d(cell) x_d,
-// the sum in FP64 over the block's data in their order (alpha, beta, rho)
+// every cell of the simbox: result = trend + sum_d gamma_d(cell) x_d over
+// the data of the cell's block, in the system's order (alpha, beta, rho
+// data), the sum in FP64; cells of blocks without data are left alone
 template <typename T>
-__global__ __launch_bounds__(kKrigThreads) void k_krig_cells(T* trend, long long sj, long long sk, const KTile* tiles,
-                                                             const KBlock* blocks, const int4* dpos, const double* dx,
-                                                             KTabs tabs, int nxp, int nyp, int nzp,
+__global__ __launch_bounds__(kKrigThreads) void k_krig_cells(T* trend, KGeo g, const KBlock* blocks,
+                                                             const int4* dpos, const double* dx, KTabs tabs,
                                                              unsigned long long* counts) {
-  __shared__ int4 sp[kKrigStage];
-  __shared__ double sx[kKrigStage];
-  const KTile tl = tiles[blockIdx.x];
-  const KBlock bk = blocks[tl.block];
-  const int t = threadIdx.x;
-  const int ci = tl.i0 + t % tl.ni, cj = tl.j0 + (t / tl.ni) % tl.nj, ck = tl.k0 + t / (tl.ni * tl.nj);
-  const bool mine = t < tl.ni * tl.nj * tl.nk;
-  double s = 0.0;
-  for (int c0 = 0; c0 < bk.n; c0 += kKrigStage) {
-    const int cn = min(kKrigStage, bk.n - c0);
-    for (int e = t; e < cn; e += kKrigThreads) {
-      sp[e] = dpos[bk.off + c0 + e];
-      sx[e] = dx[bk.off + c0 + e];
-    }
-    __syncthreads();
-    if (mine) {
-      for (int e = 0; e < cn; ++e) {
-        const int4 d = sp[e];
+  const long long ncell = (long long)g.nx * g.ny * g.nz;
+  const long long c = (long long)blockIdx.x * kKrigThreads + threadIdx.x;
+  bool updated = false, missing = false;
+  if (c < ncell) {
+    const int ci = (int)(c % g.nx);
+    const long long r = c / g.nx;
+    const int cj = (int)(r % g.ny), ck = (int)(r / g.ny);
+    const KBlock bk = blocks[ci / g.dxB + g.nbx * (cj / g.dyB + g.nby * (ck / g.dzB))];
+    if (bk.n > 0) {
+      double s = 0.0;
+      for (int e = 0; e < bk.n; ++e) {
+        const int4 d = dpos[bk.off + e];
         const int ty = d.w;
         const int sg = tabs.flip[ty] ? -1 : 1;
         int di = sg * (d.x - ci), dj = sg * (d.y - cj), dk = sg * (d.z - ck);
-        float g = 0.0f;
-        if (abs(di) < nxp / 2 && abs(dj) < nyp / 2 && abs(dk) < nzp / 2) {
-          if (di < 0) di += nxp;
-          if (dj < 0) dj += nyp;
-          if (dk < 0) dk += nzp;
-          g = tabs.xy[ty][di + dj * nxp] * tabs.z[ty][dk];
+        float gv = 0.0f;
+        if (abs(di) < g.nxp / 2 && abs(dj) < g.nyp / 2 && abs(dk) < g.nzp / 2) {
+          if (di < 0) di += g.nxp;
+          if (dj < 0) dj += g.nyp;
+          if (dk < 0) dk += g.nzp;
+          gv = tabs.xy[ty][di + dj * g.nxp] * tabs.z[ty][dk];
         }
-        s = fma((double)g, sx[e], s);
+        s = fma((double)gv, dx[bk.off + e], s);
+      }
+      T* cell = trend + ci + g.sj * cj + g.sk * ck;
+      const T v = *cell;
+      if (v == (T)kMissing) {
+        missing = true;
+      } else {
+        if constexpr (sizeof(T) == 4) *cell = v + (float)s;  // float result + float(k . x), as the reference
+        else *cell = v + s;
+        updated = true;
       }
     }
-    __syncthreads();
   }
-  if (mine) {
-    T* c = trend + ci + sj * cj + sk * ck;
-    const T v = *c;
-    if (v == (T)kMissing) {
-      atomicAdd(&counts[1], 1ull);
-    } else {
-      if constexpr (sizeof(T) == 4) *c = v + (float)s;  // float result + float(k . x), as the reference
-      else *c = v + s;
-      atomicAdd(&counts[0], 1ull);
-    }
+  // the counts, one atomic per wave
+  const unsigned long long bu = __ballot(updated), bm = __ballot(missing);
+  if ((threadIdx.x & 63) == 0) {
+    if (bu) atomicAdd(&counts[0], (unsigned long long)__popcll(bu));
+    if (bm) atomicAdd(&counts[1], (unsigned long long)__popcll(bm));
   }
 }
 
@@ -587,15 +599,25 @@ extern "C" int crv_krig_post(crv_grid* mean[3], crv_grid* cov[6], const crv_well
       for (int m = 0; m < np; ++m)
         if (p.pts[m].nv[a] > 0 && tr[a * np + m] != kMissing) p.pts[m].v[a] -= tr[a * np + m];
   }
-  // per parameter: the blocks' systems, then every cell on the device
+  // per parameter: the blocks' systems (host threads over the blocks), then
+  // every cell on the device
   const int nbx = (p.nx + p.dxB - 1) / p.dxB, nby = (p.ny + p.dyB - 1) / p.dyB, nbz = (p.nz + p.dzB - 1) / p.dzB;
-  st.blocks = nbx * nby * nbz;
+  const long long nblocks = (long long)nbx * nby * nbz;
+  if (nblocks > (1LL << 30)) return crv_set_error(ctx, CRV_E_UNSUPPORTED, "kriging: too many blocks");
+  st.blocks = (int)nblocks;
+  p.by_i.resize(np);
+  for (int m = 0; m < np; ++m) p.by_i[m] = m;
+  std::sort(p.by_i.begin(), p.by_i.end(), [&](int x, int y) {
+    return p.pts[x].i != p.pts[y].i ? p.pts[x].i < p.pts[y].i : x < y;
+  });
+  p.ival.resize(np);
+  for (int m = 0; m < np; ++m) p.ival[m] = p.pts[p.by_i[m]].i;
   // covariance table per (kriged parameter, data type) and its direction
-  // (SetKrigVector: alpha uses aa, ab, ar; beta ab~, bb~, br; rho ar~, br~, rr~)
+  // (SetKrigVector: alpha uses aa, ab, ar; beta ab~, bb~, br; rho ar~, br~,
+  // rr~); the system's upper triangle uses table tab_of[row type][column
+  // type] from the row datum to the column datum (SetMatrix)
   const int tab_of[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
   const int flip_of[3][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}};
-  // upper-triangle entries of the system (SetMatrix): table and direction
-  // of the covariance between data of types (row, column), row <= column
   float* dtab = nullptr;
   const size_t nxy = (size_t)p.cov[0].nxp * p.cov[0].nyp, nzp = p.cov[0].nzp;
   CRV_HIP_CHECK(ctx, hipMalloc(&dtab, 6 * (nxy + nzp) * sizeof(float)));
@@ -615,81 +637,101 @@ extern "C" int crv_krig_post(crv_grid* mean[3], crv_grid* cov[6], const crv_well
     (void)hipFree(dtab);
     (void)hipFree(dcount);
   };
+  const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   for (int g = 0; g < 3; ++g) {
-    std::vector<KTile> tiles;
-    std::vector<KBlock> blocks;
+    // thread t plans blocks [t nblocks / nthr, (t + 1) nblocks / nthr)
+    struct Part {
+      std::vector<int4> pos;
+      std::vector<double> x;
+      int empty = 0, maxn = 0;
+      bool fail = false;
+    };
+    std::vector<Part> part(nthr);
+    std::vector<KBlock> blocks(nblocks);
+    auto plan_range = [&](int t) {
+      Part& P = part[t];
+      std::vector<int> lists[3], hit, idx, ty;
+      std::vector<double> K, x;
+      const long long b0 = nblocks * t / nthr, b1 = nblocks * (t + 1) / nthr;
+      for (long long b = b0; b < b1 && !P.fail; ++b) {
+        const int bi = (int)(b % nbx), bj = (int)((b / nbx) % nby), bk = (int)(b / ((long long)nbx * nby));
+        const int i1 = bi * p.dxB, j1 = bj * p.dyB, k1 = bk * p.dzB;
+        const Box block = make_box(i1, j1, k1, i1 + p.dxB - 1, j1 + p.dyB - 1, k1 + p.dzB - 1, dims);
+        const Box data = make_box(i1 - p.dxE, j1 - p.dyE, k1 - p.dzE, i1 + p.dxB + p.dxE - 1, j1 + p.dyB + p.dyE - 1,
+                                  k1 + p.dzB + p.dzE - 1, dims);
+        find_data_loop(p, g, block, data, lists, hit);
+        const int n = (int)(lists[0].size() + lists[1].size() + lists[2].size());
+        blocks[b] = KBlock{(int)P.pos.size(), n};
+        if (n == 0) {
+          ++P.empty;
+          continue;
+        }
+        P.maxn = std::max(P.maxn, n);
+        // the order of the system: alpha data, beta data, rho data
+        idx.clear();
+        ty.clear();
+        for (int a = 0; a < 3; ++a)
+          for (int m : lists[a]) {
+            idx.push_back(m);
+            ty.push_back(a);
+          }
+        K.assign((size_t)n * n, 0.0);
+        x.assign(n, 0.0);
+        for (int r = 0; r < n; ++r) {
+          const WellPt& wr = p.pts[idx[r]];
+          x[r] = wr.v[ty[r]];
+          for (int c = r; c < n; ++c) {
+            const WellPt& wc = p.pts[idx[c]];
+            K[(size_t)r * n + c] = p.cov[tab_of[ty[r]][ty[c]]].g2(wr.i, wr.j, wr.k, wc.i, wc.j, wc.k);
+          }
+        }
+        if (!chol_solve(K, n, x)) {
+          P.fail = true;
+          break;
+        }
+        for (int r = 0; r < n; ++r) {
+          const WellPt& w = p.pts[idx[r]];
+          P.pos.push_back(make_int4(w.i, w.j, w.k, ty[r]));
+          P.x.push_back(x[r]);
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; ++t) pool.emplace_back(plan_range, t);
+    plan_range(0);
+    for (auto& th : pool) th.join();
     std::vector<int4> dposv;
     std::vector<double> dxv;
-    std::vector<int> lists[3];
-    for (int bk = 0; bk < nbz; ++bk)
-      for (int bj = 0; bj < nby; ++bj)
-        for (int bi = 0; bi < nbx; ++bi) {
-          const int i1 = bi * p.dxB, j1 = bj * p.dyB, k1 = bk * p.dzB;
-          const Box block = make_box(i1, j1, k1, i1 + p.dxB - 1, j1 + p.dyB - 1, k1 + p.dzB - 1, dims);
-          const Box data = make_box(i1 - p.dxE, j1 - p.dyE, k1 - p.dzE, i1 + p.dxB + p.dxE - 1,
-                                    j1 + p.dyB + p.dyE - 1, k1 + p.dzB + p.dzE - 1, dims);
-          find_data_loop(p, g, block, data, lists);
-          const int n = (int)(lists[0].size() + lists[1].size() + lists[2].size());
-          if (n == 0) {
-            st.empty_blocks[g]++;
-            continue;
-          }
-          st.max_block_data[g] = std::max(st.max_block_data[g], n);
-          // the order of the system: alpha data, beta data, rho data
-          std::vector<int> idx, ty;
-          for (int a = 0; a < 3; ++a)
-            for (int m : lists[a]) {
-              idx.push_back(m);
-              ty.push_back(a);
-            }
-          std::vector<double> K((size_t)n * n), x(n);
-          for (int r = 0; r < n; ++r) {
-            const WellPt& wr = p.pts[idx[r]];
-            x[r] = wr.v[ty[r]];
-            for (int c = r; c < n; ++c) {
-              const WellPt& wc = p.pts[idx[c]];
-              // SetMatrix's upper blocks: aa, ab, ar, bb, br, rr, each from
-              // the row datum to the column datum
-              const int q = tab_of[ty[r]][ty[c]];
-              K[(size_t)r * n + c] = p.cov[q].g2(wr.i, wr.j, wr.k, wc.i, wc.j, wc.k);
-            }
-          }
-          if (!chol_solve(K, n, x)) {
-            cleanup();
-            return crv_set_error(ctx, CRV_E_INVALID, "kriging: block system not positive definite");
-          }
-          const int off = (int)dposv.size();
-          for (int r = 0; r < n; ++r) {
-            const WellPt& w = p.pts[idx[r]];
-            dposv.push_back(make_int4(w.i, w.j, w.k, ty[r]));
-            dxv.push_back(x[r]);
-          }
-          const int bid = (int)blocks.size();
-          blocks.push_back(KBlock{off, n});
-          // tiles of at most 256 cells: up to 64 along x, then y, then z
-          const int bnx = block.i1 - block.i0 + 1, bny = block.j1 - block.j0 + 1, bnz = block.k1 - block.k0 + 1;
-          const int ti = std::min(bnx, 64), tj = std::min(bny, kKrigThreads / ti),
-                    tk = std::min(bnz, kKrigThreads / (ti * tj));
-          for (int k = 0; k < bnz; k += tk)
-            for (int j = 0; j < bny; j += tj)
-              for (int i = 0; i < bnx; i += ti)
-                tiles.push_back(KTile{bid, block.i0 + i, block.j0 + j, block.k0 + k, std::min(ti, bnx - i),
-                                      std::min(tj, bny - j), std::min(tk, bnz - k)});
-        }
-    if (tiles.empty()) continue;
-    KTile* dt = nullptr;
+    std::vector<long long> base(nthr, 0);
+    for (int t = 0; t < nthr; ++t) {
+      if (part[t].fail) {
+        cleanup();
+        return crv_set_error(ctx, CRV_E_INVALID, "kriging: block system not positive definite");
+      }
+      base[t] = (long long)dposv.size();
+      dposv.insert(dposv.end(), part[t].pos.begin(), part[t].pos.end());
+      dxv.insert(dxv.end(), part[t].x.begin(), part[t].x.end());
+      st.empty_blocks[g] += part[t].empty;
+      st.max_block_data[g] = std::max(st.max_block_data[g], part[t].maxn);
+    }
+    if (dposv.size() > (size_t)INT32_MAX) {
+      cleanup();
+      return crv_set_error(ctx, CRV_E_UNSUPPORTED, "kriging: too many block data");
+    }
+    for (int t = 0; t < nthr; ++t) {
+      const long long b0 = nblocks * t / nthr, b1 = nblocks * (t + 1) / nthr;
+      for (long long b = b0; b < b1; ++b) blocks[b].off += (int)base[t];
+    }
+    if (dposv.empty()) continue;
     KBlock* db = nullptr;
     int4* dp = nullptr;
     double* dxp = nullptr;
-    CRV_HIP_CHECK(ctx, hipMalloc(&dt, tiles.size() * sizeof(KTile)));
-    keep.push_back(dt);
     CRV_HIP_CHECK(ctx, hipMalloc(&db, blocks.size() * sizeof(KBlock)));
     keep.push_back(db);
     CRV_HIP_CHECK(ctx, hipMalloc(&dp, dposv.size() * sizeof(int4)));
     keep.push_back(dp);
     CRV_HIP_CHECK(ctx, hipMalloc(&dxp, dxv.size() * sizeof(double)));
     keep.push_back(dxp);
-    CRV_HIP_CHECK(ctx, hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(KTile), hipMemcpyHostToDevice, ctx->stream));
     CRV_HIP_CHECK(ctx, hipMemcpyAsync(db, blocks.data(), blocks.size() * sizeof(KBlock), hipMemcpyHostToDevice, ctx->stream));
     CRV_HIP_CHECK(ctx, hipMemcpyAsync(dp, dposv.data(), dposv.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream));
     CRV_HIP_CHECK(ctx, hipMemcpyAsync(dxp, dxv.data(), dxv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -700,15 +742,21 @@ extern "C" int crv_krig_post(crv_grid* mean[3], crv_grid* cov[6], const crv_well
       tb.z[a] = dtab + q * (nxy + nzp) + nxy;
       tb.flip[a] = flip_of[g][a];
     }
-    const long long sj = g0->rnxp, sk = (long long)g0->rnxp * g0->nyp;
+    KGeo geo;
+    geo.nx = p.nx; geo.ny = p.ny; geo.nz = p.nz;
+    geo.dxB = p.dxB; geo.dyB = p.dyB; geo.dzB = p.dzB; geo.nbx = nbx; geo.nby = nby;
+    geo.nxp = g0->nxp; geo.nyp = g0->nyp; geo.nzp = g0->nzp;
+    geo.sj = g0->rnxp; geo.sk = (long long)g0->rnxp * g0->nyp;
+    const long long ncell = (long long)p.nx * p.ny * p.nz;
+    const unsigned nb = (unsigned)((ncell + kKrigThreads - 1) / kKrigThreads);
     {
       ProfScope ps(ctx, "krig_cells");
       if (g0->precision == CRV_FP64)
-        hipLaunchKernelGGL(k_krig_cells<double>, dim3((unsigned)tiles.size()), dim3(kKrigThreads), 0, ctx->stream,
-                           (double*)mean[g]->d, sj, sk, dt, db, dp, dxp, tb, g0->nxp, g0->nyp, g0->nzp, dcount + 2 * g);
+        hipLaunchKernelGGL(k_krig_cells<double>, dim3(nb), dim3(kKrigThreads), 0, ctx->stream, (double*)mean[g]->d,
+                           geo, db, dp, dxp, tb, dcount + 2 * g);
       else
-        hipLaunchKernelGGL(k_krig_cells<float>, dim3((unsigned)tiles.size()), dim3(kKrigThreads), 0, ctx->stream,
-                           (float*)mean[g]->d, sj, sk, dt, db, dp, dxp, tb, g0->nxp, g0->nyp, g0->nzp, dcount + 2 * g);
+        hipLaunchKernelGGL(k_krig_cells<float>, dim3(nb), dim3(kKrigThreads), 0, ctx->stream, (float*)mean[g]->d,
+                           geo, db, dp, dxp, tb, dcount + 2 * g);
     }
     if (hipGetLastError() != hipSuccess) {
       cleanup();
