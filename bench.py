@@ -94,8 +94,9 @@ def timed(ctx, G, kw, mode, steps, warmup, seed):
 
 def general_leg(ctx, s, sv, kw, steps, warmup):
     """The drop-in path as the reference feeds it: six real-space prior
-    covariance grids and the error-correlation grid go in and are transformed
-    inside (FFTCovGrids, seismicparametersholder.cpp:273-322; errCorr,
+    covariance grids and the error-correlation grid, filled on the device as
+    the reference fills them with correlation gradients (0.02, -0.015 samples
+    per cell: a non-separable prior), go in and are transformed inside (FFTCovGrids, seismicparametersholder.cpp:273-322; errCorr,
     crava.cpp:1062), the general posterior kernel reads them per wavenumber,
     and the realization comes from the simulation kernel over the stored
     covariances.  Timed like the headline step (FP32 refine mode, inputs
@@ -110,13 +111,14 @@ def general_leg(ctx, s, sv, kw, steps, warmup):
         g.copyFrom(src)
     for g, src in zip(G["data_in"], sv.data):
         g.copyFrom(src)
-    # prior covariance spectra -> real-space grids; errCorr: the lateral
-    # correlation spectrum on every kz plane (what the separable plan uses)
-    hip.fill_prior_spectra(ctx, sv.sigma0, sv.cxy, sv.ct, cov_in)
-    hip.fft_c2r_batch(cov_in)
-    hip.fill_prior_spectra(ctx, np.eye(3), sv.cxy, np.ones(s.nzp), G["cov"])
-    err_in[0].copyFrom(G["cov"][0])
-    hip.fft_c2r_batch(err_in)
+    # the six prior covariance grids and errCorr as the reference fills them
+    # (initializeCorrelations / fillInErrCorr), with lateral correlation
+    # gradients, which make the prior non-separable
+    from crava_amd import model
+    cxy = model.corr_xy(s)
+    ct = model.circ_corr_t(s).astype(np.float32)
+    hip.fill_param_corr(ctx, cov_in, cxy, ct, 0.02, -0.015, sv.sigma0)
+    hip.fill_err_corr(ctx, err_in[0], cxy, 0.02, -0.015)
     gkw = {k: v for k, v in kw.items() if k not in ("cxy", "ct")}
     plan = hip.PostPlan(ctx, s.nxp, s.nyp, s.nzp, 32, s.n, mode=L.CRV_MODE_FP32_REFINE, separable=False,
                         skip_mean_inverse=True, **gkw)

@@ -37,7 +37,7 @@ EXPORTS = [
     "crv_storm_header", "crv_grid_write_storm",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
     "crv_fill_prior_spectra_noalloc", "crv_circ_corr_t", "crv_facies_density_kernel",
-    "crv_invert_noise", "crv_noise_guard", "crv_krig_post",
+    "crv_invert_noise", "crv_noise_guard", "crv_krig_post", "crv_fill_param_corr", "crv_fill_err_corr",
 ]
 CRV_SMOOTH_FOURIER, CRV_SMOOTH_SAMPLED = 0, 1
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
@@ -203,6 +203,9 @@ def load(path: Path | None = None) -> C.CDLL:
                                  C.POINTER(NoiseSpec), i, C.POINTER(vp), i]),
         "crv_noise_guard": (i, [vp, C.c_double, i, C.POINTER(C.c_longlong)]),
         "crv_krig_post": (i, [C.POINTER(vp), C.POINTER(vp), C.POINTER(WellPoint), i, i, i, C.POINTER(KrigStats)]),
+        "crv_fill_param_corr": (i, [C.POINTER(vp), C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_float, C.c_float,
+                                    C.POINTER(C.c_double)]),
+        "crv_fill_err_corr": (i, [vp, C.POINTER(C.c_double), C.c_float, C.c_float]),
         "crv_fft_autotune": (i, [vp, i, i, i, i, i, C.POINTER(FftShape)]),
         "crv_grid_create_batch": (i, [vp, i, i, i, i, i, i, i, i, i, C.POINTER(vp)]),
         "crv_simulate_batch": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.c_ulonglong, C.c_ulonglong, i, i,

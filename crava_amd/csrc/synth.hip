@@ -124,6 +124,95 @@ __global__ void k_moments(const T* g, int nx, int ny, int nz, int rnxp, int nyp,
 }
 
 // Argument checks shared by the two prior-spectra entry points.
+// The lateral index of column i as the reference's fills take it: i, or
+// i - n for the upper half (the cyclic offset nearest zero, ties to i).
+__device__ __forceinline__ int cycle_of(int i, int n) {
+  const int c = i - n;
+  return (i < -c) ? i : c;
+}
+
+constexpr float kRMissing = -99999.0f;  // RMISSING (src/definitions.h)
+
+// SeismicParametersHolder::initializeCorrelations (seismicparametersholder.
+// cpp:197-216): FFTGrid::fillInParamCorr (fftgrid.cpp:1072-1117) -- the
+// temporal correlation shifted by the lateral offset times the correlation
+// gradient, interpolated linearly and cyclically, times the lateral
+// correlation -- then each grid scaled by its prior covariance (float
+// multiplyByScalar over every element, padding included).  Float
+// arithmetic in the reference's order, contraction off.
+struct ParamCorrArgs {
+  void* g[6];
+  float s[6];
+  const float* cxy;   // float(priorCorrXY(i + nxp j))
+  const float* ct;    // circCorrT[k]
+  float gi, gj;
+  int nxp, nyp, nzp, rnxp;
+  long long rsize;
+};
+
+template <typename T>
+__global__ void k_param_corr(ParamCorrArgs a) {
+#pragma clang fp contract(off)
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < a.rsize;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % a.rnxp);
+    const long long r = t / a.rnxp;
+    const int j = (int)(r % a.nyp), k = (int)(r / a.nyp);
+    float value;
+    if (i < a.nxp) {
+      const int ci = cycle_of(i, a.nxp), cj = cycle_of(j, a.nyp);
+      float subK = (float)k + (float)ci * a.gi;
+      subK = subK + (float)cj * a.gj;
+      int baseK = (int)floorf(subK);
+      subK -= (float)baseK;
+      while (baseK < 0) baseK += a.nzp;
+      while (baseK >= a.nzp) baseK -= a.nzp;
+      value = (1.0f - subK) * a.ct[baseK];
+      value += subK * a.ct[baseK != a.nzp - 1 ? baseK + 1 : 0];
+      value *= a.cxy[i + a.nxp * j];
+    } else {
+      value = kRMissing;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) ((T*)a.g[q])[t] = (T)(value * a.s[q]);
+  }
+}
+
+// FFTGrid::fillInErrCorr (fftgrid.cpp:1018-1069): the lateral correlation
+// times the weight of plane 0 at the shifted position s: within one sample
+// of plane 0 or nzp, 1 - (s minus its truncation toward zero), moved into
+// [-1, 1) cyclically (a negative base weighs by the other side); 0 elsewhere.
+template <typename T>
+__global__ void k_err_corr(T* g, const float* cxy, float gi, float gj, int nxp, int nyp, int nzp, int rnxp,
+                           long long rsize) {
+#pragma clang fp contract(off)
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < rsize;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % rnxp);
+    const long long r = t / rnxp;
+    const int j = (int)(r % nyp), k = (int)(r / nyp);
+    const int ci = cycle_of(i, nxp), cj = cycle_of(j, nyp);
+    float subK = (float)k + (float)ci * gi;
+    subK = subK + (float)cj * gj;
+    float value;
+    if (i < nxp) {
+      if (fabsf(subK) < 1.0f || fabsf(subK - (float)nzp) < 1.0f) {
+        int baseK = (int)subK;
+        subK -= (float)baseK;
+        while (baseK < -1) baseK += nzp;
+        while (baseK >= 1) baseK -= nzp;
+        if (baseK < 0) subK = 1.0f - subK;
+        value = (1.0f - subK) * cxy[i + nxp * j];
+      } else {
+        value = 0.0f;
+      }
+    } else {
+      value = kRMissing;
+    }
+    g[t] = (T)value;
+  }
+}
+
 int check_prior_args(crv_ctx* ctx, const double* sigma0, const double* cxy, const double* ct, crv_grid* cov[6]) {
   if (!sigma0 || !cxy || !ct || !cov) return crv_set_error(ctx, CRV_E_INVALID, "NULL argument");
   crv_grid* g0 = cov[0];
@@ -230,6 +319,81 @@ extern "C" int crv_fill_prior_spectra_noalloc(crv_ctx* ctx, const double* sigma0
   rc = launch_prior_spectra(ctx, sigma0, cxy, ct, cov, d);
   if (rc != CRV_OK) return rc;
   for (int q = 0; q < 6; ++q) { cov[q]->transformed = true; cov[q]->cubetype = CRV_COVARIANCE; }
+  return CRV_OK;
+}
+
+// float(priorCorrXY) on the device (nxp nyp values)
+static int upload_corr_xy(crv_ctx* ctx, const double* corr_xy, int nxp, int nyp, float** out) {
+  std::vector<float> h((size_t)nxp * nyp);
+  for (size_t q = 0; q < h.size(); ++q) h[q] = (float)corr_xy[q];
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)out, h.size() * sizeof(float), ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(*out, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  // the host array goes out of scope: finish the copy first
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return CRV_OK;
+}
+
+extern "C" int crv_fill_param_corr(crv_grid* cov[6], const double* corr_xy, const float* circ_corr_t, float grad_i,
+                                   float grad_j, const double sigma0[6]) {
+  if (!cov || !cov[0]) return CRV_E_INVALID;
+  crv_ctx* ctx = cov[0]->ctx;
+  if (!corr_xy || !circ_corr_t || !sigma0) return crv_set_error(ctx, CRV_E_INVALID, "param corr: NULL argument");
+  crv_grid* g0 = cov[0];
+  for (int q = 0; q < 6; ++q)
+    if (!cov[q] || cov[q]->nxp != g0->nxp || cov[q]->nyp != g0->nyp || cov[q]->nzp != g0->nzp ||
+        cov[q]->precision != g0->precision)
+      return crv_set_error(ctx, CRV_E_INVALID, "param corr: grids inconsistent");
+  if (!std::isfinite(grad_i) || !std::isfinite(grad_j) || std::fabs(grad_i) * g0->nxp > 1e6f ||
+      std::fabs(grad_j) * g0->nyp > 1e6f)
+    return crv_set_error(ctx, CRV_E_INVALID, "param corr: correlation gradient out of range");
+  ParamCorrArgs a;
+  for (int q = 0; q < 6; ++q) {
+    a.g[q] = cov[q]->d;
+    a.s[q] = (float)sigma0[q];
+  }
+  float* dxy = nullptr;
+  int rc = upload_corr_xy(ctx, corr_xy, g0->nxp, g0->nyp, &dxy);
+  if (rc != CRV_OK) return rc;
+  float* dct = nullptr;
+  CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dct, g0->nzp * sizeof(float), ctx->stream));
+  CRV_HIP_CHECK(ctx, hipMemcpyAsync(dct, circ_corr_t, g0->nzp * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  a.cxy = dxy; a.ct = dct; a.gi = grad_i; a.gj = grad_j;
+  a.nxp = g0->nxp; a.nyp = g0->nyp; a.nzp = g0->nzp; a.rnxp = g0->rnxp; a.rsize = (long long)g0->rsize;
+  if (g0->precision == CRV_FP64)
+    hipLaunchKernelGGL(k_param_corr<double>, dim3(nblocks(a.rsize)), dim3(kThreads), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_param_corr<float>, dim3(nblocks(a.rsize)), dim3(kThreads), 0, ctx->stream, a);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  CRV_HIP_CHECK(ctx, hipFreeAsync(dxy, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipFreeAsync(dct, ctx->stream));
+  CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < 6; ++q) {
+    cov[q]->transformed = false;
+    cov[q]->cubetype = CRV_COVARIANCE;
+  }
+  return CRV_OK;
+}
+
+extern "C" int crv_fill_err_corr(crv_grid* err, const double* corr_xy, float grad_i, float grad_j) {
+  if (!err) return CRV_E_INVALID;
+  crv_ctx* ctx = err->ctx;
+  if (!corr_xy) return crv_set_error(ctx, CRV_E_INVALID, "err corr: NULL argument");
+  if (!std::isfinite(grad_i) || !std::isfinite(grad_j))
+    return crv_set_error(ctx, CRV_E_INVALID, "err corr: correlation gradient not finite");
+  float* dxy = nullptr;
+  int rc = upload_corr_xy(ctx, corr_xy, err->nxp, err->nyp, &dxy);
+  if (rc != CRV_OK) return rc;
+  const long long rs = (long long)err->rsize;
+  if (err->precision == CRV_FP64)
+    hipLaunchKernelGGL(k_err_corr<double>, dim3(nblocks(rs)), dim3(kThreads), 0, ctx->stream, (double*)err->d, dxy,
+                       grad_i, grad_j, err->nxp, err->nyp, err->nzp, err->rnxp, rs);
+  else
+    hipLaunchKernelGGL(k_err_corr<float>, dim3(nblocks(rs)), dim3(kThreads), 0, ctx->stream, (float*)err->d, dxy,
+                       grad_i, grad_j, err->nxp, err->nyp, err->nzp, err->rnxp, rs);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  CRV_HIP_CHECK(ctx, hipFreeAsync(dxy, ctx->stream));
+  err->transformed = false;
+  err->cubetype = CRV_COVARIANCE;
   return CRV_OK;
 }
 

@@ -254,6 +254,28 @@ def simulate(ctx: Context, cov, post_mean, out, *, noise_mode=L.CRV_NOISE_COUNTE
     return st.value
 
 
+def fill_param_corr(ctx: Context, cov, corr_xy, circ_corr_t, grad_i, grad_j, sigma0):
+    """crv_fill_param_corr: the six real prior covariance grids of the
+    general prior (fillInParamCorr times Sigma0).  corr_xy (nyp, nxp),
+    circ_corr_t (nzp,), sigma0 (3, 3) or its six entries aa bb rr ab ar br."""
+    cx = np.ascontiguousarray(corr_xy, np.float64)
+    ct = np.ascontiguousarray(circ_corr_t, np.float32)
+    S = np.asarray(sigma0, np.float64)
+    s6 = S.reshape(-1) if S.size == 6 else np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    s6 = np.ascontiguousarray(s6, np.float64)
+    covs = (C.c_void_p * 6)(*[g.h.value for g in cov])
+    ctx.check(ctx.lib.crv_fill_param_corr(covs, cx.ctypes.data_as(C.POINTER(C.c_double)),
+                                          ct.ctypes.data_as(C.POINTER(C.c_float)), float(grad_i), float(grad_j),
+                                          s6.ctypes.data_as(C.POINTER(C.c_double))), "fill_param_corr")
+
+
+def fill_err_corr(ctx: Context, err, corr_xy, grad_i, grad_j):
+    """crv_fill_err_corr: the real error correlation grid (fillInErrCorr)."""
+    cx = np.ascontiguousarray(corr_xy, np.float64)
+    ctx.check(ctx.lib.crv_fill_err_corr(err.h, cx.ctypes.data_as(C.POINTER(C.c_double)), float(grad_i),
+                                        float(grad_j)), "fill_err_corr")
+
+
 def krig_post(ctx: Context, mean, cov, points, data_target, smooth=False) -> dict:
     """crv_krig_post (Crava::doPostKriging): the real posterior means mean[3]
     kriged in place to the well points (rows i, j, k, log Vp, log Vs,

@@ -638,6 +638,26 @@ int crv_storm_header(const crv_storm_info* si, int cubetype, int nx, int ny, int
 int crv_grid_write_storm(crv_grid* g, const char* path, const crv_storm_info* si, int ascii, int padding,
                          int scientific);
 
+/* ---- prior and error correlation grids (general prior) ------------------
+ * SeismicParametersHolder::initializeCorrelations (src/seismicparameters-
+ * holder.cpp:197-216): the six real prior covariance grids cov[6] (aa, bb,
+ * rr, ab, ar, br) as FFTGrid::fillInParamCorr (src/fftgrid.cpp:1072-1117)
+ * fills them -- value(i, j, k) = corrXY(i, j) times the temporal
+ * correlation circ_corr_t (nzp values, crv_circ_corr_t) at the cyclic,
+ * linearly interpolated shift k + ci grad_i + cj grad_j (ci, cj the lateral
+ * offsets nearest zero), RMISSING in the padding -- each multiplied by
+ * float(sigma0[q]).  corr_xy: the lateral correlation surface, nxp * nyp
+ * values (i + nxp j).  Float arithmetic as the reference (bit for bit).  */
+int crv_fill_param_corr(crv_grid* cov[6], const double* corr_xy, const float* circ_corr_t, float grad_i,
+                        float grad_j, const double sigma0[6]);
+/* FFTGrid::fillInErrCorr (src/fftgrid.cpp:1018-1069): the real error
+ * correlation grid, corrXY(i, j) times the weight of plane 0 at the shifted
+ * position s = k + ci grad_i + cj grad_j: within one sample of plane 0 (or
+ * of plane nzp, cyclically) 1 - (s minus s truncated toward zero), as the
+ * reference's int() gives it (so 1 + |s| for s in (-1, 0)); 0 elsewhere;
+ * RMISSING in the padding.                                                */
+int crv_fill_err_corr(crv_grid* err, const double* corr_xy, float grad_i, float grad_j);
+
 /* ---- prediction kriging to wells ---------------------------------------
  * Crava::doPostKriging (src/crava.cpp:1736-1765): the posterior means
  * mean[3] (real, updated in place) kriged to well data with the separable

@@ -344,3 +344,63 @@ def rock_forward(x, params):
     o.or_rock_forward_batch(C.byref(p), n, x.ctypes.data_as(dp), out.ctypes.data_as(dp),
                             st.ctypes.data_as(C.POINTER(C.c_int)))
     return out, st
+
+
+# ---- prior and error correlation grids of the general prior ----------------
+
+def _cycle(n):
+    """The offsets the fills give index i of an axis of n: i, or i - n when
+    that is nearer zero (ties to i)."""
+    i = np.arange(n)
+    return np.where(i < -(i - n), i, i - n)
+
+
+def param_corr_grids(corr_xy, circ_corr_t, grad_i, grad_j, sigma0, nxp, nyp, nzp, rnxp):
+    """SeismicParametersHolder::initializeCorrelations (reference
+    src/seismicparametersholder.cpp:197-216): FFTGrid::fillInParamCorr
+    (src/fftgrid.cpp:1072-1117) then multiplyByScalar(float(Sigma0)) over
+    every element, in numpy float32 with the reference's operation order.
+    Six (nzp, nyp, rnxp) float32 grids (aa bb rr ab ar br)."""
+    f = np.float32
+    ci = _cycle(nxp).astype(np.float32)[None, None, :]
+    cj = _cycle(nyp).astype(np.float32)[None, :, None]
+    k = np.arange(nzp).astype(np.float32)[:, None, None]
+    subk = (k + ci * f(grad_i)).astype(np.float32)
+    subk = (subk + cj * f(grad_j)).astype(np.float32)
+    base = np.floor(subk).astype(np.int64)
+    frac = (subk - base.astype(np.float32)).astype(np.float32)
+    base = base % nzp
+    ct = np.asarray(circ_corr_t, np.float32)
+    nxt = np.where(base != nzp - 1, base + 1, 0)
+    v = ((f(1) - frac) * ct[base]).astype(np.float32)
+    v = (v + frac * ct[nxt]).astype(np.float32)
+    v = (v * np.asarray(corr_xy, np.float64).astype(np.float32)[None]).astype(np.float32)
+    full = np.full((nzp, nyp, rnxp), f(-99999.0), np.float32)
+    full[:, :, :nxp] = v
+    S = np.asarray(sigma0, np.float64)
+    s6 = S.reshape(-1) if S.size == 6 else [S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]]
+    return [(full * f(s)).astype(np.float32) for s in s6]
+
+
+def err_corr_grid(corr_xy, grad_i, grad_j, nxp, nyp, nzp, rnxp):
+    """FFTGrid::fillInErrCorr (reference src/fftgrid.cpp:1018-1069) in numpy
+    float32: within one sample of plane 0 (or of plane nzp, cyclically) the
+    weight 1 - (shift minus its truncation toward zero), with the
+    reference's int() truncation: a shift f in (-1, 0) keeps fraction f and
+    weighs 1 - f = 1 + |f|; (nzp, nyp, rnxp) float32."""
+    f = np.float32
+    ci = _cycle(nxp).astype(np.float32)[None, None, :]
+    cj = _cycle(nyp).astype(np.float32)[None, :, None]
+    k = np.arange(nzp).astype(np.float32)[:, None, None]
+    subk = (k + ci * f(grad_i)).astype(np.float32)
+    subk = (subk + cj * f(grad_j)).astype(np.float32)
+    near = (np.abs(subk) < f(1)) | (np.abs((subk - f(nzp)).astype(np.float32)) < f(1))
+    base = np.trunc(subk).astype(np.int64)
+    frac = (subk - base.astype(np.float32)).astype(np.float32)
+    base = (base + 1) % nzp - 1        # the reference's cyclic moves into [-1, 1)
+    frac = np.where(base < 0, (f(1) - frac).astype(np.float32), frac)
+    v = ((f(1) - frac) * np.asarray(corr_xy, np.float64).astype(np.float32)[None]).astype(np.float32)
+    v = np.where(near, v, f(0)).astype(np.float32)
+    full = np.full((nzp, nyp, rnxp), f(-99999.0), np.float32)
+    full[:, :, :nxp] = v
+    return full
