@@ -43,6 +43,7 @@ CRV_SMOOTH_FOURIER, CRV_SMOOTH_SAMPLED = 0, 1
 CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BLOCKS = 1, 2, 3, 4, 5
 CRV_TUNE_BATCH_MB = 6
 CRV_TUNE_ORDER = 7
+CRV_TUNE_ZPOST = 8
 
 
 class GridInfo(C.Structure):

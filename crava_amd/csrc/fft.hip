@@ -24,6 +24,12 @@
 // Twiddles of all stages come from a per-size table computed on the host in
 // long double and copied to LDS by each block; the index arithmetic uses
 // multiply-shift division, no runtime modulo.
+// Floating-point contraction only within an expression (fused multiply-add
+// where the source writes a * b + c): the rounding of every butterfly then
+// follows from the source alone, not from the kernel it is inlined into, so
+// different kernels that run the same stages (the z passes and the fused z
+// pass of crv_invert) produce the same bits.
+#pragma clang fp contract(on)
 #include "internal.hpp"
 #include "dft.hpp"
 #include "eiggen.hpp"
@@ -609,7 +615,6 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
 }
 
 // ---- y / z passes: strided complex pencils ---------------------------------
-constexpr int kRecMaxNGen = 6;   // angles of the eigen-basis records
 template <typename T>
 struct ColArgs {
   cplx_t<T>* data;
@@ -702,10 +707,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
               const float lam = synL[c];
               const float* tb = synT + synB[e] * a.syn_rw;
               if (!eig_queued(a.syn_geo, lam, tb[1], tb[2], tb[3], x0 + c, a.syn_j0 + o, e)) {
-                float w[kRecMaxNGen];
+                float w[kEigMaxN];
 #pragma unroll
-                for (int p = 0; p < kRecMaxNGen; ++p) w[p] = p < n ? eig_weight(lam, tb[4 + p]) : 0.0f;
-                const float2 g = eig_cov_entry<kRecMaxNGen>(n, q < 3, lam * tb[0], lam * lam, w, tb + 4 + n,
+                for (int p = 0; p < kEigMaxN; ++p) w[p] = p < n ? eig_weight(lam, tb[4 + p]) : 0.0f;
+                const float2 g = eig_cov_entry<kEigMaxN>(n, q < 3, lam * tb[0], lam * lam, w, tb + 4 + n,
                                                             tb + 4 + 2 * n);
                 v = cmake<C>((T)g.x, (T)g.y);
                 gen = true;
@@ -738,6 +743,199 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
     if (c < nc) base[(size_t)e * a.estride + c] = v;
   };
   transform_tile<T, INV, false, false, RF, RL, H>(lds, TX, a.cshift, a.f, a.sd, twl, load, store);
+}
+
+// ---- z pass with the posterior of the means (crv_invert) ---------------------
+// One tile is TX consecutive x columns of one y row, whole z pencils, for
+// the n data cubes and the 3 means (FP32, eigen-basis records).  The pencil
+// sets of a tile go through LDS one after another: all threads load a set
+// (lanes along the columns) and write it to LDS in natural order; the loads
+// of the next two sets are in flight meanwhile.  A data set's elements
+// outside the inverted band are their own residual and go straight back to
+// HBM, the band's stay in registers; the mean sets are transformed into three
+// slots.  The band's wavenumbers then get eig_mean_update (posterior means
+// into the slots, residuals to HBM), and the slots are inverse transformed
+// and stored.  The blocks are persistent: the loads of a block's next tile
+// are issued before the posterior and the inverse transforms of its current
+// one.  Each element sees the arithmetic of the separate z forward pass,
+// k_posterior_eig and z inverse pass, so the results are bit-equal; the
+// spectra of the means and the data never make the HBM round trip between
+// them.
+constexpr int kZThreads = 512;
+constexpr int kZQ = 2;    // band wavenumbers per thread at most
+constexpr int kZPF = 5;   // elements per thread of a pencil set at most
+
+struct ZPostArgs {
+  float2* data[kEigMaxN];
+  float2* mean[3];
+  Fft1D f;
+  StageDiv sd;
+  const float2* tw;
+  int estride;         // cnxp nyp
+  int ncols, ntx, ntiles, cshift, rev;
+  int n, nband;
+  const float* lam;    // [nyp][cnxp]
+  const float* erec;   // plane records, kEigRec floats per plane
+  const int* bidx;     // [nzp] row among the band planes, -1 outside the band
+  const int* bplanes;  // [nband] band planes
+  unsigned long long* stats;
+};
+
+template <int RF, int RL>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kZThreads), amdgpu_waves_per_eu(4))) void k_zpost(
+    ZPostArgs a) {
+  using C = float2;
+  extern __shared__ unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const int TX = 1 << a.cshift, L = a.f.L, LT = L * TX;
+  C* twl = lds + 3 * LT;
+  int* sbidx = reinterpret_cast<int*>(twl + a.f.twn);
+  int* sbpl = sbidx + L;
+  copy_to_lds<float>(twl, a.tw + a.f.tw0, a.f.twn);
+  for (int e = threadIdx.x; e < L; e += kZThreads) sbidx[e] = a.bidx[e];
+  for (int e = threadIdx.x; e < a.nband; e += kZThreads) sbpl[e] = a.bplanes[e];
+  const int nbw = a.nband * TX;
+  const int es = a.estride, cm = TX - 1;
+  const int nset = a.n + 3;
+  const int n = a.n;
+  const int OC = 2 * n * n + 22 * n + 12;   // EigLayout<n>::OC
+  // the tile's y row and first column, its column count and element offset
+  struct Tile { int o, x0, nc; size_t off; };
+  auto tile_of = [&](int vb) {
+    const int tile = xcd_remap(vb, a.ntiles, a.rev);
+    Tile T;
+    T.o = tile / a.ntx;
+    T.x0 = (tile - T.o * a.ntx) * TX;
+    T.nc = min(TX, a.ncols - T.x0);
+    T.off = (size_t)T.o * a.ncols + T.x0;
+    return T;
+  };
+  auto set_ptr = [&](const Tile& T, int s) -> C* { return (s < n ? a.data[s] : a.mean[s - n]) + T.off; };
+  // element u of this thread in a pencil set: t = threadIdx.x + u kZThreads,
+  // column t & cm, plane t >> cshift.  csh: the tile's column shift, made
+  // opaque where it is used in a loop: otherwise the element and LDS
+  // offsets, the same for every set, are hoisted out of the loops and held
+  // in registers across them
+  C pa[kZPF], pb[kZPF];
+  auto prefetch = [&](C (&pf)[kZPF], const Tile& T, int s, int csh) {
+    const C* base = set_ptr(T, s);
+    const int cmk = (1 << csh) - 1;
+#pragma unroll
+    for (int u = 0; u < kZPF; ++u) {
+      const int tt = threadIdx.x + u * kZThreads;
+      const int c = tt & cmk, e = tt >> csh;
+      pf[u] = (tt < LT && c < T.nc) ? base[e * es + c] : make_float2(0.0f, 0.0f);
+    }
+  };
+  C d[kZQ][kEigMaxN];
+  auto step = [&](C (&pf)[kZPF], const Tile& T, int s) {
+    int csh = a.cshift;
+    asm volatile("" : "+v"(csh));
+    const int txs = 1 << csh, cmk = txs - 1;
+    C* slot = lds + (s < n ? 0 : s - n) * LT;
+#pragma unroll
+    for (int u = 0; u < kZPF; ++u) {
+      const int tt = threadIdx.x + u * kZThreads;
+      if (tt < LT) slot[tt] = pf[u];
+    }
+    __syncthreads();
+    if (s + 2 < nset) prefetch(pf, T, s + 2, csh);
+    transform_tile_lds<float, false, RF, RL, false>(slot, txs, csh, a.f, a.sd, twl);
+    if (s < n) {
+      // outside the band the spectrum is the residual
+      C* base = set_ptr(T, s);
+      for (int tt = threadIdx.x; tt < LT; tt += kZThreads) {
+        const int c = tt & cmk, e = tt >> csh;
+        if (c < T.nc && sbidx[e] < 0) base[e * es + c] = slot[tt];
+      }
+#pragma unroll
+      for (int q = 0; q < kZQ; ++q) {
+        const int b = threadIdx.x + q * kZThreads;
+        const C v = b < nbw ? slot[sbpl[b >> csh] * txs + (b & cmk)] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int ll = 0; ll < kEigMaxN; ++ll)
+          if (ll == s) d[q][ll] = v;
+      }
+      __syncthreads();
+    }
+  };
+  double cmax = 0.0;
+  int vb = blockIdx.x;
+  Tile T = tile_of(vb);
+  prefetch(pa, T, 0, a.cshift);
+  prefetch(pb, T, 1, a.cshift);   // nset >= 4
+  while (vb < a.ntiles) {
+    for (int s = 0; s < nset; s += 2) {
+      step(pa, T, s);
+      if (s + 1 < nset) step(pb, T, s + 1);
+    }
+    // posterior of the band's wavenumbers, one after the other
+    int pcs = a.cshift;
+    asm volatile("" : "+v"(pcs));
+#pragma unroll
+    for (int q = 0; q < kZQ; ++q) {
+      const int b = threadIdx.x + q * kZThreads;
+      const int c = b & ((1 << pcs) - 1);
+      if (b < nbw && c < T.nc) {
+        const int e = sbpl[b >> pcs];
+        const float* er = a.erec + (size_t)e * kEigRec;
+        const float lam = a.lam[T.off + c];
+        C mu[3], mp[3], res[kEigMaxN];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) mu[m] = lds[m * LT + (e << pcs) + c];
+        eig_mean_update<kEigMaxN>(n, er, lam, mu, d[q], mp, res);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) lds[m * LT + (e << pcs) + c] = mp[m];
+#pragma unroll
+        for (int l = 0; l < kEigMaxN; ++l)
+          if (l < n) a.data[l][T.off + (size_t)e * es + c] = res[l];
+        cmax = fmax(cmax, (double)eig_cond(lam, er[OC], er[OC + 1], er[OC + 2]));
+      }
+      // keep the two wavenumbers' record loads apart (register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    // the next tile's first two sets load while this one's means are
+    // inverse transformed and stored
+    const int vn = vb + (int)gridDim.x;
+    const Tile Tn = tile_of(vn < a.ntiles ? vn : vb);
+    if (vn < a.ntiles) {
+      int csh = a.cshift;
+      asm volatile("" : "+v"(csh));
+      prefetch(pa, Tn, 0, csh);
+      prefetch(pb, Tn, 1, csh);
+    }
+    // means: inverse transform of the slots
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      int csh = a.cshift;
+      asm volatile("" : "+v"(csh));
+      const int txs = 1 << csh, cmk = txs - 1;
+      C* slot = lds + m * LT;
+      transform_tile_lds<float, true, RF, RL, false>(slot, txs, csh, a.f, a.sd, twl);
+      C* base = a.mean[m] + T.off;
+      for (int tt = threadIdx.x; tt < LT; tt += kZThreads) {
+        const int c = tt & cmk, e = tt >> csh;
+        if (c < T.nc) base[e * es + c] = slot[tt];
+      }
+    }
+    // the slots' stores above read LDS that the next tile's first set writes
+    __syncthreads();
+    vb = vn;
+    T = Tn;
+  }
+  // largest condition estimate of the block
+  __shared__ double s_c[kZThreads / 64];
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1) cmax = fmax(cmax, __shfl_down(cmax, sh, 64));
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = cmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double cm2 = 0.0;
+#pragma unroll
+    for (int w = 0; w < kZThreads / 64; ++w) cm2 = fmax(cm2, s_c[w]);
+    if (cm2 > 0.0) atomicMax(&a.stats[1], (unsigned long long)__double_as_longlong(cm2));
+  }
 }
 
 // ---- planning ----------------------------------------------------------------
@@ -1208,6 +1406,70 @@ bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw) {
 }
 
 // One pass of a transform over part of a batch (see internal.hpp).
+// ---- fused z pass of crv_invert -----------------------------------------------
+size_t zpost_lds_bytes(const FftPlan* p, int nband, int tx) {
+  return ((size_t)3 * p->fz.L * tx + p->fz.twn) * sizeof(float2) + (size_t)(p->fz.L + nband) * sizeof(int);
+}
+
+const void* zpost_kernel(const Fft1D& f) {
+  if (f.nst != 2) return nullptr;
+#define CRV_PAIR_ZPOST(A, B) \
+  if (f.R[0] == (A) && f.R[1] == (B)) return (const void*)k_zpost<A, B>;
+  CRV_FFT_PAIRS(CRV_PAIR_ZPOST)
+#undef CRV_PAIR_ZPOST
+  return nullptr;
+}
+
+int crv_zpost_tx(crv_ctx* ctx, const FftPlan* p, int nband) {
+  const int want = ctx->tune[CRV_TUNE_ZPOST];
+  if (want == 0 || p->prec != CRV_FP32 || nband < 1 || !zpost_kernel(p->fz)) return 0;
+  const int L = p->fz.L;
+  // the widest tile (16 columns at most, or the tuned width) whose band
+  // wavenumbers the threads can hold, whose stages the block covers and
+  // whose LDS leaves room for a second block per CU (the tuned width may
+  // take the whole 160 KB)
+  if ((long long)(p->nxp / 2 + 1) * p->nyp * p->nzp >= (1LL << 31)) return 0;
+  for (int t = want >= 2 ? pow2_floor(want) : 16; t >= 1; t /= 2) {
+    if ((long long)nband * t > (long long)kZQ * kZThreads || L * t > kZPF * kZThreads) continue;
+    bool ok = true;
+    for (int st = 0; st < 2; ++st) {
+      const int R = p->fz.R[st];
+      if ((L / R) * t > ((Ept<float>::value + R - 1) / R) * kZThreads) ok = false;
+    }
+    if (ok && zpost_lds_bytes(p, nband, t) <= (want >= 2 ? 160u : 80u) * 1024u) return t;
+  }
+  return 0;
+}
+
+int crv_launch_zpost(crv_ctx* ctx, FftPlan* p, const ZPost& z, int tx) {
+  const void* kern = zpost_kernel(p->fz);
+  if (!kern || tx < 1 || z.n < 1 || z.n > kEigMaxN)
+    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "fused z pass: unsupported shape");
+  ZPostArgs a{};
+  for (int l = 0; l < z.n; ++l) a.data[l] = (float2*)z.data[l];
+  for (int m = 0; m < 3; ++m) a.mean[m] = (float2*)z.mean[m];
+  a.f = p->fz; a.sd = make_stage_div(p->fz); a.tw = (const float2*)p->tw;
+  const int cnxp = p->nxp / 2 + 1;
+  a.estride = cnxp * p->nyp; a.ncols = cnxp;
+  a.cshift = ilog2(tx); a.ntx = (cnxp + tx - 1) / tx;
+  a.rev = (p->order >> 2) & 1;
+  a.n = z.n; a.nband = z.nband; a.lam = z.lam; a.erec = z.erec; a.bidx = z.bidx; a.bplanes = z.bplanes;
+  a.stats = z.stats;
+  const size_t lds = zpost_lds_bytes(p, z.nband, tx);
+  if (lds > 64u * 1024u)
+    CRV_HIP_CHECK(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  a.ntiles = a.ntx * p->nyp;
+  // persistent: as many blocks as fit at once (a multiple of the 8 XCDs)
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kZThreads, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int grid = std::min(a.ntiles, std::max(8, per_cu * ctx->num_cu / 8 * 8));
+  void* args[] = {&a};
+  ProfScope ps(ctx, "fft_z_post");
+  CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(grid), dim3(kZThreads), args, lds, ctx->stream));
+  return CRV_OK;
+}
+
 int crv_launch_pass(crv_ctx* ctx, FftPlan* plan, int pass, void* buf, const void* src, int nbatch, long long bstride,
                     int o0, int o1, double scale, const void* const* add, int nadd, const CovGen* gen) {
   if (nbatch < 1 || nbatch > 65535 || nadd < 0 || nadd > 3 || pass < 0 || pass > 5 || o0 < 0 || o1 < o0)

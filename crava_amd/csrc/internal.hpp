@@ -83,7 +83,7 @@ struct ProfEvent {
 };
 
 struct crv_ctx {
-  int tune[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // CRV_TUNE_* overrides (0 = default)
+  int tune[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // CRV_TUNE_* overrides (0 = default)
   bool prof_on = false;
   std::vector<ProfEvent> prof;
   int device = 0;
@@ -188,6 +188,27 @@ struct CovGen {
 // LDS of the generating z inverse of a plan with nband band planes of rw
 // floats each fits the column kernels' 64 KB
 bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw);
+
+// The z pass of crv_invert with the eigen-basis posterior of the means in
+// between (FP32 plans with eigen-basis records where no wavenumber can be
+// queued for the FP64 solve): data[l], after their x and y forward passes,
+// become the residual spectra; mean[m], likewise, the z inverse transforms
+// of the posterior mean spectra.  stats[1] receives the largest condition
+// estimate (atomic max of the double's bits).
+struct ZPost {
+  void* data[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* mean[3] = {nullptr, nullptr, nullptr};
+  int n = 0, nband = 0;
+  const float* lam = nullptr;     // [nyp][cnxp] |cxy|
+  const float* erec = nullptr;    // eigen-basis plane records
+  const int* bidx = nullptr;      // [nzp] row among the band planes, -1 outside
+  const int* bplanes = nullptr;   // [nband] the band planes
+  unsigned long long* stats = nullptr;
+};
+// tile width of the fused z pass for a plan with nband band planes, 0 when
+// it does not apply (or CRV_TUNE_ZPOST is 1)
+int crv_zpost_tx(crv_ctx* ctx, const FftPlan* p, int nband);
+int crv_launch_zpost(crv_ctx* ctx, FftPlan* p, const ZPost& z, int tx);
 
 // One pass of a transform over part of a batch of nbatch grids bstride reals
 // apart: pass 0 x R2C (rows [o0, o1), row = j + nyp k; reads src, scales),

@@ -108,6 +108,9 @@ struct PostArgs {
   // eigen-basis kernel: the covariances inside the band are left unwritten
   // too, but for the wavenumbers of the FP64 list (generated likewise)
   int skip_band_cov;
+  // eigen-basis kernel: the realizations only, the means and residuals
+  // being updated by the z pass (crv_invert's fused z pass)
+  int sim_only;
 };
 
 // the noise of realization r at wavenumber idx: the counter stream, or the
@@ -508,21 +511,6 @@ __global__ __launch_bounds__(kThreads) void k_posterior_rec(PostArgs<float> a) {
 // where G_p = conj(V_pa) V_pb per covariance q = (a, b).  The solve is
 // diagonal in that basis.  The condition estimate is the bound
 // cond(D) (lam l_max + 1) / (lam l_min + 1) >= cond(M).
-constexpr int kEigRec = 256;  // floats per plane record
-template <int N>
-struct EigLayout {
-  static constexpr int OT = 0;              // T, N x N complex [p][l]
-  static constexpr int OV = OT + 2 * N * N; // V, N x 3 complex [p][m]
-  static constexpr int OK = OV + 6 * N;     // K, N x 3 complex [l][m]
-  static constexpr int OG = OK + 6 * N;     // G_p: 3 real, then 3 complex (ab, ar, br)
-  static constexpr int OW = OG + 9 * N;     // l_p
-  static constexpr int OS = OW + N;         // |ct| Sigma0 (aa bb rr ab ar br)
-  static constexpr int OF = OS + 6;         // factor of S (fused_sim_prior)
-  static constexpr int OC = OF + 6;         // cond(D), l_max, l_min
-  static constexpr int OBAND = kEigRec - 1; // 1: inverted plane
-  static_assert(OC + 3 <= OBAND, "record overflow");
-};
-
 // the record of plane k as scalar (wave-uniform) loads
 typedef const __attribute__((address_space(4))) float* EigRecPtr;
 
@@ -571,61 +559,21 @@ __global__ __launch_bounds__(kThreads) void k_posterior_eig(PostArgs<float> a) {
     }
     if (!live) continue;
     cmax = fmax(cmax, (double)eig_cond(lam, er[Lo::OC], er[Lo::OC + 1], er[Lo::OC + 2]));
-    C mu[3], r[N];
+    if (!a.sim_only) {
+      C mu[3], d[N], mp[3], res[N];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
+      for (int m = 0; m < 3; ++m) mu[m] = a.mean[m][idx];
 #pragma unroll
-    for (int l = 0; l < N; ++l) {
-      C x = a.data[l][idx];
+      for (int l = 0; l < N; ++l) d[l] = a.data[l][idx];
+      eig_mean_update<N>(N, er, lam, mu, d, mp, res);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const float kr = er[Lo::OK + 2 * (l * 3 + m)], ki = er[Lo::OK + 2 * (l * 3 + m) + 1];
-        x.x = fmaf(-kr, mu[m].x, fmaf(ki, mu[m].y, x.x));
-        x.y = fmaf(-kr, mu[m].y, fmaf(-ki, mu[m].x, x.y));
-      }
-      r[l] = x;
+      for (int m = 0; m < 3; ++m) a.mean[m][idx] = mp[m];
+#pragma unroll
+      for (int l = 0; l < N; ++l) a.data[l][idx] = res[l];
     }
-    // y = lam diag(w) T r
-    C y[N];
     float w[N];
 #pragma unroll
-    for (int p = 0; p < N; ++p) {
-      C z = make_float2(0.0f, 0.0f);
-#pragma unroll
-      for (int l = 0; l < N; ++l) {
-        const float tr = er[Lo::OT + 2 * (p * N + l)], ti = er[Lo::OT + 2 * (p * N + l) + 1];
-        z.x = fmaf(tr, r[l].x, fmaf(-ti, r[l].y, z.x));
-        z.y = fmaf(tr, r[l].y, fmaf(ti, r[l].x, z.y));
-      }
-      w[p] = eig_weight(lam, er[Lo::OW + p]);
-      const float g = lam * w[p];
-      y[p] = make_float2(g * z.x, g * z.y);
-    }
-    // mean update V^H y, residual r - K dm
-    C dm[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      C x = make_float2(0.0f, 0.0f);
-#pragma unroll
-      for (int p = 0; p < N; ++p) {
-        const float vr = er[Lo::OV + 2 * (p * 3 + m)], vi = er[Lo::OV + 2 * (p * 3 + m) + 1];
-        x.x = fmaf(vr, y[p].x, fmaf(vi, y[p].y, x.x));
-        x.y = fmaf(vr, y[p].y, fmaf(-vi, y[p].x, x.y));
-      }
-      dm[m] = x;
-      a.mean[m][idx] = make_float2(mu[m].x + x.x, mu[m].y + x.y);
-    }
-#pragma unroll
-    for (int l = 0; l < N; ++l) {
-      C x = r[l];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const float kr = er[Lo::OK + 2 * (l * 3 + m)], ki = er[Lo::OK + 2 * (l * 3 + m) + 1];
-        x.x = fmaf(-kr, dm[m].x, fmaf(ki, dm[m].y, x.x));
-        x.y = fmaf(-kr, dm[m].y, fmaf(-ki, dm[m].x, x.y));
-      }
-      a.data[l][idx] = x;
-    }
+    for (int p = 0; p < N; ++p) w[p] = eig_weight(lam, er[Lo::OW + p]);
     // covariances lam S - lam^2 sum_p w_p G_p
     const float l2 = lam * lam;
     C cq[6];
@@ -1133,6 +1081,38 @@ bool eig_records(const HostTables& h, int n, int nzp, std::vector<float>& out) {
   return true;
 }
 
+// True when the eigen-basis posterior can queue no wavenumber of the grid
+// for the FP64 solve (eig_queued false everywhere), shown on the host from
+// the tables the kernels read: the low-|k| test is smallest at i = 0, ky = 0
+// and the plane's smallest |kz|, where the kernel's r2 is exactly fz * fz;
+// the condition bound cond(D) (lam l_max + 1) / (lam l_min + 1) grows with
+// lam (l_max >= l_min >= 0), so it is largest at the table's largest lam,
+// where it must stay a relative 1e-5 below the threshold (far more than the
+// few float roundings of the kernel's test).
+bool eig_never_queued(const std::vector<float>& er, const std::vector<int>& inv, const std::vector<double>& lam,
+                      int n, int nzp, float lowk2, float thr) {
+  const int OC = 2 * n * n + 22 * n + 12;
+  float lmax_tab = 0.0f;
+  for (double v : lam) {
+    const float f = (float)v;
+    if (!std::isfinite(f) || f < 0.0f) return false;
+    lmax_tab = std::max(lmax_tab, f);
+  }
+  const float rz = 2.0f / (float)nzp;
+  for (int k = 0; k < nzp; ++k) {
+    if (!inv[k]) continue;
+    const int kz = std::min(k, nzp - k);
+    const float fz = (float)kz * rz;
+    if (fz * fz < lowk2) return false;
+    const float* rk = er.data() + (size_t)k * kEigRec;
+    const double cd = rk[OC], lmx = rk[OC + 1], lmn = rk[OC + 2];
+    if (!(lmn >= 0.0) || !(lmx >= lmn) || !std::isfinite(cd)) return false;
+    const double bound = cd * ((double)lmax_tab * lmx + 1.0) / ((double)lmax_tab * lmn + 1.0);
+    if (!(bound <= (double)thr * (1.0 - 1e-5))) return false;
+  }
+  return true;
+}
+
 struct crv_post_plan {
   crv_ctx* ctx = nullptr;
   int n = 0, nxp = 0, nyp = 0, nzp = 0, cnxp = 0, precision = 0;
@@ -1146,6 +1126,12 @@ struct crv_post_plan {
   // inverse (CovGen): row index per plane, then 6 x nband rows of gw floats
   size_t oGI = 0, oGR = 0;
   int nband = 0, gw = 0;
+  // with eigen-basis records: the band planes (nband ints), and whether no
+  // wavenumber of the grid can be queued for the FP64 solve (zpost_ok), in
+  // which case crv_invert may run the posterior of the means inside its z
+  // pass (crv_launch_zpost)
+  size_t oBP = 0;
+  bool zpost_ok = false;
   unsigned int* dlist = nullptr;
   double sig0[6] = {0, 0, 0, 0, 0, 0};
   float cond_thr = 1e5f;
@@ -1235,6 +1221,13 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
       pl->oGR = (blob.size() + 255) & ~(size_t)255;
       blob.resize(pl->oGR + rows.size() * sizeof(float), 0);
       memcpy(blob.data() + pl->oGR, rows.data(), rows.size() * sizeof(float));
+      std::vector<int> bplanes;
+      for (int k = 0; k < nzp; ++k)
+        if (h.inv[k]) bplanes.push_back(k);
+      pl->oBP = (blob.size() + 255) & ~(size_t)255;
+      blob.resize(pl->oBP + std::max<size_t>(1, bplanes.size()) * sizeof(int), 0);
+      if (!bplanes.empty()) memcpy(blob.data() + pl->oBP, bplanes.data(), bplanes.size() * sizeof(int));
+      pl->zpost_ok = nb > 0 && eig_never_queued(er, h.inv, h.lam, n, nzp, (float)pl->lowk2, pl->cond_thr);
     }
   }
   pl->oI = blob.size();
@@ -1312,6 +1305,7 @@ struct FusedSim {
   const long long* prefix = nullptr;
   bool skip_prior_cov = false;
   bool skip_band_cov = false;
+  bool sim_only = false;
 };
 
 int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, crv_grid* err_corr,
@@ -1358,6 +1352,7 @@ int exec_rows(crv_post_plan* pl, crv_grid* const* mean, crv_grid* const* data, c
       a.cpp = (pl->cnxp * nyr + 63) / 64;
       a.skip_prior_cov = fs.skip_prior_cov ? 1 : 0;
       a.skip_band_cov = (fs.skip_band_cov && pl->oE) ? 1 : 0;
+      a.sim_only = (fs.sim_only && pl->oE) ? 1 : 0;
       if (fs.nsim > 0) {
         a.sim = fs.sim; a.sstride = fs.sstride; a.nsim = fs.nsim; a.seed = fs.seed; a.real0 = fs.real0;
         a.geo = make_noise_geom_dims(pl->cnxp, pl->nyp, pl->nzp);
@@ -1564,16 +1559,53 @@ extern "C" int crv_invert_noise(crv_post_plan* pl, crv_grid* mean[3], crv_grid**
   // with eigen-basis records the band's planes as well (but for the
   // wavenumbers of the FP64 list), when their tables fit the z inverse's LDS
   const bool synth_band = synth && pl->oE != 0 && pl->oGR != 0 && crv_cov_gen_fits(fp, pl->nband, pl->gw);
-  // forward x and y passes of the data and the means
-  if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src, !chunked)) != CRV_OK)
+  // whole grids on a plan where no wavenumber can need the FP64 solve: the
+  // posterior of the means and the residuals runs inside the z pass
+  // (crv_launch_zpost), so their spectra never make the HBM round trip
+  // between the z forward pass, the posterior and the z inverse pass
+  const int ztx = (!chunked && synth && pl->oE != 0 && pl->zpost_ok && n <= 6) ? crv_zpost_tx(ctx, fp, pl->nband) : 0;
+  const bool zfused = ztx > 0;
+  // forward x and y passes of the data and the means (and their z passes
+  // unless the fused z pass does them)
+  const bool fwd_z = !chunked && !zfused;
+  if ((rc = family_xy(ctx, fp, fd, false, crv_scale_forward(data[0]), nullptr, 0, data_src, fwd_z)) != CRV_OK)
     return rc;
-  if ((rc = family_xy(ctx, fp, fm, false, crv_scale_forward(mean[0]), nullptr, 0, mean_src, !chunked)) != CRV_OK)
+  if ((rc = family_xy(ctx, fp, fm, false, crv_scale_forward(mean[0]), nullptr, 0, mean_src, fwd_z)) != CRV_OK)
     return rc;
   for (int q = 0; q < 6; ++q) cov[q]->cubetype = CRV_COVARIANCE;
   const void* cv[6];
   for (int q = 0; q < 6; ++q) cv[q] = cov[q]->d;
+  if (zfused) {
+    unsigned long long* dstats = (unsigned long long*)(pl->dtab + pl->oS);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, dstats, 32);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+    ZPost z;
+    for (int l = 0; l < n; ++l) z.data[l] = data[l]->d;
+    for (int m = 0; m < 3; ++m) z.mean[m] = mean[m]->d;
+    z.n = n; z.nband = pl->nband;
+    z.lam = tab_at<float>(pl->dtab, pl->oT).lam;
+    z.erec = (const float*)(pl->dtab + pl->oE);
+    z.bidx = (const int*)(pl->dtab + pl->oGI);
+    z.bplanes = (const int*)(pl->dtab + pl->oBP);
+    z.stats = dstats;
+    if ((rc = crv_launch_zpost(ctx, fp, z, ztx)) != CRV_OK) return rc;
+    // the realizations' spectra (and the band's covariances where the z
+    // inverse does not generate them) from the posterior kernel alone
+    if (nsim > 0 || !synth_band) {
+      FusedSim fsim;
+      if (nsim > 0) {
+        fsim.sim = (float2*)sims[0]->d; fsim.sstride = (long long)sims[0]->rsize / 2; fsim.nsim = nreal;
+        fsim.seed = seed; fsim.real0 = realization0;
+        fsim.normals = normals; fsim.draws = fp->noise_draws; fsim.prefix = fp->noise_prefix;
+      }
+      fsim.skip_prior_cov = synth;
+      fsim.skip_band_cov = synth_band;
+      fsim.sim_only = true;
+      if ((rc = exec_rows(pl, mean, data, nullptr, cov, 0, nyp, false, fsim)) != CRV_OK) return rc;
+    }
+  }
   // per chunk of y rows: z forward, posterior, simulation, z inverse
-  for (int j0 = 0; j0 < nyp; j0 += chunk_rows) {
+  for (int j0 = 0; j0 < nyp && !zfused; j0 += chunk_rows) {
     const int j1 = std::min(nyp, j0 + chunk_rows);
     if (chunked) {
       if ((rc = family_pass(ctx, fp, fd, 2, j0, j1, 1.0, nullptr, 0)) != CRV_OK) return rc;
@@ -1601,7 +1633,7 @@ extern "C" int crv_invert_noise(crv_post_plan* pl, crv_grid* mean[3], crv_grid**
   // whose real values the realizations add in their last pass; the
   // realizations' noise when not chunked after the covariances' spectra are
   // complete and before their inverse transforms
-  if ((rc = family_xy(ctx, fp, fm, true, crv_scale_inverse(mean[0]), nullptr, 0, nullptr, !chunked)) != CRV_OK)
+  if ((rc = family_xy(ctx, fp, fm, true, crv_scale_inverse(mean[0]), nullptr, 0, nullptr, fwd_z)) != CRV_OK)
     return rc;
   if (!chunked && nsim && !fused &&
       (rc = crv_launch_sim_rows(ctx, fp, cv, sims[0]->d, (long long)sims[0]->rsize / 2, nreal, seed, realization0,

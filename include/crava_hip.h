@@ -113,6 +113,13 @@ int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
  *                      on the tiles its predecessor wrote last finds them in
  *                      the memory-side cache (0 = the library's default)   */
 #define CRV_TUNE_ORDER    7
+/*   CRV_TUNE_ZPOST     crv_invert's z pass with the posterior of the means
+ *                      inside it (FP32 refine plans whose wavenumbers all
+ *                      stay in FP32; results bit-equal to the separate z
+ *                      forward pass, posterior and z inverse pass): 0 = off
+ *                      (the default), 1 = on with the default tile width,
+ *                      2, 4, ... = on with that many x columns per tile    */
+#define CRV_TUNE_ZPOST    8
 int crv_ctx_tune(crv_ctx* ctx, int key, int value);
 
 /* Autotuner for one grid size: times the candidate launch shapes of each
