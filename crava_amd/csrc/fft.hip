@@ -638,7 +638,7 @@ struct ColArgs {
   int syn_q0;
   const int* syn_bidx;
   const float* syn_rows;
-  int syn_nband, syn_rw, syn_n, syn_j0;
+  int syn_nband, syn_rw, syn_n, syn_j0, syn_noqueue;
   int syn_nrows;       // y rows of the launch (persistent SYN blocks walk ntx x nrows tiles)
   EigQueueGeo syn_geo;
 };
@@ -664,14 +664,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
     // band; with the band generated too, each plane's row index and the band
     // planes' rows) are staged in LDS once, then the block walks the tiles
     // vb = blockIdx.x + i gridDim.x (gridDim.x a multiple of the 8 XCDs, so
-    // that xcd_remap keeps each XCD on a contiguous range).  Each tile is
-    // made in LDS in natural order with the lanes along its columns (the
-    // lanes of a wave share one or a few planes), transformed LDS -> LDS and
-    // stored with the lanes along the columns.
+    // that xcd_remap keeps each XCD on a contiguous range).  Each tile goes
+    // through the transform as a z inverse pass's does, its first stage
+    // generating the elements where that one loads them (the lanes of a wave
+    // share a few planes) and its last stage storing to HBM.
     float* synS = reinterpret_cast<float*>(twl + a.f.twn);
     float* synL = synS + a.f.L;
     int* synB = reinterpret_cast<int*>(synL + TX);
-    float* synT = reinterpret_cast<float*>(synB + a.f.L);
+    // band rows at a 16-byte boundary (read as float4)
+    float* synT = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(synB + a.f.L) + 15) & ~(uintptr_t)15);
     const int q = a.syn_q0 + blockIdx.y;
     for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x)
       synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
@@ -683,50 +684,50 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
     const int n = a.syn_n;
     const int ntiles = a.ntx * a.syn_nrows;
     for (int vb = blockIdx.x; vb < ntiles; vb += gridDim.x) {
+      // the column shift, opaque in each tile: otherwise the element and LDS
+      // offsets, the same for every tile, are hoisted out of the loop and
+      // held in registers across it
+      int csh = a.cshift;
+      asm volatile("" : "+v"(csh));
+      const int txs = 1 << csh;
       const int tile = xcd_remap(vb, ntiles, a.rev);
       const int o = tile / a.ntx, tt = tile - o * a.ntx;
-      const int x0 = tt * TX;
-      const int nc = min(TX, a.ncols - x0);
+      const int x0 = tt * txs;
+      const int nc = min(txs, a.ncols - x0);
       C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
-      // the previous tile's stores have read the LDS tile and synL
+      // the previous tile's last stage has read the LDS tile and synL
       __syncthreads();
-      for (int c = threadIdx.x; c < TX; c += (int)blockDim.x)
+      for (int c = threadIdx.x; c < txs; c += (int)blockDim.x)
         synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
       __syncthreads();
-      for (int t = threadIdx.x; t < a.f.L * TX; t += (int)blockDim.x) {
-        const int c = t & (TX - 1), e = t >> a.cshift;
-        C v = cmake<C>((T)0, (T)0);
-        if (c < nc) {
-          const float sv = synS[e];
-          if (!__builtin_isnan(sv)) {
-            v = cmake<C>((T)(synL[c] * sv), (T)0);
-          } else {
-            bool gen = false;
-            if (a.syn_rows) {
-              // a band plane: the eigen-basis posterior's covariance entry
-              const float lam = synL[c];
-              const float* tb = synT + synB[e] * a.syn_rw;
-              if (!eig_queued(a.syn_geo, lam, tb[1], tb[2], tb[3], x0 + c, a.syn_j0 + o, e)) {
-                float w[kEigMaxN];
+      auto gen = [&](int c, int e) -> C {
+        if (c >= nc) return cmake<C>((T)0, (T)0);
+        const float sv = synS[e];
+        if (!__builtin_isnan(sv)) return cmake<C>((T)(synL[c] * sv), (T)0);
+        if (a.syn_rows) {
+          // a band plane: the eigen-basis posterior's covariance entry
+          const float lam = synL[c];
+          const float4* tb = reinterpret_cast<const float4*>(synT + synB[e] * a.syn_rw);
+          const float4 h = tb[0];   // S_q, cond(D), l_max, l_min
+          if (a.syn_noqueue || !eig_queued(a.syn_geo, lam, h.y, h.z, h.w, x0 + c, a.syn_j0 + o, e)) {
+            float w[kEigMaxN], gr[kEigMaxN], gi[kEigMaxN];
 #pragma unroll
-                for (int p = 0; p < kEigMaxN; ++p) w[p] = p < n ? eig_weight(lam, tb[4 + p]) : 0.0f;
-                const float2 g = eig_cov_entry<kEigMaxN>(n, q < 3, lam * tb[0], lam * lam, w, tb + 4 + n,
-                                                            tb + 4 + 2 * n);
-                v = cmake<C>((T)g.x, (T)g.y);
-                gen = true;
-              }
+            for (int p = 0; p < kEigMaxN; ++p) {
+              const float4 g = p < n ? tb[1 + p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+              w[p] = p < n ? eig_weight(lam, g.x) : 0.0f;
+              gr[p] = g.y;
+              gi[p] = g.z;
             }
-            if (!gen) v = base[(size_t)e * a.estride + c];
+            const float2 g = eig_cov_entry<kEigMaxN>(n, q < 3, lam * h.x, lam * lam, w, gr, gi);
+            return cmake<C>((T)g.x, (T)g.y);
           }
         }
-        lds[e * TX + c] = v;
-      }
-      __syncthreads();
-      transform_tile_lds<T, INV, RF, RL, H>(lds, TX, a.cshift, a.f, a.sd, twl);
-      for (int t = threadIdx.x; t < a.f.L * TX; t += (int)blockDim.x) {
-        const int c = t & (TX - 1), e = t >> a.cshift;
-        if (c < nc) base[(size_t)e * a.estride + c] = lds[e * TX + c];
-      }
+        return base[(size_t)e * a.estride + c];
+      };
+      auto store = [&](int c, int e, C v) {
+        if (c < nc) base[(size_t)e * a.estride + c] = v;
+      };
+      transform_tile<T, INV, false, false, RF, RL, H>(lds, txs, csh, a.f, a.sd, twl, gen, store);
     }
     return;
   }
@@ -1106,7 +1107,7 @@ size_t cols_lds_bytes(const Fft1D& f, int tx, size_t elem, const CovGen* gen) {
   size_t lds = ((size_t)f.L * tx + f.twn) * elem;
   if (gen) {
     lds += ((size_t)f.L + tx) * sizeof(float);
-    if (gen->rows) lds += (size_t)f.L * sizeof(int) + (size_t)gen->nband * gen->rw * sizeof(float);
+    if (gen->rows) lds += (size_t)f.L * sizeof(int) + 16 + (size_t)gen->nband * gen->rw * sizeof(float);
   }
   return lds;
 }
@@ -1149,7 +1150,7 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
     a.syn_q0 = zs->q0;
     a.syn_bidx = zs->bidx;
     a.syn_rows = zs->rows;
-    a.syn_nband = zs->nband; a.syn_rw = zs->rw; a.syn_n = zs->n; a.syn_j0 = o0;
+    a.syn_nband = zs->nband; a.syn_rw = zs->rw; a.syn_n = zs->n; a.syn_j0 = o0; a.syn_noqueue = zs->noqueue;
     a.syn_geo = EigQueueGeo{zs->rx, zs->ry, zs->rz, zs->lowk2, zs->cond_thr, p->nyp, p->nzp};
     a.syn_nrows = o1 - o0;
     kern = cols_kernel_syn(a.f, half, wide);

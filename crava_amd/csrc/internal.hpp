@@ -171,10 +171,11 @@ int crv_sub_batch(const FftPlan* p, int nbatch);
 // covariances q0 + b.  Outside the band (band[k] == 0) the element of column
 // i, y row j is lam[j cnxp + i] S[6 k + q].  With rows (eigen-basis plans)
 // the band's planes are generated too: plane k's row bidx[k] of covariance
-// q, rows + (q nband + bidx[k]) rw, holds S_q, cond(D), l_max, l_min, the n
-// eigenvalues l_p and the real and imaginary parts of the n G_pq, and the
-// element is lam S_q - lam^2 sum_p w_p G_pq unless the posterior queued the
-// wavenumber for its FP64 solve (same rule), whose value is read.
+// q, rows + (q nband + bidx[k]) rw (rw = 4 + 4 n), holds S_q, cond(D),
+// l_max, l_min, then for each of the n eigenvalues l_p, Re G_pq, Im G_pq, 0,
+// and the element is lam S_q - lam^2 sum_p w_p G_pq unless the posterior
+// queued the wavenumber for its FP64 solve (same rule), whose value is read;
+// noqueue: the plan has shown that no wavenumber can be queued.
 struct CovGen {
   const float* lam = nullptr;
   const float* S = nullptr;
@@ -184,6 +185,7 @@ struct CovGen {
   const float* rows = nullptr;
   int nband = 0, rw = 0, n = 0;
   float rx = 0, ry = 0, rz = 0, lowk2 = 0, cond_thr = 0;
+  int noqueue = 0;
 };
 // LDS of the generating z inverse of a plan with nband band planes of rw
 // floats each fits the column kernels' 64 KB

@@ -1191,14 +1191,15 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
       memcpy(blob.data() + pl->oE, er.data(), er.size() * sizeof(float));
       // the generating z inverse's copy of the band planes' values: the same
       // floats as the records, per covariance q: S_q, cond(D), l_max, l_min,
-      // l_p, Re G_pq, Im G_pq (CovGen)
+      // then per eigenvalue p: l_p, Re G_pq, Im G_pq, 0 (CovGen; groups of
+      // four floats, read as float4)
       const int n = p->ntheta;
       const int OG = 2 * n * n + 12 * n, OW = OG + 9 * n, OS = OW + n, OC = OS + 12;
       std::vector<int> bidx(nzp, -1);
       int nb = 0;
       for (int k = 0; k < nzp; ++k)
         if (h.inv[k]) bidx[k] = nb++;
-      const int gw = 4 + 3 * n;
+      const int gw = 4 + 4 * n;
       std::vector<float> rows((size_t)6 * nb * gw, 0.0f);
       for (int k = 0; k < nzp; ++k) {
         if (bidx[k] < 0) continue;
@@ -1208,9 +1209,9 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
           t[0] = rk[OS + q];
           t[1] = rk[OC]; t[2] = rk[OC + 1]; t[3] = rk[OC + 2];
           for (int pp = 0; pp < n; ++pp) {
-            t[4 + pp] = rk[OW + pp];
-            t[4 + n + pp] = rk[OG + 9 * pp + (q < 3 ? q : 3 + 2 * (q - 3))];
-            t[4 + 2 * n + pp] = q < 3 ? 0.0f : rk[OG + 9 * pp + 4 + 2 * (q - 3)];
+            t[4 + 4 * pp] = rk[OW + pp];
+            t[5 + 4 * pp] = rk[OG + 9 * pp + (q < 3 ? q : 3 + 2 * (q - 3))];
+            t[6 + 4 * pp] = q < 3 ? 0.0f : rk[OG + 9 * pp + 4 + 2 * (q - 3)];
           }
         }
       }
@@ -1655,6 +1656,7 @@ extern "C" int crv_invert_noise(crv_post_plan* pl, crv_grid* mean[3], crv_grid**
       cs.nband = pl->nband; cs.rw = pl->gw; cs.n = pl->n;
       cs.rx = 2.0f / (float)pl->nxp; cs.ry = 2.0f / (float)pl->nyp; cs.rz = 2.0f / (float)pl->nzp;
       cs.lowk2 = (float)pl->lowk2; cs.cond_thr = pl->cond_thr;
+      cs.noqueue = pl->zpost_ok ? 1 : 0;
     }
   }
   if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked,
