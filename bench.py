@@ -575,13 +575,21 @@ def bench_c4(a):
     Pr = np.stack([p.download_real()[:, :, :nx] for p in P_rp])
     dP = np.abs(Pn - Pr)
     agree = float(np.mean(Pn.argmax(0) == Pr.argmax(0)))
+    # with the reference's undefined class (p_undef 0.01, faciesprob.cpp:2008)
+    Pn_u, Un = facies.probabilities(ctx, d_nn, prior, box, mean, p_undefined=0.01, with_undefined=True)
+    Pr_u, Ur = facies.probabilities(ctx, d_rp, prior, box, mean, p_undefined=0.01, with_undefined=True)
+    dPu = np.abs(np.stack([p.download_real()[:, :, :nx] for p in Pn_u + [Un]])
+                 - np.stack([p.download_real()[:, :, :nx] for p in Pr_u + [Ur]]))
     out["facies"] = {"cube": [sv_s.nxp, sv_s.nyp, sv_s.nzp], "posterior_s": t_post, "densities_and_probabilities_s":
                      t_fac, "bins": list(bins), "mean_abs_dP_vs_rplib": float(dP.mean()),
                      "p999_abs_dP_vs_rplib": float(np.quantile(dP, 0.999)), "max_abs_dP_vs_rplib": float(dP.max()),
-                     "most_probable_facies_agree": agree}
+                     "most_probable_facies_agree": agree,
+                     "undefined_class_0.01": {"max_abs_dP_vs_rplib": float(dPu.max()),
+                                              "mean_abs_dP_vs_rplib": float(dPu.mean())}}
     log(f"[bench c4] posterior {t_post * 1e3:.1f} ms, densities + probabilities {t_fac * 1e3:.1f} ms; "
         f"|P(surrogate) - P(rplib)| mean {dP.mean():.2e} p99.9 {np.quantile(dP, 0.999):.2e} max {dP.max():.2e}; "
-        f"most probable facies agree at {100 * agree:.3f} % of voxels")
+        f"most probable facies agree at {100 * agree:.3f} % of voxels; with the undefined class (0.01) "
+        f"max {dPu.max():.2e}")
     out["value"] = n_all / t_nn
     out["config"] = {"workload": "rock-physics surrogate 5-128-128-128-3 tanh, 3 facies x 1e7 samples, "
                                  "facies probabilities over a 256x256x200 FP32 posterior"}

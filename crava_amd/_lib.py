@@ -32,7 +32,7 @@ EXPORTS = [
     "crv_mlp_gradients", "crv_mlp_train_step", "crv_mlp_predict", "crv_mlp_flops", "crv_rock_refine",
     "crv_post_plan_create", "crv_post_plan_destroy", "crv_posterior_exec", "crv_post_plan_stats", "crv_invert",
     "crv_fft_autotune", "crv_grid_create_batch", "crv_simulate_batch", "crv_graph_begin", "crv_graph_end",
-    "crv_graph_launch", "crv_graph_destroy", "crv_facies_density", "crv_facies_prob",
+    "crv_graph_launch", "crv_graph_destroy", "crv_facies_density", "crv_facies_prob", "crv_facies_prob_undef",
     "crv_grid_derived", "crv_crava_header", "crv_grid_write_crava", "crv_grid_read_crava",
     "crv_storm_header", "crv_grid_write_storm",
     "crv_fft_r2c_batch", "crv_fft_r2c_batch_from", "crv_fft_c2r_batch",
@@ -221,6 +221,8 @@ def load(path: Path | None = None) -> C.CDLL:
         "crv_fft_r2c_batch_from": (i, [C.POINTER(vp), C.POINTER(vp), i]),
         "crv_fft_c2r_batch": (i, [C.POINTER(vp), i]),
         "crv_facies_prob": (i, [vp, i, C.POINTER(vp), _dp, C.POINTER(FaciesBox), C.POINTER(vp), C.POINTER(vp)]),
+        "crv_facies_prob_undef": (i, [vp, i, C.POINTER(vp), _dp, C.c_double, C.POINTER(FaciesBox), C.POINTER(vp),
+                                      C.POINTER(vp), vp]),
         "crv_grid_derived": (i, [vp, i, vp, vp, vp, vp]),
         "crv_crava_header": (i, [C.POINTER(SimboxInfo), i, i, i, C.c_char_p, C.c_size_t]),
         "crv_grid_write_crava": (i, [vp, C.c_char_p, C.POINTER(SimboxInfo)]),

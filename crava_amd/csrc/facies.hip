@@ -107,6 +107,8 @@ __global__ void k_sum(const float* __restrict__ d, int nx, long long rows, int r
 // src/posteriorelasticpdf3d.cpp:369-420): the Gaussian exp(-x^T Si x / 2)
 // sampled at the cyclic bin lags (j for j <= n/2, j - n above), stored in
 // float; the sum is taken separately and the kernel divided by it.
+__global__ void k_zero_d(double* p) { *p = 0.0; }
+
 __global__ void k_smoother(float* __restrict__ s, int nx, int ny, int nz, int rnx, double dx, double dy, double dz,
                            double i00, double i11, double i22, double i10, double i20, double i21) {
   const long long n = (long long)rnx * ny * nz;
@@ -162,23 +164,33 @@ struct ProbArgs {
   Box b;
   const float* m[3];   // posterior means (real grids, FP32)
   float* p[8];         // outputs (real grids)
+  float* pu;           // undefined class output (may be null)
+  // undefined class (FaciesProb::calculateFaciesProb, faciesprob.cpp:2008,
+  // 2020-2050): p_undef times the mean bin value of the first density --
+  // p_undef / (number of bins) for densities that sum to 1 over the box, as
+  // the reference's, and the same uniform share for unit-integral ones
+  float p_undef;
+  const double* dsum;  // sum of the first density over its bins (p_undef > 0)
+  double nbins;
   int nx, ny, nz, rnx; // voxel grid (padded, row length)
 };
 
 __global__ void k_prob(ProbArgs a) {
   const long long n = (long long)a.nx * a.ny * a.nz;
+  const float undef = a.p_undef > 0.0f ? (float)((double)a.p_undef * (*a.dsum / a.nbins)) : 0.0f;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(t % a.nx);
     const long long r = t / a.nx;
     const size_t i = (size_t)x + (size_t)a.rnx * r;
     const float v0 = a.m[0][i], v1 = a.m[1][i], v2 = a.m[2][i];
-    float q[8], s = 0.0f;
+    float q[8], s = undef;
     for (int f = 0; f < a.nf; ++f) {
       q[f] = a.prior[f] * trilinear(a.dens[f], a.b, v0, v1, v2);
       s += q[f];
     }
     const float inv = s > 0.0f ? 1.0f / s : 0.0f;
     for (int f = 0; f < a.nf; ++f) a.p[f][i] = s > 0.0f ? q[f] * inv : 1.0f / (float)a.nf;
+    if (a.pu) a.pu[i] = s > 0.0f ? undef * inv : 0.0f;
   }
 }
 
@@ -324,23 +336,27 @@ extern "C" int crv_facies_density_kernel(crv_ctx* ctx, const float* d_samples, l
   return facies_density_sampled(ctx, d_samples, n, box, smooth_cov, density);
 }
 
-extern "C" int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
-                               const crv_facies_box* box, crv_grid* mean[3], crv_grid** prob) {
+extern "C" int crv_facies_prob_undef(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
+                                     double p_undefined, const crv_facies_box* box, crv_grid* mean[3],
+                                     crv_grid** prob, crv_grid* prob_undef) {
   if (!ctx) return CRV_E_INVALID;
-  if (nf < 1 || nf > 8 || !density || !prior || !box || !mean || !prob)
+  if (nf < 1 || nf > 8 || !density || !prior || !box || !mean || !prob || !(p_undefined >= 0.0))
     return crv_set_error(ctx, CRV_E_INVALID, "facies_prob: bad args");
   const crv_grid* d0 = density[0];
   const crv_grid* m0 = mean[0];
   if (!d0 || !m0) return crv_set_error(ctx, CRV_E_INVALID, "facies_prob: NULL grid");
+  auto same_out = [&](const crv_grid* p) {
+    return p && p->nxp == m0->nxp && p->nyp == m0->nyp && p->nzp == m0->nzp && p->precision == CRV_FP32;
+  };
   for (int f = 0; f < nf; ++f) {
     const crv_grid* d = density[f];
     if (!d || d->nxp != d0->nxp || d->nyp != d0->nyp || d->nzp != d0->nzp || d->precision != CRV_FP32 ||
         d->transformed)
       return crv_set_error(ctx, CRV_E_INVALID, "facies_prob: densities must be real FP32 grids of one size");
-    const crv_grid* p = prob[f];
-    if (!p || p->nxp != m0->nxp || p->nyp != m0->nyp || p->nzp != m0->nzp || p->precision != CRV_FP32)
-      return crv_set_error(ctx, CRV_E_INVALID, "facies_prob: output grids inconsistent");
+    if (!same_out(prob[f])) return crv_set_error(ctx, CRV_E_INVALID, "facies_prob: output grids inconsistent");
   }
+  if (prob_undef && !same_out(prob_undef))
+    return crv_set_error(ctx, CRV_E_INVALID, "facies_prob: undefined-class grid inconsistent");
   for (int g = 0; g < 3; ++g) {
     const crv_grid* m = mean[g];
     if (!m || m->nxp != m0->nxp || m->nyp != m0->nyp || m->nzp != m0->nzp || m->precision != CRV_FP32 ||
@@ -358,10 +374,34 @@ extern "C" int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, c
     a.p[f] = (float*)prob[f]->d;
     prob[f]->transformed = false;
   }
+  if (prob_undef) {
+    a.pu = (float*)prob_undef->d;
+    prob_undef->transformed = false;
+  }
   for (int g = 0; g < 3; ++g) a.m[g] = (const float*)mean[g]->d;
   a.nx = m0->nxp; a.ny = m0->nyp; a.nz = m0->nzp; a.rnx = m0->rnxp;
-  ProfScope ps_(ctx, "facies_prob");
-  hipLaunchKernelGGL(k_prob, dim3(nblocks((long long)a.nx * a.ny * a.nz)), dim3(kThreads), 0, ctx->stream, a);
-  CRV_HIP_CHECK(ctx, hipGetLastError());
+  a.p_undef = (float)p_undefined;
+  a.nbins = (double)d0->nxp * d0->nyp * d0->nzp;
+  double* dsum = nullptr;
+  if (p_undefined > 0.0) {
+    CRV_HIP_CHECK(ctx, hipMallocAsync((void**)&dsum, sizeof(double), ctx->stream));
+    // a kernel clears the accumulator (no memset node in a captured graph)
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, ctx->stream, dsum);
+    hipLaunchKernelGGL(k_sum, dim3(256), dim3(kThreads), 0, ctx->stream, (const float*)d0->d, d0->nxp,
+                       (long long)d0->nyp * d0->nzp, d0->rnxp, dsum);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+    a.dsum = dsum;
+  }
+  {
+    ProfScope ps_(ctx, "facies_prob");
+    hipLaunchKernelGGL(k_prob, dim3(nblocks((long long)a.nx * a.ny * a.nz)), dim3(kThreads), 0, ctx->stream, a);
+    CRV_HIP_CHECK(ctx, hipGetLastError());
+  }
+  if (dsum) CRV_HIP_CHECK(ctx, hipFreeAsync(dsum, ctx->stream));
   return CRV_OK;
+}
+
+extern "C" int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
+                               const crv_facies_box* box, crv_grid* mean[3], crv_grid** prob) {
+  return crv_facies_prob_undef(ctx, nf, density, prior, 0.0, box, mean, prob, nullptr);
 }

@@ -41,14 +41,18 @@ def density(ctx: Context, d_samples, box: L.FaciesBox, smooth_cov, bins=(100, 10
     return g
 
 
-def probabilities(ctx: Context, densities, prior, box: L.FaciesBox, mean) -> list:
-    """P(facies | posterior mean) per voxel, one real FP32 grid per facies."""
+def probabilities(ctx: Context, densities, prior, box: L.FaciesBox, mean, p_undefined=0.0, with_undefined=False):
+    """P(facies | posterior mean) per voxel, one real FP32 grid per facies.
+    p_undefined > 0 adds FaciesProb's undefined class (0.01 in the
+    reference's defaults); with_undefined also returns its probability grid."""
     m0 = mean[0]
     out = [FFTGrid(ctx, m0.nxp, m0.nyp, m0.nzp, precision=32, cubetype=L.CRV_PARAMETER) for _ in densities]
+    und = FFTGrid(ctx, m0.nxp, m0.nyp, m0.nzp, precision=32, cubetype=L.CRV_PARAMETER) if with_undefined else None
     pr = np.ascontiguousarray(prior, np.float64)
     ds = (C.c_void_p * len(densities))(*[g.h.value for g in densities])
     ms = (C.c_void_p * 3)(*[g.h.value for g in mean])
     ps = (C.c_void_p * len(out))(*[g.h.value for g in out])
-    ctx.check(ctx.lib.crv_facies_prob(ctx.h, len(densities), ds, pr.ctypes.data_as(C.POINTER(C.c_double)),
-                                      C.byref(box), ms, ps), "facies_prob")
-    return out
+    ctx.check(ctx.lib.crv_facies_prob_undef(ctx.h, len(densities), ds, pr.ctypes.data_as(C.POINTER(C.c_double)),
+                                            float(p_undefined), C.byref(box), ms, ps,
+                                            und.h if und is not None else None), "facies_prob")
+    return (out, und) if with_undefined else out

@@ -542,6 +542,19 @@ int crv_facies_density(crv_ctx* ctx, const float* d_samples, long long n, const 
                        const double* smooth_cov, crv_grid* density);
 int crv_facies_prob(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
                     const crv_facies_box* box, crv_grid* mean[3], crv_grid** prob);
+/* crv_facies_prob with FaciesProb's undefined class (calculateFaciesProb,
+ * src/faciesprob.cpp:2008-2056, p_undefined = ModelSettings p_undef_, 0.01
+ * by default, modelsettings.cpp:119): the denominator gains
+ * p_undefined / (number of bins) for densities that sum to 1 over the box
+ * (the reference's PosteriorElasticPDF3D mass per bin; in general
+ * p_undefined times the mean bin value of density[0], the same share for
+ * unit-integral densities), so voxels far from every facies get
+ * probabilities near 0 instead of ratios of vanishing densities.
+ * prob_undef (may be NULL) receives the undefined class's probability
+ * (faciesProbUndef_).  p_undefined = 0 is crv_facies_prob.                 */
+int crv_facies_prob_undef(crv_ctx* ctx, int nf, crv_grid* const* density, const double* prior,
+                          double p_undefined, const crv_facies_box* box, crv_grid* mean[3], crv_grid** prob,
+                          crv_grid* prob_undef);
 /* The density with a choice of smoothing kernel:
  *   CRV_SMOOTH_FOURIER  crv_facies_density (the Gaussian's Fourier transform,
  *                       clamped at 0, normalised to unit integral);

@@ -72,6 +72,21 @@ def probabilities(dens, prior, lo, hi, bins, m):
     return np.where(s > 0, q / np.where(s > 0, s, 1.0), 1.0 / len(dens))
 
 
+def probabilities_undef(dens, prior, lo, hi, bins, m, p_undefined):
+    """With FaciesProb's undefined class (reference src/faciesprob.cpp:
+    2008-2056): the denominator gains undef = p_undefined / (number of bins)
+    for densities summing to 1 over the box -- in general p_undefined times
+    the mean bin value of the first density; returns (P per facies, P of the
+    undefined class)."""
+    pr = np.asarray(prior, np.float64)
+    pr = pr / pr.sum()
+    d0 = np.asarray(dens[0], np.float64)
+    undef = p_undefined * d0.sum() / d0.size
+    q = np.stack([p * trilinear(d, lo, hi, bins, m) for p, d in zip(pr, dens)], 0)
+    s = q.sum(0) + undef
+    return q / s, undef / s
+
+
 def density_pdf3d(samples, lo, hi, sigma, bins):
     """PosteriorElasticPDF3D's density (reference
     src/posteriorelasticpdf3d.cpp:12-131, 369-420), (nz, ny, nx): the

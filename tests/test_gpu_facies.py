@@ -100,6 +100,17 @@ def test_probabilities_match_oracle(ctx):
     sig = ev >= 1e-3 * max(p * d.max() for p, d in zip(pr, dens_o))
     assert sig.mean() > 0.05
     assert np.max(np.abs(got - want)[:, sig]) < 2e-3
+    # with the reference's undefined class (p_undef 0.01) the tails are
+    # bounded: compare every voxel
+    Pd, Pu = facies.probabilities(ctx, dens, prior, box, mean, p_undefined=0.01, with_undefined=True)
+    got_d = np.stack([p.download_real()[:, :, :nx].ravel() for p in Pd])
+    got_u = Pu.download_real()[:, :, :nx].ravel()
+    want_d, want_u = F.probabilities_undef(dens_o, prior, lo, hi, bins, pts, 0.01)
+    assert np.allclose(got_d.sum(0) + got_u, 1.0, atol=1e-5)
+    err = max(np.max(np.abs(got_d - want_d)), np.max(np.abs(got_u - want_u)))
+    print(f"facies probabilities with the undefined class: max error over all voxels {err:.2e}, "
+          f"undefined-class share {got_u.mean():.3f}")
+    assert err < 2e-3
 
 
 def test_surrogate_gain_in_facies_probabilities(ctx):
@@ -154,7 +165,16 @@ def test_surrogate_gain_in_facies_probabilities(ctx):
     diff = max(np.max(np.abs(a.download_real()[:, :, :nx].ravel() - b.download_real()[:, :, :nx].ravel())[sig])
                for a, b in zip(P_rp, P_nn))
     refined = [o[1] / 10_000_000 for o in outs]
-    print(f"facies probabilities: max |surrogate - rplib| = {diff:.2e}; samples: rplib {t_rplib * 1e3:.1f} ms, "
+    # every voxel, with the reference's undefined class (p_undef 0.01),
+    # which keeps the facies probabilities of voxels far from every facies
+    # near 0 instead of ratios of vanishing densities
+    Pu_rp, U_rp = facies.probabilities(ctx, d_rp, prior, box, mean, p_undefined=0.01, with_undefined=True)
+    Pu_nn, U_nn = facies.probabilities(ctx, d_nn, prior, box, mean, p_undefined=0.01, with_undefined=True)
+    diff_all = max(np.max(np.abs(a.download_real()[:, :, :nx] - b.download_real()[:, :, :nx]))
+                   for a, b in zip(Pu_rp + [U_rp], Pu_nn + [U_nn]))
+    print(f"facies probabilities: max |surrogate - rplib| = {diff:.2e} (significant evidence), {diff_all:.2e} "
+          f"(every voxel, undefined class 0.01); samples: rplib {t_rplib * 1e3:.1f} ms, "
           f"surrogate + gate {t_nn * 1e3:.1f} ms (refined {', '.join(f'{f:.1%}' for f in refined)})")
     assert diff < 0.03
+    assert diff_all < 0.03
     assert t_nn < t_rplib

@@ -771,6 +771,32 @@ def test_facies_density_pdf3d_closed_forms():
     assert np.max(np.abs(d[core] - mass[core]) / mass[core]) < 0.05
 
 
+def test_facies_probabilities_undefined_class_closed_forms():
+    """The undefined class of FaciesProb (faciesprob.cpp:2008-2056): with
+    uniform facies densities summing to 1 over the box every density is
+    1/nbins and undef = p/nbins, so P_f = prior_f / (1 + p) and
+    P_undef = p / (1 + p) everywhere; far from every facies (densities 0)
+    P_f = 0 and P_undef = 1; and P_undef + sum P_f = 1."""
+    from oracle import facies_oracle as F
+    lo, hi, bins = np.array([0.0, 0.0, 0.0]), np.array([1.0, 2.0, 0.5]), (10, 8, 6)
+    nb = bins[0] * bins[1] * bins[2]
+    uni = np.full((bins[2], bins[1], bins[0]), 1.0 / nb)
+    prior = [0.2, 0.5, 0.3]
+    rng = np.random.default_rng(7)
+    m = np.column_stack([rng.uniform(lo[k], hi[k], 500) for k in range(3)])
+    for p in (0.01, 0.5):
+        P, Pu = F.probabilities_undef([uni, uni, uni], prior, lo, hi, bins, m, p)
+        assert np.allclose(P, np.asarray(prior)[:, None] / (1 + p), rtol=1e-12)
+        assert np.allclose(Pu, p / (1 + p), rtol=1e-12)
+    # a facies density that vanishes on half the box
+    half = uni * 2.0
+    half[:, :, bins[0] // 2:] = 0.0
+    P, Pu = F.probabilities_undef([half, half], [0.5, 0.5], lo, hi, bins, m, 0.01)
+    assert np.allclose(P.sum(0) + Pu, 1.0)
+    far = m[:, 0] > 0.6   # beyond the half and the interpolation cell next to it
+    assert np.all(P[:, far] == 0.0) and np.all(Pu[far] == 1.0)
+
+
 def test_facies_probabilities_logistic():
     """Two Gaussian facies densities with one covariance T: P(1 | m) is the
     logistic function of the linear discriminant."""
