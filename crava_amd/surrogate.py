@@ -61,6 +61,73 @@ def sample_inputs(n: int, rng: np.random.Generator, facies: str | None = None) -
 FACIES_INPUTS = ("shale", "brine_sand", "gas_sand")
 
 
+def _quantile_inputs(u: np.ndarray, facies: str | None) -> np.ndarray:
+    """The inputs of sample_inputs as functions of five uniforms u (n, 5):
+    each parameter is its distribution's quantile at its own u (the
+    reference draws a rock from such quantiles, DistributionsRockDEM::
+    GenerateSamplePrivate, rplib/distributionsrockdem.cpp:125-156)."""
+    from scipy.special import betaincinv
+    uni = lambda q, a, b: a + (b - a) * q
+    phi, clay, sw, lasp, pe = (uni(u[:, 0], 0.05, 0.35), betaincinv(2, 5, u[:, 1]), uni(u[:, 2], 0.0, 1.0),
+                               uni(u[:, 3], -2.0, 0.0), uni(u[:, 4], 10.0, 40.0))
+    if facies == "shale":
+        clay = 0.5 + 0.5 * betaincinv(4, 2, u[:, 1])
+        sw = np.ones(len(u))
+        phi = uni(u[:, 0], 0.05, 0.2)
+        lasp = uni(u[:, 3], -2.0, -1.0)
+    elif facies in ("brine_sand", "gas_sand"):
+        clay = 0.25 * betaincinv(2, 5, u[:, 1])
+        sw = np.ones(len(u)) if facies == "brine_sand" else uni(u[:, 2], 0.05, 0.4)
+        phi = uni(u[:, 0], 0.15, 0.35)
+        lasp = uni(u[:, 3], -0.7, 0.0)
+    return np.column_stack([phi, clay, sw, 10.0 ** lasp, pe]).astype(np.float32)
+
+
+def synthetic_wells(nwells: int, length: int, dz: float, rng: np.random.Generator, facies=FACIES_INPUTS,
+                    corr: float = 0.5, uniforms=None):
+    """Synthetic well logs as FaciesProb::GenerateSyntWellData makes them
+    (reference src/faciesprob.cpp:2494-2559): a well is a sequence of facies
+    runs until it holds at least `length` samples (the last run is kept
+    whole); each run's facies is floor(nf U), its length ceil(-ln U / lambda)
+    with lambda = -ln(1 - dz / 10) (geometric, mean 10 / dz samples; dz must
+    be below 10), and its samples come from DistributionsRock::
+    GenerateWellSample (rplib/distributionsrock.cpp:42-59) with corr = 0.5:
+    the first from fresh uniforms, each next one by DEMTools::UpdateU
+    (rplib/demmodelling.cpp:820-850), u' = Phi(corr Phi^-1(u) +
+    sqrt(1 - corr^2) Z) per parameter.  The uniforms and normals come from
+    rng (PCG64), not the reference's dSFMT stream; uniforms (a callable
+    returning the next uniform) replaces rng's for the facies and lengths.
+
+    Returns (facies index per sample, inputs (n, 5) float32, well index per
+    sample, run index per sample)."""
+    from scipy.special import ndtr, ndtri
+    if not 0.0 < dz < 10.0:
+        raise ValueError("synthetic wells: dz must be in (0, 10)")
+    lam = -math.log(1.0 - dz / 10.0)
+    unif = uniforms if uniforms is not None else rng.random
+    nf = len(facies)
+    fac, us, well, run = [], [], [], []
+    eps = math.sqrt(1.0 - corr * corr)
+    r = 0
+    for w in range(nwells):
+        k = 0
+        while k < length:
+            f = int(math.floor(nf * unif()))
+            L = int(math.ceil(-math.log(unif()) / lam))
+            u = np.empty((L, 5))
+            u[0] = rng.random(5)
+            for i in range(1, L):
+                u[i] = ndtr(corr * ndtri(u[i - 1]) + eps * rng.standard_normal(5))
+            fac.append(np.full(L, f))
+            us.append((f, u))
+            well.append(np.full(L, w))
+            run.append(np.full(L, r))
+            r += 1
+            k += L
+    x = np.concatenate([_quantile_inputs(u, facies[f]) for f, u in us]) if us else np.zeros((0, 5), np.float32)
+    return np.concatenate(fac), x, np.concatenate(well), np.concatenate(run)
+
+
 def training_inputs(n: int, rng: np.random.Generator) -> np.ndarray:
     """(n, 5) training inputs: half from the broad box, the other half split
     evenly over the facies distributions that the sampling step queries,
