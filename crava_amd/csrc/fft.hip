@@ -641,6 +641,9 @@ struct ColArgs {
   int syn_nband, syn_rw, syn_n, syn_j0, syn_noqueue;
   int syn_nrows;       // y rows of the launch (persistent SYN blocks walk ntx x nrows tiles)
   EigQueueGeo syn_geo;
+  // complex per-column factor outside the band in place of syn_lam (the y
+  // inverse transform of lam, [y row][column]; CovGen::lamy)
+  const float2* syn_lamy;
 };
 
 // W: 512-thread blocks (tiles of up to 8192 complex FP32 values), so that
@@ -673,6 +676,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
     int* synB = reinterpret_cast<int*>(synL + TX);
     // band rows at a 16-byte boundary (read as float4)
     float* synT = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(synB + a.f.L) + 15) & ~(uintptr_t)15);
+    float2* synLy = reinterpret_cast<float2*>(synT + (a.syn_rows ? a.syn_nband * a.syn_rw : 0));
     const int q = a.syn_q0 + blockIdx.y;
     for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x)
       synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
@@ -697,13 +701,18 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
       C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
       // the previous tile's last stage has read the LDS tile and synL
       __syncthreads();
-      for (int c = threadIdx.x; c < txs; c += (int)blockDim.x)
+      for (int c = threadIdx.x; c < txs; c += (int)blockDim.x) {
         synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
+        if (a.syn_lamy) synLy[c] = (c < nc) ? a.syn_lamy[(size_t)o * a.ncols + x0 + c] : make_float2(0.0f, 0.0f);
+      }
       __syncthreads();
       auto gen = [&](int c, int e) -> C {
         if (c >= nc) return cmake<C>((T)0, (T)0);
         const float sv = synS[e];
-        if (!__builtin_isnan(sv)) return cmake<C>((T)(synL[c] * sv), (T)0);
+        if (!__builtin_isnan(sv)) {
+          if (a.syn_lamy) return cmake<C>((T)(synLy[c].x * sv), (T)(synLy[c].y * sv));
+          return cmake<C>((T)(synL[c] * sv), (T)0);
+        }
         if (a.syn_rows) {
           // a band plane: the eigen-basis posterior's covariance entry
           const float lam = synL[c];
@@ -744,6 +753,71 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
     if (c < nc) base[(size_t)e * a.estride + c] = v;
   };
   transform_tile<T, INV, false, false, RF, RL, H>(lds, TX, a.cshift, a.f, a.sd, twl, load, store);
+}
+
+// ---- y inverse of the covariances' band planes (crv_invert) ------------------
+// The covariance spectra of the inverted band are the eigen-basis posterior's
+// entries lam S_q - lam^2 sum_p w_p G_pq, generated from the plane's row of
+// tables; outside the band they are the separable prior lam(i, j) |ct(k)|
+// Sigma0[q].  The y and z inverse transforms commute, so the y pass runs on
+// the band's planes only (generating them in its first stage), and the z
+// inverse pass then generates the other planes from the y transform of lam
+// (CovGen::lamy): 40 % of a y pass per covariance instead of a whole one.
+struct BandYArgs {
+  float2* data;        // covariance grid of batch entry 0 (blockIdx.y: q = q0 + blockIdx.y)
+  long long bstride;   // complex elements between the grids of the batch
+  Fft1D f;
+  StageDiv sd;
+  const float2* tw;
+  long long pstride;   // cnxp nyp
+  int ncols, ntx, cshift, rev;
+  int nband, rw, n, q0, noqueue;
+  const int* bplanes;
+  const float* lam;    // [nyp][cnxp]
+  const float* rows;   // rows + (q nband + b) rw: S_q, cond(D), l_max, l_min, then (l_p, Re G, Im G, 0)
+  EigQueueGeo geo;
+};
+
+template <int RF, int RL, bool W>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThreads))) void k_cov_band_y(
+    BandYArgs a) {
+  using C = float2;
+  extern __shared__ unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const int TX = 1 << a.cshift;
+  C* twl = lds + a.f.L * TX;
+  float4* row = reinterpret_cast<float4*>((reinterpret_cast<uintptr_t>(twl + a.f.twn) + 15) & ~(uintptr_t)15);
+  copy_to_lds<float>(twl, a.tw + a.f.tw0, a.f.twn);
+  const int tile = xcd_remap(blockIdx.x, gridDim.x, a.rev);
+  const int b = tile / a.ntx, t = tile - b * a.ntx;
+  const int k = a.bplanes[b];
+  const int q = a.q0 + blockIdx.y;
+  const float* rq = a.rows + ((size_t)q * a.nband + b) * a.rw;
+  for (int i = threadIdx.x; i < a.rw; i += (int)blockDim.x) reinterpret_cast<float*>(row)[i] = rq[i];
+  __syncthreads();
+  const int x0 = t * TX, nc = min(TX, a.ncols - x0);
+  C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)k * a.pstride + x0;
+  const float4 h = row[0];   // S_q, cond(D), l_max, l_min
+  const int n = a.n;
+  auto gen = [&](int c, int j) -> C {
+    if (c >= nc) return make_float2(0.0f, 0.0f);
+    const float lam = a.lam[(size_t)j * a.ncols + x0 + c];
+    if (!a.noqueue && eig_queued(a.geo, lam, h.y, h.z, h.w, x0 + c, j, k))
+      return base[(size_t)j * a.ncols + c];   // the FP64 solve's value
+    float w[kEigMaxN], gr[kEigMaxN], gi[kEigMaxN];
+#pragma unroll
+    for (int p = 0; p < kEigMaxN; ++p) {
+      const float4 g = p < n ? row[1 + p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      w[p] = p < n ? eig_weight(lam, g.x) : 0.0f;
+      gr[p] = g.y;
+      gi[p] = g.z;
+    }
+    return eig_cov_entry<kEigMaxN>(n, q < 3, lam * h.x, lam * lam, w, gr, gi);
+  };
+  auto store = [&](int c, int j, C v) {
+    if (c < nc) base[(size_t)j * a.ncols + c] = v;
+  };
+  transform_tile<float, true, false, false, RF, RL, false>(lds, TX, a.cshift, a.f, a.sd, twl, gen, store);
 }
 
 // ---- z pass with the posterior of the means (crv_invert) ---------------------
@@ -1108,6 +1182,8 @@ size_t cols_lds_bytes(const Fft1D& f, int tx, size_t elem, const CovGen* gen) {
   if (gen) {
     lds += ((size_t)f.L + tx) * sizeof(float);
     if (gen->rows) lds += (size_t)f.L * sizeof(int) + 16 + (size_t)gen->nband * gen->rw * sizeof(float);
+    else lds += (size_t)f.L * sizeof(int) + 16;
+    if (gen->lamy) lds += (size_t)tx * sizeof(float2);
   }
   return lds;
 }
@@ -1151,6 +1227,7 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
     a.syn_bidx = zs->bidx;
     a.syn_rows = zs->rows;
     a.syn_nband = zs->nband; a.syn_rw = zs->rw; a.syn_n = zs->n; a.syn_j0 = o0; a.syn_noqueue = zs->noqueue;
+    a.syn_lamy = zs->lamy ? zs->lamy + (size_t)o0 * cnxp : nullptr;
     a.syn_geo = EigQueueGeo{zs->rx, zs->ry, zs->rz, zs->lowk2, zs->cond_thr, p->nyp, p->nzp};
     a.syn_nrows = o1 - o0;
     kern = cols_kernel_syn(a.f, half, wide);
@@ -1407,6 +1484,43 @@ bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw) {
 }
 
 // One pass of a transform over part of a batch (see internal.hpp).
+// ---- y inverse of the covariances' band planes -------------------------------
+const void* cov_band_y_kernel(const Fft1D& f, bool wide) {
+  if (f.nst != 2) return nullptr;
+#define CRV_PAIR_BANDY(A, B)                                                                       \
+  if (f.R[0] == (A) && f.R[1] == (B))                                                              \
+    return wide ? (const void*)k_cov_band_y<A, B, true> : (const void*)k_cov_band_y<A, B, false>;
+  CRV_FFT_PAIRS(CRV_PAIR_BANDY)
+#undef CRV_PAIR_BANDY
+  return nullptr;
+}
+
+bool crv_cov_band_y_fits(const FftPlan* p) {
+  return p->prec == CRV_FP32 && p->nyp > 1 && p->nzp > 1 && cov_band_y_kernel(p->fy, p->wide[1]) != nullptr;
+}
+
+int crv_launch_cov_band_y(crv_ctx* ctx, FftPlan* p, void* buf, int nbatch, long long bstride, const CovGen& g) {
+  const void* kern = crv_cov_band_y_fits(p) ? cov_band_y_kernel(p->fy, p->wide[1]) : nullptr;
+  if (!kern || !g.rows || !g.bplanes || g.nband < 1 || nbatch < 1 || nbatch > 6 || g.rw % 4 != 0)
+    return crv_set_error(ctx, CRV_E_UNSUPPORTED, "covariance band y pass: unsupported plan");
+  BandYArgs a{};
+  const int cnxp = p->nxp / 2 + 1;
+  a.data = (float2*)buf; a.bstride = bstride / 2;
+  a.f = p->fy; a.sd = make_stage_div(p->fy); a.tw = (const float2*)p->tw;
+  a.pstride = (long long)cnxp * p->nyp;
+  a.ncols = cnxp; a.cshift = ilog2(p->tx_y); a.ntx = (cnxp + p->tx_y - 1) / p->tx_y;
+  a.rev = (p->order >> 4) & 1;
+  a.nband = g.nband; a.rw = g.rw; a.n = g.n; a.q0 = g.q0; a.noqueue = g.noqueue;
+  a.bplanes = g.bplanes; a.lam = g.lam; a.rows = g.rows;
+  a.geo = EigQueueGeo{g.rx, g.ry, g.rz, g.lowk2, g.cond_thr, p->nyp, p->nzp};
+  const size_t lds = ((size_t)p->fy.L * p->tx_y + p->fy.twn) * sizeof(float2) + 16 + (size_t)g.rw * sizeof(float);
+  void* args[] = {&a};
+  ProfScope ps(ctx, "fft_y_inv_band");
+  CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(a.ntx * g.nband, nbatch), dim3(p->wide[1] ? 2 * kThreads : kThreads),
+                                     args, lds, ctx->stream));
+  return CRV_OK;
+}
+
 // ---- fused z pass of crv_invert -----------------------------------------------
 size_t zpost_lds_bytes(const FftPlan* p, int nband, int tx) {
   return ((size_t)3 * p->fz.L * tx + p->fz.twn) * sizeof(float2) + (size_t)(p->fz.L + nband) * sizeof(int);

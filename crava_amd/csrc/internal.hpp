@@ -186,7 +186,21 @@ struct CovGen {
   int nband = 0, rw = 0, n = 0;
   float rx = 0, ry = 0, rz = 0, lowk2 = 0, cond_thr = 0;
   int noqueue = 0;
+  // the band planes (nband ints; crv_launch_cov_band_y)
+  const int* bplanes = nullptr;
+  // with lamy (complex, [nyp][cnxp]) the z inverse's input outside the band
+  // is lamy[o cnxp + i] S[6 k + q], for covariances whose y inverse has been
+  // done already: lamy is the y inverse transform of lam (the separable
+  // prior's plane after its y pass)
+  const float2* lamy = nullptr;
 };
+// The y inverse pass of the band planes only of nbatch covariance grids
+// (FP32, eigen-basis records, covariances q0 + b): the elements are generated
+// as the eigen-basis posterior's entries (the queued ones read from the grid),
+// transformed along y and stored in place; the other planes are left as they
+// are.  Needs a two-stage kernel for nyp.
+int crv_launch_cov_band_y(crv_ctx* ctx, FftPlan* plan, void* buf, int nbatch, long long bstride, const CovGen& g);
+bool crv_cov_band_y_fits(const FftPlan* p);
 // LDS of the generating z inverse of a plan with nband band planes of rw
 // floats each fits the column kernels' 64 KB
 bool crv_cov_gen_fits(const FftPlan* p, int nband, int rw);

@@ -838,6 +838,11 @@ KzTab<U> tab_at(unsigned char* base, const TabOffsets& o) {
   return t;
 }
 
+__global__ void k_real_to_complex(const float* __restrict__ x, float2* __restrict__ z, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    z[i] = make_float2(x[i], 0.0f);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
@@ -1132,6 +1137,11 @@ struct crv_post_plan {
   // pass (crv_launch_zpost)
   size_t oBP = 0;
   bool zpost_ok = false;
+  // with eigen-basis records (plans made by crv_post_plan_create): the y
+  // inverse transform of lam as a complex plane (nyp x cnxp), from which the
+  // covariances' z inverse generates the planes outside the band after the
+  // band's y pass (crv_launch_cov_band_y)
+  float2* dlamy = nullptr;
   unsigned int* dlist = nullptr;
   double sig0[6] = {0, 0, 0, 0, 0, 0};
   float cond_thr = 1e5f;
@@ -1250,6 +1260,25 @@ int plan_make(crv_ctx* ctx, const crv_post_params* p, int nxp, int nyp, int nzp,
     crv_post_plan_destroy(pl);
     return crv_set_error(ctx, CRV_E_HIP, std::string("posterior plan: ") + hipGetErrorString(e));
   }
+  if (!async && pl->oE && pl->oGR && pl->oBP && precision == CRV_FP32) {
+    // lam's y inverse transform, with the transforms' own y pass
+    FftPlan* fp = nullptr;
+    const size_t nl = (size_t)pl->cnxp * nyp;
+    if ((rc = crv_get_plan(ctx, nxp, nyp, nzp, CRV_FP32, &fp)) == CRV_OK && crv_cov_band_y_fits(fp)) {
+      e = hipMalloc((void**)&pl->dlamy, nl * sizeof(float2));
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_real_to_complex, dim3((unsigned)std::min<size_t>((nl + 255) / 256, 4096)), dim3(256), 0,
+                           ctx->stream, tab_at<float>(pl->dtab, pl->oT).lam, pl->dlamy, (long long)nl);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) rc = crv_launch_pass(ctx, fp, 4, pl->dlamy, pl->dlamy, 1, 0, 0, 1, 1.0, nullptr, 0);
+      if (e == hipSuccess && rc == CRV_OK) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess || rc != CRV_OK) {
+        crv_post_plan_destroy(pl);
+        return rc != CRV_OK ? rc : crv_set_error(ctx, CRV_E_HIP, std::string("posterior plan: ") + hipGetErrorString(e));
+      }
+    }
+  }
   *out = pl;
   return CRV_OK;
 }
@@ -1274,6 +1303,7 @@ extern "C" int crv_post_plan_destroy(crv_post_plan* pl) {
     (void)hipStreamSynchronize(ctx->stream);
     if (pl->dtab) (void)hipFree(pl->dtab);
     if (pl->dlist) (void)hipFree(pl->dlist);
+    if (pl->dlamy) (void)hipFree(pl->dlamy);
   }
   delete pl;
   return CRV_OK;
@@ -1659,9 +1689,29 @@ extern "C" int crv_invert_noise(crv_post_plan* pl, crv_grid* mean[3], crv_grid**
       cs.noqueue = pl->zpost_ok ? 1 : 0;
     }
   }
-  if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked,
-                      synth ? &cs : nullptr)) != CRV_OK)
+  if (synth_band && pl->dlamy && crv_cov_band_y_fits(fp)) {
+    // per covariance: the y inverse of the band's planes, generated there;
+    // the z inverse, generating the planes outside the band from the y
+    // transform of lam and reading the band's; the x C2R pass
+    cs.bplanes = (const int*)(pl->dtab + pl->oBP);
+    for (int q = 0; q < 6; ++q) {
+      CovGen g = cs;
+      g.q0 = q;
+      if ((rc = crv_launch_cov_band_y(ctx, fp, cov[q]->d, 1, 0, g)) != CRV_OK) return rc;
+      CovGen gz = cs;
+      gz.q0 = q;
+      gz.rows = nullptr;
+      gz.lamy = pl->dlamy;
+      if ((rc = crv_launch_pass(ctx, fp, 3, cov[q]->d, cov[q]->d, 1, 0, 0, nyp, 1.0, nullptr, 0, &gz)) != CRV_OK)
+        return rc;
+      if ((rc = crv_launch_pass(ctx, fp, 5, cov[q]->d, cov[q]->d, 1, 0, 0, nyp * pl->nzp, crv_scale_inverse(cov[q]),
+                                nullptr, 0)) != CRV_OK)
+        return rc;
+    }
+  } else if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked,
+                             synth ? &cs : nullptr)) != CRV_OK) {
     return rc;
+  }
   for (int l = 0; l < n; ++l) data[l]->transformed = true;
   for (int m = 0; m < 3; ++m) mean[m]->transformed = false;
   for (int q = 0; q < 6; ++q) cov[q]->transformed = false;
