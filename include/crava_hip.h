@@ -327,8 +327,13 @@ int crv_post_plan_stats(crv_post_plan* plan, crv_post_stats* stats);
  * up to 6 angles), the realizations outside the inverted band scale the noise by
  * sqrt(lam) times the plane's factor of the prior, where
  * crv_simulate_batch factors the stored FP32 product lam S: those values
- * differ by rounding (about 1e-7 relative).  Launches only (capturable);
- * statistics through crv_post_plan_stats.                                  */
+ * differ by rounding (about 1e-7 relative).  And on such plans whose nyp has
+ * a two-stage kernel the covariances' inverse transform runs its y pass on
+ * the inverted band's planes first and its z pass after it, generating the
+ * other planes from the y transform of |cxy| (the sequence does z, then y):
+ * the covariances then differ from the sequence's by rounding (below 1e-6
+ * relative RMS).  Launches only (capturable); statistics through
+ * crv_post_plan_stats.                                                      */
 int crv_invert(crv_post_plan* plan, crv_grid* mean[3], crv_grid** data, crv_grid* const* mean_src,
                crv_grid* const* data_src, crv_grid* cov[6], unsigned long long seed,
                unsigned long long realization0, int nreal, crv_grid** sims, int chunk_rows);
