@@ -1694,18 +1694,23 @@ extern "C" int crv_invert_noise(crv_post_plan* pl, crv_grid* mean[3], crv_grid**
     // the z inverse, generating the planes outside the band from the y
     // transform of lam and reading the band's; the x C2R pass
     cs.bplanes = (const int*)(pl->dtab + pl->oBP);
-    for (int q = 0; q < 6; ++q) {
+    // in sub-batches of contiguous grids sized to the memory-side cache (one
+    // grid at a time otherwise), each pass reading its predecessor's output
+    const int sb = fc.contiguous ? crv_sub_batch(fp, 6) : 1;
+    const long long rs = (long long)cov[0]->rsize;
+    for (int q0 = 0; q0 < 6; q0 += sb) {
+      const int nb = std::min(sb, 6 - q0);
       CovGen g = cs;
-      g.q0 = q;
-      if ((rc = crv_launch_cov_band_y(ctx, fp, cov[q]->d, 1, 0, g)) != CRV_OK) return rc;
+      g.q0 = q0;
+      if ((rc = crv_launch_cov_band_y(ctx, fp, cov[q0]->d, nb, rs, g)) != CRV_OK) return rc;
       CovGen gz = cs;
-      gz.q0 = q;
+      gz.q0 = q0;
       gz.rows = nullptr;
       gz.lamy = pl->dlamy;
-      if ((rc = crv_launch_pass(ctx, fp, 3, cov[q]->d, cov[q]->d, 1, 0, 0, nyp, 1.0, nullptr, 0, &gz)) != CRV_OK)
+      if ((rc = crv_launch_pass(ctx, fp, 3, cov[q0]->d, cov[q0]->d, nb, rs, 0, nyp, 1.0, nullptr, 0, &gz)) != CRV_OK)
         return rc;
-      if ((rc = crv_launch_pass(ctx, fp, 5, cov[q]->d, cov[q]->d, 1, 0, 0, nyp * pl->nzp, crv_scale_inverse(cov[q]),
-                                nullptr, 0)) != CRV_OK)
+      if ((rc = crv_launch_pass(ctx, fp, 5, cov[q0]->d, cov[q0]->d, nb, rs, 0, nyp * pl->nzp,
+                                crv_scale_inverse(cov[q0]), nullptr, 0)) != CRV_OK)
         return rc;
     }
   } else if ((rc = family_xy(ctx, fp, fc, true, crv_scale_inverse(cov[0]), nullptr, 0, nullptr, !chunked,
