@@ -197,11 +197,14 @@ def transforms_per_step(name, s, names=()):
     passes carry that name)."""
     if name in ("fft_x_r2c", "fft_y_fwd", "fft_z_fwd"):
         return s.n + 3
-    if name in ("fft_x_c2r", "fft_y_inv"):
+    if name == "fft_x_c2r":
         return 12
+    if name == "fft_y_inv":
+        # the covariances' y pass runs on their band planes ("fft_y_inv_band")
+        return 6 if "fft_y_inv_band" in names else 12
     if name == "fft_z_inv":
         return 6 if "fft_z_inv_prior" in names else 12
-    if name == "fft_z_inv_prior":
+    if name in ("fft_z_inv_prior", "fft_y_inv_band"):
         return 6
     return None
 
@@ -232,6 +235,9 @@ def kernel_bytes(name, s, prec, ninv, names=()):
     if name == "fft_z_inv_prior":
         # reads the band's planes only, writes every plane
         return 2 * e * ninv + 2 * e * Nc
+    if name == "fft_y_inv_band":
+        # generates and writes the band's planes, reads nothing
+        return 2 * e * ninv
     if name == "fft_x_r2c":
         return e * N + 2 * e * Nc
     if name == "fft_x_c2r":
@@ -243,9 +249,12 @@ def kernel_bytes(name, s, prec, ninv, names=()):
     if name in ("simulate_scale", "simulate_scale_batch"):
         return Nc * (6 + 3) * 2 * e
     if name == "posterior_sim_fp32":
-        # the posterior plus the realization's 3 spectra it writes
+        # the posterior plus the realization's 3 spectra it writes; with the
+        # covariance planes generated by their inverse passes it writes no
+        # covariance at all
+        cov_in = 6 * 2 * e if prior_cov_written else 0
         outside = (Nc - ninv) * 6 * 2 * e if prior_cov_written else 0
-        return ninv * (2 * (3 + n) * 2 * e + 6 * 2 * e) + outside + Nc * 3 * 2 * e
+        return ninv * (2 * (3 + n) * 2 * e + cov_in) + outside + Nc * 3 * 2 * e
     return None
 
 
@@ -449,6 +458,7 @@ KERNEL_SYMBOLS = {
     "fft_y_inv": "k_cols<float, true, 25, 18, false, false, true>",
     "fft_z_inv": "k_cols<float, true, 20, 16, false, false, true>",
     "fft_z_inv_prior": "k_cols<float, true, 20, 16, false, true, true>",
+    "fft_y_inv_band": "k_cov_band_y<25, 18, true>", "fft_z_post": "k_zpost<20, 16>",
     "posterior_fp32": "k_posterior_eig<5, 0>", "posterior_sim_fp32": "k_posterior_eig<5, 2>",
     "simulate_scale_batch": "k_simulate<float, true>",
 }
