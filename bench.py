@@ -479,7 +479,7 @@ def read_traffic(kernel):
     return v.get("bytes") if isinstance(v, dict) else None
 
 
-def bench_c4(a):
+def bench_c4(a, emit=True):
     """The learned rock-physics surrogate: BF16 MFMA training on 1e6 rplib
     samples, 1e7 samples per facies (shale, brine sand, gas sand) from the
     surrogate with rplib re-evaluating every sample whose predicted ln vp
@@ -603,10 +603,12 @@ def bench_c4(a):
     out["value"] = n_all / t_nn
     out["config"] = {"workload": "rock-physics surrogate 5-128-128-128-3 tanh, 3 facies x 1e7 samples, "
                                  "facies probabilities over a 256x256x200 FP32 posterior"}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def bench_c3(a):
+def bench_c3(a, emit=True):
     """Batched and graph-captured runs: 64 realizations of the 256x256x200
     5-angle inversion (one launch per stage for all realizations), and 12
     uneven volumes in one captured graph; the FFT launch shapes of every
@@ -758,7 +760,67 @@ def bench_c3(a):
                     "voxels_per_s": vox / graphed2,
                     "autotune": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in shapes.items()}}
     log(f"[bench c3] part 2: graph {graphed2 * 1e3:.2f} ms, eager {eager2 * 1e3:.2f} ms for {len(vols)} volumes")
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
+
+
+def c0_leg():
+    """The 64x64x128 3-angle survey (the reference's CPU-runnable case):
+    crv_invert in FP32 refine and in FP64 on identical inputs, voxels/s, and
+    the FP32 outputs' errors against FP64."""
+    from crava_amd import _lib as L
+    from crava_amd import hip, model
+    from crava_amd.survey import GpuSurvey
+    s = model.config0()
+    ctx = hip.Context(0)
+    sv = GpuSurvey(ctx, s)
+    kw = sv.posterior_kwargs()
+    res = {}
+    for prec, mode in ((32, L.CRV_MODE_FP32_REFINE), (64, L.CRV_MODE_PLAIN)):
+        G = make_grids(ctx, s, prec)
+        G["plan"] = hip.PostPlan(ctx, s.nxp, s.nyp, s.nzp, prec, s.n, mode=mode, separable=True,
+                                 skip_mean_inverse=True, **kw)
+        G["path"], G["chunk_rows"] = "invert", 0
+        for g, src in zip(G["mean_in"] + G["data_in"], sv.mean + sv.data):
+            if prec == 32:
+                g.copyFrom(src)
+            else:
+                tmp = hip.FFTGrid(ctx, s.nxp, s.nyp, s.nzp, precision=32, cubetype=g.getType())
+                tmp.copyFrom(src)
+                g.copyFrom(tmp)
+                tmp.close()
+        sec = timed(ctx, G, kw, mode, 20, 3, seed=s.seed_sim)
+        step(ctx, G, kw, mode, 77, s.seed_sim)
+        ctx.synchronize()
+        res[prec] = (sec, G)
+    G32, G64 = res[32][1], res[64][1]
+    err = {}
+    for a_, nm in enumerate(("mean_vp", "mean_vs", "mean_rho")):
+        err[nm] = compare(G64["mean"][a_], G32["mean"][a_], s.nxp, s.mu[a_])[0]
+    for q, nm in enumerate(("cov_aa", "cov_bb", "cov_rr", "cov_ab", "cov_ar", "cov_br")):
+        err[nm] = compare(G64["cov"][q], G32["cov"][q], s.nxp)[0]
+    for a_, nm in enumerate(("real_vp", "real_vs", "real_rho")):
+        err[nm] = compare(G64["sim"][a_], G32["sim"][a_], s.nxp, s.mu[a_])[0]
+    return {"workload": f"AVO inversion {s.nxp}x{s.nyp}x{s.nzp} padded, {s.n} angles, mean + 6 cov + 1 realization",
+            "fp32_refine_ms_per_step": res[32][0] * 1e3, "voxels_per_s": s.voxels / res[32][0],
+            "fp64_ms_per_step": res[64][0] * 1e3, "rel_rms_vs_fp64": err}
+
+
+def other_configs(a):
+    """The other workloads, each measured as its own --config run does
+    (bounded: about half a minute together), for the record of the default
+    run; a failure is recorded, not raised."""
+    out = {}
+    strip = lambda d: {k: v for k, v in d.items() if k not in ("autotune", "kernels_ms")} if isinstance(d, dict) else d
+    for name, fn in (("c0", c0_leg), ("c3", lambda: bench_c3(a, emit=False)), ("c4", lambda: bench_c4(a, emit=False))):
+        log(f"[bench] other workload {name}")
+        try:
+            r = fn()
+            out[name] = {k: strip(v) for k, v in r.items()}
+        except Exception as ex:  # recorded in the output line
+            out[name] = {"error": repr(ex)}
+    return out
 
 
 def main():
@@ -775,6 +837,8 @@ def main():
                     help="skip the step with the reference RandomGen stream for the realization")
     ap.add_argument("--no-sequence", action="store_true",
                     help="skip the one-call sequence timing (the like-for-like FP32 / FP64 ratio)")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the other workloads (64x64x128 survey, batched realizations and volumes, surrogate)")
     ap.add_argument("--no-general", action="store_true",
                     help="skip the drop-in general path (prior covariance grids + errCorr, transformed inside)")
     ap.add_argument("--fp32-mode", choices=["refine", "plain"], default="refine",
@@ -1032,6 +1096,7 @@ def main():
         "kernels_fp64": ({k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in res[64]["prof"].items()}
                          if 64 in res else None),
     }
+    out["other_configs"] = None if (a.no_other or a.config != "c1") else other_configs(a)
     if not a.no_cpu:
         log("[bench] CPU baseline (bounded sample)")
         out["cpu_baseline"] = cpu_baseline(s, a.cpu_threads)
