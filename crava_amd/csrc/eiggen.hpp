@@ -63,17 +63,23 @@ __device__ __forceinline__ float eig_weight(float lam, float lp) { return __buil
 
 // entry q of the posterior covariance: s = lam S_q, l2 = lam^2, w the
 // weights, gr / gi the real and imaginary parts of G_pq (gi unused for the
-// diagonal entries, whose imaginary part is 0)
+// diagonal entries, whose imaginary part is 0).  Terms p >= n are computed
+// and discarded by a select (their inputs may hold anything): a branch on
+// the run-time n would split the caller's code into blocks that the
+// compiler schedules one after another.
 template <int NMAX>
 __device__ __forceinline__ float2 eig_cov_entry(int n, bool diag, float s, float l2, const float* w, const float* gr,
                                                 const float* gi) {
   float xr = 0.0f, xi = 0.0f;
 #pragma unroll
-  for (int p = 0; p < NMAX; ++p)
-    if (p < n) {
-      xr = fmaf(w[p], gr[p], xr);
-      if (!diag) xi = fmaf(w[p], gi[p], xi);
+  for (int p = 0; p < NMAX; ++p) {
+    const float tr = fmaf(w[p], gr[p], xr);
+    xr = p < n ? tr : xr;
+    if (!diag) {
+      const float ti = fmaf(w[p], gi[p], xi);
+      xi = p < n ? ti : xi;
     }
+  }
   return diag ? make_float2(fmaf(-l2, xr, s), 0.0f) : make_float2(fmaf(-l2, xr, s), -l2 * xi);
 }
 

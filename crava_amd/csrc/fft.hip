@@ -152,6 +152,43 @@ __device__ __forceinline__ void stage_first(cplx_t<T>* lds, int S, int cshift, i
   }
 }
 
+// first stage whose inputs are made in two steps: pre(c, e) for every
+// element of the thread's butterflies (the loads, all in flight together),
+// then gen(c, e, p) from what pre returned.  A generator whose value needs a
+// load and then a branch on it would otherwise wait for each load in turn.
+template <typename T, int R, bool INV, int E, typename Pre, typename Gen>
+__device__ __forceinline__ void stage_first_pre(cplx_t<T>* lds, int S, int cshift, int L, Pre pre, Gen gen) {
+  using C = cplx_t<T>;
+  using P = decltype(pre(0, 0));
+  constexpr int MAXB = (E + R - 1) / R;
+  const int Lr = L / R;
+  const int total = Lr << cshift;
+  const int cm = (1 << cshift) - 1;
+  P pv[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * (int)blockDim.x;
+    if (b < total) {
+      const int c = b & cm, j = b >> cshift;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pv[q][r] = pre(c, j + r * Lr);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = threadIdx.x + q * (int)blockDim.x;
+    if (b < total) {
+      const int c = b & cm, j = b >> cshift;
+      C v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = gen(c, j + r * Lr, pv[q][r]);
+      Dft<T, R, INV>::run(v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[(j * R + r) * S + c] = v[r];
+    }
+  }
+}
+
 // middle stage: LDS -> LDS (pencil-fast); ends with the tile written.
 // TW = false: a first stage (Ns = 1), whose twiddles are all 1.
 template <typename T, int R, bool INV, bool TW, int E>
@@ -338,6 +375,19 @@ __device__ __forceinline__ void transform_tile(cplx_t<T>* lds, int S, int cshift
   }
 }
 
+// Whole two-stage transform of a pencil-fast tile whose first stage makes
+// its inputs with pre / gen (stage_first_pre) and whose last stage stores.
+template <typename T, bool INV, int RF, int RL, typename Pre, typename Gen, typename Store>
+__device__ __forceinline__ void transform_tile_pre(cplx_t<T>* lds, int S, int cshift, const Fft1D& f,
+                                                   const StageDiv& sd, const cplx_t<T>* twl, Pre pre, Gen gen,
+                                                   Store store) {
+  static_assert(RF > 0, "two-stage lengths only");
+  constexpr int E = ept<T, RF, false>();
+  stage_first_pre<T, RF, INV, E>(lds, S, cshift, f.L, pre, gen);
+  __syncthreads();
+  stage_last<T, RL, INV, false, E>(lds, S, cshift, f.L, sd.lr[1], twl + (f.tw[1] - f.tw0), store);
+}
+
 // Whole transform of a tile already in LDS in natural order (lds[e*S + c]),
 // left there in natural order; every stage runs LDS -> LDS.  The caller
 // synchronises before; the tile is complete on return.
@@ -357,6 +407,16 @@ __device__ __forceinline__ void transform_tile_lds(cplx_t<T>* lds, int S, int cs
     const cplx_t<T>* t = twl + (f.tw[s] - f.tw0);
     CRV_RADIX_SWITCH(f.R[s], (stage_mid<T, R_, INV, true, E>(lds, S, cshift, L, f.Ns[s], sd.ns[s], t)));
   }
+}
+
+// p (inside the dynamic LDS block base) rounded up to a 16-byte boundary,
+// kept an LDS pointer: a round trip through an integer would make it a
+// generic one, whose loads are flat loads that the compiler waits for
+// together with the global loads in flight
+template <typename U, typename V>
+__device__ __forceinline__ U* lds_align16(unsigned char* base, V* p) {
+  const unsigned off = (unsigned)(reinterpret_cast<unsigned char*>(p) - base);
+  return reinterpret_cast<U*>(base + ((off + 15u) & ~15u));
 }
 
 template <typename T>
@@ -394,7 +454,7 @@ template <typename T, int RF, int RL, bool H>
 __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   using C = cplx_t<T>;
   constexpr int E = ept<T, RF, H>();
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
   const int L = a.f.L, K = a.cnxp;
@@ -467,7 +527,7 @@ template <typename T, int RF, int RL, bool H>
 __global__ __launch_bounds__(kThreads) void k_rows_c2r(RowArgs<T> a) {
   using C = cplx_t<T>;
   constexpr int E = ept<T, RF, H>();
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int P = 1 << a.pshift, S = P + 1;
   const int L = a.f.L, K = a.cnxp;
@@ -657,7 +717,7 @@ template <typename T, bool INV, int RF, int RL, bool H, bool SYN = false, bool W
 __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThreads),
                           amdgpu_waves_per_eu(cols_waves_per_eu(W, SYN, RF * RL)))) void k_cols(ColArgs<T> a) {
   using C = cplx_t<T>;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int TX = 1 << a.cshift;
   C* twl = lds + a.f.L * TX;
@@ -672,11 +732,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
     // generating the elements where that one loads them (the lanes of a wave
     // share a few planes) and its last stage storing to HBM.
     float* synS = reinterpret_cast<float*>(twl + a.f.twn);
-    float* synL = synS + a.f.L;
-    int* synB = reinterpret_cast<int*>(synL + TX);
+    int* synB = reinterpret_cast<int*>(synS + a.f.L);
     // band rows at a 16-byte boundary (read as float4)
-    float* synT = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(synB + a.f.L) + 15) & ~(uintptr_t)15);
-    float2* synLy = reinterpret_cast<float2*>(synT + (a.syn_rows ? a.syn_nband * a.syn_rw : 0));
+    float* synT = lds_align16<float>(smem_raw, synB + a.f.L);
     const int q = a.syn_q0 + blockIdx.y;
     for (int e = threadIdx.x; e < a.f.L; e += (int)blockDim.x)
       synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
@@ -699,44 +757,68 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
       const int x0 = tt * txs;
       const int nc = min(txs, a.ncols - x0);
       C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)o * a.ostride + x0;
-      // the previous tile's last stage has read the LDS tile and synL
+      // where the block size is a multiple of TX, the first stage gives each
+      // thread butterflies of one column, c = threadIdx.x mod TX: its lam and
+      // lam's y transform are loaded into registers with the band reads
+      // (otherwise each element loads them)
+      const bool percol = txs <= (int)blockDim.x;
+      const size_t lrow = (size_t)o * a.ncols + x0;
+      const size_t li = lrow + min((int)threadIdx.x & (txs - 1), nc - 1);
+      const float cl0 = a.syn_lam[li];
+      const float2 cly0 = a.syn_lamy ? a.syn_lamy[li] : make_float2(0.0f, 0.0f);
+      auto col_lam = [&](int c) { return percol ? cl0 : a.syn_lam[lrow + c]; };
+      auto col_lamy = [&](int c) { return percol ? cly0 : a.syn_lamy[lrow + c]; };
+      // the previous tile's last stage has read the LDS tile
       __syncthreads();
-      for (int c = threadIdx.x; c < txs; c += (int)blockDim.x) {
-        synL[c] = (c < nc) ? a.syn_lam[(size_t)o * a.ncols + x0 + c] : 0.0f;
-        if (a.syn_lamy) synLy[c] = (c < nc) ? a.syn_lamy[(size_t)o * a.ncols + x0 + c] : make_float2(0.0f, 0.0f);
-      }
-      __syncthreads();
-      auto gen = [&](int c, int e) -> C {
+      // an element of a band plane that the generator does not make itself
+      // (no tables here, or the wavenumber went to the FP64 solve) is read
+      auto band_read = [&](int c, int e) -> bool {
+        if (!a.syn_rows) return true;
+        if (a.syn_noqueue) return false;
+        const float4 h = reinterpret_cast<const float4*>(synT + synB[e] * a.syn_rw)[0];
+        return eig_queued(a.syn_geo, col_lam(c), h.y, h.z, h.w, x0 + c, a.syn_j0 + o, e);
+      };
+      // the reads first, all in flight together; then the generated values
+      auto pre = [&](int c, int e) -> C {
+        if (c < nc && __builtin_isnan(synS[e]) && band_read(c, e)) return base[(size_t)e * a.estride + c];
+        return cmake<C>((T)0, (T)0);
+      };
+      auto gen = [&](int c, int e, C v) -> C {
         if (c >= nc) return cmake<C>((T)0, (T)0);
         const float sv = synS[e];
         if (!__builtin_isnan(sv)) {
-          if (a.syn_lamy) return cmake<C>((T)(synLy[c].x * sv), (T)(synLy[c].y * sv));
-          return cmake<C>((T)(synL[c] * sv), (T)0);
-        }
-        if (a.syn_rows) {
-          // a band plane: the eigen-basis posterior's covariance entry
-          const float lam = synL[c];
-          const float4* tb = reinterpret_cast<const float4*>(synT + synB[e] * a.syn_rw);
-          const float4 h = tb[0];   // S_q, cond(D), l_max, l_min
-          if (a.syn_noqueue || !eig_queued(a.syn_geo, lam, h.y, h.z, h.w, x0 + c, a.syn_j0 + o, e)) {
-            float w[kEigMaxN], gr[kEigMaxN], gi[kEigMaxN];
-#pragma unroll
-            for (int p = 0; p < kEigMaxN; ++p) {
-              const float4 g = p < n ? tb[1 + p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-              w[p] = p < n ? eig_weight(lam, g.x) : 0.0f;
-              gr[p] = g.y;
-              gi[p] = g.z;
-            }
-            const float2 g = eig_cov_entry<kEigMaxN>(n, q < 3, lam * h.x, lam * lam, w, gr, gi);
-            return cmake<C>((T)g.x, (T)g.y);
+          if (a.syn_lamy) {
+            const float2 ly = col_lamy(c);
+            return cmake<C>((T)(ly.x * sv), (T)(ly.y * sv));
           }
+          return cmake<C>((T)(col_lam(c) * sv), (T)0);
         }
-        return base[(size_t)e * a.estride + c];
+        if (band_read(c, e)) return v;
+        // a band plane: the eigen-basis posterior's covariance entry (the
+        // rows are padded to kEigMaxN angles; terms p >= n are discarded)
+        const float lam = col_lam(c);
+        const float4* tb = reinterpret_cast<const float4*>(synT + synB[e] * a.syn_rw);
+        const float4 h = tb[0];   // S_q, cond(D), l_max, l_min
+        float w[kEigMaxN], gr[kEigMaxN], gi[kEigMaxN];
+#pragma unroll
+        for (int p = 0; p < kEigMaxN; ++p) {
+          const float4 g = p < n ? tb[1 + p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          w[p] = eig_weight(lam, g.x);
+          gr[p] = g.y;
+          gi[p] = g.z;
+        }
+        const float2 g = eig_cov_entry<kEigMaxN>(n, q < 3, lam * h.x, lam * lam, w, gr, gi);
+        return cmake<C>((T)g.x, (T)g.y);
       };
       auto store = [&](int c, int e, C v) {
         if (c < nc) base[(size_t)e * a.estride + c] = v;
       };
-      transform_tile<T, INV, false, false, RF, RL, H>(lds, txs, csh, a.f, a.sd, twl, gen, store);
+      if constexpr (RF > 0) {
+        transform_tile_pre<T, INV, RF, RL>(lds, txs, csh, a.f, a.sd, twl, pre, gen, store);
+      } else {
+        auto load = [&](int c, int e) -> C { return gen(c, e, pre(c, e)); };
+        transform_tile<T, INV, false, false, RF, RL, H>(lds, txs, csh, a.f, a.sd, twl, load, store);
+      }
     }
     return;
   }
@@ -782,42 +864,52 @@ template <int RF, int RL, bool W>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThreads))) void k_cov_band_y(
     BandYArgs a) {
   using C = float2;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int TX = 1 << a.cshift;
   C* twl = lds + a.f.L * TX;
-  float4* row = reinterpret_cast<float4*>((reinterpret_cast<uintptr_t>(twl + a.f.twn) + 15) & ~(uintptr_t)15);
+  float4* row = lds_align16<float4>(smem_raw, twl + a.f.twn);
   copy_to_lds<float>(twl, a.tw + a.f.tw0, a.f.twn);
   const int tile = xcd_remap(blockIdx.x, gridDim.x, a.rev);
   const int b = tile / a.ntx, t = tile - b * a.ntx;
   const int k = a.bplanes[b];
   const int q = a.q0 + blockIdx.y;
   const float* rq = a.rows + ((size_t)q * a.nband + b) * a.rw;
-  for (int i = threadIdx.x; i < a.rw; i += (int)blockDim.x) reinterpret_cast<float*>(row)[i] = rq[i];
+  // the row, padded to kEigMaxN angles (the terms p >= n are discarded)
+  for (int i = threadIdx.x; i < 4 + 4 * kEigMaxN; i += (int)blockDim.x)
+    reinterpret_cast<float*>(row)[i] = i < a.rw ? rq[i] : 0.0f;
   __syncthreads();
   const int x0 = t * TX, nc = min(TX, a.ncols - x0);
   C* base = a.data + (size_t)blockIdx.y * a.bstride + (size_t)k * a.pstride + x0;
-  const float4 h = row[0];   // S_q, cond(D), l_max, l_min
-  const int n = a.n;
-  auto gen = [&](int c, int j) -> C {
-    if (c >= nc) return make_float2(0.0f, 0.0f);
-    const float lam = a.lam[(size_t)j * a.ncols + x0 + c];
-    if (!a.noqueue && eig_queued(a.geo, lam, h.y, h.z, h.w, x0 + c, j, k))
-      return base[(size_t)j * a.ncols + c];   // the FP64 solve's value
-    float w[kEigMaxN], gr[kEigMaxN], gi[kEigMaxN];
+  // the plane's row, the same for the whole block, in scalar registers
+  auto uni = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+  const float4 h4 = row[0];
+  const float4 h = make_float4(uni(h4.x), uni(h4.y), uni(h4.z), uni(h4.w));   // S_q, cond(D), l_max, l_min
+  float lp[kEigMaxN], gr[kEigMaxN], gi[kEigMaxN];
 #pragma unroll
-    for (int p = 0; p < kEigMaxN; ++p) {
-      const float4 g = p < n ? row[1 + p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      w[p] = p < n ? eig_weight(lam, g.x) : 0.0f;
-      gr[p] = g.y;
-      gi[p] = g.z;
-    }
-    return eig_cov_entry<kEigMaxN>(n, q < 3, lam * h.x, lam * lam, w, gr, gi);
+  for (int p = 0; p < kEigMaxN; ++p) {
+    const float4 g = row[1 + p];
+    lp[p] = uni(g.x);
+    gr[p] = uni(g.y);
+    gi[p] = uni(g.z);
+  }
+  const int n = a.n;
+  // lam of every element first (columns past the grid's read the last
+  // column's and are zeroed below), then the entries
+  auto pre = [&](int c, int j) -> float { return a.lam[(size_t)j * a.ncols + x0 + min(c, nc - 1)]; };
+  auto gen = [&](int c, int j, float lam) -> C {
+    if (c < nc && !a.noqueue && eig_queued(a.geo, lam, h.y, h.z, h.w, x0 + c, j, k))
+      return base[(size_t)j * a.ncols + c];   // the FP64 solve's value
+    float w[kEigMaxN];
+#pragma unroll
+    for (int p = 0; p < kEigMaxN; ++p) w[p] = eig_weight(lam, lp[p]);
+    const float2 v = eig_cov_entry<kEigMaxN>(n, q < 3, lam * h.x, lam * lam, w, gr, gi);
+    return c < nc ? v : make_float2(0.0f, 0.0f);
   };
   auto store = [&](int c, int j, C v) {
     if (c < nc) base[(size_t)j * a.ncols + c] = v;
   };
-  transform_tile<float, true, false, false, RF, RL, false>(lds, TX, a.cshift, a.f, a.sd, twl, gen, store);
+  transform_tile_pre<float, true, RF, RL>(lds, TX, a.cshift, a.f, a.sd, twl, pre, gen, store);
 }
 
 // ---- z pass with the posterior of the means (crv_invert) ---------------------
@@ -860,7 +952,7 @@ template <int RF, int RL>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kZThreads), amdgpu_waves_per_eu(4))) void k_zpost(
     ZPostArgs a) {
   using C = float2;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int TX = 1 << a.cshift, L = a.f.L, LT = L * TX;
   C* twl = lds + 3 * LT;
@@ -1176,14 +1268,12 @@ int launch_rows(crv_ctx* ctx, FftPlan* p, bool inv, T* dst, const T* src, const 
 }
 
 // LDS of a column tile of TX columns, with the generating z inverse's
-// tables (gen != nullptr: S, lam columns; rows: row indices and band rows)
+// tables (gen != nullptr: S and the planes' row indices; rows: band rows)
 size_t cols_lds_bytes(const Fft1D& f, int tx, size_t elem, const CovGen* gen) {
   size_t lds = ((size_t)f.L * tx + f.twn) * elem;
   if (gen) {
-    lds += ((size_t)f.L + tx) * sizeof(float);
-    if (gen->rows) lds += (size_t)f.L * sizeof(int) + 16 + (size_t)gen->nband * gen->rw * sizeof(float);
-    else lds += (size_t)f.L * sizeof(int) + 16;
-    if (gen->lamy) lds += (size_t)tx * sizeof(float2);
+    lds += (size_t)f.L * (sizeof(float) + sizeof(int)) + 16;
+    if (gen->rows) lds += (size_t)gen->nband * gen->rw * sizeof(float);
   }
   return lds;
 }
@@ -1501,7 +1591,8 @@ bool crv_cov_band_y_fits(const FftPlan* p) {
 
 int crv_launch_cov_band_y(crv_ctx* ctx, FftPlan* p, void* buf, int nbatch, long long bstride, const CovGen& g) {
   const void* kern = crv_cov_band_y_fits(p) ? cov_band_y_kernel(p->fy, p->wide[1]) : nullptr;
-  if (!kern || !g.rows || !g.bplanes || g.nband < 1 || nbatch < 1 || nbatch > 6 || g.rw % 4 != 0)
+  if (!kern || !g.rows || !g.bplanes || g.nband < 1 || nbatch < 1 || nbatch > 6 || g.rw % 4 != 0 ||
+      g.rw > 4 + 4 * kEigMaxN)
     return crv_set_error(ctx, CRV_E_UNSUPPORTED, "covariance band y pass: unsupported plan");
   BandYArgs a{};
   const int cnxp = p->nxp / 2 + 1;
@@ -1513,7 +1604,7 @@ int crv_launch_cov_band_y(crv_ctx* ctx, FftPlan* p, void* buf, int nbatch, long 
   a.nband = g.nband; a.rw = g.rw; a.n = g.n; a.q0 = g.q0; a.noqueue = g.noqueue;
   a.bplanes = g.bplanes; a.lam = g.lam; a.rows = g.rows;
   a.geo = EigQueueGeo{g.rx, g.ry, g.rz, g.lowk2, g.cond_thr, p->nyp, p->nzp};
-  const size_t lds = ((size_t)p->fy.L * p->tx_y + p->fy.twn) * sizeof(float2) + 16 + (size_t)g.rw * sizeof(float);
+  const size_t lds = ((size_t)p->fy.L * p->tx_y + p->fy.twn) * sizeof(float2) + 16 + (4 + 4 * kEigMaxN) * sizeof(float);
   void* args[] = {&a};
   ProfScope ps(ctx, "fft_y_inv_band");
   CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(a.ntx * g.nband, nbatch), dim3(p->wide[1] ? 2 * kThreads : kThreads),
