@@ -44,6 +44,7 @@ CRV_TUNE_CHUNK_MB, CRV_TUNE_TX_Y, CRV_TUNE_TX_Z, CRV_TUNE_ROWS, CRV_TUNE_POST_BL
 CRV_TUNE_BATCH_MB = 6
 CRV_TUNE_ORDER = 7
 CRV_TUNE_ZPOST = 8
+CRV_TUNE_ZINV = 9
 
 
 class GridInfo(C.Structure):

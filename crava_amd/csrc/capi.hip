@@ -482,7 +482,7 @@ extern "C" int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len) {
 
 extern "C" int crv_ctx_tune(crv_ctx* ctx, int key, int value) {
   if (!ctx) return CRV_E_INVALID;
-  if (key < CRV_TUNE_CHUNK_MB || key > CRV_TUNE_ZPOST || value < 0 || (key == CRV_TUNE_ORDER && value > 64))
+  if (key < CRV_TUNE_CHUNK_MB || key > CRV_TUNE_ZINV || value < 0 || (key == CRV_TUNE_ORDER && value > 64))
     return crv_set_error(ctx, CRV_E_INVALID, "crv_ctx_tune: bad key or value");
   CRV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (key == CRV_TUNE_POST_BLOCKS) {

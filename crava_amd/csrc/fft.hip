@@ -37,7 +37,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
+#include <unordered_map>
 
 namespace {
 
@@ -912,6 +914,120 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, W ? 2 * kThreads : kThr
   transform_tile_pre<float, true, RF, RL>(lds, TX, a.cshift, a.f, a.sd, twl, pre, gen, store);
 }
 
+// the element off bytes after p
+__device__ __forceinline__ float2& at_off(float2* p, unsigned off) {
+  return *reinterpret_cast<float2*>(reinterpret_cast<char*>(p) + off);
+}
+
+// ---- z inverse pass with the next tile's loads in flight ---------------------
+// FP32 two-stage z lengths.  Persistent blocks walk the tiles vb =
+// blockIdx.x + i gridDim.x (gridDim.x a multiple of the 8 XCDs).  The
+// elements of a block's next tile are loaded into registers in natural
+// order (element i = e TX + c of the tile in thread i mod blockDim) while
+// its current tile goes through both stages and its stores, so that their
+// latency hides behind that work instead of being waited for once per tile;
+// at the next tile they are written to LDS in the lds[e TX + c] layout and
+// the first stage runs LDS -> LDS.  Each element goes through the same
+// butterflies as in k_cols, so the results are bit-equal.  SYN: the
+// covariances' z inverse after the band's y pass (no band rows): the planes
+// outside the band (S finite) are generated as f(column) S[e], f = lam's y
+// transform (syn_lamy) or lam, and only the band's are read.
+// elements per thread held for the next tile: a 16-column tile over the
+// block size the plan gives it (512 threads above 4096 values); the
+// launcher uses the kernel only for tiles within that
+constexpr int zinv_nat(int L) { return L * 16 > 4096 ? (L * 16 + 511) / 512 : (L * 16 + 255) / 256; }
+
+template <int RF, int RL, bool SYN>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 2 * kThreads), amdgpu_waves_per_eu(4))) void k_zinv_pf(
+    ColArgs<float> a) {
+  using C = float2;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  constexpr int L = RF * RL;
+  C* twl = lds + L * (1 << a.cshift);
+  float* synS = reinterpret_cast<float*>(twl + a.f.twn);
+  copy_to_lds<float>(twl, a.tw + a.f.tw0, a.f.twn);
+  if constexpr (SYN) {
+    const int q = a.syn_q0 + blockIdx.y;
+    for (int e = threadIdx.x; e < L; e += (int)blockDim.x)
+      synS[e] = a.syn_band[e] ? __builtin_nanf("") : a.syn_S[6 * e + q];
+  }
+  __syncthreads();
+  constexpr int E = ept<float, RF, false>();
+  const int NT = (int)blockDim.x;
+  const int ntiles = a.ntx * a.syn_nrows;
+  C* const gbase = a.data + (size_t)blockIdx.y * a.bstride;
+  // byte offsets from the tile's first column fit 32 bits (the launcher
+  // checks the grid's size), so each address is one register over the
+  // block-uniform base
+  const unsigned es8 = (unsigned)a.estride * 8u;
+  struct Tile { C* base; size_t frow; int x0, nc; };
+  auto tile_of = [&](int vb, int csh) {
+    const int tile = xcd_remap(vb, ntiles, a.rev);
+    const int o = tile / a.ntx;
+    Tile t;
+    t.x0 = (tile - o * a.ntx) << csh;
+    t.nc = min(1 << csh, a.ncols - t.x0);
+    t.base = gbase + (size_t)o * a.ostride + t.x0;
+    t.frow = (size_t)o * a.ncols + t.x0;
+    return t;
+  };
+  // the next tile's elements (zero where generated or past the grid's
+  // columns) and, for SYN, the factor of this thread's column c =
+  // threadIdx.x mod TX (the block size is a multiple of TX)
+  constexpr int NAT = zinv_nat(L);
+  C pv[NAT];
+  float2 fv = make_float2(0.0f, 0.0f);
+  auto issue = [&](const Tile& t, int csh) {
+    const int cm = (1 << csh) - 1, lt = L << csh;
+#pragma unroll
+    for (int u = 0; u < NAT; ++u) {
+      const int i = (int)threadIdx.x + u * NT;
+      if (i < lt) {
+        const int e = i >> csh, c = i & cm;
+        const bool rd = c < t.nc && (!SYN || __builtin_isnan(synS[e]));
+        pv[u] = rd ? at_off(t.base, (unsigned)e * es8 + (unsigned)c * 8u) : make_float2(0.0f, 0.0f);
+      }
+    }
+    if constexpr (SYN) {
+      const size_t fi = t.frow + min((int)threadIdx.x & cm, t.nc - 1);
+      fv = a.syn_lamy ? a.syn_lamy[fi] : make_float2(a.syn_lam[fi], 0.0f);
+    }
+  };
+  int vb = blockIdx.x;
+  if (vb < ntiles) issue(tile_of(vb, a.cshift), a.cshift);
+  for (; vb < ntiles; vb += (int)gridDim.x) {
+    // the column shift, opaque in each tile (otherwise the offsets, the same
+    // for every tile, are hoisted out of the loop and held in registers)
+    int csh = a.cshift;
+    asm volatile("" : "+v"(csh));
+    const int txs = 1 << csh, cm = txs - 1, lt = L << csh;
+    const Tile t = tile_of(vb, csh);
+    // the previous tile's last stage has read the LDS tile
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NAT; ++u) {
+      const int i = (int)threadIdx.x + u * NT;
+      if (i < lt) {
+        C v = pv[u];
+        if constexpr (SYN) {
+          const float sv = synS[i >> csh];
+          if ((i & cm) < t.nc && !__builtin_isnan(sv))
+            v = a.syn_lamy ? make_float2(fv.x * sv, fv.y * sv) : make_float2(fv.x * sv, 0.0f);
+        }
+        lds[i] = v;
+      }
+    }
+    if (vb + (int)gridDim.x < ntiles) issue(tile_of(vb + (int)gridDim.x, csh), csh);
+    __syncthreads();
+    stage_mid<float, RF, true, false, E>(lds, txs, csh, L, 1, a.sd.ns[0], twl);
+    auto store = [&](int c, int e, C v) {
+      if (c < t.nc) at_off(t.base, (unsigned)e * es8 + (unsigned)c * 8u) = v;
+    };
+    stage_last<float, RL, true, false, E>(lds, txs, csh, L, a.sd.lr[1], twl + (a.f.tw[1] - a.f.tw0), store);
+  }
+}
+
 // ---- z pass with the posterior of the means (crv_invert) ---------------------
 // One tile is TX consecutive x columns of one y row, whole z pencils, for
 // the n data cubes and the 3 means (FP32, eigen-basis records).  The pencil
@@ -1212,6 +1328,30 @@ const void* cols_kernel_syn(const Fft1D& f, bool half, bool wide) {
   return half ? (const void*)k_cols<float, true, 0, 0, true, true> : (const void*)k_cols<float, true, 0, 0, false, true>;
 }
 
+// the prefetching z inverse kernel of a length, or nullptr where there is
+// none or where it spills registers to scratch (its registers would then
+// wait for the loads it means to keep in flight)
+const void* zinv_pf_kernel(const Fft1D& f, bool syn) {
+  if (f.nst != 2) return nullptr;
+  const void* k = nullptr;
+#define CRV_PAIR_ZPF(A, B)                                                                              \
+  if (!k && f.R[0] == (A) && f.R[1] == (B))                                                              \
+    k = syn ? (const void*)k_zinv_pf<A, B, true> : (const void*)k_zinv_pf<A, B, false>;
+  CRV_FFT_PAIRS(CRV_PAIR_ZPF)
+#undef CRV_PAIR_ZPF
+  if (!k) return nullptr;
+  static std::mutex mu;
+  static std::unordered_map<const void*, bool> spills;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = spills.find(k);
+  if (it == spills.end()) {
+    hipFuncAttributes at{};
+    const bool sp = hipFuncGetAttributes(&at, k) != hipSuccess || at.localSizeBytes > 0;
+    it = spills.emplace(k, sp).first;
+  }
+  return it->second ? nullptr : k;
+}
+
 template <typename T, bool INV>
 const void* cols_kernel(const Fft1D& f, bool half, bool wide) {
   if (f.nst == 2) {
@@ -1308,6 +1448,8 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
   const bool half = p->half[zaxis ? 2 : 1];
   const bool wide = p->wide[zaxis ? 2 : 1];
   const void* kern = inv ? cols_kernel<T, true>(a.f, half, wide) : cols_kernel<T, false>(a.f, half, wide);
+  const int threads = wide ? 2 * kThreads : kThreads;
+  a.syn_nrows = o1 - o0;
   if (syn) {
     // the generated elements of row o0 + o use the tables of rows from o0
     a.syn_lam = zs->lam + (size_t)o0 * cnxp;
@@ -1319,16 +1461,34 @@ int launch_cols(crv_ctx* ctx, FftPlan* p, bool inv, bool zaxis, cplx_t<T>* data,
     a.syn_nband = zs->nband; a.syn_rw = zs->rw; a.syn_n = zs->n; a.syn_j0 = o0; a.syn_noqueue = zs->noqueue;
     a.syn_lamy = zs->lamy ? zs->lamy + (size_t)o0 * cnxp : nullptr;
     a.syn_geo = EigQueueGeo{zs->rx, zs->ry, zs->rz, zs->lowk2, zs->cond_thr, p->nyp, p->nzp};
-    a.syn_nrows = o1 - o0;
-    kern = cols_kernel_syn(a.f, half, wide);
-    // persistent: as many blocks as fit at once (a multiple of 8, the XCDs)
-    const int threads = wide ? 2 * kThreads : kThreads;
+  }
+  // persistent launches: as many blocks as fit at once (a multiple of 8, the
+  // XCDs), at most one per tile
+  auto persistent = [&](const void* k, size_t bytes) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, bytes) != hipSuccess || per_cu < 1)
       per_cu = 1;
     const int want = std::max(8, (per_cu * ctx->num_cu / std::max(1, bt.n) + 7) / 8 * 8);
-    const int gx = std::min(nb, want);
-    CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(gx, bt.n), dim3(threads), args, lds, ctx->stream));
+    return hipLaunchKernel(k, dim3(std::min(nb, want), bt.n), dim3(threads), args, bytes, ctx->stream);
+  };
+  // FP32 two-stage z inverse: the kernel with the next tile's loads in
+  // flight for the covariances' generating pass (without band rows), and
+  // on request for the others, whose reads it measured slower
+  if constexpr (std::is_same<T, float>::value) {
+    const int zt = ctx->tune[CRV_TUNE_ZINV];
+    const size_t lpf = ((size_t)a.f.L * tx + a.f.twn) * sizeof(C) + (syn ? (size_t)a.f.L * sizeof(float) : 0);
+    const void* pf = (zaxis && inv && (zt == 1 || (zt == 0 && syn)) && tx <= threads && lpf <= 65536 &&
+                      (size_t)a.f.L * tx <= (size_t)zinv_nat(a.f.L) * threads &&
+                      (!syn || !zs->rows) && (size_t)a.f.L * a.estride * sizeof(C) < (1ull << 32))
+                         ? zinv_pf_kernel(a.f, syn)
+                         : nullptr;
+    if (pf) {
+      CRV_HIP_CHECK(ctx, persistent(pf, lpf));
+      return CRV_OK;
+    }
+  }
+  if (syn) {
+    CRV_HIP_CHECK(ctx, persistent(cols_kernel_syn(a.f, half, wide), lds));
     return CRV_OK;
   }
   CRV_HIP_CHECK(ctx, hipLaunchKernel(kern, dim3(nb, bt.n), dim3(wide ? 2 * kThreads : kThreads), args, lds,

@@ -83,7 +83,7 @@ struct ProfEvent {
 };
 
 struct crv_ctx {
-  int tune[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // CRV_TUNE_* overrides (0 = default)
+  int tune[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // CRV_TUNE_* overrides (0 = default)
   bool prof_on = false;
   std::vector<ProfEvent> prof;
   int device = 0;

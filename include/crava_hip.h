@@ -120,6 +120,13 @@ int crv_ctx_profile_report(crv_ctx* ctx, char* buf, size_t len);
  *                      (the default), 1 = on with the default tile width,
  *                      2, 4, ... = on with that many x columns per tile    */
 #define CRV_TUNE_ZPOST    8
+/*   CRV_TUNE_ZINV      z inverse passes of FP32 two-stage z lengths as
+ *                      persistent blocks that load a tile's inputs while
+ *                      the previous tile is transformed (results bit-equal):
+ *                      0 = for the covariances' generating z inverse of
+ *                      crv_invert (the default), 1 = for every such z
+ *                      inverse, 2 = off                                     */
+#define CRV_TUNE_ZINV     9
 int crv_ctx_tune(crv_ctx* ctx, int key, int value);
 
 /* Autotuner for one grid size: times the candidate launch shapes of each
