@@ -212,9 +212,13 @@ __device__ __forceinline__ void sincos_2pi_unit(float u, float& sn, float& cs) {
   pc = fmaf(t2, pc, 4.1666667e-2f);                    // 1/4!
   pc = fmaf(t2, pc, -0.5f);
   const float c = fmaf(t2, pc, 1.0f);
-  const int k = (int)q & 3;
-  sn = (k == 0) ? s : (k == 1) ? c : (k == 2) ? -s : -c;
-  cs = (k == 0) ? c : (k == 1) ? -s : (k == 2) ? -c : s;
+  // quarter turn k = q mod 4: (sin, cos) = (s, c), (c, -s), (-s, -c), (-c, s),
+  // by a swap on odd k and sign flips (k & 2 for sin, (k + 1) & 2 for cos)
+  // rather than branches on k, which differs between the lanes of a wave
+  const unsigned k = (unsigned)(int)q;
+  const float a = (k & 1u) ? c : s, b = (k & 1u) ? s : c;
+  sn = __uint_as_float(__float_as_uint(a) ^ ((k & 2u) << 30));
+  cs = __uint_as_float(__float_as_uint(b) ^ (((k + 1u) & 2u) << 30));
 }
 
 // Box-Muller on two 21-bit fields
