@@ -457,7 +457,7 @@ KERNEL_SYMBOLS = {
     "fft_z_fwd": "k_cols<float, false, 20, 16, false, false, true>",
     "fft_y_inv": "k_cols<float, true, 25, 18, false, false, true>",
     "fft_z_inv": "k_cols<float, true, 20, 16, false, false, true>",
-    "fft_z_inv_prior": "k_cols<float, true, 20, 16, false, true, true>",
+    "fft_z_inv_prior": "k_zinv_pf<20, 16, true>",
     "fft_y_inv_band": "k_cov_band_y<25, 18, true>", "fft_z_post": "k_zpost<20, 16>",
     "posterior_fp32": "k_posterior_eig<5, 0>", "posterior_sim_fp32": "k_posterior_eig<5, 2>",
     "simulate_scale_batch": "k_simulate<float, true>",
