@@ -213,10 +213,12 @@ template <typename T, bool INV> struct Dft<T, 6, INV> : DftAB<T, 2, 3, INV> {};
 template <typename T, bool INV> struct Dft<T, 9, INV> : DftAB<T, 3, 3, INV> {};
 template <typename T, bool INV> struct Dft<T, 10, INV> : DftAB<T, 2, 5, INV> {};
 template <typename T, bool INV> struct Dft<T, 12, INV> : DftAB<T, 4, 3, INV> {};
+template <typename T, bool INV> struct Dft<T, 14, INV> : DftAB<T, 2, 7, INV> {};
 template <typename T, bool INV> struct Dft<T, 15, INV> : DftAB<T, 3, 5, INV> {};
 template <typename T, bool INV> struct Dft<T, 16, INV> : DftAB<T, 4, 4, INV> {};
 template <typename T, bool INV> struct Dft<T, 18, INV> : DftAB<T, 2, 9, INV> {};
 template <typename T, bool INV> struct Dft<T, 20, INV> : DftAB<T, 4, 5, INV> {};
+template <typename T, bool INV> struct Dft<T, 21, INV> : DftAB<T, 3, 7, INV> {};
 template <typename T, bool INV> struct Dft<T, 25, INV> : DftAB<T, 5, 5, INV> {};
 
 }  // namespace dft

@@ -329,7 +329,8 @@ __device__ __forceinline__ void stage_single(int cshift, Load load, Store store)
 // radices of the general kernel, whose stage loop dispatches at run time.
 #define CRV_FFT_PAIRS(X)                                                                              \
   X(25, 18) X(15, 15) X(20, 16) X(16, 16) X(16, 8) X(20, 20) X(25, 16) X(18, 16) X(16, 12) X(25, 8)   \
-  X(18, 15) X(18, 12) X(15, 9) X(12, 9) X(25, 10) X(25, 15) X(20, 9) X(15, 10) X(12, 7) X(15, 8)
+  X(18, 15) X(18, 12) X(15, 9) X(12, 9) X(25, 10) X(25, 15) X(20, 9) X(15, 10) X(12, 7) X(15, 8)      \
+  X(15, 14) X(21, 15) X(12, 5) X(7, 6) X(15, 5)
 constexpr int kSmallRadices[] = {8, 4, 2, 5, 3, 7};
 
 // Whole transform of a tile.  load/store move HBM data; with LAST_TO_LDS the

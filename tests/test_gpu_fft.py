@@ -55,7 +55,10 @@ def test_c2r_matches_fftw(ctx, golden, shape, ct):
 
 
 SHAPES = [(64, 64, 128), (450, 6, 10), (343, 7, 5), (14, 49, 25), (1, 1, 1), (2, 3, 5), (30, 1, 7),
-          (96, 84, 150), (160, 240, 100), (256, 20, 200), (7, 9, 343)]
+          (96, 84, 150), (160, 240, 100), (256, 20, 200), (7, 9, 343),
+          # the pairs 12 x 5, 7 x 6, 15 x 5 (x half lengths 60, 42, 75; y 42, 75), 15 x 14 and 21 x 15
+          # (z and y 210, z 315)
+          (120, 42, 210), (84, 75, 315), (150, 210, 7)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
