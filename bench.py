@@ -864,6 +864,8 @@ def main():
     ap.add_argument("--chunk-rows", type=int, default=0, help="crv_invert: y rows per chunk (0: whole grids)")
     ap.add_argument("--order", type=int, default=-1, help="CRV_TUNE_ORDER bit mask of reversed passes")
     ap.add_argument("--rows", type=int, default=0, help="CRV_TUNE_ROWS: x rows per tile of the x passes (0: default)")
+    ap.add_argument("--post-blocks", type=int, default=0,
+                    help="CRV_TUNE_POST_BLOCKS: posterior blocks per CU (0: the occupancy limit)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain 1 GiB device copies (the reference point for HBM counter readings)")
@@ -907,6 +909,8 @@ def main():
         ctx.tune(L.CRV_TUNE_ORDER, a.order + 1)
     if a.rows > 0:
         ctx.tune(L.CRV_TUNE_ROWS, a.rows)
+    if a.post_blocks > 0:
+        ctx.tune(L.CRV_TUNE_POST_BLOCKS, a.post_blocks)
     mode32 = L.CRV_MODE_FP32_REFINE if a.fp32_mode == "refine" else L.CRV_MODE_PLAIN
     for prec, mode in ((32, mode32), (64, L.CRV_MODE_PLAIN)):
         if prec == 64 and a.no_fp64:
