@@ -20,6 +20,20 @@ def test_header_and_binding_agree():
     assert sorted(_lib.EXPORTS) == _declared()
 
 
+def test_header_constants_match_the_binding():
+    """Every CRV_* integer constant the header defines has the same value in
+    the Python binding (tuning keys, modes, grid types, error codes)."""
+    from crava_amd import _lib
+    txt = (ROOT / "include" / "crava_hip.h").read_text()
+    consts = dict(re.findall(r"^#define (CRV_[A-Z0-9_]+)\s+(-?\d+)\s*$", txt, flags=re.M))
+    assert any(k.startswith("CRV_TUNE_") for k in consts)
+    for name, value in consts.items():
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == int(value), name
+    tune = [k for k in consts if k.startswith("CRV_TUNE_")]
+    assert all(hasattr(_lib, k) for k in tune), [k for k in tune if not hasattr(_lib, k)]
+
+
 def test_library_exports_every_symbol():
     from crava_amd import _lib
     lib = ctypes.CDLL(str(_lib.LIBPATH))
