@@ -128,16 +128,18 @@ def synthetic_wells(nwells: int, length: int, dz: float, rng: np.random.Generato
     return np.concatenate(fac), x, np.concatenate(well), np.concatenate(run)
 
 
-def training_inputs(n: int, rng: np.random.Generator) -> np.ndarray:
+def training_inputs(n: int, rng: np.random.Generator, weights=(1.0, 1.0, 1.0)) -> np.ndarray:
     """(n, 5) training inputs: half from the broad box, the other half split
-    evenly over the facies distributions that the sampling step queries,
-    shuffled (a surrogate is trained on the distribution it will see)."""
+    over the facies distributions that the sampling step queries in the
+    proportions `weights` (FACIES_INPUTS order), shuffled (a surrogate is
+    trained on the distribution it will see)."""
     nb = n // 2
-    nf = (n - nb) // len(FACIES_INPUTS)
+    w = np.asarray(weights, float) / float(np.sum(weights))
+    counts = np.floor(w * (n - nb)).astype(int)
+    counts[-1] = n - nb - int(counts[:-1].sum())
     parts = [sample_inputs(nb, rng)]
-    for i, f in enumerate(FACIES_INPUTS):
-        m = nf if i < len(FACIES_INPUTS) - 1 else n - nb - nf * (len(FACIES_INPUTS) - 1)
-        parts.append(sample_inputs(m, rng, facies=f))
+    for f, m in zip(FACIES_INPUTS, counts):
+        parts.append(sample_inputs(int(m), rng, facies=f))
     x = np.concatenate(parts)
     return x[rng.permutation(n)]
 
@@ -248,7 +250,7 @@ class TrainResult:
 
 def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int = 16384, lr: float = 2e-3,
           seed: int = 7, n_val: int = 200_000, params: dict | None = None, log=None,
-          vp_weight: float = 1.0, mix_facies: bool = True) -> TrainResult:
+          vp_weight: float = 1.0, mix_facies: bool = True, facies_weights=(1.0, 1.0, 1.0)) -> TrainResult:
     """Generate n_train + n_val rplib samples on the GPU (crv_rock_forward)
     (training_inputs, or the broad box only without mix_facies) and fit the MLP with Adam, cosine learning-rate decay and reshuffled
     mini-batches.  vp_weight > 1 shrinks the normalisation scale of ln vp,
@@ -258,7 +260,8 @@ def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int =
     import torch
     dev = _dev()
     rng = np.random.default_rng(seed)
-    x = training_inputs(n_train + n_val, rng) if mix_facies else sample_inputs(n_train + n_val, rng)
+    x = (training_inputs(n_train + n_val, rng, facies_weights) if mix_facies
+         else sample_inputs(n_train + n_val, rng))
     d_x = torch.from_numpy(x).to(dev)
     d_y, st = rplib_targets(ctx, d_x, x.shape[0], params)
     keep = (st == 0)
