@@ -65,6 +65,7 @@ constexpr int kKa = 144;         // augmented K of layers 2..4
 constexpr int kK1 = 16;          // augmented K of layer 1
 constexpr int kNo = 32;          // padded output rows
 constexpr int kLd = kKa + 8;     // LDS row stride (bf16) of activations and W2..W4
+static_assert(kK1 == 16 && (kLd * 2) % 16 == 0, "tile rows start at 16-byte boundaries; layer 1 reads 16 columns");
 constexpr int kLd1 = kK1 + 8;    // LDS row stride of W1
 constexpr int kOffW1 = 0, kOffW2 = kHid * kK1, kOffW3 = kOffW2 + kHid * kKa, kOffW4 = kOffW3 + kHid * kKa;
 constexpr int kParams = kOffW4 + kNo * kKa;  // 43520
@@ -166,26 +167,49 @@ __global__ __launch_bounds__(kT) void k_mlp_forward(FwdArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float lsum = 0.0f;
   const long long ntiles = (a.n + kTile - 1) / kTile;
+  // the inputs of sample threadIdx.x of a tile (threads below kTile), loaded
+  // one tile ahead so that their latency hides behind the current tile's
+  // layers
+  float xin[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  auto load_in = [&](long long tile) {
+    const long long gr = tile * kTile + threadIdx.x;
+    if (threadIdx.x < kTile && tile < ntiles && gr < a.n) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) xin[f] = a.x[gr * 5 + f];
+    }
+  };
+  load_in(blockIdx.x);
+  // columns kHid..kKa-1 of the tile rows: the bias column of layers 2..4 (1)
+  // and zero padding.  No layer writes them, so they are set once; layer 1
+  // reads columns 0..kK1-1 only, set for each tile below, and writes 0..kHid-1
+  for (int e = threadIdx.x; e < kTile * (kKa - kHid); e += kT) {
+    const int r = e / (kKa - kHid), c = kHid + e % (kKa - kHid);
+    act[r * kLd + c] = (__bf16)(c == kHid ? 1.0f : 0.0f);
+  }
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long long row0 = tile * kTile;
     __syncthreads();
     if (threadIdx.x < kTile) {
       const int r = threadIdx.x;
       const long long gr = row0 + r;
-      __bf16* ar = act + r * kLd;
+      // the 5 scaled inputs, the layer-1 bias column (1) and zero padding
+      bf16x8 v0, v1;
 #pragma unroll
-      for (int c = 0; c < kKa; ++c) ar[c] = (__bf16)0.0f;
+      for (int c = 0; c < 8; ++c) { v0[c] = (__bf16)0.0f; v1[c] = (__bf16)0.0f; }
       if (gr < a.n) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-          float v = a.x[gr * 5 + f];
+          float v = xin[f];
           if (a.nm.in_log_mask & (1 << f)) v = log10f(v);
-          ar[f] = (__bf16)((v - a.nm.in_lo[f]) * a.nm.in_scale[f] - 1.0f);
+          v0[f] = (__bf16)((v - a.nm.in_lo[f]) * a.nm.in_scale[f] - 1.0f);
         }
       }
-      ar[5] = (__bf16)1.0f;      // layer-1 bias column
-      ar[kHid] = (__bf16)1.0f;   // bias column of layers 2..4 (kept through the layers)
+      v0[5] = (__bf16)1.0f;
+      bf16x8* ar = reinterpret_cast<bf16x8*>(act + r * kLd);   // rows are 16-byte aligned (kLd = 152)
+      ar[0] = v0;
+      ar[1] = v1;
     }
+    load_in(tile + gridDim.x);
     __syncthreads();
     if (a.train) store_rows(a.xa, act, row0, a.n, kK1);
     mlp_layer<1, 2, true>(act, w1, kLd1, nullptr);
