@@ -34,7 +34,9 @@ def _setup(ctx, n, seed):
     return mlp, p, x, t, d_x, d_t, S.norm_dict(nm)
 
 
-@pytest.mark.parametrize("n", [4096, 1000])
+# 70001: more 128-sample tiles than blocks (each block walks several, the
+# next one's inputs loaded ahead) and a partial last tile
+@pytest.mark.parametrize("n", [4096, 1000, 70001])
 def test_forward_matches_oracle(ctx, n):
     import torch
     mlp, p, x, t, d_x, d_t, nd = _setup(ctx, n, 5)
