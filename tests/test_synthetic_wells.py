@@ -81,3 +81,20 @@ def test_injected_uniforms_give_the_reference_formulas():
     # runs: facies 0 of 3 samples, facies 2 of 1, facies 1 of 7 (reaches 11)
     assert list(f) == [0] * 3 + [2] + [1] * 7
     assert list(np.bincount(r)) == [3, 1, 7]
+
+
+def test_training_mix_follows_the_facies_weights():
+    """Half the training inputs come from the broad box, the other half from
+    the facies in proportion to the weights; Sw = 1 exactly marks shale and
+    brine sand (the box draws Sw uniformly), clay >= 0.5 among those marks
+    shale (brine sand's clay is at most 0.25)."""
+    n = 60000
+    x = S.training_inputs(n, np.random.default_rng(4), weights=(3.0, 1.0, 1.0))
+    assert x.shape == (n, 5)
+    sat = x[:, 2] == 1.0
+    assert int(sat.sum()) == 18000 + 6000
+    assert int((sat & (x[:, 1] >= 0.5)).sum()) == 18000
+    # equal weights: the thirds of the original mix
+    y = S.training_inputs(n, np.random.default_rng(4))
+    sat = y[:, 2] == 1.0
+    assert int(sat.sum()) == 20000 and int((sat & (y[:, 1] >= 0.5)).sum()) == 10000
