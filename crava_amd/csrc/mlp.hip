@@ -24,7 +24,8 @@
 //  * k_gemm: tiled 64 x 64 BF16 product with LDS staging, used for the
 //    backward pass: dZ = (dZ' W) * (1 - H^2) and split-K weight gradients
 //    dW = dZ'^T H (partial sums per K slice).
-//  * k_reduce / k_adam: sum the slices, Adam step, BF16 refresh.
+//  * k_reduce / k_reduce_adam: sum the slices (and with it the Adam step and
+//    the BF16 refresh of a training step).
 #include "internal.hpp"
 
 #include <algorithm>
@@ -268,12 +269,13 @@ constexpr int kGT = 256;   // 4 waves, 64 x 64 tile
 constexpr int kGK = 128;   // K per LDS tile: 8 loads of 16 B in flight per thread, 8 MFMA steps per wave
 constexpr int kGLd = kGK + 8;
 
-__global__ __launch_bounds__(kGT) void k_gemm(GemmArgs g) {
+// one 64 x 64 output tile (bx, by) of K slice bz
+__device__ __forceinline__ void gemm_block(const GemmArgs& g, int bx, int by, int bz) {
   __shared__ __bf16 As[64 * kGLd];
   __shared__ __bf16 Bs[64 * kGLd];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
-  const long long kb = (long long)blockIdx.z * g.kchunk;
+  const int m0 = bx * 64, n0 = by * 64;
+  const long long kb = (long long)bz * g.kchunk;
   const long long ke = (g.K < kb + g.kchunk) ? g.K : kb + g.kchunk;
   const int wm = wave & 1, wn = wave >> 1;
   floatx16 acc = floatx16{};
@@ -367,11 +369,27 @@ __global__ __launch_bounds__(kGT) void k_gemm(GemmArgs g) {
     const int row = m0 + 32 * wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
     if (row >= g.M || col >= g.N) continue;
     if (g.mode == 0) {
-      g.P[((size_t)blockIdx.z * g.M + row) * g.N + col] = acc[i];
+      g.P[((size_t)bz * g.M + row) * g.N + col] = acc[i];
     } else {
       const float h = (float)g.H[(size_t)row * g.ldh + col];
       g.C[(size_t)row * g.ldc + col] = (__bf16)(acc[i] * (1.0f - h * h));
     }
+  }
+}
+
+__global__ __launch_bounds__(kGT) void k_gemm(GemmArgs g) { gemm_block(g, blockIdx.x, blockIdx.y, blockIdx.z); }
+
+// Two independent products in one launch: blocks below n0 = gx0 gy0 gz0
+// take tiles of g0, the others tiles of g1 (gx1 x gy1 tiles per K slice)
+__global__ __launch_bounds__(kGT) void k_gemm_pair(GemmArgs g0, int gx0, int gy0, int gz0, GemmArgs g1, int gx1,
+                                                   int gy1) {
+  const int n0 = gx0 * gy0 * gz0;
+  int b = blockIdx.x;
+  if (b < n0) {
+    gemm_block(g0, b % gx0, (b / gx0) % gy0, b / (gx0 * gy0));
+  } else {
+    b -= n0;
+    gemm_block(g1, b % gx1, (b / gx1) % gy1, b / (gx1 * gy1));
   }
 }
 
@@ -395,10 +413,23 @@ __global__ void k_reduce(ReduceArgs a) {
   }
 }
 
-__global__ void k_adam(float* w, float* m, float* v, const float* g, __bf16* wb, float lr_t, float b1, float b2,
-                       float eps) {
+// the reduction of k_reduce and the Adam step (m, v, the bias-corrected
+// step lr_t and the BF16 refresh) in one pass over the parameters
+__global__ void k_reduce_adam(ReduceArgs a, float* w, float* m, float* v, __bf16* wb, float lr_t, float b1, float b2,
+                              float eps) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kParams; p += gridDim.x * blockDim.x) {
-    const float gi = g[p];
+    int l = 0;
+    while (l < 3 && p >= a.s[l + 1].off) ++l;
+    const int q = p - a.s[l].off;
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const int ns = a.s[l].slices;
+    int z = 0;
+    for (; z + 8 <= ns; z += 8)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += a.s[l].P[(size_t)(z + u) * a.s[l].size + q];
+    for (; z < ns; ++z) acc[0] += a.s[l].P[(size_t)z * a.s[l].size + q];
+    const float gi = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    a.g[p] = gi;
     const float mi = b1 * m[p] + (1.0f - b1) * gi;
     const float vi = b2 * v[p] + (1.0f - b2) * gi * gi;
     m[p] = mi;
@@ -438,40 +469,61 @@ int launch_forward(crv_mlp* h, const float* x, const float* t, float* y, long lo
   return CRV_OK;
 }
 
-int launch_gemm(crv_ctx* ctx, GemmArgs g, int slices, const char* name) {
-  dim3 grid((g.M + 63) / 64, (g.N + 63) / 64, slices);
+// a product with its grid: 64 x 64 output tiles, `slices` K slices
+struct GemmLaunch {
+  GemmArgs g;
+  int gx, gy, gz;
+};
+
+GemmLaunch make_gemm(GemmArgs g, int slices) {
   auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
   g.vec = al16(g.A) && al16(g.B) && g.lda % 8 == 0 && g.ldb % 8 == 0;
+  return GemmLaunch{g, (g.M + 63) / 64, (g.N + 63) / 64, slices};
+}
+
+int launch_gemm(crv_ctx* ctx, const GemmLaunch& l, const char* name) {
   ProfScope ps(ctx, name);
-  hipLaunchKernelGGL(k_gemm, grid, dim3(kGT), 0, ctx->stream, g);
+  hipLaunchKernelGGL(k_gemm, dim3(l.gx, l.gy, l.gz), dim3(kGT), 0, ctx->stream, l.g);
+  CRV_HIP_CHECK(ctx, hipGetLastError());
+  return CRV_OK;
+}
+
+// two independent products in one launch
+int launch_gemm_pair(crv_ctx* ctx, const GemmLaunch& a, const GemmLaunch& b, const char* name) {
+  const int nb = a.gx * a.gy * a.gz + b.gx * b.gy * b.gz;
+  ProfScope ps(ctx, name);
+  hipLaunchKernelGGL(k_gemm_pair, dim3(nb), dim3(kGT), 0, ctx->stream, a.g, a.gx, a.gy, a.gz, b.g, b.gx, b.gy);
   CRV_HIP_CHECK(ctx, hipGetLastError());
   return CRV_OK;
 }
 
 // dW[M=out][N=in] partials = dZ^T (rows x out) . H (rows x in)
-int weight_grad(crv_mlp* h, const __bf16* dz, int ldz, int nout, const __bf16* hin, int ldh, int nin, long long n,
-                float* P, const char* name) {
+GemmLaunch weight_grad(const __bf16* dz, int ldz, int nout, const __bf16* hin, int ldh, int nin, long long n,
+                       float* P) {
   GemmArgs g{};
   g.A = dz; g.lda = ldz; g.a_trans = 1;
   g.B = hin; g.ldb = ldh; g.b_rowk = 1;
   g.M = nout; g.N = nin; g.K = n; g.kchunk = kKChunk; g.mode = 0; g.P = P;
-  const int slices = (int)((n + kKChunk - 1) / kKChunk);
-  return launch_gemm(h->ctx, g, slices, name);
+  return make_gemm(g, (int)((n + kKChunk - 1) / kKChunk));
 }
 
 // dZ_prev[rows][128] = (dZ . W[:, :128]) * (1 - H_prev^2)
-int back_prop(crv_mlp* h, const __bf16* dz, int ldz, int kdim, const __bf16* w, int ldw, const __bf16* hprev,
-              __bf16* out, long long n, const char* name) {
+GemmLaunch back_prop(const __bf16* dz, int ldz, int kdim, const __bf16* w, int ldw, const __bf16* hprev, __bf16* out,
+                     long long n) {
   GemmArgs g{};
   g.A = dz; g.lda = ldz; g.a_trans = 0;
   g.B = w; g.ldb = ldw; g.b_rowk = 1;
   g.M = (int)n; g.N = kHid; g.K = kdim; g.kchunk = kdim; g.mode = 1;
   g.C = out; g.ldc = kHid; g.H = hprev; g.ldh = kKa;
-  return launch_gemm(h->ctx, g, 1, name);
+  return make_gemm(g, 1);
 }
 
-// forward (training) + backward; leaves the summed gradient in h->g
-int gradients(crv_mlp* h, const float* x, const float* t, long long n) {
+// forward (training) + backward products: the per-slice partial gradients,
+// and in r what sums them.  Each layer's weight gradient and the next
+// layer's back-propagation share a launch (they read the same dZ and write
+// different arrays), so the chain is the forward pass, three paired
+// launches and the first layer's weight gradient.
+int partials(crv_mlp* h, const float* x, const float* t, long long n, ReduceArgs& r) {
   crv_ctx* ctx = h->ctx;
   if (n < 1 || n > h->max_batch) return crv_set_error(ctx, CRV_E_INVALID, "mlp: batch size out of range");
   CRV_HIP_CHECK(ctx, hipMemsetAsync(h->loss, 0, sizeof(float), ctx->stream));
@@ -485,29 +537,41 @@ int gradients(crv_mlp* h, const float* x, const float* t, long long n) {
   const __bf16* W2 = h->wb + kOffW2;
   const __bf16* W3 = h->wb + kOffW3;
   const __bf16* W4 = h->wb + kOffW4;
-  if ((rc = weight_grad(h, h->dy, kNo, kNo, h->h[2], kKa, kKa, n, P4, "mlp_dW4")) != CRV_OK) return rc;
-  if ((rc = back_prop(h, h->dy, kNo, kNo, W4, kKa, h->h[2], h->dz[0], n, "mlp_dZ3")) != CRV_OK) return rc;
-  if ((rc = weight_grad(h, h->dz[0], kHid, kHid, h->h[1], kKa, kKa, n, P3, "mlp_dW3")) != CRV_OK) return rc;
-  if ((rc = back_prop(h, h->dz[0], kHid, kHid, W3, kKa, h->h[1], h->dz[1], n, "mlp_dZ2")) != CRV_OK) return rc;
-  if ((rc = weight_grad(h, h->dz[1], kHid, kHid, h->h[0], kKa, kKa, n, P2, "mlp_dW2")) != CRV_OK) return rc;
-  if ((rc = back_prop(h, h->dz[1], kHid, kHid, W2, kKa, h->h[0], h->dz[0], n, "mlp_dZ1")) != CRV_OK) return rc;
-  if ((rc = weight_grad(h, h->dz[0], kHid, kHid, h->xa, kK1, kK1, n, P1, "mlp_dW1")) != CRV_OK) return rc;
-  ReduceArgs r{};
+  if ((rc = launch_gemm_pair(ctx, weight_grad(h->dy, kNo, kNo, h->h[2], kKa, kKa, n, P4),
+                              back_prop(h->dy, kNo, kNo, W4, kKa, h->h[2], h->dz[0], n), "mlp_dW4_dZ3")) != CRV_OK)
+    return rc;
+  if ((rc = launch_gemm_pair(ctx, weight_grad(h->dz[0], kHid, kHid, h->h[1], kKa, kKa, n, P3),
+                              back_prop(h->dz[0], kHid, kHid, W3, kKa, h->h[1], h->dz[1], n), "mlp_dW3_dZ2")) !=
+      CRV_OK)
+    return rc;
+  if ((rc = launch_gemm_pair(ctx, weight_grad(h->dz[1], kHid, kHid, h->h[0], kKa, kKa, n, P2),
+                              back_prop(h->dz[1], kHid, kHid, W2, kKa, h->h[0], h->dz[0], n), "mlp_dW2_dZ1")) !=
+      CRV_OK)
+    return rc;
+  if ((rc = launch_gemm(ctx, weight_grad(h->dz[0], kHid, kHid, h->xa, kK1, kK1, n, P1), "mlp_dW1")) != CRV_OK)
+    return rc;
+  r = ReduceArgs{};
   r.s[0] = Seg{kOffW1, kHid * kK1, slices, P1};
   r.s[1] = Seg{kOffW2, kHid * kKa, slices, P2};
   r.s[2] = Seg{kOffW3, kHid * kKa, slices, P3};
   r.s[3] = Seg{kOffW4, kNo * kKa, slices, P4};
   r.g = h->g;
-  {
-    ProfScope ps(ctx, "mlp_reduce");
-    hipLaunchKernelGGL(k_reduce, dim3(85), dim3(512), 0, ctx->stream, r);
-    CRV_HIP_CHECK(ctx, hipGetLastError());
-  }
   const double rows = (double)slices * kKChunk;
   // padded MFMA work: weight gradients over every row, back-propagation
   // products over 64-row tiles
   h->flops_bwd = 2.0 * rows * ((double)kNo * kKa + 2.0 * kHid * kKa + kHid * kK1) +
                  2.0 * (double)((n + 63) / 64 * 64) * ((double)kNo * kHid + 2.0 * kHid * kHid);
+  return CRV_OK;
+}
+
+// the summed gradient in h->g
+int gradients(crv_mlp* h, const float* x, const float* t, long long n) {
+  ReduceArgs r;
+  int rc = partials(h, x, t, n, r);
+  if (rc != CRV_OK) return rc;
+  ProfScope ps(h->ctx, "mlp_reduce");
+  hipLaunchKernelGGL(k_reduce, dim3(85), dim3(512), 0, h->ctx->stream, r);
+  CRV_HIP_CHECK(h->ctx, hipGetLastError());
   return CRV_OK;
 }
 
@@ -600,14 +664,15 @@ int crv_mlp_train_step(crv_mlp* h, const float* d_x, const float* d_t, long long
                        float* loss) {
   if (!h || !d_x || !d_t || step < 1) return CRV_E_INVALID;
   crv_ctx* ctx = h->ctx;
-  int rc = gradients(h, d_x, d_t, n);
+  ReduceArgs r;
+  int rc = partials(h, d_x, d_t, n, r);
   if (rc != CRV_OK) return rc;
   const double b1 = 0.9, b2 = 0.999;
   const float lr_t = (float)(lr * std::sqrt(1.0 - std::pow(b2, step)) / (1.0 - std::pow(b1, step)));
   {
-    ProfScope ps(ctx, "mlp_adam");
-    hipLaunchKernelGGL(k_adam, dim3(85), dim3(512), 0, ctx->stream, h->w, h->m, h->v, (const float*)h->g, h->wb,
-                       lr_t, (float)b1, (float)b2, 1e-8f);
+    ProfScope ps(ctx, "mlp_reduce_adam");
+    hipLaunchKernelGGL(k_reduce_adam, dim3(85), dim3(512), 0, ctx->stream, r, h->w, h->m, h->v, h->wb, lr_t,
+                       (float)b1, (float)b2, 1e-8f);
     CRV_HIP_CHECK(ctx, hipGetLastError());
   }
   if (loss) {

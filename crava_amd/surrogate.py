@@ -308,7 +308,7 @@ def train(ctx: Context, n_train: int = 1_000_000, epochs: int = 40, batch: int =
     ctx.profile(False)
     rep = ctx.profile_report()
     res.step_kernels_ms = {k: v[1] for k, v in rep.items()}
-    ms = sum(v[1] for k, v in rep.items() if k.startswith("mlp_") and k not in ("mlp_reduce", "mlp_adam"))
+    ms = sum(v[1] for k, v in rep.items() if k.startswith("mlp_") and k not in ("mlp_reduce", "mlp_reduce_adam"))
     f, b_ = mlp.flops()
     if ms > 0:
         res.mfma_tflops = (f + b_) / (ms * 1e-3) / 1e12
