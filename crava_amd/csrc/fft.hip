@@ -97,6 +97,16 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks, int rev) {
   return x * q + min(x, r) + (rev ? cnt - 1 - i : i);
 }
 
+// Two consecutive reals (p aligned to their pair) in one streaming load: two
+// scalar loads of p and p + 1 in a kernel that also reads single reals on
+// another branch are not merged by the compiler.
+template <typename T>
+__device__ __forceinline__ cplx_t<T> load_pair_nt(const T* p) {
+  typedef T pair_t __attribute__((ext_vector_type(2)));
+  const pair_t r = __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(p));
+  return cmake<cplx_t<T>>(r.x, r.y);
+}
+
 template <typename T>
 __device__ __forceinline__ cplx_t<T> twiddle(cplx_t<T> a, cplx_t<T> w, bool inv) {
   return inv ? cmulc(a, w) : cmul(a, w);
@@ -483,8 +493,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
         // displacing the spectrum this pass writes (the next pass's input)
         // in the memory-side cache
         if (a.xeven) {
-          const T* p2 = row + 2 * e;
-          v[q] = cmake<C>(__builtin_nontemporal_load(p2), __builtin_nontemporal_load(p2 + 1));
+          v[q] = load_pair_nt(row + 2 * e);
         } else {
           v[q] = cmake<C>(__builtin_nontemporal_load(row + e), (T)0);
         }
