@@ -514,24 +514,44 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(RowArgs<T> a) {
   __syncthreads();
   transform_tile_lds<T, false, RF, RL, H>(lds, S, a.pshift, a.f, a.sd, twl);
   __syncthreads();
-  // post-process and store cnxp complex per row
-  for (int idx = threadIdx.x; idx < P * K; idx += kThreads) {
-    const int c = fdiv(idx, a.divK), k = idx - c * K;
-    if (c >= nr) continue;
-    C X;
-    if (a.xeven) {
-      const int M = L;
-      C za = lds[(k == M ? 0 : k) * S + c];
-      C zb = cconj(lds[(k == 0 ? 0 : M - k) * S + c]);
-      C d = csub(za, zb);
-      C t = cmul(wpl[k], d);
-      // X = 0.5 (za + zb) - 0.5 i t
-      X = cmake<C>((T)0.5 * (za.x + zb.x + t.y), (T)0.5 * (za.y + zb.y - t.x));
-    } else {
-      X = lds[k * S + c];
+  // post-process and store cnxp complex per row.  Even lengths: each thread
+  // reads the LDS values of four of its elements before combining and
+  // storing them (clamped indexes, which stay inside the tile, keep the
+  // reads free of branches so that they are in flight together).
+  const int PK = P * K;
+  if (a.xeven) {
+    const int M = L;
+    constexpr int U = 4;
+    for (int i0 = threadIdx.x; i0 < PK; i0 += U * kThreads) {
+      C za[U], zb[U], w[U];
+      int cu[U], ku[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = min(i0 + u * kThreads, PK - 1);
+        const int c = fdiv(idx, a.divK), k = idx - c * K;
+        cu[u] = c;
+        ku[u] = k;
+        za[u] = lds[(k == M ? 0 : k) * S + c];
+        zb[u] = lds[(k == 0 ? 0 : M - k) * S + c];
+        w[u] = wpl[k];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (i0 + u * kThreads >= PK || cu[u] >= nr) continue;
+        const C zc = cconj(zb[u]);
+        const C d = csub(za[u], zc);
+        const C t = cmul(w[u], d);
+        // X = 0.5 (za + zb) - 0.5 i t
+        const C X = cmake<C>((T)0.5 * (za[u].x + zc.x + t.y), (T)0.5 * (za[u].y + zc.y - t.x));
+        reinterpret_cast<C*>(dst0 + (size_t)(row0 + cu[u]) * a.rnxp)[ku[u]] = X;
+      }
     }
-    C* out = reinterpret_cast<C*>(dst0 + (size_t)(row0 + c) * a.rnxp);
-    out[k] = X;
+  } else {
+    for (int idx = threadIdx.x; idx < PK; idx += kThreads) {
+      const int c = fdiv(idx, a.divK), k = idx - c * K;
+      if (c >= nr) continue;
+      reinterpret_cast<C*>(dst0 + (size_t)(row0 + c) * a.rnxp)[k] = lds[k * S + c];
+    }
   }
 }
 
